@@ -1,0 +1,135 @@
+/*
+ * rsx_capi.h -- C ABI of the MI355X Debye-SAXS engine (libresaxs_amd.so).
+ *
+ * Plain pointers and sizes only.  Every function returns an int status:
+ * 0 on success, or one of the negative codes below (the first six are the
+ * reference's resaxs::error codes, resaxslib/include/resaxs.hpp:213-221,
+ * with the same values).  rsx_last_error() gives the message of the last
+ * failure on an engine.
+ *
+ * Bodies are 16-byte {x, y, z, w} records (FP32) with the atom-type index
+ * stored in the bit pattern of w, as the reference's cl_float4 bodies
+ * (resaxslib/include/utils.hpp:61-88).  Factor tables are row-major
+ * [n_q][n_types] (resaxslib/include/ff_coef.hpp:129-132).
+ *
+ * Threading: one engine owns one HIP stream; calls on an engine are
+ * synchronous unless named *_async, and an engine must not be used from two
+ * host threads at once (the reference's queue-per-algorithm rule).
+ */
+#ifndef RSX_CAPI_H
+#define RSX_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define RSX_OK                    0
+#define RSX_INVALID_DEVICE       -1   /* resaxs::error::SAXS_INVALID_DEVICE */
+#define RSX_ALG_NOT_INITIALIZED  -2   /* SAXS_ALG_NOT_INITIALIZED */
+#define RSX_INVALID_ARG          -3   /* SAXS_INVALID_ARG */
+#define RSX_PROGRAM_NOT_LOADED   -4   /* SAXS_PROGRAM_NOT_LOADED */
+#define RSX_ARGS_NOT_SET         -5   /* SAXS_ARGS_NOT_SET */
+#define RSX_NO_INITIAL_CALC      -6   /* SAXS_NO_INITIAL_CALC */
+#define RSX_HIP_ERROR          -100   /* a HIP runtime call failed */
+#define RSX_OUT_OF_MEMORY      -101
+#define RSX_NOT_SUPPORTED      -102
+
+/* dot-sphere kinds for SASA */
+#define RSX_DOTS_DODECA_ICOSA     0   /* 32 dots, rsasa_algorithm.cpp:157-208 (reference GPU default) */
+#define RSX_DOTS_FIBONACCI        1   /* 2n dots, rsasa_algorithm.cpp:96-116 */
+
+typedef struct rsx_engine rsx_engine;
+
+/* ---- engine lifetime (replaces cl_base<FLT>::initialize, re_cl_algorithm_base.cpp:38-46) ---- */
+int rsx_device_count(int *count);
+int rsx_create(int device, rsx_engine **out);
+int rsx_destroy(rsx_engine *h);
+const char *rsx_last_error(const rsx_engine *h);
+/* HIP stream of the engine (hipStream_t), for callers that order their own work with it */
+void *rsx_stream(rsx_engine *h);
+
+/* ---- parameters (saxs_params / form_factor_params / implicit_water_params,
+ *      resaxslib/include/saxs_algorithm.hpp:33-132) ---- */
+int rsx_set_q(rsx_engine *h, const float *q, int n_q);
+/* table [n_q][n_types]; water_index < 0: no hydration factor */
+int rsx_set_factors(rsx_engine *h, const float *table, int n_types, int water_index);
+int rsx_set_water_weight(rsx_engine *h, float w);
+/* skip the SASA recomputation on recalc once one exists (implicit_water_params::no_rsasa_recalc_) */
+int rsx_set_no_rsasa_recalc(rsx_engine *h, int flag);
+/* SASA set-up used for hydration: per-type radii, probe radius, dot sphere.
+ * kind RSX_DOTS_FIBONACCI uses n_dots = 2*n_half; RSX_DOTS_DODECA_ICOSA ignores n_half. */
+int rsx_set_sasa_params(rsx_engine *h, const float *radii, int n_types, float probe, int kind, int n_half);
+/* full-recompute period of the running incremental I(q) (0 = never) */
+int rsx_set_rebaseline(rsx_engine *h, int every_n_recalcs);
+/* size the engine for n_bodies (saxs_cl_gpu::initialize + set_args, saxs_alg_cl_gpu_pt_wf.cpp:176-263) */
+int rsx_initialize(rsx_engine *h, int n_bodies);
+
+/* ---- Debye I(q) (i_saxs, saxs_algorithm.hpp:147-183) ---- */
+/* saxs_cl_gpu::calc_curve, saxs_alg_cl_gpu_pt_wf.cpp:414-458 */
+int rsx_calc_curve(rsx_engine *h, const float *bodies, int n, float *out_iq);
+int rsx_calc_curve_f64(rsx_engine *h, const float *bodies, int n, double *out_iq);
+/* saxs_cl_gpu::recalc_curve, saxs_alg_cl_gpu_pt_wf.cpp:468-577: bodies[start, start+len) moved */
+int rsx_recalc_curve(rsx_engine *h, const float *bodies, int n, int start, int len, float *out_iq);
+int rsx_recalc_curve_f64(rsx_engine *h, const float *bodies, int n, int start, int len, double *out_iq);
+/* saxs_cl_gpu::commit / revert, saxs_alg_cl_gpu_pt_wf.cpp:592-620 */
+int rsx_commit(rsx_engine *h);
+int rsx_revert(rsx_engine *h);
+/* exposed fraction per body from the last hydration update (implicit_water_params::access_rsasa) */
+int rsx_get_rsasa(rsx_engine *h, float *out, int n);
+/* engine state: 0 blank, 1 initialized, 2 program loaded, 3 args set, 4 computing (re_algorithm.hpp:34-54) */
+int rsx_state(const rsx_engine *h);
+
+/* ---- device-resident variants (inputs already in HBM; no host copies) ---- */
+/* bodies_dev: device pointer to n float4 bodies in the caller's order */
+int rsx_calc_curve_dev_async(rsx_engine *h, const void *bodies_dev, int n, double *out_dev);
+int rsx_synchronize(rsx_engine *h);
+
+/* ---- SASA (i_rsasa::calc_rsasa, rsasa_algorithm.hpp:42-61; kernel rsasa_dot_spheres.cl:94-163) ---- */
+/* radii per type, dots [n_types][n_dots][4] templates relative to the atom centre;
+ * out_counts exposed dots per body, out_frac = count / n_dots (either may be NULL) */
+int rsx_calc_rsasa(rsx_engine *h, const float *bodies, int n, const float *radii, int n_types,
+                   float probe, const float *dots, int n_dots, unsigned *out_counts, float *out_frac);
+
+/* ---- host helpers (no device work) ---- */
+/* rsasa_algorithm.cpp:96-116: 2*n_half dots {x,y,z,radius} on a sphere of radius+probe */
+int rsx_fibonacci_dots(int n_half, float radius, float probe, float *out);
+/* rsasa_algorithm.cpp:157-208: 32 dots {x,y,z,radius} */
+int rsx_dodeca_icosa_dots(float radius, float probe, float *out);
+/* ff_coef.hpp:112-211: default table [n_q][16] and the 16 type radii */
+int rsx_atomic_form_factors(const float *q, int n_q, float avg_radius, float expansion, float *out);
+int rsx_atomic_radii(float *out16);
+
+/* ---- batched ensemble (no reference counterpart; calc_saxs/calc_saxs.hpp:611-624 loops frames) ---- */
+/* frames: bodies concatenated, frame f is [offsets[f], offsets[f+1]); uploads and lays them out */
+int rsx_batch_set_frames(rsx_engine *h, const float *bodies, const int *offsets, int n_frames);
+/* per-frame I(q) [n_frames][n_q] and the uniform-weight mean [n_q]; either may be NULL */
+int rsx_batch_calc(rsx_engine *h, double *out_frames, double *out_mean);
+/* same on resident frames, results left on the device (no host copy); graph-captured */
+int rsx_batch_calc_dev_async(rsx_engine *h, double *out_frames_dev, double *out_mean_dev);
+
+/* ---- differentiable fit (no reference counterpart; uses saxs_profile.hpp:137-166 scale/chi) ---- */
+/* chi2 = sum_k (I_exp - c I)^2 / sigma^2 at the least-squares c; grad = dchi2/dx [n][3] */
+int rsx_chi2_grad(rsx_engine *h, const float *bodies, int n, const float *i_exp, const float *sigma,
+                  double *out_chi2, double *out_scale, float *out_grad, double *out_iq);
+/* adjoint only: grad_i = sum_k w_k dI_k/dx_i for caller weights w [n_q] (backward of I(q)) */
+int rsx_debye_vjp(rsx_engine *h, const float *bodies, int n, const double *w, float *out_grad);
+
+/* ---- tuning and measurement ---- */
+/* choose q-per-thread and column-range length for the current n_q / size; cache in a text file */
+int rsx_autotune(rsx_engine *h, const char *cache_path);
+/* tuning knobs: k (q per thread, 0 = automatic), col_range (atoms per job, 0 = automatic) */
+int rsx_set_tuning(rsx_engine *h, int k, int col_range);
+/* average device time (ms) of the last timed region, measured with HIP events on the engine stream */
+int rsx_time_forward(rsx_engine *h, int reps, float *ms_total, float *ms_pair_kernel);
+/* device memory helpers for callers without another allocator */
+int rsx_dev_alloc(rsx_engine *h, void **ptr, long long bytes);
+int rsx_dev_free(rsx_engine *h, void *ptr);
+int rsx_memcpy_h2d(rsx_engine *h, void *dst, const void *src, long long bytes);
+int rsx_memcpy_d2h(rsx_engine *h, void *dst, const void *src, long long bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RSX_CAPI_H */

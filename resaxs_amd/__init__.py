@@ -1,0 +1,8 @@
+"""MI355X-native Debye-SAXS engine: hand-written gfx950 HIP kernels behind a C ABI.
+
+The compute path lives in libresaxs_amd.so (resaxs_amd/csrc); this package is
+its Python handle (ctypes) plus seeded workload generators.
+"""
+from ._lib import RsxError, RSX_DOTS_DODECA_ICOSA, RSX_DOTS_FIBONACCI  # noqa: F401
+from .engine import (Engine, atomic_form_factors, atomic_radii, device_count, dodeca_icosa_dots,  # noqa: F401
+                     fibonacci_dots, make_bodies)

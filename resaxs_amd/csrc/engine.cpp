@@ -1,0 +1,1672 @@
+// Host engine of the MI355X Debye-SAXS library and its C ABI (include/rsx_capi.h).
+//
+// Device layout (HBM):
+//   cols      float4 [n]      atoms in "slot" order: grouped by form-factor class,
+//                              {x, y, z, class bits}; the pair kernels read it both
+//                              as rows (lanes) and as columns (scalar loads)
+//   ftab      float  [n_class][ldf]  F_class(q_k), zero padded to ldf = n_chunks*K
+//   partials  double [n_slots][ldf]  one row per pair job
+//   iq        double [n_q]    running I(q) (FP64)
+// Classes: without hydration class = type; with hydration
+// class = type * (n_dots + 1) + exposed-dot count, so F_class(q) =
+// f_type(q) + (count / n_dots) * w * f_water(q) exactly as
+// resaxslib/src/gpu_saxs_pt_wf.cl:130-138 composes it.
+
+#include "kernels.h"
+#include "../../include/rsx_capi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+using rsx::ChunkConst;
+using rsx::CLS_SKIP;
+using rsx::PairJob;
+
+constexpr double PI = 3.14159265358979323846;
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& msg)
+{
+    throw Fail{code, msg};
+}
+
+void hipchk(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) fail(RSX_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    void ensure(size_t n)
+    {
+        if (n <= cap && p) return;
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
+        if (e != hipSuccess) fail(RSX_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+        cap = n;
+    }
+    void release()
+    {
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+inline uint32_t fbits(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+inline float bitsf(uint32_t u)
+{
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// ---------------------------------------------------------------------------
+// host-side generators (dot spheres and the default atomic factor table)
+// ---------------------------------------------------------------------------
+
+// rsasa_algorithm.cpp:96-116, float arithmetic of the reference's float build
+void fibonacci_dots(int nhalf, float radius, float probe, float* out)
+{
+    const float phi = (1.0f + std::sqrt(5.0f)) / 2.0f;
+    const float ext = radius + probe;
+    int d = 0;
+    for (int i = -(nhalf - 1); i <= nhalf; ++i, ++d) {
+        const float theta = 2.0f * (float)PI * (float)i / phi;
+        const float sphi = (float)i / (float)nhalf;
+        const float cphi = (float)(std::sqrt((double)((nhalf + i) * (nhalf - i))) / nhalf);
+        out[4 * d + 0] = cphi * std::sin(theta) * ext;
+        out[4 * d + 1] = cphi * std::cos(theta) * ext;
+        out[4 * d + 2] = sphi * ext;
+        out[4 * d + 3] = radius;
+    }
+}
+
+// rsasa_algorithm.cpp:157-208: dodecahedron (20) + icosahedron (12) vertices
+void dodeca_icosa_dots(float radius, float probe, float* out)
+{
+    const float ext = radius + probe;
+    const float phi = (1.0f + std::sqrt(5.0f)) / 2.0f;
+    const float v1 = (1.0f / std::sqrt(3.0f)) * ext;
+    const float v2 = (1.0f / phi) * v1, v3 = phi * v1;
+    const double cc = (double)phi * std::cos(3.0 * PI / 10.0);
+    const float ico = (float)((double)((std::sqrt(5.0f) + 5.0f) / 10.0f) / std::sqrt(3.0 - 1.0 / std::pow(cc, 2.0)) *
+                              (double)ext);
+    const float a = ico, b = phi * ico;
+    const float sgn[2] = {1.0f, -1.0f};
+    int d = 0;
+    auto put = [&](float x, float y, float z) {
+        out[4 * d + 0] = x;
+        out[4 * d + 1] = y;
+        out[4 * d + 2] = z;
+        out[4 * d + 3] = radius;
+        ++d;
+    };
+    for (float sx : sgn)
+        for (float sy : sgn)
+            for (float sz : sgn) put(sx * v1, sy * v1, sz * v1);
+    for (float sy : sgn)
+        for (float sz : sgn) put(0.0f, sy * v2, sz * v3);
+    for (float sx : sgn)
+        for (float sy : sgn) put(sx * v2, sy * v3, 0.0f);
+    for (float sx : sgn)
+        for (float sz : sgn) put(sx * v3, 0.0f, sz * v2);
+    for (float sx : sgn)
+        for (float sz : sgn) put(sx * a, 0.0f, sz * b);
+    for (float sy : sgn)
+        for (float sz : sgn) put(0.0f, sy * b, sz * a);
+    for (float sx : sgn)
+        for (float sy : sgn) put(sx * b, sy * a, 0.0f);
+}
+
+// ff_coef.hpp:39-48 Cromer-Mann a1..a4, c, b1..b4 (H, C, N, O, P, S, Zn); the
+// excluded volumes and complex types of constants.hpp:61-81.  Tabulated
+// physical constants, kept identical to the reference's.
+const float kCM[7][9] = {
+    {0.4930020f, 0.3229120f, 0.1401910f, 4.0810000E-02f, 3.0380001E-03f, 10.51090f, 26.12570f, 3.142360f, 57.79970f},
+    {2.310000f, 1.020000f, 1.588600f, 0.8650000f, 0.2156000f, 20.84390f, 10.20750f, 0.5687000f, 51.65120f},
+    {12.21260f, 3.132200f, 2.012500f, 1.166300f, -11.52900f, 5.7000001E-03f, 9.893300f, 28.99750f, 0.5826000f},
+    {3.048500f, 2.286800f, 1.546300f, 0.8670000f, 0.2508000f, 13.27710f, 5.701100f, 0.3239000f, 32.90890f},
+    {6.434500f, 4.179100f, 1.780000f, 1.490800f, 1.114900f, 1.906700f, 27.15700f, 0.5260000f, 68.16450f},
+    {6.905300f, 5.203400f, 1.437900f, 1.586300f, 0.8669000f, 1.467900f, 22.21510f, 0.2536000f, 56.17200f},
+    {14.07430f, 7.031800f, 5.165200f, 2.410000f, 1.304100f, 3.265500f, 0.2333000f, 10.31630f, 58.70970f}};
+const float kExclVol[7] = {5.15f, 16.44f, 2.49f, 9.13f, 5.73f, 19.86f, 9.85f};
+const int kComplex[9][2] = {{1, 1}, {1, 2}, {1, 3}, {2, 1}, {2, 2}, {2, 3}, {3, 1}, {3, 2}, {5, 1}};
+constexpr float kRho0 = 0.334f;
+
+float excl_ff(int t, float k_sq)
+{
+    return kRho0 * kExclVol[t] * std::exp(-(float)PI * std::pow(kExclVol[t], 2.0f / 3.0f) * k_sq);
+}
+
+void atomic_radii(float* out16)
+{
+    float fe[16];
+    for (int t = 0; t < 7; ++t) fe[t] = excl_ff(t, 0.0f);
+    for (int t = 7; t < 16; ++t) fe[t] = fe[kComplex[t - 7][0]] + (float)kComplex[t - 7][1] * fe[0];
+    const float coef = (float)(3.0 / (4.0 * PI * 0.334f));
+    for (int t = 0; t < 16; ++t) out16[t] = std::pow(coef * fe[t], (float)(1.0 / 3));
+}
+
+void atomic_ff(const float* q, int nq, float avg_radius, float c, float* out)
+{
+    const float k43 = std::pow(4.0f * (float)PI / 3.0f, 2.0f / 3.0f);
+    for (int i = 0; i < nq; ++i) {
+        float* page = out + (size_t)i * 16;
+        const float k = q[i] / (4.0f * (float)PI);
+        const float k_sq = (float)(k * k);
+        float E = 1.0f;
+        if (c != 1.0f) {
+            const float c_sq = c * c;
+            E = c_sq * c * std::exp(-(float)PI * k43 * k_sq * avg_radius * avg_radius * (c_sq - 1.0f));
+        }
+        for (int t = 0; t < 7; ++t) {
+            float f = kCM[t][4];
+            for (int a = 0; a < 4; ++a) f += kCM[t][a] * std::exp(-kCM[t][5 + a] * k_sq);
+            page[t] = f - excl_ff(t, k_sq) * E;
+        }
+        for (int t = 7; t < 16; ++t) page[t] = page[kComplex[t - 7][0]] + (float)kComplex[t - 7][1] * page[0];
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// engine
+// ---------------------------------------------------------------------------
+struct rsx_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    std::string err;
+    int state = 0;  // 0 blank, 1 initialized, 3 args set, 4 computing
+
+    // q grid
+    std::vector<float> q;
+    int nq = 0;
+    bool uniform_grid = false;  // q_k = q_0 + k dq
+    bool uniform = false;       // the recurrence path is active
+    double centre[3] = {0, 0, 0};
+    double radius = 0.0;        // bounding radius of the current structure
+    double dq = 0.0;
+    int q0_index = -1;
+
+    // factors
+    std::vector<float> table;
+    int n_types = 0;
+    int water = -1;
+    float w = 0.0f;
+    bool params_dirty = true;  // factors / weight changed since the last full calc
+    bool w_dirty = false;
+    bool no_rsasa_recalc = false;
+
+    // SASA set-up
+    bool sasa_set = false;
+    std::vector<float> radii;  // per type
+    float probe = 1.8f;
+    int dot_kind = RSX_DOTS_DODECA_ICOSA;
+    int n_half = 0;
+    int n_dots = 0;
+    std::vector<float> dots;  // [n_types][n_dots][4]
+    std::vector<float> r2, rext;
+
+    // tuning
+    int k_req = 0, range_req = 0;
+    int K = 0, n_chunks = 0, ldf = 0;
+    int rebase_every = 200;
+
+    // bodies (host mirror, caller order)
+    int n = 0;
+    std::vector<float> bodies;  // n*4, type bits in w
+    std::vector<uint32_t> type;
+    std::vector<uint32_t> counts;
+    std::vector<float> rsasa;
+    bool have_rsasa = false;
+    std::vector<uint32_t> cls;      // class per caller index
+    std::vector<int> slot_of, caller_of;
+    int d1 = 1;                      // classes per type
+    int n_class = 0;
+    std::vector<double> f64;         // [n_class][nq] FP64 class factors
+    bool hydrate = false;
+
+    // device buffers
+    DBuf<float4> d_cols, d_saved_cols, d_rows, d_vals, d_sasa_atoms;
+    DBuf<float> d_ftab, d_qrev, d_qv, d_r2, d_rext;
+    DBuf<ChunkConst> d_chunks;
+    DBuf<double> d_invq, d_partials, d_iq, d_saved_iq, d_out, d_wq, d_q0;
+    DBuf<PairJob> d_jobs, d_fjobs;  // incremental jobs; forward (triangle) jobs
+    DBuf<double> d_f0col;           // FP64 F_class(q = 0) per class
+    DBuf<int> d_slot_begin_f;  // [0, n forward jobs]
+    DBuf<int> d_slot_begin, d_idx, d_bucket_of, d_bucket_count, d_bucket_start, d_bucket_fill, d_sorted, d_s2c;
+    DBuf<uint32_t> d_counts;
+    DBuf<float4> d_dots;
+    DBuf<float> d_gpart, d_grad;
+    size_t ftab_rows = 0;
+
+    // last forward launch (for timing)
+    std::vector<PairJob> last_jobs;
+    int last_slots = 0;
+    double last_q0 = 0.0;
+
+    // transaction
+    bool xaction = false;
+    struct Saved {
+        std::vector<float> bodies;
+        std::vector<uint32_t> counts, cls;
+        std::vector<float> rsasa;
+        std::vector<int> slot_of, caller_of;
+        bool have_rsasa = false;
+        int since_rebase = 0;
+    } saved;
+    int since_rebase = 0;
+
+    // batch
+    int b_frames = 0;
+    std::vector<int> b_offsets;
+    DBuf<float4> b_cols;
+    DBuf<PairJob> b_jobs;
+    DBuf<double> b_partials, b_out, b_mean, b_q0;
+    DBuf<int> b_slot_begin;
+    int b_njobs = 0, b_slots = 0;
+    hipGraph_t b_graph = nullptr;
+    hipGraphExec_t b_exec = nullptr;
+    double* b_graph_out = nullptr;
+    double* b_graph_mean = nullptr;
+};
+
+namespace {
+
+void set_device(rsx_engine* h)
+{
+    hipchk(hipSetDevice(h->device), "hipSetDevice");
+}
+
+// ---- q grid and chunking -------------------------------------------------
+void choose_chunks(rsx_engine* h)
+{
+    const bool uni = h->uniform;
+    const int nk = rsx::debye_k_count(uni);
+    int bestK = 0, bestC = 0;
+    double best = 1e300;
+    for (int i = 0; i < nk; ++i) {
+        const int K = rsx::debye_k_value(uni, i);
+        if (h->k_req > 0 && K != h->k_req) continue;
+        if (h->k_req <= 0 && K > 56) continue;  // 64 needs more than 256 VGPRs
+        const int C = (h->nq + K - 1) / K;
+        // per (pair, chunk): 2K recurrence ops + ~40 for distance and seeds
+        const double cost = uni ? C * (2.0 * K + 40.0) : C * (6.0 * K + 12.0);
+        if (cost < best) {
+            best = cost;
+            bestK = K;
+            bestC = C;
+        }
+    }
+    if (bestK == 0) fail(RSX_INVALID_ARG, "unsupported K (q values per thread)");
+    h->K = bestK;
+    h->n_chunks = bestC;
+    h->ldf = bestK * bestC;
+}
+
+// Upload the per-chunk constants of the active path (uniform recurrence or direct).
+void apply_path(rsx_engine* h, bool uniform)
+{
+    const int nq = h->nq;
+    h->uniform = uniform;
+    choose_chunks(h);
+    const int ldf = h->ldf;
+    std::vector<ChunkConst> cc(h->n_chunks);
+    for (int c = 0; c < h->n_chunks; ++c) {
+        const int k0 = c * h->K;
+        const double a = k0 < nq ? (double)h->q[k0] : 0.0;
+        cc[c].a = (float)a;
+        cc[c].a_rev = (float)(a / (2.0 * PI));
+        cc[c].mid_rev = (float)((a + 0.5 * h->dq) / (2.0 * PI));
+        cc[c].pad = 0.0f;
+    }
+    std::vector<double> invq(ldf, 0.0);
+    std::vector<float> qrev(ldf, 0.0f), qv(ldf, 0.0f);
+    for (int k = 0; k < nq; ++k) {
+        invq[k] = h->q[k] != 0.0f ? 1.0 / (double)h->q[k] : 0.0;
+        qrev[k] = (float)((double)h->q[k] / (2.0 * PI));
+        qv[k] = h->q[k];
+    }
+    h->d_chunks.ensure(cc.size());
+    h->d_invq.ensure(ldf);
+    h->d_qrev.ensure(ldf);
+    h->d_qv.ensure(ldf);
+    hipchk(hipMemcpy(h->d_chunks.p, cc.data(), cc.size() * sizeof(ChunkConst), hipMemcpyHostToDevice), "upload chunks");
+    hipchk(hipMemcpy(h->d_invq.p, invq.data(), ldf * sizeof(double), hipMemcpyHostToDevice), "upload invq");
+    hipchk(hipMemcpy(h->d_qrev.p, qrev.data(), ldf * sizeof(float), hipMemcpyHostToDevice), "upload qrev");
+    hipchk(hipMemcpy(h->d_qv.p, qv.data(), ldf * sizeof(float), hipMemcpyHostToDevice), "upload qv");
+    h->params_dirty = true;  // the class table is laid out with stride ldf
+}
+
+void setup_q(rsx_engine* h)
+{
+    const int nq = h->nq;
+    h->q0_index = -1;
+    for (int k = 0; k < nq; ++k)
+        if (h->q[k] == 0.0f) {
+            h->q0_index = k;
+            break;
+        }
+    // uniform grid test: every q within 4 ulps (of the largest |q|) of q_0 + k dq
+    h->uniform_grid = false;
+    h->dq = 0.0;
+    if (nq >= 2) {
+        const double dq = ((double)h->q[nq - 1] - (double)h->q[0]) / (nq - 1);
+        double qmax = 0.0;
+        for (float v : h->q) qmax = std::max(qmax, (double)std::fabs(v));
+        const double tol = 4.0 * std::ldexp(1.0, -23) * qmax;
+        bool ok = dq > 0.0 && h->q[0] >= 0.0f;
+        for (int k = 0; k < nq && ok; ++k)
+            if (std::fabs((double)h->q[k] - ((double)h->q[0] + k * dq)) > tol) ok = false;
+        h->uniform_grid = ok;
+        h->dq = ok ? dq : 0.0;
+    }
+    apply_path(h, h->uniform_grid);
+    h->d_iq.ensure(nq);
+    h->d_saved_iq.ensure(nq);
+    h->d_out.ensure(nq);
+    h->d_q0.ensure(64);
+}
+
+// The recurrence's step angle dq r must stay below 3 rad (its FP64 sin
+// polynomial covers dq r / 2 <= 1.5 and the three-term form loses accuracy
+// as dq r -> pi).  r <= 2 R for R the largest distance from the centroid.
+constexpr double MAX_STEP_ANGLE = 3.0;
+
+bool fits_recurrence(const rsx_engine* h, double radius)
+{
+    return h->uniform_grid && h->dq * 2.0 * radius <= MAX_STEP_ANGLE;
+}
+
+double bounding_radius(const float* bodies, int n, double* centre)
+{
+    double c[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) c[d] += bodies[4 * (size_t)i + d];
+    for (int d = 0; d < 3; ++d) c[d] = n ? c[d] / n : 0.0;
+    double r2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double v = bodies[4 * (size_t)i + d] - c[d];
+            s += v * v;
+        }
+        r2 = std::max(r2, s);
+    }
+    if (centre)
+        for (int d = 0; d < 3; ++d) centre[d] = c[d];
+    return std::sqrt(r2) * (1.0 + 1e-6) + 1e-6;
+}
+
+// choose the path for a structure of the given bounding radius; true if it changed
+bool select_path(rsx_engine* h, double radius)
+{
+    const bool want = fits_recurrence(h, radius);
+    if (want == h->uniform) return false;
+    apply_path(h, want);
+    return true;
+}
+
+// ---- classes ---------------------------------------------------------------
+bool hydration_on(const rsx_engine* h)
+{
+    return h->w != 0.0f && h->water >= 0 && h->sasa_set;
+}
+
+void build_class_table(rsx_engine* h)
+{
+    h->hydrate = hydration_on(h);
+    h->d1 = h->hydrate ? h->n_dots + 1 : 1;
+    h->n_class = h->n_types * h->d1;
+    const int nq = h->nq, ldf = h->ldf;
+    h->f64.assign((size_t)h->n_class * nq, 0.0);
+    std::vector<float> ft((size_t)h->n_class * ldf, 0.0f);
+    for (int t = 0; t < h->n_types; ++t)
+        for (int c = 0; c < h->d1; ++c) {
+            const int id = t * h->d1 + c;
+            // exposed fraction as the reference forms it: float(count) / n_dots
+            const float s = h->hydrate ? (float)c / (float)h->n_dots : 0.0f;
+            for (int k = 0; k < nq; ++k) {
+                double f = h->table[(size_t)k * h->n_types + t];
+                if (h->hydrate) f += (double)s * ((double)h->w * (double)h->table[(size_t)k * h->n_types + h->water]);
+                h->f64[(size_t)id * nq + k] = f;
+                ft[(size_t)id * ldf + k] = (float)f;
+            }
+        }
+    h->d_ftab.ensure(ft.size());
+    hipchk(hipMemcpy(h->d_ftab.p, ft.data(), ft.size() * sizeof(float), hipMemcpyHostToDevice), "upload ftab");
+    std::vector<double> f0(h->n_class, 0.0);
+    if (h->q0_index >= 0)
+        for (int c = 0; c < h->n_class; ++c) f0[c] = h->f64[(size_t)c * nq + h->q0_index];
+    h->d_f0col.ensure(h->n_class);
+    hipchk(hipMemcpy(h->d_f0col.p, f0.data(), f0.size() * sizeof(double), hipMemcpyHostToDevice), "upload f0");
+}
+
+uint32_t class_of(const rsx_engine* h, int i)
+{
+    return h->type[i] * (uint32_t)h->d1 + (h->hydrate ? h->counts[i] : 0u);
+}
+
+// stable sort of atoms by class -> slot maps; upload the column records
+void assign_slots(rsx_engine* h)
+{
+    const int n = h->n;
+    h->cls.resize(n);
+    for (int i = 0; i < n; ++i) h->cls[i] = class_of(h, i);
+    std::vector<int> cnt(h->n_class + 1, 0);
+    for (int i = 0; i < n; ++i) cnt[h->cls[i] + 1]++;
+    for (int c = 0; c < h->n_class; ++c) cnt[c + 1] += cnt[c];
+    h->caller_of.resize(n);
+    h->slot_of.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const int s = cnt[h->cls[i]]++;
+        h->caller_of[s] = i;
+        h->slot_of[i] = s;
+    }
+    std::vector<float4> cols(n);
+    for (int s = 0; s < n; ++s) {
+        const int i = h->caller_of[s];
+        cols[s] = make_float4(h->bodies[4 * i], h->bodies[4 * i + 1], h->bodies[4 * i + 2], bitsf(h->cls[i]));
+    }
+    h->d_cols.ensure(n);
+    hipchk(hipMemcpyAsync(h->d_cols.p, cols.data(), n * sizeof(float4), hipMemcpyHostToDevice, h->stream),
+           "upload cols");
+    h->d_s2c.ensure(n);
+    hipchk(hipMemcpyAsync(h->d_s2c.p, h->caller_of.data(), n * sizeof(int), hipMemcpyHostToDevice, h->stream),
+           "upload slot map");
+    hipchk(hipStreamSynchronize(h->stream), "sync");
+}
+
+double sum_f0(const rsx_engine* h, const std::vector<uint32_t>& cls_list)
+{
+    double s = 0.0;
+    if (h->q0_index < 0) return 0.0;
+    for (uint32_t c : cls_list)
+        if (c != CLS_SKIP) s += h->f64[(size_t)c * h->nq + h->q0_index];
+    return s;
+}
+
+// ---- SASA ----------------------------------------------------------------
+void upload_sasa_tables(rsx_engine* h)
+{
+    const int nt = (int)h->radii.size();
+    h->r2.resize(nt);
+    h->rext.resize(nt);
+    for (int t = 0; t < nt; ++t) {
+        const float R = h->probe + h->radii[t];
+        h->r2[t] = R * R;
+        h->rext[t] = (float)std::sqrt((double)h->r2[t]);
+    }
+    h->d_r2.ensure(nt);
+    h->d_rext.ensure(nt);
+    h->d_dots.ensure((size_t)nt * h->n_dots);
+    hipchk(hipMemcpy(h->d_r2.p, h->r2.data(), nt * sizeof(float), hipMemcpyHostToDevice), "upload r2");
+    hipchk(hipMemcpy(h->d_rext.p, h->rext.data(), nt * sizeof(float), hipMemcpyHostToDevice), "upload rext");
+    hipchk(hipMemcpy(h->d_dots.p, h->dots.data(), h->dots.size() * sizeof(float), hipMemcpyHostToDevice),
+           "upload dots");
+}
+
+// counts for n atoms already on the device in d_sasa_atoms
+void run_sasa_device(rsx_engine* h, int n)
+{
+    float rmax = 0.0f;
+    for (float v : h->rext) rmax = std::max(rmax, v);
+    int nb = 1024;
+    while (nb < 2 * n) nb <<= 1;
+    h->d_bucket_of.ensure(n);
+    h->d_bucket_count.ensure(nb + 1);
+    h->d_bucket_start.ensure(nb + 1);
+    h->d_bucket_fill.ensure(nb);
+    h->d_sorted.ensure(n);
+    h->d_counts.ensure(n);
+    rsx::SasaArgs a;
+    a.atoms = h->d_sasa_atoms.p;
+    a.n = n;
+    a.r2 = h->d_r2.p;
+    a.rext = h->d_rext.p;
+    a.dots = h->d_dots.p;
+    a.n_dots = h->n_dots;
+    a.cell = 2.0f * rmax + 0.02f;
+    a.inv_cell = 1.0f / a.cell;
+    a.n_buckets = nb;
+    a.bucket_of = h->d_bucket_of.p;
+    a.bucket_count = h->d_bucket_count.p;
+    a.bucket_start = h->d_bucket_start.p;
+    a.bucket_fill = h->d_bucket_fill.p;
+    a.sorted = h->d_sorted.p;
+    a.counts = h->d_counts.p;
+    hipchk(rsx::launch_sasa(a, h->stream), "sasa kernels");
+}
+
+// recompute counts / rsasa for the host mirror bodies (caller order)
+void compute_sasa(rsx_engine* h)
+{
+    const int n = h->n;
+    for (int i = 0; i < n; ++i)
+        if (h->type[i] >= h->radii.size()) fail(RSX_INVALID_ARG, "body type index outside the SASA radius table");
+    h->d_sasa_atoms.ensure(n);
+    hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, h->bodies.data(), (size_t)n * 16, hipMemcpyHostToDevice, h->stream),
+           "upload sasa atoms");
+    run_sasa_device(h, n);
+    h->counts.resize(n);
+    hipchk(hipMemcpyAsync(h->counts.data(), h->d_counts.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream),
+           "download counts");
+    hipchk(hipStreamSynchronize(h->stream), "sync sasa");
+    h->rsasa.resize(n);
+    for (int i = 0; i < n; ++i) h->rsasa[i] = (float)h->counts[i] / (float)h->n_dots;
+    h->have_rsasa = true;
+}
+
+// ---- pair jobs ---------------------------------------------------------------
+int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles)
+{
+    if (h->range_req > 0) return h->range_req;
+    // aim for ~8192 waves (4 per SIMD at 2 waves/SIMD occupancy over 1024 SIMDs)
+    const double jobs = 8192.0 / std::max(1, h->n_chunks);
+    double L = column_atoms_total / std::max(1.0, jobs);
+    (void)row_tiles;
+    L = std::max(64.0, std::min(4096.0, L));
+    return ((int)L + 63) / 64 * 64;
+}
+
+// triangle jobs over records [0, m) of cols (rows == cols); appends to jobs
+void triangle_jobs(std::vector<PairJob>& jobs, const float4* cols, int m, int L, float scale)
+{
+    const int T = (m + 63) / 64;
+    for (int I = 0; I < T; ++I) {
+        const int r0 = I * 64;
+        for (int c0 = r0; c0 < m; c0 += L) {
+            PairJob j;
+            j.rows = cols + r0;
+            j.nrow = std::min(64, m - r0);
+            j.cols = cols + c0;
+            j.ncol = std::min(L, m - c0);
+            j.diag = c0 == r0 ? 1 : 0;
+            j.scale = scale;
+            j.slot = 0;
+            j.pad = 0;
+            jobs.push_back(j);
+        }
+    }
+}
+
+// rows [0, nr) against cols [0, nc) (full rectangle)
+void rect_jobs(std::vector<PairJob>& jobs, const float4* rows, int nr, const float4* cols, int nc, int L, float scale)
+{
+    for (int r0 = 0; r0 < nr; r0 += 64)
+        for (int c0 = 0; c0 < nc; c0 += L) {
+            PairJob j;
+            j.rows = rows + r0;
+            j.nrow = std::min(64, nr - r0);
+            j.cols = cols + c0;
+            j.ncol = std::min(L, nc - c0);
+            j.diag = 0;
+            j.scale = scale;
+            j.slot = 0;
+            j.pad = 0;
+            jobs.push_back(j);
+        }
+}
+
+// order by cost (longest first) and number the slots in execution order
+void finalize_jobs(std::vector<PairJob>& jobs)
+{
+    std::stable_sort(jobs.begin(), jobs.end(),
+                     [](const PairJob& a, const PairJob& b) { return (long)a.ncol * a.nrow > (long)b.ncol * b.nrow; });
+    for (size_t s = 0; s < jobs.size(); ++s) jobs[s].slot = (int)s;
+}
+
+rsx::DebyeArgs debye_args(rsx_engine* h, const PairJob* d_jobs, double* partials)
+{
+    rsx::DebyeArgs a;
+    a.jobs = d_jobs;
+    a.n_chunks = h->n_chunks;
+    a.ftab = h->d_ftab.p;
+    a.ldf = h->ldf;
+    a.chunks = h->d_chunks.p;
+    a.half_dq = 0.5 * h->dq;
+    a.dq = (float)h->dq;
+    a.invq = h->d_invq.p;
+    a.qrev = h->d_qrev.p;
+    a.qv = h->d_qv.p;
+    a.partials = partials;
+    return a;
+}
+
+// upload jobs, launch the pair kernel and reduce into iq (single output)
+void run_jobs(rsx_engine* h, DBuf<PairJob>& jbuf, const std::vector<PairJob>& jobs, double* iq, int accumulate,
+              double q0val)
+{
+    const int nj = (int)jobs.size();
+    jbuf.ensure(nj);
+    h->d_partials.ensure((size_t)std::max(nj, 1) * h->ldf);
+    h->d_slot_begin.ensure(2);
+    if (nj > 0)
+        hipchk(hipMemcpyAsync(jbuf.p, jobs.data(), nj * sizeof(PairJob), hipMemcpyHostToDevice, h->stream),
+               "upload jobs");
+    int sb[2] = {0, nj};
+    hipchk(hipMemcpyAsync(h->d_slot_begin.p, sb, sizeof(sb), hipMemcpyHostToDevice, h->stream), "upload slots");
+    hipchk(hipMemcpyAsync(h->d_q0.p, &q0val, sizeof(double), hipMemcpyHostToDevice, h->stream), "upload q0");
+    const rsx::DebyeArgs a = debye_args(h, jbuf.p, h->d_partials.p);
+    hipchk(rsx::launch_debye(a, nj, h->K, h->uniform, h->stream), "debye kernel");
+    if (nj > 0) {
+        hipchk(rsx::launch_reduce(h->d_partials.p, h->ldf, h->nq, h->d_slot_begin.p, 1, iq, h->nq, accumulate,
+                                  h->d_q0.p, h->q0_index, h->stream),
+               "reduce kernel");
+    } else if (!accumulate) {
+        std::vector<double> z(h->nq, 0.0);
+        if (h->q0_index >= 0) z[h->q0_index] = q0val;
+        hipchk(hipMemcpyAsync(iq, z.data(), h->nq * sizeof(double), hipMemcpyHostToDevice, h->stream), "zero iq");
+    }
+    hipchk(hipStreamSynchronize(h->stream), "sync pair kernels");
+}
+
+void full_forward(rsx_engine* h)
+{
+    const int n = h->n;
+    std::vector<PairJob> jobs;
+    const int L = pick_range(h, (double)n * n / 2.0, (n + 63) / 64);
+    triangle_jobs(jobs, h->d_cols.p, n, L, 2.0f);
+    finalize_jobs(jobs);
+    const double sf = sum_f0(h, h->cls);
+    run_jobs(h, h->d_fjobs, jobs, h->d_iq.p, 0, sf * sf);
+    h->last_jobs = jobs;
+    h->last_q0 = sf * sf;
+    int sbf[2] = {0, (int)jobs.size()};
+    h->d_slot_begin_f.ensure(2);
+    hipchk(hipMemcpy(h->d_slot_begin_f.p, sbf, sizeof(sbf), hipMemcpyHostToDevice), "upload forward slots");
+    h->since_rebase = 0;
+}
+
+void copy_out(rsx_engine* h, float* out32, double* out64)
+{
+    std::vector<double> tmp(h->nq);
+    hipchk(hipMemcpy(tmp.data(), h->d_iq.p, h->nq * sizeof(double), hipMemcpyDeviceToHost), "download iq");
+    for (int k = 0; k < h->nq; ++k) {
+        if (out32) out32[k] = (float)tmp[k];
+        if (out64) out64[k] = tmp[k];
+    }
+}
+
+void check_args_set(rsx_engine* h)
+{
+    if (h->state < 3) fail(RSX_ARGS_NOT_SET, "engine arguments not set (rsx_initialize)");
+}
+
+void load_bodies(rsx_engine* h, const float* bodies, int start, int len)
+{
+    for (int i = start; i < start + len; ++i) {
+        uint32_t t;
+        std::memcpy(&t, &bodies[4 * i + 3], 4);
+        if ((int)t >= h->n_types) fail(RSX_INVALID_ARG, "body type index outside the factor table");
+        std::memcpy(&h->bodies[4 * i], &bodies[4 * i], 16);
+        h->type[i] = t;
+    }
+}
+
+void calc_curve_impl(rsx_engine* h, const float* bodies, int n, float* out32, double* out64)
+{
+    check_args_set(h);
+    if (n != h->n) fail(RSX_INVALID_ARG, "different number of bodies expected in calc_curve()");
+    if (!bodies) fail(RSX_INVALID_ARG, "null bodies");
+    load_bodies(h, bodies, 0, n);
+    set_device(h);
+    h->radius = bounding_radius(h->bodies.data(), n, h->centre);
+    select_path(h, h->radius);
+    if (h->params_dirty || h->d_ftab.p == nullptr) build_class_table(h);
+    if (h->hydrate) compute_sasa(h);
+    assign_slots(h);
+    full_forward(h);
+    h->params_dirty = false;
+    h->w_dirty = false;
+    h->state = 4;
+    copy_out(h, out32, out64);
+}
+
+void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int len, float* out32, double* out64)
+{
+    if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "recalc_curve() called before an initial call to calc_curve()");
+    if (start < 0 || len < 0 || (long)start + len > n)
+        fail(RSX_INVALID_ARG, "bodies vector too small in recalc_curve()");
+    if ((long)start + len > h->n) fail(RSX_INVALID_ARG, "update region exceeds the number of bodies in recalc_curve()");
+    set_device(h);
+    const std::vector<float> old_bodies(h->bodies.begin() + 4 * (size_t)start,
+                                        h->bodies.begin() + 4 * (size_t)(start + len));
+    load_bodies(h, bodies, start, len);
+    for (int i = start; i < start + len; ++i) {
+        double s2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double v = h->bodies[4 * (size_t)i + d] - h->centre[d];
+            s2 += v * v;
+        }
+        h->radius = std::max(h->radius, std::sqrt(s2) * (1.0 + 1e-6) + 1e-6);
+    }
+    if (select_path(h, h->radius)) h->params_dirty = true;
+
+    // hydration update (saxs_alg_cl_gpu_pt_wf.cpp:333-401)
+    const bool hyd_before = h->hydrate;
+    bool sasa_now = false;
+    if (hydration_on(h)) {
+        if (len > 0)
+            sasa_now = true;
+        else
+            sasa_now = h->w_dirty && !h->have_rsasa;
+        if (h->no_rsasa_recalc && h->have_rsasa) sasa_now = false;
+    }
+
+    if (h->params_dirty || hyd_before != hydration_on(h)) {
+        // new parameters force a full recalculation
+        build_class_table(h);
+        if (h->hydrate && (sasa_now || !h->have_rsasa)) compute_sasa(h);
+        assign_slots(h);
+        full_forward(h);
+        h->params_dirty = false;
+        h->w_dirty = false;
+        copy_out(h, out32, out64);
+        return;
+    }
+
+    std::vector<uint32_t> old_counts;
+    if (sasa_now) {
+        old_counts = h->counts;
+        compute_sasa(h);
+    }
+    // changed set: moved atoms plus atoms whose class changed
+    std::vector<int> M;
+    std::vector<char> in_m(h->n, 0);
+    for (int i = start; i < start + len; ++i) {
+        M.push_back(i);
+        in_m[i] = 1;
+    }
+    if (sasa_now)
+        for (int i = 0; i < h->n; ++i)
+            if (!in_m[i] && old_counts[i] != h->counts[i]) {
+                M.push_back(i);
+                in_m[i] = 1;
+            }
+    if (M.empty()) {
+        copy_out(h, out32, out64);
+        return;
+    }
+    if (h->rebase_every > 0 && h->since_rebase + 1 >= h->rebase_every) {
+        assign_slots(h);
+        full_forward(h);
+        copy_out(h, out32, out64);
+        return;
+    }
+
+    const int m = (int)M.size();
+    const int mp = (m + 63) / 64 * 64;
+    // rows: [new (mp) | old (mp)], pads carry the skip class
+    std::vector<float4> rows(2 * (size_t)mp, make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP)));
+    std::vector<uint32_t> cls_new(m), cls_old(m);
+    for (int a = 0; a < m; ++a) {
+        const int i = M[a];
+        cls_old[a] = h->cls[i];
+        cls_new[a] = class_of(h, i);
+        const float* nb = &h->bodies[4 * (size_t)i];
+        const float* ob = (i >= start && i < start + len) ? &old_bodies[4 * (size_t)(i - start)] : nb;
+        rows[a] = make_float4(nb[0], nb[1], nb[2], bitsf(cls_new[a]));
+        rows[mp + a] = make_float4(ob[0], ob[1], ob[2], bitsf(cls_old[a]));
+    }
+    // column edits: first remove M from the columns, later write the new records
+    std::vector<int> slots(m);
+    std::vector<float4> vals(2 * (size_t)m);
+    for (int a = 0; a < m; ++a) {
+        slots[a] = h->slot_of[M[a]];
+        vals[a] = make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP));
+        vals[m + a] = rows[a];
+    }
+    h->d_rows.ensure(rows.size());
+    h->d_idx.ensure(m);
+    h->d_vals.ensure(2 * (size_t)m);
+    hipchk(hipMemcpyAsync(h->d_rows.p, rows.data(), rows.size() * sizeof(float4), hipMemcpyHostToDevice, h->stream),
+           "upload rows");
+    hipchk(hipMemcpyAsync(h->d_idx.p, slots.data(), m * sizeof(int), hipMemcpyHostToDevice, h->stream),
+           "upload idx");
+    hipchk(hipMemcpyAsync(h->d_vals.p, vals.data(), vals.size() * sizeof(float4), hipMemcpyHostToDevice, h->stream),
+           "upload vals");
+    hipchk(rsx::launch_scatter(h->d_cols.p, h->d_idx.p, h->d_vals.p, m, h->stream), "scatter skip");
+
+    std::vector<PairJob> jobs;
+    const int L = pick_range(h, 2.0 * mp * (double)h->n / 64.0 * 64.0 / 2.0, 2 * mp / 64);
+    rect_jobs(jobs, h->d_rows.p, mp, h->d_cols.p, h->n, L, 2.0f);            // new x rest
+    rect_jobs(jobs, h->d_rows.p + mp, mp, h->d_cols.p, h->n, L, -2.0f);      // old x rest
+    rect_jobs(jobs, h->d_rows.p, mp, h->d_rows.p, mp, L, 1.0f);             // new x new
+    rect_jobs(jobs, h->d_rows.p + mp, mp, h->d_rows.p + mp, mp, L, -1.0f);  // old x old
+    finalize_jobs(jobs);
+
+    // q = 0: (sum F_new)^2 - (sum F_old)^2 from the class lists
+    double q0d = 0.0;
+    if (h->q0_index >= 0) {
+        const double s_old = sum_f0(h, h->cls);
+        double s_new = s_old;
+        for (int a = 0; a < m; ++a) {
+            s_new -= h->f64[(size_t)cls_old[a] * h->nq + h->q0_index];
+            s_new += h->f64[(size_t)cls_new[a] * h->nq + h->q0_index];
+        }
+        q0d = s_new * s_new - s_old * s_old;
+    }
+    run_jobs(h, h->d_jobs, jobs, h->d_iq.p, 1, q0d);
+    hipchk(rsx::launch_scatter(h->d_cols.p, h->d_idx.p, h->d_vals.p + m, m, h->stream), "scatter new");
+    hipchk(hipStreamSynchronize(h->stream), "sync scatter");
+    for (int a = 0; a < m; ++a) h->cls[M[a]] = cls_new[a];
+    h->since_rebase++;
+    copy_out(h, out32, out64);
+}
+
+void commit_impl(rsx_engine* h)
+{
+    if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "commit() called before an initial call to calc_curve()");
+    set_device(h);
+    h->d_saved_cols.ensure(h->n);
+    hipchk(hipMemcpyAsync(h->d_saved_cols.p, h->d_cols.p, (size_t)h->n * sizeof(float4), hipMemcpyDeviceToDevice,
+                          h->stream),
+           "save cols");
+    hipchk(hipMemcpyAsync(h->d_saved_iq.p, h->d_iq.p, h->nq * sizeof(double), hipMemcpyDeviceToDevice, h->stream),
+           "save iq");
+    hipchk(hipStreamSynchronize(h->stream), "sync commit");
+    h->saved.bodies = h->bodies;
+    h->saved.counts = h->counts;
+    h->saved.cls = h->cls;
+    h->saved.rsasa = h->rsasa;
+    h->saved.slot_of = h->slot_of;
+    h->saved.caller_of = h->caller_of;
+    h->saved.have_rsasa = h->have_rsasa;
+    h->saved.since_rebase = h->since_rebase;
+    h->xaction = true;
+}
+
+void revert_impl(rsx_engine* h)
+{
+    if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "revert() called before an initial call to calc_curve()");
+    if (!h->xaction) return;
+    set_device(h);
+    const bool layout_changed = h->saved.caller_of != h->caller_of;
+    h->bodies = h->saved.bodies;
+    h->counts = h->saved.counts;
+    h->cls = h->saved.cls;
+    h->rsasa = h->saved.rsasa;
+    h->slot_of = h->saved.slot_of;
+    h->caller_of = h->saved.caller_of;
+    h->have_rsasa = h->saved.have_rsasa;
+    h->since_rebase = h->saved.since_rebase;
+    for (int i = 0; i < h->n; ++i) {
+        uint32_t t;
+        std::memcpy(&t, &h->bodies[4 * i + 3], 4);
+        h->type[i] = t;
+    }
+    hipchk(hipMemcpyAsync(h->d_cols.p, h->d_saved_cols.p, (size_t)h->n * sizeof(float4), hipMemcpyDeviceToDevice,
+                          h->stream),
+           "restore cols");
+    hipchk(hipMemcpyAsync(h->d_iq.p, h->d_saved_iq.p, h->nq * sizeof(double), hipMemcpyDeviceToDevice, h->stream),
+           "restore iq");
+    if (layout_changed)
+        hipchk(hipMemcpyAsync(h->d_s2c.p, h->caller_of.data(), h->n * sizeof(int), hipMemcpyHostToDevice, h->stream),
+               "restore slot map");
+    hipchk(hipStreamSynchronize(h->stream), "sync revert");
+}
+
+// ---- standalone SASA -----------------------------------------------------------
+void calc_rsasa_impl(rsx_engine* h, const float* bodies, int n, const float* radii, int nt, float probe,
+                     const float* dots, int n_dots, unsigned* out_counts, float* out_frac)
+{
+    if (!bodies || !radii || !dots || n < 0 || nt <= 0 || n_dots <= 0 || n_dots > 512)
+        fail(RSX_INVALID_ARG, "calc_rsasa: invalid argument");
+    for (int i = 0; i < n; ++i) {
+        uint32_t t;
+        std::memcpy(&t, &bodies[4 * i + 3], 4);
+        if ((int)t >= nt) fail(RSX_INVALID_ARG, "calc_rsasa: type index outside the radius table");
+    }
+    set_device(h);
+    // use a private copy of the SASA tables
+    const std::vector<float> keep_radii = h->radii, keep_dots = h->dots, keep_r2 = h->r2, keep_rext = h->rext;
+    const float keep_probe = h->probe;
+    const int keep_nd = h->n_dots;
+    h->radii.assign(radii, radii + nt);
+    h->probe = probe;
+    h->n_dots = n_dots;
+    h->dots.assign(dots, dots + (size_t)nt * n_dots * 4);
+    upload_sasa_tables(h);
+    if (n > 0) {
+        h->d_sasa_atoms.ensure(n);
+        hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, bodies, (size_t)n * 16, hipMemcpyHostToDevice, h->stream),
+               "upload atoms");
+        run_sasa_device(h, n);
+        std::vector<uint32_t> c(n);
+        hipchk(hipMemcpyAsync(c.data(), h->d_counts.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream),
+               "download counts");
+        hipchk(hipStreamSynchronize(h->stream), "sync");
+        for (int i = 0; i < n; ++i) {
+            if (out_counts) out_counts[i] = c[i];
+            if (out_frac) out_frac[i] = (float)c[i] / (float)n_dots;
+        }
+    }
+    h->radii = keep_radii;
+    h->dots = keep_dots;
+    h->probe = keep_probe;
+    h->n_dots = keep_nd;
+    h->r2 = keep_r2;
+    h->rext = keep_rext;
+    if (h->sasa_set) upload_sasa_tables(h);
+}
+
+// ---- batch --------------------------------------------------------------------
+void batch_release_graph(rsx_engine* h)
+{
+    if (h->b_exec) hipGraphExecDestroy(h->b_exec);
+    if (h->b_graph) hipGraphDestroy(h->b_graph);
+    h->b_exec = nullptr;
+    h->b_graph = nullptr;
+}
+
+void batch_set_frames_impl(rsx_engine* h, const float* bodies, const int* offsets, int nf)
+{
+    if (h->nq <= 0 || h->n_types <= 0) fail(RSX_ARGS_NOT_SET, "batch: q and factors must be set first");
+    if (nf <= 0 || !offsets || !bodies) fail(RSX_INVALID_ARG, "batch: invalid frames");
+    set_device(h);
+    batch_release_graph(h);
+    double rmax = 0.0;
+    for (int f = 0; f < nf; ++f)
+        if (offsets[f + 1] > offsets[f])
+            rmax = std::max(rmax, bounding_radius(bodies + 4 * (size_t)offsets[f], offsets[f + 1] - offsets[f], nullptr));
+    select_path(h, rmax);
+    if (h->params_dirty || h->d_ftab.p == nullptr || h->hydrate) {
+        // frames use the plain (type) classes
+        const float keep_w = h->w;
+        h->w = 0.0f;
+        build_class_table(h);
+        h->w = keep_w;
+        h->params_dirty = true;  // the single-structure path rebuilds its table on next use
+    }
+    const int total = offsets[nf];
+    std::vector<float4> cols(std::max(total, 1));
+    std::vector<double> q0(nf, 0.0);
+    for (int f = 0; f < nf; ++f) {
+        const int b = offsets[f], e = offsets[f + 1];
+        if (e < b) fail(RSX_INVALID_ARG, "batch: offsets must not decrease");
+        std::vector<int> idx(e - b);
+        std::iota(idx.begin(), idx.end(), b);
+        std::vector<uint32_t> t(e - b);
+        for (int i = b; i < e; ++i) {
+            std::memcpy(&t[i - b], &bodies[4 * (size_t)i + 3], 4);
+            if ((int)t[i - b] >= h->n_types) fail(RSX_INVALID_ARG, "batch: type index outside the factor table");
+        }
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return t[x - b] < t[y - b]; });
+        double s = 0.0;
+        for (int a = 0; a < e - b; ++a) {
+            const int i = idx[a];
+            cols[b + a] = make_float4(bodies[4 * (size_t)i], bodies[4 * (size_t)i + 1], bodies[4 * (size_t)i + 2],
+                                      bitsf(t[i - b]));
+            if (h->q0_index >= 0) s += h->f64[(size_t)t[i - b] * h->nq + h->q0_index];
+        }
+        q0[f] = s * s;
+    }
+    h->b_cols.ensure(cols.size());
+    hipchk(hipMemcpy(h->b_cols.p, cols.data(), cols.size() * sizeof(float4), hipMemcpyHostToDevice), "upload frames");
+    // jobs grouped by frame: slots of frame f are [sb[f], sb[f+1])
+    double work = 0.0;
+    for (int f = 0; f < nf; ++f) {
+        const double m = offsets[f + 1] - offsets[f];
+        work += m * m / 2.0;
+    }
+    const int L = pick_range(h, work, 0);
+    std::vector<PairJob> jobs;
+    std::vector<int> sb(nf + 1, 0);
+    for (int f = 0; f < nf; ++f) {
+        std::vector<PairJob> fj;
+        triangle_jobs(fj, h->b_cols.p + offsets[f], offsets[f + 1] - offsets[f], L, 2.0f);
+        for (auto& j : fj) {
+            j.slot = (int)jobs.size();
+            jobs.push_back(j);
+        }
+        sb[f + 1] = (int)jobs.size();
+    }
+    // execution order: longest first; slots keep the frame grouping
+    std::stable_sort(jobs.begin(), jobs.end(),
+                     [](const PairJob& a, const PairJob& b) { return (long)a.ncol * a.nrow > (long)b.ncol * b.nrow; });
+    h->b_njobs = (int)jobs.size();
+    h->b_slots = h->b_njobs;
+    h->b_jobs.ensure(jobs.size());
+    h->b_partials.ensure((size_t)std::max(1, h->b_slots) * h->ldf);
+    h->b_slot_begin.ensure(nf + 1);
+    h->b_out.ensure((size_t)nf * h->nq);
+    h->b_mean.ensure(h->nq);
+    h->b_q0.ensure(nf);
+    hipchk(hipMemcpy(h->b_jobs.p, jobs.data(), jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice), "upload jobs");
+    hipchk(hipMemcpy(h->b_slot_begin.p, sb.data(), sb.size() * sizeof(int), hipMemcpyHostToDevice), "upload sb");
+    hipchk(hipMemcpy(h->b_q0.p, q0.data(), nf * sizeof(double), hipMemcpyHostToDevice), "upload q0");
+    h->b_frames = nf;
+    h->b_offsets.assign(offsets, offsets + nf + 1);
+}
+
+void batch_enqueue(rsx_engine* h, double* out_frames, double* out_mean)
+{
+    const rsx::DebyeArgs a = debye_args(h, h->b_jobs.p, h->b_partials.p);
+    hipchk(rsx::launch_debye(a, h->b_njobs, h->K, h->uniform, h->stream), "batch debye");
+    hipchk(rsx::launch_reduce(h->b_partials.p, h->ldf, h->nq, h->b_slot_begin.p, h->b_frames, out_frames, h->nq, 0,
+                              h->b_q0.p, h->q0_index, h->stream),
+           "batch reduce");
+    hipchk(rsx::launch_frame_mean(out_frames, h->b_frames, h->nq, out_mean, h->stream), "frame mean");
+}
+
+void batch_run_dev(rsx_engine* h, double* out_frames, double* out_mean)
+{
+    if (h->b_frames <= 0) fail(RSX_ARGS_NOT_SET, "batch: no frames set");
+    set_device(h);
+    if (!h->b_exec || h->b_graph_out != out_frames || h->b_graph_mean != out_mean) {
+        batch_release_graph(h);
+        hipchk(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+            batch_enqueue(h, out_frames, out_mean);
+        } catch (...) {
+            hipGraph_t g;
+            hipStreamEndCapture(h->stream, &g);
+            if (g) hipGraphDestroy(g);
+            throw;
+        }
+        hipchk(hipStreamEndCapture(h->stream, &h->b_graph), "end capture");
+        hipchk(hipGraphInstantiate(&h->b_exec, h->b_graph, nullptr, nullptr, 0), "instantiate graph");
+        h->b_graph_out = out_frames;
+        h->b_graph_mean = out_mean;
+    }
+    hipchk(hipGraphLaunch(h->b_exec, h->stream), "graph launch");
+}
+
+// ---- adjoint -----------------------------------------------------------------
+void adjoint_impl(rsx_engine* h, const std::vector<double>& wk, float* out_grad)
+{
+    const int n = h->n;
+    set_device(h);
+    int K = 0;
+    for (int i = 0; i < rsx::adjoint_k_count(); ++i) {
+        const int k = rsx::adjoint_k_value(i);
+        if (k <= 48 && k >= std::min(h->K, 48)) {
+            K = k;
+            break;
+        }
+    }
+    if (K == 0) K = 32;
+    const int nch = (h->nq + K - 1) / K;
+    const int ldf = h->ldf;
+    if (nch * K > ldf) {
+        // the class table is padded to ldf; recompute chunk set-up for the adjoint's K
+    }
+    std::vector<double> wq(std::max(ldf, nch * K), 0.0);
+    for (int k = 0; k < h->nq; ++k) wq[k] = h->q[k] == 0.0f ? 0.0 : wk[k];
+    // chunk constants for the adjoint's chunking
+    std::vector<ChunkConst> cc(nch);
+    for (int c = 0; c < nch; ++c) {
+        const int k0 = c * K;
+        const double a = k0 < h->nq ? (double)h->q[k0] : 0.0;
+        cc[c].a = (float)a;
+        cc[c].a_rev = (float)(a / (2.0 * PI));
+        cc[c].mid_rev = (float)((a + 0.5 * h->dq) / (2.0 * PI));
+        cc[c].pad = 0.0f;
+    }
+    // class table and 1/q with the adjoint's padding
+    const int ldfa = nch * K;
+    std::vector<float> ft((size_t)h->n_class * ldfa, 0.0f);
+    for (int c = 0; c < h->n_class; ++c)
+        for (int k = 0; k < h->nq; ++k) ft[(size_t)c * ldfa + k] = (float)h->f64[(size_t)c * h->nq + k];
+    std::vector<double> invq(ldfa, 0.0);
+    for (int k = 0; k < h->nq; ++k) invq[k] = h->q[k] != 0.0f ? 1.0 / (double)h->q[k] : 0.0;
+    DBuf<float> ftab;
+    DBuf<ChunkConst> dcc;
+    DBuf<double> dinv;
+    ftab.ensure(ft.size());
+    dcc.ensure(nch);
+    dinv.ensure(ldfa);
+    h->d_wq.ensure(ldfa);
+    try {
+        hipchk(hipMemcpy(ftab.p, ft.data(), ft.size() * sizeof(float), hipMemcpyHostToDevice), "upload ftab");
+        hipchk(hipMemcpy(dcc.p, cc.data(), nch * sizeof(ChunkConst), hipMemcpyHostToDevice), "upload chunks");
+        hipchk(hipMemcpy(dinv.p, invq.data(), ldfa * sizeof(double), hipMemcpyHostToDevice), "upload invq");
+        hipchk(hipMemcpy(h->d_wq.p, wq.data(), ldfa * sizeof(double), hipMemcpyHostToDevice), "upload w");
+        rsx::AdjointArgs a;
+        a.rows = h->d_cols.p;
+        a.cols = h->d_cols.p;
+        a.n = n;
+        a.col_range = 2048;
+        a.n_colblocks = (n + a.col_range - 1) / a.col_range;
+        a.n_chunks = nch;
+        a.ftab = ftab.p;
+        a.ldf = ldfa;
+        a.chunks = dcc.p;
+        a.half_dq = 0.5 * h->dq;
+        a.dq = (float)h->dq;
+        a.wq = h->d_wq.p;
+        a.invq = dinv.p;
+        const int parts = a.n_colblocks * nch;
+        h->d_gpart.ensure((size_t)parts * n * 4);
+        h->d_grad.ensure((size_t)n * 3);
+        a.grad_partial = h->d_gpart.p;
+        hipchk(rsx::launch_adjoint(a, K, h->stream), "adjoint kernel");
+        hipchk(rsx::launch_grad_reduce(h->d_gpart.p, parts, n, h->d_s2c.p, h->d_grad.p, h->stream), "grad reduce");
+        hipchk(hipMemcpyAsync(out_grad, h->d_grad.p, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, h->stream),
+               "download grad");
+        hipchk(hipStreamSynchronize(h->stream), "sync adjoint");
+    } catch (...) {
+        ftab.release();
+        dcc.release();
+        dinv.release();
+        throw;
+    }
+    ftab.release();
+    dcc.release();
+    dinv.release();
+}
+
+// Device-resident forward step: (SASA) -> class records in slot order -> pair
+// kernel -> fixed-order reduction.  No host synchronisation.
+void enqueue_device_forward(rsx_engine* h, const float4* bodies_dev, double* out)
+{
+    const int n = h->n;
+    const uint32_t* counts = nullptr;
+    if (h->hydrate) {
+        h->d_sasa_atoms.ensure(n);
+        hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, bodies_dev, (size_t)n * 16, hipMemcpyDeviceToDevice, h->stream),
+               "copy atoms");
+        run_sasa_device(h, n);
+        counts = h->d_counts.p;
+    }
+    hipchk(rsx::launch_gather_cols(h->d_cols.p, bodies_dev, h->d_s2c.p, counts, h->d1, n, h->stream), "gather");
+    hipchk(rsx::launch_q0(h->d_cols.p, n, h->d_f0col.p, h->d_q0.p, h->stream), "q0 kernel");
+    const int nj = (int)h->last_jobs.size();
+    const rsx::DebyeArgs a = debye_args(h, h->d_fjobs.p, h->d_partials.p);
+    hipchk(rsx::launch_debye(a, nj, h->K, h->uniform, h->stream), "debye kernel");
+    hipchk(rsx::launch_reduce(h->d_partials.p, h->ldf, h->nq, h->d_slot_begin_f.p, 1, out, h->nq, 0, h->d_q0.p,
+                              h->q0_index, h->stream),
+           "reduce kernel");
+}
+
+template <typename F>
+int guarded(rsx_engine* h, F&& f)
+{
+    if (!h) return RSX_INVALID_ARG;
+    try {
+        f();
+        h->err.clear();
+        return RSX_OK;
+    } catch (const Fail& e) {
+        h->err = e.msg;
+        return e.code;
+    } catch (const std::exception& e) {
+        h->err = e.what();
+        return RSX_HIP_ERROR;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int rsx_device_count(int* count)
+{
+    if (!count) return RSX_INVALID_ARG;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return RSX_OK;
+}
+
+int rsx_create(int device, rsx_engine** out)
+{
+    if (!out) return RSX_INVALID_ARG;
+    *out = nullptr;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || device < 0 || device >= c) return RSX_INVALID_DEVICE;
+    rsx_engine* h = new rsx_engine();
+    h->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
+        hipEventCreate(&h->ev2) != hipSuccess || hipEventCreate(&h->ev3) != hipSuccess) {
+        delete h;
+        return RSX_HIP_ERROR;
+    }
+    h->state = 1;
+    *out = h;
+    return RSX_OK;
+}
+
+int rsx_destroy(rsx_engine* h)
+{
+    if (!h) return RSX_INVALID_ARG;
+    hipSetDevice(h->device);
+    batch_release_graph(h);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    DBuf<float4>* f4[] = {&h->d_cols, &h->d_saved_cols, &h->d_rows, &h->d_vals, &h->d_sasa_atoms, &h->d_dots, &h->b_cols};
+    for (auto* b : f4) b->release();
+    DBuf<float>* ff[] = {&h->d_ftab, &h->d_qrev, &h->d_qv, &h->d_r2, &h->d_rext, &h->d_gpart, &h->d_grad};
+    for (auto* b : ff) b->release();
+    DBuf<double>* fd[] = {&h->d_invq, &h->d_partials, &h->d_iq, &h->d_saved_iq, &h->d_out, &h->d_wq, &h->d_q0,
+                          &h->b_partials, &h->b_out, &h->b_mean, &h->b_q0};
+    for (auto* b : fd) b->release();
+    DBuf<int>* fi[] = {&h->d_slot_begin, &h->d_idx, &h->d_bucket_of, &h->d_bucket_count, &h->d_bucket_start,
+                       &h->d_bucket_fill, &h->d_sorted, &h->d_s2c, &h->b_slot_begin};
+    for (auto* b : fi) b->release();
+    h->d_chunks.release();
+    h->d_jobs.release();
+    h->d_fjobs.release();
+    h->d_f0col.release();
+    h->d_slot_begin_f.release();
+    h->b_jobs.release();
+    h->d_counts.release();
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->ev2) hipEventDestroy(h->ev2);
+    if (h->ev3) hipEventDestroy(h->ev3);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return RSX_OK;
+}
+
+const char* rsx_last_error(const rsx_engine* h)
+{
+    return h ? h->err.c_str() : "null engine";
+}
+
+void* rsx_stream(rsx_engine* h)
+{
+    return h ? (void*)h->stream : nullptr;
+}
+
+int rsx_state(const rsx_engine* h)
+{
+    return h ? h->state : RSX_INVALID_ARG;
+}
+
+int rsx_set_q(rsx_engine* h, const float* q, int n_q)
+{
+    return guarded(h, [&] {
+        if (!q || n_q <= 0) fail(RSX_INVALID_ARG, "q vector must not be empty");
+        for (int k = 0; k < n_q; ++k)
+            if (!(q[k] >= 0.0f) || !std::isfinite(q[k])) fail(RSX_INVALID_ARG, "q values must be finite and >= 0");
+        set_device(h);
+        h->q.assign(q, q + n_q);
+        h->nq = n_q;
+        setup_q(h);
+        h->params_dirty = true;
+        if (h->state > 3) h->state = 3;
+        batch_release_graph(h);
+        h->b_frames = 0;
+    });
+}
+
+int rsx_set_factors(rsx_engine* h, const float* table, int n_types, int water_index)
+{
+    return guarded(h, [&] {
+        if (h->nq <= 0) fail(RSX_ARGS_NOT_SET, "set q before the factor table");
+        if (!table || n_types <= 0 || water_index >= n_types) fail(RSX_INVALID_ARG, "invalid factor table");
+        h->table.assign(table, table + (size_t)h->nq * n_types);
+        h->n_types = n_types;
+        h->water = water_index;
+        h->params_dirty = true;
+    });
+}
+
+int rsx_set_water_weight(rsx_engine* h, float w)
+{
+    return guarded(h, [&] {
+        if (!std::isfinite(w)) fail(RSX_INVALID_ARG, "water weight must be finite");
+        if (w != h->w) {
+            h->w = w;
+            h->params_dirty = true;
+            h->w_dirty = true;
+        }
+    });
+}
+
+int rsx_set_no_rsasa_recalc(rsx_engine* h, int flag)
+{
+    return guarded(h, [&] { h->no_rsasa_recalc = flag != 0; });
+}
+
+int rsx_set_sasa_params(rsx_engine* h, const float* radii, int n_types, float probe, int kind, int n_half)
+{
+    return guarded(h, [&] {
+        if (!radii || n_types <= 0 || !(probe >= 0.0f)) fail(RSX_INVALID_ARG, "invalid SASA parameters");
+        int nd = 0;
+        if (kind == RSX_DOTS_DODECA_ICOSA)
+            nd = 32;
+        else if (kind == RSX_DOTS_FIBONACCI) {
+            if (n_half <= 0 || 2 * n_half > 512) fail(RSX_INVALID_ARG, "Fibonacci sphere: 1 <= n_half <= 256");
+            nd = 2 * n_half;
+        } else
+            fail(RSX_INVALID_ARG, "unknown dot-sphere kind");
+        set_device(h);
+        h->radii.assign(radii, radii + n_types);
+        h->probe = probe;
+        h->dot_kind = kind;
+        h->n_half = n_half;
+        h->n_dots = nd;
+        h->dots.assign((size_t)n_types * nd * 4, 0.0f);
+        for (int t = 0; t < n_types; ++t) {
+            float* d = &h->dots[(size_t)t * nd * 4];
+            if (kind == RSX_DOTS_DODECA_ICOSA)
+                dodeca_icosa_dots(radii[t], probe, d);
+            else
+                fibonacci_dots(n_half, radii[t], probe, d);
+            // templates are offsets from the atom centre; w carries the radius
+        }
+        upload_sasa_tables(h);
+        h->sasa_set = true;
+        h->have_rsasa = false;
+        h->params_dirty = true;
+    });
+}
+
+int rsx_set_rebaseline(rsx_engine* h, int every)
+{
+    return guarded(h, [&] {
+        if (every < 0) fail(RSX_INVALID_ARG, "rebaseline period must be >= 0");
+        h->rebase_every = every;
+    });
+}
+
+int rsx_set_tuning(rsx_engine* h, int k, int col_range)
+{
+    return guarded(h, [&] {
+        if (k < 0 || col_range < 0) fail(RSX_INVALID_ARG, "tuning values must be >= 0");
+        if (col_range > 0 && col_range % 64 != 0) fail(RSX_INVALID_ARG, "col_range must be a multiple of 64");
+        h->k_req = k;
+        h->range_req = col_range;
+        if (h->nq > 0) {
+            set_device(h);
+            setup_q(h);
+            h->params_dirty = true;
+            if (h->state > 3) h->state = 3;
+            batch_release_graph(h);
+            h->b_frames = 0;
+        }
+    });
+}
+
+int rsx_initialize(rsx_engine* h, int n_bodies)
+{
+    return guarded(h, [&] {
+        if (n_bodies < 0) fail(RSX_INVALID_ARG, "negative body count");
+        if (h->nq <= 0 || h->n_types <= 0) fail(RSX_ARGS_NOT_SET, "q and factor table must be set before initialize");
+        set_device(h);
+        h->n = n_bodies;
+        h->bodies.assign((size_t)n_bodies * 4, 0.0f);
+        h->type.assign(n_bodies, 0);
+        h->counts.assign(n_bodies, 0);
+        h->rsasa.assign(n_bodies, 0.0f);
+        h->have_rsasa = false;
+        h->xaction = false;
+        h->params_dirty = true;
+        h->d_cols.ensure(n_bodies);
+        h->d_s2c.ensure(n_bodies);
+        h->state = 3;
+    });
+}
+
+int rsx_calc_curve(rsx_engine* h, const float* bodies, int n, float* out)
+{
+    return guarded(h, [&] { calc_curve_impl(h, bodies, n, out, nullptr); });
+}
+
+int rsx_calc_curve_f64(rsx_engine* h, const float* bodies, int n, double* out)
+{
+    return guarded(h, [&] { calc_curve_impl(h, bodies, n, nullptr, out); });
+}
+
+int rsx_recalc_curve(rsx_engine* h, const float* bodies, int n, int start, int len, float* out)
+{
+    return guarded(h, [&] { recalc_curve_impl(h, bodies, n, start, len, out, nullptr); });
+}
+
+int rsx_recalc_curve_f64(rsx_engine* h, const float* bodies, int n, int start, int len, double* out)
+{
+    return guarded(h, [&] { recalc_curve_impl(h, bodies, n, start, len, nullptr, out); });
+}
+
+int rsx_commit(rsx_engine* h)
+{
+    return guarded(h, [&] { commit_impl(h); });
+}
+
+int rsx_revert(rsx_engine* h)
+{
+    return guarded(h, [&] { revert_impl(h); });
+}
+
+int rsx_get_rsasa(rsx_engine* h, float* out, int n)
+{
+    return guarded(h, [&] {
+        if (!out || n < h->n) fail(RSX_INVALID_ARG, "rsasa output too small");
+        for (int i = 0; i < h->n; ++i) out[i] = h->have_rsasa ? h->rsasa[i] : 0.0f;
+    });
+}
+
+int rsx_calc_curve_dev_async(rsx_engine* h, const void* bodies_dev, int n, double* out_dev)
+{
+    return guarded(h, [&] {
+        check_args_set(h);
+        if (n != h->n || !bodies_dev) fail(RSX_INVALID_ARG, "calc_curve_dev: body count mismatch");
+        if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "calc_curve_dev needs one host calc_curve to lay out the atoms");
+        set_device(h);
+        enqueue_device_forward(h, (const float4*)bodies_dev, out_dev ? out_dev : h->d_iq.p);
+    });
+}
+
+int rsx_synchronize(rsx_engine* h)
+{
+    return guarded(h, [&] { hipchk(hipStreamSynchronize(h->stream), "synchronize"); });
+}
+
+int rsx_calc_rsasa(rsx_engine* h, const float* bodies, int n, const float* radii, int n_types, float probe,
+                   const float* dots, int n_dots, unsigned* out_counts, float* out_frac)
+{
+    return guarded(h, [&] { calc_rsasa_impl(h, bodies, n, radii, n_types, probe, dots, n_dots, out_counts, out_frac); });
+}
+
+int rsx_fibonacci_dots(int n_half, float radius, float probe, float* out)
+{
+    if (n_half <= 0 || !out) return RSX_INVALID_ARG;
+    fibonacci_dots(n_half, radius, probe, out);
+    return RSX_OK;
+}
+
+int rsx_dodeca_icosa_dots(float radius, float probe, float* out)
+{
+    if (!out) return RSX_INVALID_ARG;
+    dodeca_icosa_dots(radius, probe, out);
+    return RSX_OK;
+}
+
+int rsx_atomic_form_factors(const float* q, int n_q, float avg_radius, float expansion, float* out)
+{
+    if (!q || n_q <= 0 || !out) return RSX_INVALID_ARG;
+    atomic_ff(q, n_q, avg_radius, expansion, out);
+    return RSX_OK;
+}
+
+int rsx_atomic_radii(float* out16)
+{
+    if (!out16) return RSX_INVALID_ARG;
+    atomic_radii(out16);
+    return RSX_OK;
+}
+
+int rsx_batch_set_frames(rsx_engine* h, const float* bodies, const int* offsets, int n_frames)
+{
+    return guarded(h, [&] { batch_set_frames_impl(h, bodies, offsets, n_frames); });
+}
+
+int rsx_batch_calc_dev_async(rsx_engine* h, double* out_frames_dev, double* out_mean_dev)
+{
+    return guarded(h, [&] {
+        double* of = out_frames_dev ? out_frames_dev : (h->b_out.ensure((size_t)h->b_frames * h->nq), h->b_out.p);
+        double* om = out_mean_dev ? out_mean_dev : (h->b_mean.ensure(h->nq), h->b_mean.p);
+        batch_run_dev(h, of, om);
+    });
+}
+
+int rsx_batch_calc(rsx_engine* h, double* out_frames, double* out_mean)
+{
+    return guarded(h, [&] {
+        if (h->b_frames <= 0) fail(RSX_ARGS_NOT_SET, "batch: no frames set");
+        h->b_out.ensure((size_t)h->b_frames * h->nq);
+        h->b_mean.ensure(h->nq);
+        batch_run_dev(h, h->b_out.p, h->b_mean.p);
+        if (out_frames)
+            hipchk(hipMemcpyAsync(out_frames, h->b_out.p, (size_t)h->b_frames * h->nq * sizeof(double),
+                                  hipMemcpyDeviceToHost, h->stream),
+                   "download frames");
+        if (out_mean)
+            hipchk(hipMemcpyAsync(out_mean, h->b_mean.p, h->nq * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+                   "download mean");
+        hipchk(hipStreamSynchronize(h->stream), "sync batch");
+    });
+}
+
+int rsx_chi2_grad(rsx_engine* h, const float* bodies, int n, const float* i_exp, const float* sigma, double* out_chi2,
+                  double* out_scale, float* out_grad, double* out_iq)
+{
+    return guarded(h, [&] {
+        if (!i_exp || !sigma) fail(RSX_INVALID_ARG, "chi2_grad: null profile");
+        if (!h->uniform) fail(RSX_NOT_SUPPORTED, "chi2_grad needs a uniform q grid");
+        std::vector<double> I(h->nq);
+        calc_curve_impl(h, bodies, n, nullptr, I.data());
+        double num = 0.0, den = 0.0;
+        for (int k = 0; k < h->nq; ++k) {
+            const double p2 = 1.0 / ((double)sigma[k] * (double)sigma[k]);
+            num += (double)i_exp[k] * I[k] * p2;
+            den += I[k] * I[k] * p2;
+        }
+        const double c = num / den;
+        double x2 = 0.0;
+        std::vector<double> wk(h->nq);
+        for (int k = 0; k < h->nq; ++k) {
+            const double s2 = (double)sigma[k] * (double)sigma[k];
+            const double d = (double)i_exp[k] - c * I[k];
+            x2 += d * d / s2;
+            wk[k] = -2.0 * c * d / s2;
+        }
+        if (out_chi2) *out_chi2 = x2;
+        if (out_scale) *out_scale = c;
+        if (out_iq) std::copy(I.begin(), I.end(), out_iq);
+        if (out_grad) adjoint_impl(h, wk, out_grad);
+    });
+}
+
+int rsx_debye_vjp(rsx_engine* h, const float* bodies, int n, const double* w, float* out_grad)
+{
+    return guarded(h, [&] {
+        if (!w || !out_grad) fail(RSX_INVALID_ARG, "vjp: null argument");
+        if (!h->uniform) fail(RSX_NOT_SUPPORTED, "vjp needs a uniform q grid");
+        calc_curve_impl(h, bodies, n, nullptr, nullptr);
+        std::vector<double> wk(w, w + h->nq);
+        adjoint_impl(h, wk, out_grad);
+    });
+}
+
+int rsx_autotune(rsx_engine* h, const char* cache_path)
+{
+    (void)cache_path;
+    return guarded(h, [&] { fail(RSX_NOT_SUPPORTED, "autotune not built yet"); });
+}
+
+int rsx_time_forward(rsx_engine* h, int reps, float* ms_total, float* ms_pair)
+{
+    return guarded(h, [&] {
+        if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "time_forward needs a prior calc_curve");
+        if (reps <= 0) fail(RSX_INVALID_ARG, "reps must be > 0");
+        set_device(h);
+        const int nj = (int)h->last_jobs.size();
+        hipchk(hipMemcpy(h->d_q0.p, &h->last_q0, sizeof(double), hipMemcpyHostToDevice), "upload q0");
+        const rsx::DebyeArgs a = debye_args(h, h->d_fjobs.p, h->d_partials.p);
+        float pair_ms = 0.0f;
+        hipchk(hipEventRecord(h->ev0, h->stream), "event");
+        for (int r = 0; r < reps; ++r) {
+            hipchk(hipEventRecord(h->ev2, h->stream), "event");
+            hipchk(rsx::launch_debye(a, nj, h->K, h->uniform, h->stream), "debye kernel");
+            hipchk(hipEventRecord(h->ev3, h->stream), "event");
+            hipchk(rsx::launch_reduce(h->d_partials.p, h->ldf, h->nq, h->d_slot_begin_f.p, 1, h->d_iq.p, h->nq, 0,
+                                      h->d_q0.p, h->q0_index, h->stream),
+                   "reduce kernel");
+            hipchk(hipEventSynchronize(h->ev3), "event sync");
+            float t = 0.0f;
+            hipchk(hipEventElapsedTime(&t, h->ev2, h->ev3), "elapsed");
+            pair_ms += t;
+        }
+        hipchk(hipEventRecord(h->ev1, h->stream), "event");
+        hipchk(hipEventSynchronize(h->ev1), "event sync");
+        float tot = 0.0f;
+        hipchk(hipEventElapsedTime(&tot, h->ev0, h->ev1), "elapsed");
+        if (ms_total) *ms_total = tot / reps;
+        if (ms_pair) *ms_pair = pair_ms / reps;
+    });
+}
+
+int rsx_dev_alloc(rsx_engine* h, void** ptr, long long bytes)
+{
+    return guarded(h, [&] {
+        if (!ptr || bytes < 0) fail(RSX_INVALID_ARG, "dev_alloc");
+        set_device(h);
+        hipError_t e = hipMalloc(ptr, (size_t)std::max(bytes, 1LL));
+        if (e != hipSuccess) fail(RSX_OUT_OF_MEMORY, hipGetErrorString(e));
+    });
+}
+
+int rsx_dev_free(rsx_engine* h, void* ptr)
+{
+    return guarded(h, [&] {
+        set_device(h);
+        hipchk(hipFree(ptr), "hipFree");
+    });
+}
+
+int rsx_memcpy_h2d(rsx_engine* h, void* dst, const void* src, long long bytes)
+{
+    return guarded(h, [&] {
+        set_device(h);
+        hipchk(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice), "memcpy h2d");
+    });
+}
+
+int rsx_memcpy_d2h(rsx_engine* h, void* dst, const void* src, long long bytes)
+{
+    return guarded(h, [&] {
+        set_device(h);
+        hipchk(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost), "memcpy d2h");
+    });
+}
+
+}  // extern "C"

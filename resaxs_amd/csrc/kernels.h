@@ -1,0 +1,127 @@
+// Device-side data structures and kernel launchers of the Debye-SAXS engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace rsx {
+
+// Class id stored in the w bits of a column/row record that contributes nothing
+// (padding, atoms temporarily removed from a column set).
+constexpr uint32_t CLS_SKIP = 0x7fffffffu;
+
+// One unit of pair-sum work: up to 64 row atoms (one per lane) against a
+// contiguous run of column atoms.  Records are float4 {x, y, z, class bits}.
+// slot selects the row of the partial-sum table the job writes; scale
+// multiplies its sum.  diag != 0 means cols[0..63] are the same atoms as
+// rows[0..63] (the diagonal tile of a triangle), so only j > i counts fully,
+// j == i counts half and j < i not at all (the job scale is then 2).
+struct PairJob {
+    const float4* rows;
+    const float4* cols;
+    int nrow;
+    int ncol;
+    int slot;
+    float scale;
+    int diag;
+    int pad;
+};
+
+// Per-q-chunk constants of the uniform-grid path: a = first q of the chunk,
+// *_rev = value / (2 pi) (v_sin/v_cos take revolutions).
+struct ChunkConst {
+    float a_rev;
+    float mid_rev;  // (a + dq/2) / (2 pi)
+    float a;
+    float pad;
+};
+
+struct DebyeArgs {
+    const PairJob* jobs;
+    int n_chunks;         // q chunks per job (one wave each)
+    const float* ftab;    // [n_class][ldf] FP32 form factors per class, zero padded
+    int ldf;              // n_chunks * K
+    const ChunkConst* chunks;
+    double half_dq;       // dq / 2 (FP64)
+    float dq;
+    const double* invq;   // [ldf] 1/q, 0 for q == 0 and padding
+    const float* qrev;    // [ldf] q / (2 pi)  (direct path)
+    const float* qv;      // [ldf] q           (direct path)
+    double* partials;     // [n_slots][ldf]
+};
+
+// Debye pair sums.  uniform: q_k = a + k dq within each chunk (FP64
+// three-term recurrence); otherwise direct FP32 sin per term.
+hipError_t launch_debye(const DebyeArgs& a, int n_jobs, int K, bool uniform, hipStream_t s);
+// K values the launcher accepts, ascending.
+int debye_k_count(bool uniform);
+int debye_k_value(bool uniform, int i);
+
+// out[o*out_ld + k] (+)= sum_{s in [slot_begin[o], slot_begin[o+1])} partials[s*ldf + k]
+// in a fixed order.  q0_index >= 0: that k is set to q0_val[o] instead (the
+// exact (sum F)^2 rule for q == 0, added when accumulate != 0).
+hipError_t launch_reduce(const double* partials, int ldf, int n_q, const int* slot_begin, int n_out,
+                         double* out, int out_ld, int accumulate, const double* q0_val, int q0_index,
+                         hipStream_t s);
+
+// out[i] = mean over frames (uniform weights) of in[f*n_q + i]
+hipError_t launch_frame_mean(const double* in, int n_frames, int n_q, double* out, hipStream_t s);
+
+// cols[slot[t]] = vals[t]
+hipError_t launch_scatter(float4* cols, const int* slot, const float4* vals, int m, hipStream_t s);
+
+// cols[s] = {bodies[src[s]].xyz, class}, class = type(bodies[src[s]]) * d1 + (counts ? counts[src[s]] : 0)
+hipError_t launch_gather_cols(float4* cols, const float4* bodies, const int* src, const uint32_t* counts,
+                              int d1, int n, hipStream_t s);
+
+// ---- SASA ----
+struct SasaArgs {
+    const float4* atoms;     // x, y, z, type bits
+    int n;
+    const float* r2;         // [n_types] (probe + r)^2
+    const float* rext;       // [n_types] sqrt(r2) (prefilter)
+    const float4* dots;      // [n_types][n_dots]
+    int n_dots;
+    float cell;              // cell edge
+    float inv_cell;
+    int n_buckets;           // power of two
+    int* bucket_of;          // [n]
+    int* bucket_count;       // [n_buckets + 1]
+    int* bucket_start;       // [n_buckets + 1]
+    int* bucket_fill;        // [n_buckets]
+    int* sorted;             // [n] atom ids grouped by bucket
+    uint32_t* counts;        // [n] exposed dots
+};
+hipError_t launch_sasa(const SasaArgs& a, hipStream_t s);
+
+// ---- adjoint ----
+struct AdjointArgs {
+    const float4* rows;      // all atoms (slot order)
+    const float4* cols;
+    int n;                   // atoms
+    int col_range;           // columns per job
+    int n_chunks;
+    const float* ftab;       // [n_class][ldf]
+    int ldf;
+    const ChunkConst* chunks;
+    double half_dq;
+    float dq;
+    const double* wq;        // [ldf] w_k / 1 (0 padding)  -- weights dchi2/dI_k
+    const double* invq;      // [ldf]
+    float* grad_partial;     // [n_colblocks * n_chunks][n][4]
+    int n_colblocks;
+};
+hipError_t launch_adjoint(const AdjointArgs& a, int K, hipStream_t s);
+int adjoint_k_count();
+int adjoint_k_value(int i);
+// grad[i] = sum over partial blocks (fixed order), written in caller order via slot map
+hipError_t launch_grad_reduce(const float* grad_partial, int n_parts, int n, const int* slot_to_caller,
+                              float* grad_out, hipStream_t s);
+
+// out[0] = (sum_s f0col[class(cols[s])])^2, fixed-order FP64 sum (the q = 0 value)
+hipError_t launch_q0(const float4* cols, int n, const double* f0col, double* out, hipStream_t s);
+
+// workspace reset kernels (used instead of memset nodes so launches can be graph-captured)
+hipError_t launch_fill_i32(int* p, int value, long long n, hipStream_t s);
+
+}  // namespace rsx

@@ -1,0 +1,255 @@
+"""Python handle over the C ABI: one Engine = one device, one HIP stream.
+
+The method names follow the reference's i_saxs / i_rsasa interface
+(resaxslib/include/saxs_algorithm.hpp:147-183, rsasa_algorithm.hpp:42-61):
+calc_curve, recalc_curve, commit, revert, calc_rsasa.  Arrays are numpy.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import RsxError
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def make_bodies(pos, types):
+    """float32 [N, 4] bodies: x, y, z and the type index in the bit pattern of w
+    (the reference's cl_float4 layout, resaxslib/include/utils.hpp:61-88)."""
+    pos = np.asarray(pos, dtype=np.float32).reshape(-1, 3)
+    b = np.zeros((len(pos), 4), dtype=np.float32)
+    b[:, :3] = pos
+    b[:, 3] = np.asarray(types, dtype=np.uint32).view(np.float32)
+    return b
+
+
+class Engine:
+    def __init__(self, device=0):
+        self._L = _lib.lib()
+        h = ctypes.c_void_p()
+        self._check(self._L.rsx_create(int(device), ctypes.byref(h)), None)
+        self.h = h
+        self.nq = 0
+        self.n = 0
+
+    # -- plumbing ---------------------------------------------------------
+    def _check(self, rc, h="self"):
+        if rc != 0:
+            msg = ""
+            if h is not None:
+                m = self._L.rsx_last_error(self.h)
+                msg = m.decode() if m else ""
+            raise RsxError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.rsx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def state(self):
+        return self._L.rsx_state(self.h)
+
+    # -- parameters (saxs_params) ------------------------------------------
+    def set_q(self, q):
+        q = _f32(q)
+        self._check(self._L.rsx_set_q(self.h, _ptr(q, ctypes.c_float), len(q)))
+        self.nq = len(q)
+
+    def set_factors(self, table, water_index=-1):
+        table = _f32(table)
+        if table.ndim != 2 or table.shape[0] != self.nq:
+            raise ValueError("factor table must be [n_q, n_types]")
+        self._check(self._L.rsx_set_factors(self.h, _ptr(table, ctypes.c_float), table.shape[1], int(water_index)))
+
+    def set_water_weight(self, w):
+        self._check(self._L.rsx_set_water_weight(self.h, float(w)))
+
+    def set_no_rsasa_recalc(self, flag):
+        self._check(self._L.rsx_set_no_rsasa_recalc(self.h, int(bool(flag))))
+
+    def set_sasa_params(self, radii, probe, kind=_lib.RSX_DOTS_FIBONACCI, n_half=128):
+        radii = _f32(radii)
+        self._check(self._L.rsx_set_sasa_params(self.h, _ptr(radii, ctypes.c_float), len(radii), float(probe),
+                                                int(kind), int(n_half)))
+
+    def set_rebaseline(self, every):
+        self._check(self._L.rsx_set_rebaseline(self.h, int(every)))
+
+    def set_tuning(self, k=0, col_range=0):
+        self._check(self._L.rsx_set_tuning(self.h, int(k), int(col_range)))
+
+    def initialize(self, n_bodies):
+        self._check(self._L.rsx_initialize(self.h, int(n_bodies)))
+        self.n = int(n_bodies)
+
+    # -- Debye I(q) (i_saxs) ------------------------------------------------
+    def calc_curve(self, bodies):
+        b = _f32(bodies).reshape(-1, 4)
+        out = np.zeros(self.nq, dtype=np.float64)
+        self._check(self._L.rsx_calc_curve_f64(self.h, _ptr(b, ctypes.c_float), len(b), _ptr(out, ctypes.c_double)))
+        return out
+
+    def calc_curve_f32(self, bodies):
+        b = _f32(bodies).reshape(-1, 4)
+        out = np.zeros(self.nq, dtype=np.float32)
+        self._check(self._L.rsx_calc_curve(self.h, _ptr(b, ctypes.c_float), len(b), _ptr(out, ctypes.c_float)))
+        return out
+
+    def recalc_curve(self, bodies, start, length):
+        b = _f32(bodies).reshape(-1, 4)
+        out = np.zeros(self.nq, dtype=np.float64)
+        self._check(self._L.rsx_recalc_curve_f64(self.h, _ptr(b, ctypes.c_float), len(b), int(start), int(length),
+                                                 _ptr(out, ctypes.c_double)))
+        return out
+
+    def commit(self):
+        self._check(self._L.rsx_commit(self.h))
+
+    def revert(self):
+        self._check(self._L.rsx_revert(self.h))
+
+    def get_rsasa(self):
+        out = np.zeros(self.n, dtype=np.float32)
+        self._check(self._L.rsx_get_rsasa(self.h, _ptr(out, ctypes.c_float), self.n))
+        return out
+
+    # -- SASA (i_rsasa) ------------------------------------------------------
+    def calc_rsasa(self, bodies, radii, probe, dots):
+        """Exposed-dot counts and fractions; dots [n_types, n_dots, 4]."""
+        b = _f32(bodies).reshape(-1, 4)
+        radii = _f32(radii)
+        dots = _f32(dots)
+        nd = dots.shape[1]
+        counts = np.zeros(len(b), dtype=np.uint32)
+        frac = np.zeros(len(b), dtype=np.float32)
+        self._check(self._L.rsx_calc_rsasa(self.h, _ptr(b, ctypes.c_float), len(b), _ptr(radii, ctypes.c_float),
+                                           len(radii), float(probe), _ptr(dots, ctypes.c_float), nd,
+                                           _ptr(counts, ctypes.c_uint), _ptr(frac, ctypes.c_float)))
+        return counts, frac
+
+    # -- batched ensemble ----------------------------------------------------
+    def batch_set_frames(self, bodies, offsets):
+        b = _f32(bodies).reshape(-1, 4)
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        self._check(self._L.rsx_batch_set_frames(self.h, _ptr(b, ctypes.c_float), _ptr(off, ctypes.c_int),
+                                                 len(off) - 1))
+        self.n_frames = len(off) - 1
+
+    def batch_calc(self):
+        frames = np.zeros((self.n_frames, self.nq), dtype=np.float64)
+        mean = np.zeros(self.nq, dtype=np.float64)
+        self._check(self._L.rsx_batch_calc(self.h, _ptr(frames, ctypes.c_double), _ptr(mean, ctypes.c_double)))
+        return frames, mean
+
+    # -- differentiable fit ---------------------------------------------------
+    def chi2_grad(self, bodies, i_exp, sigma):
+        b = _f32(bodies).reshape(-1, 4)
+        ie, sg = _f32(i_exp), _f32(sigma)
+        chi2, scale = ctypes.c_double(), ctypes.c_double()
+        grad = np.zeros((len(b), 3), dtype=np.float32)
+        iq = np.zeros(self.nq, dtype=np.float64)
+        self._check(self._L.rsx_chi2_grad(self.h, _ptr(b, ctypes.c_float), len(b), _ptr(ie, ctypes.c_float),
+                                          _ptr(sg, ctypes.c_float), ctypes.byref(chi2), ctypes.byref(scale),
+                                          _ptr(grad, ctypes.c_float), _ptr(iq, ctypes.c_double)))
+        return chi2.value, scale.value, grad, iq
+
+    def vjp(self, bodies, w):
+        b = _f32(bodies).reshape(-1, 4)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        grad = np.zeros((len(b), 3), dtype=np.float32)
+        self._check(self._L.rsx_debye_vjp(self.h, _ptr(b, ctypes.c_float), len(b), _ptr(w, ctypes.c_double),
+                                          _ptr(grad, ctypes.c_float)))
+        return grad
+
+    # -- measurement -----------------------------------------------------------
+    def time_forward(self, reps):
+        tot, pair = ctypes.c_float(), ctypes.c_float()
+        self._check(self._L.rsx_time_forward(self.h, int(reps), ctypes.byref(tot), ctypes.byref(pair)))
+        return tot.value, pair.value
+
+    def synchronize(self):
+        self._check(self._L.rsx_synchronize(self.h))
+
+    def dev_alloc(self, nbytes):
+        p = ctypes.c_void_p()
+        self._check(self._L.rsx_dev_alloc(self.h, ctypes.byref(p), int(nbytes)))
+        return p
+
+    def dev_free(self, p):
+        self._check(self._L.rsx_dev_free(self.h, p))
+
+    def h2d(self, dst, arr):
+        arr = np.ascontiguousarray(arr)
+        self._check(self._L.rsx_memcpy_h2d(self.h, dst, arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes))
+
+    def d2h(self, arr, src):
+        self._check(self._L.rsx_memcpy_d2h(self.h, arr.ctypes.data_as(ctypes.c_void_p), src, arr.nbytes))
+        return arr
+
+    def calc_curve_dev_async(self, bodies_dev, n, out_dev=None):
+        self._check(self._L.rsx_calc_curve_dev_async(self.h, bodies_dev, int(n), out_dev))
+
+    def batch_calc_dev_async(self, frames_dev=None, mean_dev=None):
+        self._check(self._L.rsx_batch_calc_dev_async(self.h, frames_dev, mean_dev))
+
+
+def device_count():
+    c = ctypes.c_int()
+    rc = _lib.lib().rsx_device_count(ctypes.byref(c))
+    if rc != 0:
+        raise RsxError(rc, "device count")
+    return c.value
+
+
+def fibonacci_dots(n_half, radius, probe):
+    out = np.zeros((2 * n_half, 4), dtype=np.float32)
+    rc = _lib.lib().rsx_fibonacci_dots(int(n_half), float(radius), float(probe), _ptr(out, ctypes.c_float))
+    if rc:
+        raise RsxError(rc, "fibonacci_dots")
+    return out
+
+
+def dodeca_icosa_dots(radius, probe):
+    out = np.zeros((32, 4), dtype=np.float32)
+    rc = _lib.lib().rsx_dodeca_icosa_dots(float(radius), float(probe), _ptr(out, ctypes.c_float))
+    if rc:
+        raise RsxError(rc, "dodeca_icosa_dots")
+    return out
+
+
+def atomic_form_factors(q, avg_radius=1.62, expansion=1.0):
+    q = _f32(q)
+    out = np.zeros((len(q), 16), dtype=np.float32)
+    rc = _lib.lib().rsx_atomic_form_factors(_ptr(q, ctypes.c_float), len(q), float(avg_radius), float(expansion),
+                                            _ptr(out, ctypes.c_float))
+    if rc:
+        raise RsxError(rc, "atomic_form_factors")
+    return out
+
+
+def atomic_radii():
+    out = np.zeros(16, dtype=np.float32)
+    rc = _lib.lib().rsx_atomic_radii(_ptr(out, ctypes.c_float))
+    if rc:
+        raise RsxError(rc, "atomic_radii")
+    return out

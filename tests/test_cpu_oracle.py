@@ -1,0 +1,226 @@
+"""CPU suite: the oracle pinned against the reference's golden fixture and
+against closed forms, brute force and invariances; the C-ABI library's
+exports and host-side generators (no GPU needed)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import rel_err
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden", "test1888.npz")
+
+
+def _sinc_sum_numpy(q, pos, F):
+    """Independent O(N^2) evaluation: full ordered double sum with numpy's sinc."""
+    pos = np.asarray(pos, dtype=np.float64)
+    d = np.sqrt(((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1))
+    out = []
+    for k, qk in enumerate(np.asarray(q, dtype=np.float64)):
+        S = np.sinc(qk * d / np.pi)          # numpy sinc(x) = sin(pi x)/(pi x)
+        out.append(F[:, k] @ S @ F[:, k])
+    return np.array(out)
+
+
+# ---------------------------------------------------------------- Debye oracle
+def test_oracle_matches_golden_fixture(oracle):
+    g = np.load(GOLDEN)
+    F = oracle.body_factors(g["pos"], g["types"], g["q"], g["table"])
+    I = oracle.debye(g["q"], g["pos"], F)
+    err = rel_err(I, g["Iq"])
+    # the fixture's positions are rounded to 1e-3 A: +-0.5e-3 A moves I by ~2-3e-5 at high q
+    assert err.max() < 5e-5
+    assert err[g["q"] <= 0.135].max() < 2e-6
+    assert np.median(err) < 1e-5
+    assert abs(I[0] - F[:, 0].sum() ** 2) < 1e-9 * I[0]
+
+
+def test_reference_loop_equals_fast_oracle(oracle):
+    g = np.load(GOLDEN)
+    n = 600
+    pos, types = g["pos"][:n], g["types"][:n]
+    F = oracle.body_factors(pos, types, g["q"], g["table"])
+    fast = oracle.debye(g["q"], pos, F)
+    ref = oracle.debye_ref_rows(g["q"], pos, types, g["table"])
+    assert rel_err(ref, fast).max() < 1e-12
+    # row ranges add up to the full sum
+    a = oracle.debye_ref_rows(g["q"], pos, types, g["table"], 0, 250)
+    b = oracle.debye_ref_rows(g["q"], pos, types, g["table"], 250, n)
+    assert rel_err(a + b, ref).max() < 1e-12
+
+
+def test_oracle_brute_force_and_closed_forms(oracle):
+    rng = np.random.default_rng(3)
+    q = np.linspace(0.0, 0.8, 17).astype(np.float32)
+    pos = rng.uniform(-8, 8, size=(40, 3)).astype(np.float32)
+    pos[5] = pos[9]                                  # a coincident pair: sinc(0) = 1
+    types = rng.integers(0, 4, size=40).astype(np.uint32)
+    table = rng.uniform(0.5, 9, size=(17, 4)).astype(np.float32)
+    F = oracle.body_factors(pos, types, q, table)
+    assert rel_err(oracle.debye(q, pos, F), _sinc_sum_numpy(q, pos, F)).max() < 1e-13
+    # one atom: F^2; two atoms: F1^2 + F2^2 + 2 F1 F2 sin(qr)/(qr)
+    one = oracle.debye(q, pos[:1], F[:1])
+    assert rel_err(one, F[0] ** 2).max() < 1e-15
+    two = oracle.debye(q, pos[:2], F[:2])
+    r = np.linalg.norm(pos[0].astype(np.float64) - pos[1])
+    x = q.astype(np.float64) * r
+    s = np.where(x == 0, 1.0, np.sin(x) / np.where(x == 0, 1, x))
+    assert rel_err(two, F[0] ** 2 + F[1] ** 2 + 2 * F[0] * F[1] * s).max() < 1e-14
+    # rows: the rows partial over all rows is the full sum
+    rows = oracle.debye_rows(q, pos, F, np.arange(40))
+    assert rel_err(rows, oracle.debye(q, pos, F)).max() < 1e-13
+
+
+def test_oracle_invariances(oracle):
+    rng = np.random.default_rng(4)
+    q = np.linspace(0.0, 0.5, 33).astype(np.float32)
+    pos = rng.uniform(-20, 20, size=(300, 3))
+    types = rng.integers(0, 3, size=300).astype(np.uint32)
+    table = rng.uniform(1, 8, size=(33, 3)).astype(np.float32)
+    F = oracle.body_factors(pos, types, q, table)
+    I0 = oracle.debye_xyz64(q, pos, F)
+    th = 1.1
+    R = np.array([[1, 0, 0], [0, np.cos(th), -np.sin(th)], [0, np.sin(th), np.cos(th)]])
+    perm = rng.permutation(300)
+    I1 = oracle.debye_xyz64(q, (pos @ R.T + 5.0)[perm], F[perm])
+    assert rel_err(I1, I0).max() < 1e-11
+    # high-q limit: I -> sum F^2 (the cross terms average out as 1/(q r))
+    qh = np.array([400.0], dtype=np.float32)
+    Fh = np.ones((300, 1))
+    Ih = oracle.debye_xyz64(qh, pos, Fh)
+    assert abs(Ih[0] - 300.0) < 0.05 * 300
+
+
+# ---------------------------------------------------------------- SASA oracle
+def test_fibonacci_dots_geometry(oracle):
+    d = oracle.fibonacci_dots(128, 1.7, 1.4)
+    assert d.shape == (256, 4)
+    rr = np.linalg.norm(d[:, :3].astype(np.float64), axis=1)
+    assert np.allclose(rr, 3.1, rtol=2e-6)
+    i = np.arange(-127, 129)
+    assert np.allclose(d[:, 2], (i / 128.0) * 3.1, atol=3e-6)
+    assert np.all(d[:, 3] == np.float32(1.7))
+    ico = oracle.dodeca_icosa_dots(1.5, 1.8)
+    assert np.allclose(np.linalg.norm(ico[:, :3].astype(np.float64), axis=1), 3.3, rtol=1e-5)
+
+
+def test_sasa_oracle_closed_forms(oracle):
+    radii = np.array([1.5], dtype=np.float32)
+    probe = 1.4
+    nh = 64
+    dots = oracle.fibonacci_dots(nh, radii[0], probe)[None]
+    R2 = ((radii + np.float32(probe)) ** 2).astype(np.float32)
+    one = oracle.sasa_counts(np.zeros((1, 3)), [0], R2, dots, cells=False)
+    assert one[0] == 2 * nh
+    # second atom on the +z axis at distance d: a dot at height z_i = (i/n) R is
+    # buried iff R^2 + d^2 - 2 R d z_i/R < R^2, i.e. i/n > d / (2R)
+    R = 2.9
+    for d in (1.0, 2.3, 4.1, 5.6):
+        pos = np.array([[0, 0, 0], [0, 0, d]], dtype=np.float32)
+        c = oracle.sasa_counts(pos, [0, 0], R2, dots, cells=False)
+        i = np.arange(-(nh - 1), nh + 1)
+        expect = int(np.sum(i / nh <= d / (2 * R)))
+        assert c[0] == expect, d
+        # the upper atom sees the lower one at -z: buried iff i/n < -d / (2R)
+        assert c[1] == int(np.sum(i / nh >= -d / (2 * R))), d
+    # cells == brute force on a random cluster
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(-7, 7, size=(200, 3)).astype(np.float32)
+    types = rng.integers(0, 2, size=200).astype(np.uint32)
+    radii2 = np.array([1.7, 1.2], dtype=np.float32)
+    dots2 = np.stack([oracle.fibonacci_dots(32, r, probe) for r in radii2])
+    R22 = ((radii2 + np.float32(probe)) ** 2).astype(np.float32)
+    a = oracle.sasa_counts(pos, types, R22, dots2, cells=False)
+    b = oracle.sasa_counts(pos, types, R22, dots2, cells=True)
+    assert np.array_equal(a, b)
+
+
+# ------------------------------------------------------------- form factors
+def test_atomic_form_factors(oracle):
+    q = np.array([0.0, 0.2, 0.5], dtype=np.float32)
+    t = oracle.atomic_ff(q)
+    # carbon at q = 0: a1+a2+a3+a4+c - rho0 V  (ff_coef.hpp:41, constants.hpp:63)
+    c0 = 2.31 + 1.02 + 1.5886 + 0.865 + 0.2156 - 0.334 * 16.44
+    assert abs(t[0, 1] - c0) < 1e-5
+    # water (OH2) at q = 0 is about 3.5 (gpu_saxs_pt_wf.cl:128 comment)
+    assert abs(t[0, 14] - 3.51) < 0.01
+    # complex types are base + n_H * H
+    assert np.allclose(t[:, 9], t[:, 1] + 3 * t[:, 0], atol=1e-6)
+    radii = oracle.atomic_radii()
+    assert abs(radii[1] - 1.5774) < 1e-3 and abs(radii[9] - 1.9672) < 1e-3
+    # expansion factor c != 1 changes only the excluded-volume part
+    t2 = oracle.atomic_ff(q, 1.62, 1.1)
+    assert np.all(t2[0, :7] != t[0, :7])
+
+
+def test_chi2_scale_and_gradient_oracle(oracle):
+    rng = np.random.default_rng(6)
+    q = np.linspace(0.02, 0.4, 24).astype(np.float32)
+    pos = rng.uniform(-6, 6, size=(30, 3))
+    types = rng.integers(0, 2, size=30).astype(np.uint32)
+    table = rng.uniform(1, 6, size=(24, 2)).astype(np.float32)
+    F = oracle.body_factors(pos.astype(np.float32), types, q, table)
+    I = oracle.debye_xyz64(q, pos, F)
+    iexp = I * 1.3 * (1 + 0.02 * rng.normal(size=24))
+    sig = 0.05 * I
+    c, x2 = oracle.scale_chi2(iexp, sig, I)
+    for dc in (-1e-3, 1e-3):
+        c2 = c * (1 + dc)
+        assert np.sum((iexp - c2 * I) ** 2 / sig ** 2) > x2
+    g, _, _, _ = oracle.chi2_grad(q, pos, F, iexp, sig)
+    comps = np.arange(0, 90, 7)
+    fd = oracle.chi2_fd(q, pos, F, iexp, sig, comps, h=1e-4)
+    assert np.allclose(g.reshape(-1)[comps], fd, rtol=1e-5, atol=1e-7 * np.abs(fd).max())
+    assert np.linalg.norm(g.sum(0)) < 1e-9 * np.linalg.norm(g)
+
+
+# -------------------------------------------------------- library (no GPU)
+def _declared_symbols():
+    text = open(os.path.join(ROOT, "include", "rsx_capi.h")).read()
+    return sorted(set(re.findall(r"\b(rsx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from resaxs_amd import _lib
+    lib_path = _lib.LIB_PATH
+    assert os.path.exists(lib_path), "build the library first (__graft_entry__.build())"
+    out = subprocess.run(["nm", "-D", "--defined-only", lib_path], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (rsx_[a-z0-9_]+)", out))
+    declared = _declared_symbols()
+    assert declared, "no declarations found"
+    missing = [s for s in declared if s not in exported]
+    assert not missing, missing
+    assert set(declared) == set(_lib.SIGNATURES), "ctypes table out of sync with the header"
+    _lib.lib()   # loads and binds every signature
+
+
+def test_host_generators_equal_oracle(oracle):
+    import resaxs_amd as ra
+    for nh, r, p in ((128, 1.7, 1.4), (32, 1.2, 1.8), (5, 0.9, 0.0)):
+        assert np.array_equal(ra.fibonacci_dots(nh, r, p), oracle.fibonacci_dots(nh, r, p))
+    for r in (1.07, 1.58, 1.97):
+        assert np.array_equal(ra.dodeca_icosa_dots(r, 1.8), oracle.dodeca_icosa_dots(r, 1.8))
+    q = np.linspace(0, 0.75, 51).astype(np.float32)
+    for c in (1.0, 0.95, 1.08):
+        assert np.array_equal(ra.atomic_form_factors(q, 1.62, c), oracle.atomic_ff(q, 1.62, c))
+    assert np.array_equal(ra.atomic_radii(), oracle.atomic_radii())
+
+
+def test_workload_generators():
+    from resaxs_amd import configs
+    w = configs.config1()
+    assert list(np.bincount(w.types)) == [604, 170, 189, 19, 906]
+    assert np.all(np.linalg.norm(w.pos, axis=1) <= 18.0 + 1e-5)
+    w2 = configs.config2(n=5000)
+    ax = configs.ellipsoid_axes(5000)
+    u = w2.pos / ax
+    assert np.all((u ** 2).sum(1) <= 1 + 1e-5)
+    assert w2.q[0] == 0 and abs(w2.q[-1] - 0.6) < 1e-7 and len(w2.q) == 256
+    assert abs(configs.ellipsoid_axes(20000)[0] - 35.66) < 0.01
+    assert w2.table.shape == (256, 6)
+    assert 2.0 < w2.w < 3.5
+    w4 = configs.config4(n_frames=16)
+    assert w4.offsets[-1] == len(w4.bodies) and np.all(np.diff(w4.offsets) >= 1000)
