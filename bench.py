@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark of the Debye-SAXS hot path on one MI355X.
+
+Workload (north-star configuration 2): 20,000 atoms at 0.045 A^-3 in a
+1:1.3:1.8 ellipsoid, SASA hydration (256 Fibonacci dots, probe 1.4 A,
+contrast 0.03 e/A^3), 256 q over 0-0.6 A^-1.  A step is one full device-
+resident forward pass: SASA dot counts -> hydration classes -> Debye pair
+kernel -> FP64 reduction, over positions already in HBM.
+
+Prints ONE JSON line.  value = forward pair.q terms/s; roofline = the pair
+kernel's algorithmic FP32 FLOP rate (P (5 n_q + 9) + 2 N n_q per launch)
+over its HIP-event duration on the engine stream, against the FP32 peak.
+"""
+import argparse
+import json
+import os
+import platform
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "debye_pair_kernel_sasa-hydrated-20000x256_counters.txt")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_engine(w, hydrate=True):
+    from resaxs_amd import RSX_DOTS_FIBONACCI, Engine
+    e = Engine(0)
+    e.set_q(w.q)
+    e.set_factors(w.table, w.water_index)
+    if hydrate:
+        e.set_sasa_params(w.radii, w.probe, RSX_DOTS_FIBONACCI, w.n_half)
+        e.set_water_weight(w.w)
+    e.initialize(len(w.bodies))
+    return e
+
+
+def read_traffic():
+    """HBM bytes per pair-kernel launch from the committed rocprofv3 counter summary."""
+    try:
+        text = open(TRAFFIC_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def forward_bench(steps, warmup):
+    from resaxs_amd import configs
+    w = configs.config2()
+    n, nq = len(w.bodies), len(w.q)
+    e = make_engine(w)
+    log(f"bench: {w.name}: host calc_curve (layout + first result)")
+    I_host = e.calc_curve(w.bodies)
+    d_b = e.dev_alloc(n * 16)
+    d_out = e.dev_alloc(nq * 8)
+    e.h2d(d_b, w.bodies)
+    for _ in range(warmup):
+        e.calc_curve_dev_async(d_b, n, d_out)
+    e.synchronize()
+    log(f"bench: timing {steps} device-resident steps")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.calc_curve_dev_async(d_b, n, d_out)
+    e.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / steps
+    I_dev = e.d2h(np.zeros(nq, dtype=np.float64), d_out)
+    # same kernels and layout; only the q = 0 value is summed in another order
+    if not np.all(np.abs(I_dev - I_host) <= 1e-12 * np.abs(I_host)):
+        raise RuntimeError("device-resident step differs from the host-path result")
+    # pair-kernel duration from HIP events on the engine stream
+    _, pair_ms = e.time_forward(10)
+    e.dev_free(d_b)
+    e.dev_free(d_out)
+    e.close()
+    return w, I_host, ms_per_step, pair_ms
+
+
+def adjoint_bench(reps=3):
+    """Forward + adjoint (chi^2 gradient) on the fit configuration (10,000 x 200)."""
+    from resaxs_amd import configs
+    w = configs.config5()
+    n, nq = len(w.bodies), len(w.q)
+    e = make_engine(w, hydrate=False)
+    I_ref = e.calc_curve(w.bodies)
+    rng = w.noise_rng
+    i_exp = (I_ref * (1.0 + 0.01 * rng.normal(size=nq))).astype(np.float32)
+    sigma = (0.01 * I_ref).astype(np.float32)
+    e.chi2_grad(w.start_bodies, i_exp, sigma)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e.chi2_grad(w.start_bodies, i_exp, sigma)
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e.calc_curve(w.start_bodies)
+    ms_fwd = (time.perf_counter() - t0) * 1e3 / reps
+    e.close()
+    terms = n * (n - 1) / 2.0 * nq
+    return {"workload": w.name, "ms_fwd_adjoint": round(ms, 3), "ms_fwd": round(ms_fwd, 3),
+            "ratio_to_fwd": round(ms / ms_fwd, 3), "terms_per_s": terms / (ms * 1e-3),
+            "timing": "host wall clock around the C-ABI call (host buffers in and out)"}
+
+
+def cpu_baseline(w, I_gpu, rows=2048):
+    """The reference CPU loop (host_debye.cpp:54-109 restated in oracle/) on a bounded
+    row sample, 1 thread; plus the FP64 oracle's full I(q) as the accuracy check."""
+    from oracle import pyoracle as po
+    nq = len(w.q)
+    log(f"bench: CPU baseline, reference loop over rows 0..{rows - 1}, 1 thread")
+    t0 = time.perf_counter()
+    po.debye_ref_rows(w.q, w.pos, w.types, w.table, 0, rows, nthreads=1)
+    t = time.perf_counter() - t0
+    terms = rows * (rows - 1) / 2.0 * nq
+    log("bench: FP64 oracle of the full workload (accuracy check)")
+    dots = np.stack([po.fibonacci_dots(w.n_half, r, w.probe) for r in w.radii])
+    rr = (w.radii.astype(np.float32) + np.float32(w.probe)).astype(np.float32)
+    counts = po.sasa_counts(w.pos, w.types, (rr * rr).astype(np.float32), dots)
+    s = (counts.astype(np.float32) / np.float32(dots.shape[1])).astype(np.float32)
+    F = po.body_factors(w.pos, w.types, w.q, w.table, w.water_index, w.w, s)
+    I_ref = po.debye(w.q, w.pos, F)
+    err = np.abs(I_gpu - I_ref) / np.abs(I_ref)
+    base = {"value": terms / t, "unit": "pair.q terms/s", "cores": 1, "kind": "port",
+            "sample": f"reference loop (host_debye.cpp:54-109) over rows 0..{rows - 1} of the "
+                      f"{len(w.pos)}-atom lower triangle x {nq} q = {terms:.3e} terms, {t:.1f} s",
+            "cpu": platform.processor() or platform.machine()}
+    acc = {"max": float(err.max()), "median": float(np.median(err)), "reference": "FP64 oracle, full I(q)"}
+    return base, acc
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of this many MiB (counter calibration)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding plan is in "
+              "DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from resaxs_amd import configs
+    if args.copy_probe:
+        from resaxs_amd import Engine
+        e = Engine(0)
+        ms = e.time_copy(args.copy_probe << 20, 3)
+        log(f"bench: copy probe {args.copy_probe} MiB: {ms:.4f} ms/launch, "
+            f"{2 * (args.copy_probe << 20) / ms / 1e6:.1f} GB/s (read + write)")
+        e.close()
+    w, I_gpu, ms_per_step, pair_ms = forward_bench(args.steps, args.warmup)
+    n, nq = len(w.pos), len(w.q)
+    terms = configs.pair_terms(n, nq)
+    flops = configs.forward_flops(n, nq)
+    achieved = flops / (pair_ms * 1e-3) / 1e12
+    line = {
+        "metric": "Debye pair.q terms/s (forward, device-resident step incl. SASA hydration)",
+        "value": terms / (ms_per_step * 1e-3),
+        "unit": "pair.q terms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded)",
+        "config": {"workload": w.name, "n_atoms": n, "n_q": nq, "q_max": float(w.q[-1]),
+                   "hydration": "SASA, 256 Fibonacci dots, probe 1.4 A, contrast 0.03 e/A^3"},
+        "roofline": {"bound": "mfma", "pipe": "VALU (FP32 + FP64 vector; no MFMA)",
+                     "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": read_traffic(),
+                     "kernel": "debye_uniform_kernel", "kernel_ms": round(pair_ms, 4),
+                     "flop_per_launch": flops},
+    }
+    if not args.no_adjoint:
+        log("bench: forward + adjoint on the fit configuration")
+        line["fwd_adjoint"] = adjoint_bench()
+    if not args.no_cpu_baseline:
+        base, acc = cpu_baseline(w, I_gpu)
+        line["cpu_baseline"] = base
+        line["rel_err_vs_fp64"] = acc
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -122,6 +122,9 @@ int rsx_autotune(rsx_engine *h, const char *cache_path);
 int rsx_set_tuning(rsx_engine *h, int k, int col_range);
 /* average device time (ms) of the last timed region, measured with HIP events on the engine stream */
 int rsx_time_forward(rsx_engine *h, int reps, float *ms_total, float *ms_pair_kernel);
+/* plain device-to-device copy of `bytes` (float4 grid-stride kernel), average ms per launch;
+ * the known-size reference for HBM counter calibration */
+int rsx_time_copy(rsx_engine *h, long long bytes, int reps, float *ms);
 /* device memory helpers for callers without another allocator */
 int rsx_dev_alloc(rsx_engine *h, void **ptr, long long bytes);
 int rsx_dev_free(rsx_engine *h, void *ptr);

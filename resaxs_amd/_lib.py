@@ -65,6 +65,7 @@ SIGNATURES = {
     "rsx_autotune": (_i, [_vp, _c.c_char_p]),
     "rsx_set_tuning": (_i, [_vp, _i, _i]),
     "rsx_time_forward": (_i, [_vp, _i, _fp, _fp]),
+    "rsx_time_copy": (_i, [_vp, _ll, _i, _fp]),
     "rsx_dev_alloc": (_i, [_vp, _c.POINTER(_vp), _ll]),
     "rsx_dev_free": (_i, [_vp, _vp]),
     "rsx_memcpy_h2d": (_i, [_vp, _vp, _vp, _ll]),

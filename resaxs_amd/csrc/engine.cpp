@@ -259,7 +259,8 @@ struct rsx_engine {
 
     // device buffers
     DBuf<float4> d_cols, d_saved_cols, d_rows, d_vals, d_sasa_atoms;
-    DBuf<float> d_ftab, d_qrev, d_qv, d_r2, d_rext;
+    DBuf<float> d_ftab, d_qv, d_r2, d_rext;
+    DBuf<double> d_qrev;
     DBuf<ChunkConst> d_chunks;
     DBuf<double> d_invq, d_partials, d_iq, d_saved_iq, d_out, d_wq, d_q0;
     DBuf<PairJob> d_jobs, d_fjobs;  // incremental jobs; forward (triangle) jobs
@@ -268,7 +269,8 @@ struct rsx_engine {
     DBuf<int> d_slot_begin, d_idx, d_bucket_of, d_bucket_count, d_bucket_start, d_bucket_fill, d_sorted, d_s2c;
     DBuf<uint32_t> d_counts;
     DBuf<float4> d_dots;
-    DBuf<float> d_gpart, d_grad;
+    DBuf<double> d_gpart;
+    DBuf<float> d_grad;
     size_t ftab_rows = 0;
 
     // last forward launch (for timing)
@@ -319,10 +321,10 @@ void choose_chunks(rsx_engine* h)
     for (int i = 0; i < nk; ++i) {
         const int K = rsx::debye_k_value(uni, i);
         if (h->k_req > 0 && K != h->k_req) continue;
-        if (h->k_req <= 0 && K > 56) continue;  // 64 needs more than 256 VGPRs
+        if (h->k_req <= 0 && !uni && K > 48) continue;  // the direct kernel spills at K = 64
         const int C = (h->nq + K - 1) / K;
-        // per (pair, chunk): 2K recurrence ops + ~40 for distance and seeds
-        const double cost = uni ? C * (2.0 * K + 40.0) : C * (6.0 * K + 12.0);
+        // per (pair, chunk): 2K recurrence ops + ~44 for distance and seeds
+        const double cost = uni ? C * (2.0 * K + 44.0) : C * (6.0 * K + 12.0);
         if (cost < best) {
             best = cost;
             bestK = K;
@@ -347,15 +349,15 @@ void apply_path(rsx_engine* h, bool uniform)
         const int k0 = c * h->K;
         const double a = k0 < nq ? (double)h->q[k0] : 0.0;
         cc[c].a = (float)a;
-        cc[c].a_rev = (float)(a / (2.0 * PI));
-        cc[c].mid_rev = (float)((a + 0.5 * h->dq) / (2.0 * PI));
+        cc[c].a_rev = a / (2.0 * PI);
+        cc[c].mid_rev = (a + 0.5 * h->dq) / (2.0 * PI);
         cc[c].pad = 0.0f;
     }
-    std::vector<double> invq(ldf, 0.0);
-    std::vector<float> qrev(ldf, 0.0f), qv(ldf, 0.0f);
+    std::vector<double> invq(ldf, 0.0), qrev(ldf, 0.0);
+    std::vector<float> qv(ldf, 0.0f);
     for (int k = 0; k < nq; ++k) {
         invq[k] = h->q[k] != 0.0f ? 1.0 / (double)h->q[k] : 0.0;
-        qrev[k] = (float)((double)h->q[k] / (2.0 * PI));
+        qrev[k] = (double)h->q[k] / (2.0 * PI);
         qv[k] = h->q[k];
     }
     h->d_chunks.ensure(cc.size());
@@ -364,7 +366,7 @@ void apply_path(rsx_engine* h, bool uniform)
     h->d_qv.ensure(ldf);
     hipchk(hipMemcpy(h->d_chunks.p, cc.data(), cc.size() * sizeof(ChunkConst), hipMemcpyHostToDevice), "upload chunks");
     hipchk(hipMemcpy(h->d_invq.p, invq.data(), ldf * sizeof(double), hipMemcpyHostToDevice), "upload invq");
-    hipchk(hipMemcpy(h->d_qrev.p, qrev.data(), ldf * sizeof(float), hipMemcpyHostToDevice), "upload qrev");
+    hipchk(hipMemcpy(h->d_qrev.p, qrev.data(), ldf * sizeof(double), hipMemcpyHostToDevice), "upload qrev");
     hipchk(hipMemcpy(h->d_qv.p, qv.data(), ldf * sizeof(float), hipMemcpyHostToDevice), "upload qv");
     h->params_dirty = true;  // the class table is laid out with stride ldf
 }
@@ -451,7 +453,7 @@ void build_class_table(rsx_engine* h)
     h->n_class = h->n_types * h->d1;
     const int nq = h->nq, ldf = h->ldf;
     h->f64.assign((size_t)h->n_class * nq, 0.0);
-    std::vector<float> ft((size_t)h->n_class * ldf, 0.0f);
+    std::vector<float> ft((size_t)(h->n_class + 1) * ldf, 0.0f);  // + an all-zero row
     for (int t = 0; t < h->n_types; ++t)
         for (int c = 0; c < h->d1; ++c) {
             const int id = t * h->d1 + c;
@@ -654,6 +656,7 @@ rsx::DebyeArgs debye_args(rsx_engine* h, const PairJob* d_jobs, double* partials
     a.n_chunks = h->n_chunks;
     a.ftab = h->d_ftab.p;
     a.ldf = h->ldf;
+    a.zero_class = h->n_class;
     a.chunks = h->d_chunks.p;
     a.half_dq = 0.5 * h->dq;
     a.dq = (float)h->dq;
@@ -1109,20 +1112,21 @@ void adjoint_impl(rsx_engine* h, const std::vector<double>& wk, float* out_grad)
 {
     const int n = h->n;
     set_device(h);
+    // q per thread for the adjoint: per (pair, chunk) 4K ops (two recurrences,
+    // combine, accumulate) plus ~60 for distance and seeds
     int K = 0;
+    double best = 1e300;
     for (int i = 0; i < rsx::adjoint_k_count(); ++i) {
         const int k = rsx::adjoint_k_value(i);
-        if (k <= 48 && k >= std::min(h->K, 48)) {
+        const int c = (h->nq + k - 1) / k;
+        const double cost = c * (4.0 * k + 60.0);
+        if (cost < best) {
+            best = cost;
             K = k;
-            break;
         }
     }
-    if (K == 0) K = 32;
     const int nch = (h->nq + K - 1) / K;
     const int ldf = h->ldf;
-    if (nch * K > ldf) {
-        // the class table is padded to ldf; recompute chunk set-up for the adjoint's K
-    }
     std::vector<double> wq(std::max(ldf, nch * K), 0.0);
     for (int k = 0; k < h->nq; ++k) wq[k] = h->q[k] == 0.0f ? 0.0 : wk[k];
     // chunk constants for the adjoint's chunking
@@ -1131,8 +1135,8 @@ void adjoint_impl(rsx_engine* h, const std::vector<double>& wk, float* out_grad)
         const int k0 = c * K;
         const double a = k0 < h->nq ? (double)h->q[k0] : 0.0;
         cc[c].a = (float)a;
-        cc[c].a_rev = (float)(a / (2.0 * PI));
-        cc[c].mid_rev = (float)((a + 0.5 * h->dq) / (2.0 * PI));
+        cc[c].a_rev = a / (2.0 * PI);
+        cc[c].mid_rev = (a + 0.5 * h->dq) / (2.0 * PI);
         cc[c].pad = 0.0f;
     }
     // class table and 1/q with the adjoint's padding
@@ -1271,7 +1275,9 @@ int rsx_destroy(rsx_engine* h)
     if (h->stream) hipStreamSynchronize(h->stream);
     DBuf<float4>* f4[] = {&h->d_cols, &h->d_saved_cols, &h->d_rows, &h->d_vals, &h->d_sasa_atoms, &h->d_dots, &h->b_cols};
     for (auto* b : f4) b->release();
-    DBuf<float>* ff[] = {&h->d_ftab, &h->d_qrev, &h->d_qv, &h->d_r2, &h->d_rext, &h->d_gpart, &h->d_grad};
+    h->d_qrev.release();
+    h->d_gpart.release();
+    DBuf<float>* ff[] = {&h->d_ftab, &h->d_qv, &h->d_r2, &h->d_rext, &h->d_grad};
     for (auto* b : ff) b->release();
     DBuf<double>* fd[] = {&h->d_invq, &h->d_partials, &h->d_iq, &h->d_saved_iq, &h->d_out, &h->d_wq, &h->d_q0,
                           &h->b_partials, &h->b_out, &h->b_mean, &h->b_q0};
@@ -1632,6 +1638,35 @@ int rsx_time_forward(rsx_engine* h, int reps, float* ms_total, float* ms_pair)
         hipchk(hipEventElapsedTime(&tot, h->ev0, h->ev1), "elapsed");
         if (ms_total) *ms_total = tot / reps;
         if (ms_pair) *ms_pair = pair_ms / reps;
+    });
+}
+
+int rsx_time_copy(rsx_engine* h, long long bytes, int reps, float* ms)
+{
+    return guarded(h, [&] {
+        if (bytes < 16 || reps <= 0) fail(RSX_INVALID_ARG, "time_copy: bytes >= 16 and reps > 0");
+        set_device(h);
+        const long long n = bytes / 16;
+        DBuf<float4> a, b;
+        a.ensure(n);
+        b.ensure(n);
+        try {
+            hipchk(rsx::launch_fill_i32((int*)a.p, 0, n * 4, h->stream), "fill");
+            hipchk(rsx::launch_copy(b.p, a.p, n, h->stream), "copy warm-up");
+            hipchk(hipEventRecord(h->ev0, h->stream), "event");
+            for (int r = 0; r < reps; ++r) hipchk(rsx::launch_copy(b.p, a.p, n, h->stream), "copy");
+            hipchk(hipEventRecord(h->ev1, h->stream), "event");
+            hipchk(hipEventSynchronize(h->ev1), "event sync");
+            float t = 0.0f;
+            hipchk(hipEventElapsedTime(&t, h->ev0, h->ev1), "elapsed");
+            if (ms) *ms = t / reps;
+        } catch (...) {
+            a.release();
+            b.release();
+            throw;
+        }
+        a.release();
+        b.release();
     });
 }
 

@@ -28,10 +28,12 @@ struct PairJob {
 };
 
 // Per-q-chunk constants of the uniform-grid path: a = first q of the chunk,
-// *_rev = value / (2 pi) (v_sin/v_cos take revolutions).
+// *_rev = value / (2 pi) (v_sin/v_cos take revolutions).  The seed phases
+// q r / (2 pi) are formed in FP64 and reduced there: an FP32 constant would
+// carry one rounding error shared by every pair, i.e. a coherent phase shift.
 struct ChunkConst {
-    float a_rev;
-    float mid_rev;  // (a + dq/2) / (2 pi)
+    double a_rev;    // a / (2 pi)
+    double mid_rev;  // (a + dq/2) / (2 pi)
     float a;
     float pad;
 };
@@ -39,13 +41,15 @@ struct ChunkConst {
 struct DebyeArgs {
     const PairJob* jobs;
     int n_chunks;         // q chunks per job (one wave each)
-    const float* ftab;    // [n_class][ldf] FP32 form factors per class, zero padded
+    const float* ftab;    // [n_class + 1][ldf] FP32 form factors per class, zero padded;
+                          // row zero_class (= n_class) is all zeros
     int ldf;              // n_chunks * K
+    int zero_class;
     const ChunkConst* chunks;
     double half_dq;       // dq / 2 (FP64)
     float dq;
     const double* invq;   // [ldf] 1/q, 0 for q == 0 and padding
-    const float* qrev;    // [ldf] q / (2 pi)  (direct path)
+    const double* qrev;   // [ldf] q / (2 pi)  (direct path)
     const float* qv;      // [ldf] q           (direct path)
     double* partials;     // [n_slots][ldf]
 };
@@ -108,18 +112,21 @@ struct AdjointArgs {
     float dq;
     const double* wq;        // [ldf] w_k / 1 (0 padding)  -- weights dchi2/dI_k
     const double* invq;      // [ldf]
-    float* grad_partial;     // [n_colblocks * n_chunks][n][4]
+    double* grad_partial;    // [n_colblocks * n_chunks][n][4] FP64 partial gradients
     int n_colblocks;
 };
 hipError_t launch_adjoint(const AdjointArgs& a, int K, hipStream_t s);
 int adjoint_k_count();
 int adjoint_k_value(int i);
 // grad[i] = sum over partial blocks (fixed order), written in caller order via slot map
-hipError_t launch_grad_reduce(const float* grad_partial, int n_parts, int n, const int* slot_to_caller,
+hipError_t launch_grad_reduce(const double* grad_partial, int n_parts, int n, const int* slot_to_caller,
                               float* grad_out, hipStream_t s);
 
 // out[0] = (sum_s f0col[class(cols[s])])^2, fixed-order FP64 sum (the q = 0 value)
 hipError_t launch_q0(const float4* cols, int n, const double* f0col, double* out, hipStream_t s);
+
+// dst[i] = src[i] for n float4 (grid-stride), the HBM calibration copy
+hipError_t launch_copy(float4* dst, const float4* src, long long n, hipStream_t s);
 
 // workspace reset kernels (used instead of memset nodes so launches can be graph-captured)
 hipError_t launch_fill_i32(int* p, int value, long long n, hipStream_t s);

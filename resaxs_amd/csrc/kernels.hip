@@ -43,17 +43,17 @@ namespace rsx {
 
 namespace {
 
-constexpr float INV_2PI = 0.15915494309189535f;
-
-__device__ __forceinline__ float sin_rev(float x)
+// sin / cos of 2 pi x for a phase x in revolutions held in FP64: the integer
+// part is removed exactly in FP64, the fraction goes to v_sin / v_cos
+// (which take revolutions; ~1.2e-7 absolute error measured on gfx950).
+__device__ __forceinline__ float sin_rev(double x)
 {
-    // v_sin_f32 takes revolutions; x >= 0 here, so fract is exact.
-    return __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(x));
+    return __builtin_amdgcn_sinf((float)__builtin_amdgcn_fract(x));
 }
 
-__device__ __forceinline__ float cos_rev(float x)
+__device__ __forceinline__ float cos_rev(double x)
 {
-    return __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(x));
+    return __builtin_amdgcn_cosf((float)__builtin_amdgcn_fract(x));
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -112,9 +112,10 @@ __device__ __forceinline__ Seeds make_seeds(float xi, float yi, float zi, const 
     const bool nz = r2 > 1e-30f;
     const float rinv = nz ? __builtin_amdgcn_rsqf(r2) : 0.0f;
     const float r = r2 * rinv;
-    const double sh = sin_poly((double)r * half_dq);  // sin(dq r / 2), r exact in FP64
-    const float sa = sin_rev(cc.a_rev * r);            // sin(a r)
-    const float cm = cos_rev(cc.mid_rev * r);          // cos((a + dq/2) r)
+    const double rd = (double)r;
+    const double sh = sin_poly(rd * half_dq);  // sin(dq r / 2)
+    const float sa = sin_rev(cc.a_rev * rd);   // sin(a r)
+    const float cm = cos_rev(cc.mid_rev * rd); // cos((a + dq/2) r)
     const float wr = WEIGHTED ? w * rinv : rinv;
     const float w1 = WEIGHTED ? w : 1.0f;
     // r == 0 (the i == j diagonal or coincident atoms): u_k -> q_k, linear in k.
@@ -180,6 +181,8 @@ __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
     const float* __restrict__ ftab = a.ftab;
     const int ldf = a.ldf;
 
+    // acc: running FP64 sums of the current class run; X: per-lane sums over
+    // flushed runs weighted by F_class (FP32)
     double acc[K];
     float X[K];
 #pragma unroll
@@ -200,9 +203,10 @@ __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
     const float4* __restrict__ cols = jb.cols;
     const int ncol = jb.ncol;
     int j = 0;
-    uint32_t cur = CLS_SKIP;
     if (jb.diag) {
+        // the diagonal tile: pair weights 1 (j > i), 1/2 (j == i), 0 (j < i)
         const int jd = ncol < 64 ? ncol : 64;
+        uint32_t cur = CLS_SKIP;
         for (; j < jd; ++j) {
             const float4 c = cols[j];
             const uint32_t cl = __float_as_uint(c.w);
@@ -215,7 +219,11 @@ __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
                 run1<K>(acc, make_seeds<true>(ri.x, ri.y, ri.z, c, cc, h_rev, dq, w));
             }
         }
+        if (cur != CLS_SKIP) flush(cur);
     }
+    // main loop: pairs of columns of one class share one recurrence pass; the
+    // running sums are flushed into X when the class changes
+    uint32_t cur = CLS_SKIP;
     while (j < ncol) {
         const float4 c0 = cols[j];
         const uint32_t cl0 = __float_as_uint(c0.w);
@@ -238,16 +246,16 @@ __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
     }
     if (cur != CLS_SKIP) flush(cur);
 
-    // row factor, 1/q, job scale; FP64 wave reduction; lane k stores q index k0 + k
-    const uint32_t ci = cls_i == CLS_SKIP ? 0u : cls_i;
+    // row factor (the extra all-zero class row serves padding lanes), 1/q, job
+    // scale; FP64 wave reduction; lane k stores q index k0 + k
+    const uint32_t ci = cls_i == CLS_SKIP ? (uint32_t)a.zero_class : cls_i;
     const float* __restrict__ fi = ftab + (size_t)ci * ldf + k0;
     const double* __restrict__ invq = a.invq + k0;
     const double scale = (double)jb.scale;
     double mine = 0.0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        double v = cls_i == CLS_SKIP ? 0.0 : (double)fi[k] * (double)X[k];
-        v = wave_sum(v) * invq[k] * scale;
+        const double v = wave_sum((double)fi[k] * (double)X[k]) * invq[k] * scale;
         mine = lane == k ? v : mine;
     }
     if (lane < K) a.partials[(size_t)jb.slot * ldf + k0 + lane] = mine;
@@ -258,7 +266,7 @@ __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
 // ---------------------------------------------------------------------------
 template <int K, bool WEIGHTED>
 __device__ __forceinline__ void direct_step(float (&acc)[K], float& zsum, float xi, float yi, float zi,
-                                            const float4 c, const float* __restrict__ qrev, float w)
+                                            const float4 c, const double* __restrict__ qrev, float w)
 {
     const float dx = xi - c.x, dy = yi - c.y, dz = zi - c.z;
     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
@@ -268,8 +276,9 @@ __device__ __forceinline__ void direct_step(float (&acc)[K], float& zsum, float 
     const float wr = WEIGHTED ? w * rinv : rinv;
     // r == 0 pairs contribute w * q_k; their weights are summed and applied at the flush
     zsum += nz ? 0.0f : (WEIGHTED ? w : 1.0f);
+    const double rd = (double)r;
 #pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = fmaf(sin_rev(qrev[k] * r), wr, acc[k]);
+    for (int k = 0; k < K; ++k) acc[k] = fmaf(sin_rev(qrev[k] * rd), wr, acc[k]);
 }
 
 template <int K>
@@ -285,7 +294,7 @@ __global__ __launch_bounds__(64, 2) void debye_direct_kernel(const DebyeArgs a)
     const int k0 = chunk * K;
     const float* __restrict__ ftab = a.ftab;
     const int ldf = a.ldf;
-    const float* __restrict__ qrev = a.qrev + k0;
+    const double* __restrict__ qrev = a.qrev + k0;
     const float* __restrict__ qv = a.qv + k0;
 
     float acc[K];
@@ -425,6 +434,13 @@ __global__ __launch_bounds__(1024) void q0_kernel(const float4* __restrict__ col
         for (int w = 0; w < 16; ++w) t += part[w];
         out[0] = t * t;
     }
+}
+
+__global__ __launch_bounds__(256) void copy_probe_kernel(float4* __restrict__ dst, const float4* __restrict__ src,
+                                                         long long n)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
 __global__ void fill_i32_kernel(int* p, int v, long long n)
@@ -587,7 +603,7 @@ __global__ __launch_bounds__(64, 2) void adjoint_kernel(const AdjointArgs a)
     const double* __restrict__ invq = a.invq + k0;
 
     double b[K];
-    float gx = 0.f, gy = 0.f, gz = 0.f;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
     const int j0 = cb * a.col_range;
     const int j1 = min(a.n, j0 + a.col_range);
     uint32_t cur = CLS_SKIP;
@@ -607,9 +623,10 @@ __global__ __launch_bounds__(64, 2) void adjoint_kernel(const AdjointArgs a)
         const bool nz = r2 > 1e-30f;
         const float rinv = nz ? __builtin_amdgcn_rsqf(r2) : 0.0f;
         const float r = r2 * rinv;
-        const double sh = sin_poly((double)r * a.half_dq);
-        const float sa = sin_rev(cc.a_rev * r), ca = cos_rev(cc.a_rev * r);
-        const float sm = sin_rev(cc.mid_rev * r), cm = cos_rev(cc.mid_rev * r);
+        const double rd = (double)r;
+        const double sh = sin_poly(rd * a.half_dq);
+        const float sa = sin_rev(cc.a_rev * rd), ca = cos_rev(cc.a_rev * rd);
+        const float sm = sin_rev(cc.mid_rev * rd), cm = cos_rev(cc.mid_rev * rd);
         const double c2 = fma(-4.0 * sh, sh, 2.0);
         // u_k = sin(q_k r) / r and v_k = cos(q_k r), both obey x_{k+1} = c2 x_k - x_{k-1}
         double up = (double)(sa * rinv), uc = up + 2.0 * (double)(cm * rinv) * sh;
@@ -627,29 +644,29 @@ __global__ __launch_bounds__(64, 2) void adjoint_kernel(const AdjointArgs a)
             vp = vc;
             vc = vn;
         }
-        const float gs = nz ? (float)G * rinv * rinv : 0.0f;
-        gx = fmaf(gs, dx, gx);
-        gy = fmaf(gs, dy, gy);
-        gz = fmaf(gs, dz, gz);
+        const double gs = nz ? G * (double)(rinv * rinv) : 0.0;
+        gx = fma(gs, (double)dx, gx);
+        gy = fma(gs, (double)dy, gy);
+        gz = fma(gs, (double)dz, gz);
     }
     if (row_ok) {
         const size_t part = (size_t)cb * a.n_chunks + chunk;
-        float* dst = a.grad_partial + (part * a.n + i) * 4;
+        double* dst = a.grad_partial + (part * a.n + i) * 4;
         dst[0] = gx;
         dst[1] = gy;
         dst[2] = gz;
-        dst[3] = 0.f;
+        dst[3] = 0.0;
     }
 }
 
-__global__ void grad_reduce_kernel(const float* __restrict__ gp, int n_parts, int n, const int* __restrict__ s2c,
+__global__ void grad_reduce_kernel(const double* __restrict__ gp, int n_parts, int n, const int* __restrict__ s2c,
                                    float* out)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     double x = 0, y = 0, z = 0;
     for (int p = 0; p < n_parts; ++p) {
-        const float* v = gp + ((size_t)p * n + s) * 4;
+        const double* v = gp + ((size_t)p * n + s) * 4;
         x += v[0];
         y += v[1];
         z += v[2];
@@ -758,6 +775,13 @@ hipError_t launch_q0(const float4* cols, int n, const double* f0col, double* out
     return hipGetLastError();
 }
 
+hipError_t launch_copy(float4* dst, const float4* src, long long n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(4096), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_i32(int* p, int value, long long n, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
@@ -794,7 +818,7 @@ hipError_t launch_adjoint(const AdjointArgs& a, int K, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_grad_reduce(const float* grad_partial, int n_parts, int n, const int* slot_to_caller,
+hipError_t launch_grad_reduce(const double* grad_partial, int n_parts, int n, const int* slot_to_caller,
                               float* grad_out, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;

@@ -187,6 +187,11 @@ class Engine:
         self._check(self._L.rsx_time_forward(self.h, int(reps), ctypes.byref(tot), ctypes.byref(pair)))
         return tot.value, pair.value
 
+    def time_copy(self, nbytes, reps=3):
+        ms = ctypes.c_float()
+        self._check(self._L.rsx_time_copy(self.h, int(nbytes), int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def synchronize(self):
         self._check(self._L.rsx_synchronize(self.h))
 

@@ -65,7 +65,13 @@ def test_adjoint_vs_analytic_and_fd(oracle):
     Fs = oracle.body_factors(start, types, q, table)
     g_ref, I_ref, c_ref, x2_ref = oracle.chi2_grad(q, xyz, Fs, i_exp, sigma)
     assert rel_err(iq, I_ref).max() < 1e-5
-    assert abs(chi2 - x2_ref) <= 1e-5 * x2_ref
+    # the engine's scale and chi^2 are the oracle's formulas applied to its own I
+    c_own, x2_own = oracle.scale_chi2(i_exp, sigma, iq)
+    assert abs(scale - c_own) <= 1e-12 * abs(c_own) and abs(chi2 - x2_own) <= 1e-12 * x2_own
+    # chi^2 vs the FP64 chi^2: first-order propagation of the 1e-5 bar on I
+    r = (i_exp - c_ref * I_ref) / sigma
+    dr = c_ref * 1e-5 * np.abs(I_ref) / sigma
+    assert abs(chi2 - x2_ref) <= 2 * np.sum(np.abs(r) * dr) + np.sum(dr * dr)
     gnorm = np.linalg.norm(g_ref)
     d = np.linalg.norm(grad.astype(np.float64) - g_ref) / gnorm
     _say(f"  |g - g_ref| / |g_ref| = {d:.3e}")

@@ -78,3 +78,19 @@ def test_hydrated_debye_vs_oracle(oracle):
     err = rel_err(I, ref)
     _say(f"  max rel err {err.max():.3e}")
     assert err.max() < 1e-5
+
+
+def test_hydrated_full_size(oracle):
+    """Configuration 2 at full size: 20,000 atoms x 256 q with SASA hydration."""
+    from resaxs_amd import configs
+    w = configs.config2()
+    _say("case: hydrated Debye 20000 x 256 (full size)")
+    dots = fib_dot_table(w.radii, w.probe, w.n_half)
+    e = engine_for(w.q, w.table, w.water_index, w.w, n=len(w.pos), radii=w.radii, probe=w.probe, n_half=w.n_half)
+    I = e.calc_curve(w.bodies)
+    counts = oracle.sasa_counts(w.pos, w.types, r2_table(w.radii, w.probe), dots)
+    ref = oracle_iq(w.q, w.pos, w.types, w.table, w.water_index, w.w, counts, 256)
+    err = rel_err(I, ref)
+    k = int(np.argmax(err))
+    _say(f"  max rel err {err.max():.3e} at q={w.q[k]:.4f}, median {np.median(err):.3e}")
+    assert err.max() < 1e-5
