@@ -120,6 +120,11 @@ int rsx_debye_vjp(rsx_engine *h, const float *bodies, int n, const double *w, fl
 int rsx_autotune(rsx_engine *h, const char *cache_path);
 /* tuning knobs: k (q per thread, 0 = automatic), col_range (atoms per job, 0 = automatic) */
 int rsx_set_tuning(rsx_engine *h, int k, int col_range);
+/* pair-kernel variant: 1 = one workgroup per job shares the pair geometry across its q chunks
+ * through LDS (default), 0 = one wave per (job, chunk) */
+int rsx_set_variant(rsx_engine *h, int shared_geometry);
+/* current q per thread, column range of the forward layout, and 1 if the recurrence path is active */
+int rsx_get_tuning(rsx_engine *h, int *k, int *col_range, int *recurrence);
 /* average device time (ms) of the last timed region, measured with HIP events on the engine stream */
 int rsx_time_forward(rsx_engine *h, int reps, float *ms_total, float *ms_pair_kernel);
 /* plain device-to-device copy of `bytes` (float4 grid-stride kernel), average ms per launch;

@@ -240,6 +240,7 @@ struct rsx_engine {
 
     // tuning
     int k_req = 0, range_req = 0;
+    bool shared_geometry = true;  // workgroup-shared pair geometry (uniform path)
     int K = 0, n_chunks = 0, ldf = 0;
     int rebase_every = 200;
 
@@ -272,6 +273,9 @@ struct rsx_engine {
     DBuf<double> d_gpart;
     DBuf<float> d_grad;
     size_t ftab_rows = 0;
+
+    std::string tune_report;
+    int last_L = 0;  // column range of the last forward job layout
 
     // last forward launch (for timing)
     std::vector<PairJob> last_jobs;
@@ -594,9 +598,11 @@ void compute_sasa(rsx_engine* h)
 int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles)
 {
     if (h->range_req > 0) return h->range_req;
-    // aim for ~8192 waves (4 per SIMD at 2 waves/SIMD occupancy over 1024 SIMDs)
-    const double jobs = 8192.0 / std::max(1, h->n_chunks);
-    double L = column_atoms_total / std::max(1.0, jobs);
+    // aim for ~16384 waves (8 per wave slot at 2 waves/SIMD over 1024 SIMDs): short
+    // enough jobs that the longest-first order leaves little tail
+    const double jobs = 16384.0 / std::max(1, h->n_chunks);
+    // column_atoms_total counts pairs; a job covers 64 rows
+    double L = column_atoms_total / 64.0 / std::max(1.0, jobs);
     (void)row_tiles;
     L = std::max(64.0, std::min(4096.0, L));
     return ((int)L + 63) / 64 * 64;
@@ -664,6 +670,7 @@ rsx::DebyeArgs debye_args(rsx_engine* h, const PairJob* d_jobs, double* partials
     a.qrev = h->d_qrev.p;
     a.qv = h->d_qv.p;
     a.partials = partials;
+    a.shared_geometry = h->shared_geometry ? 1 : 0;
     return a;
 }
 
@@ -695,13 +702,20 @@ void run_jobs(rsx_engine* h, DBuf<PairJob>& jbuf, const std::vector<PairJob>& jo
     hipchk(hipStreamSynchronize(h->stream), "sync pair kernels");
 }
 
-void full_forward(rsx_engine* h)
+std::vector<PairJob> forward_jobs(rsx_engine* h)
 {
     const int n = h->n;
     std::vector<PairJob> jobs;
     const int L = pick_range(h, (double)n * n / 2.0, (n + 63) / 64);
+    h->last_L = L;
     triangle_jobs(jobs, h->d_cols.p, n, L, 2.0f);
     finalize_jobs(jobs);
+    return jobs;
+}
+
+void full_forward(rsx_engine* h)
+{
+    const std::vector<PairJob> jobs = forward_jobs(h);
     const double sf = sum_f0(h, h->cls);
     run_jobs(h, h->d_fjobs, jobs, h->d_iq.p, 0, sf * sf);
     h->last_jobs = jobs;
@@ -863,7 +877,7 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
     hipchk(rsx::launch_scatter(h->d_cols.p, h->d_idx.p, h->d_vals.p, m, h->stream), "scatter skip");
 
     std::vector<PairJob> jobs;
-    const int L = pick_range(h, 2.0 * mp * (double)h->n / 64.0 * 64.0 / 2.0, 2 * mp / 64);
+    const int L = pick_range(h, 2.0 * mp * (double)h->n, 2 * mp / 64);
     rect_jobs(jobs, h->d_rows.p, mp, h->d_cols.p, h->n, L, 2.0f);            // new x rest
     rect_jobs(jobs, h->d_rows.p + mp, mp, h->d_cols.p, h->n, L, -2.0f);      // old x rest
     rect_jobs(jobs, h->d_rows.p, mp, h->d_rows.p, mp, L, 1.0f);             // new x new
@@ -1213,6 +1227,109 @@ void enqueue_device_forward(rsx_engine* h, const float4* bodies_dev, double* out
     hipchk(rsx::launch_reduce(h->d_partials.p, h->ldf, h->nq, h->d_slot_begin_f.p, 1, out, h->nq, 0, h->d_q0.p,
                               h->q0_index, h->stream),
            "reduce kernel");
+}
+
+// ---- autotuner -------------------------------------------------------------
+// Key of a tuning entry: device architecture, path, n_q, size bucket, hydration.
+std::string tune_key(rsx_engine* h)
+{
+    hipDeviceProp_t prop;
+    std::string arch = "unknown";
+    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) arch = prop.gcnArchName;
+    for (char& c : arch)
+        if (c == ' ' || c == ':') c = '_';
+    int bucket = 0;
+    while ((1 << (bucket + 1)) <= h->n) ++bucket;
+    std::ostringstream k;
+    k << arch << ":" << (h->uniform ? "rec" : "direct") << ":nq" << h->nq << ":n2^" << bucket << ":"
+      << (h->hydrate ? "hyd" : "plain");
+    return k.str();
+}
+
+// switch K / column range on a laid-out engine and rebuild the forward jobs
+void retune(rsx_engine* h, int K, int L)
+{
+    h->k_req = K;
+    h->range_req = L;
+    apply_path(h, h->uniform);
+    build_class_table(h);
+    h->params_dirty = false;
+    const std::vector<PairJob> jobs = forward_jobs(h);
+    const int nj = (int)jobs.size();
+    h->d_fjobs.ensure(nj);
+    h->d_partials.ensure((size_t)std::max(nj, 1) * h->ldf);
+    hipchk(hipMemcpy(h->d_fjobs.p, jobs.data(), nj * sizeof(PairJob), hipMemcpyHostToDevice), "upload jobs");
+    h->last_jobs = jobs;
+}
+
+float time_pair_kernel(rsx_engine* h, int reps)
+{
+    const rsx::DebyeArgs a = debye_args(h, h->d_fjobs.p, h->d_partials.p);
+    const int nj = (int)h->last_jobs.size();
+    hipchk(rsx::launch_debye(a, nj, h->K, h->uniform, h->stream), "warm-up");
+    hipchk(hipEventRecord(h->ev0, h->stream), "event");
+    for (int r = 0; r < reps; ++r) hipchk(rsx::launch_debye(a, nj, h->K, h->uniform, h->stream), "debye kernel");
+    hipchk(hipEventRecord(h->ev1, h->stream), "event");
+    hipchk(hipEventSynchronize(h->ev1), "event sync");
+    float ms = 0.0f;
+    hipchk(hipEventElapsedTime(&ms, h->ev0, h->ev1), "elapsed");
+    return ms / reps;
+}
+
+// Look the current problem up in cache_path ("key K L ms" lines); if absent,
+// time every (K, column range) candidate with HIP events, keep the fastest and
+// append it.  The engine must hold a computed structure (calc_curve first).
+void autotune_impl(rsx_engine* h, const char* cache_path)
+{
+    if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "autotune needs a prior calc_curve");
+    set_device(h);
+    const std::string key = tune_key(h);
+    if (cache_path && *cache_path) {
+        std::ifstream in(cache_path);
+        std::string line;
+        while (std::getline(in, line)) {
+            std::istringstream ls(line);
+            std::string k;
+            int K = 0, L = 0, V = 1;
+            if ((ls >> k >> K >> L >> V) && k == key) {
+                h->shared_geometry = V != 0;
+                retune(h, K, L);
+                full_forward(h);
+                return;
+            }
+        }
+    }
+    std::vector<int> Ks;
+    for (int i = 0; i < rsx::debye_k_count(h->uniform); ++i) {
+        const int k = rsx::debye_k_value(h->uniform, i);
+        if (k >= 16 && !(k == 64 && !h->uniform)) Ks.push_back(k);
+    }
+    const int Ls[] = {256, 512, 768, 1024, 1536, 2048, 3072, 4096};
+    int bestK = 0, bestL = 0;
+    float best = 1e30f;
+    int bestV = 1;
+    for (int V = 0; V <= 1; ++V)
+        for (int K : Ks)
+            for (int L : Ls) {
+                if (L > std::max(256, h->n)) continue;
+                h->shared_geometry = V != 0;
+                retune(h, K, L);
+                const float ms = time_pair_kernel(h, 3);
+                if (ms < best) {
+                    best = ms;
+                    bestK = K;
+                    bestL = L;
+                    bestV = V;
+                }
+            }
+    h->shared_geometry = bestV != 0;
+    retune(h, bestK, bestL);
+    full_forward(h);
+    if (cache_path && *cache_path) {
+        std::ofstream out(cache_path, std::ios::app);
+        out << key << " " << bestK << " " << bestL << " " << bestV << " " << best << "\n";
+    }
+    h->tune_report = key + " K=" + std::to_string(bestK) + " L=" + std::to_string(bestL) + " ms=" + std::to_string(best);
 }
 
 template <typename F>
@@ -1605,8 +1722,21 @@ int rsx_debye_vjp(rsx_engine* h, const float* bodies, int n, const double* w, fl
 
 int rsx_autotune(rsx_engine* h, const char* cache_path)
 {
-    (void)cache_path;
-    return guarded(h, [&] { fail(RSX_NOT_SUPPORTED, "autotune not built yet"); });
+    return guarded(h, [&] { autotune_impl(h, cache_path); });
+}
+
+int rsx_set_variant(rsx_engine* h, int shared_geometry)
+{
+    return guarded(h, [&] { h->shared_geometry = shared_geometry != 0; });
+}
+
+int rsx_get_tuning(rsx_engine* h, int* k, int* col_range, int* recurrence)
+{
+    return guarded(h, [&] {
+        if (k) *k = h->K;
+        if (col_range) *col_range = h->last_L;
+        if (recurrence) *recurrence = h->uniform ? 1 : 0;
+    });
 }
 
 int rsx_time_forward(rsx_engine* h, int reps, float* ms_total, float* ms_pair)

@@ -52,6 +52,7 @@ struct DebyeArgs {
     const double* qrev;   // [ldf] q / (2 pi)  (direct path)
     const float* qv;      // [ldf] q           (direct path)
     double* partials;     // [n_slots][ldf]
+    int shared_geometry;  // 1: one workgroup per job shares the pair geometry across its q chunks
 };
 
 // Debye pair sums.  uniform: q_k = a + k dq within each chunk (FP64

@@ -107,20 +107,22 @@ template <bool WEIGHTED>
 __device__ __forceinline__ Seeds make_seeds(float xi, float yi, float zi, const float4 c,
                                             const ChunkConst& cc, double half_dq, float dq, float w)
 {
+    // branch-free: every value stays finite and r == 0 is blended in with a 0/1 mask,
+    // so the seeds of the next column pair can be scheduled into the recurrence block
     const float dx = xi - c.x, dy = yi - c.y, dz = zi - c.z;
     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    const bool nz = r2 > 1e-30f;
-    const float rinv = nz ? __builtin_amdgcn_rsqf(r2) : 0.0f;
+    const float nzf = r2 > 1e-30f ? 1.0f : 0.0f;
+    const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f)) * nzf;
     const float r = r2 * rinv;
     const double rd = (double)r;
     const double sh = sin_poly(rd * half_dq);  // sin(dq r / 2)
     const float sa = sin_rev(cc.a_rev * rd);   // sin(a r)
     const float cm = cos_rev(cc.mid_rev * rd); // cos((a + dq/2) r)
     const float wr = WEIGHTED ? w * rinv : rinv;
-    const float w1 = WEIGHTED ? w : 1.0f;
+    const float wz = (WEIGHTED ? w : 1.0f) * (1.0f - nzf);
     // r == 0 (the i == j diagonal or coincident atoms): u_k -> q_k, linear in k.
-    const float f0 = nz ? sa * wr : w1 * cc.a;
-    const double fd = nz ? 2.0 * (double)(cm * wr) * sh : (double)(w1 * dq);
+    const float f0 = fmaf(sa, wr, wz * cc.a);
+    const double fd = fma(2.0 * (double)(cm * wr), sh, (double)(wz * dq));
     Seeds s;
     s.c2 = fma(-4.0 * sh, sh, 2.0);
     s.u0 = (double)f0;
@@ -202,28 +204,12 @@ __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
 
     const float4* __restrict__ cols = jb.cols;
     const int ncol = jb.ncol;
-    int j = 0;
-    if (jb.diag) {
-        // the diagonal tile: pair weights 1 (j > i), 1/2 (j == i), 0 (j < i)
-        const int jd = ncol < 64 ? ncol : 64;
-        uint32_t cur = CLS_SKIP;
-        for (; j < jd; ++j) {
-            const float4 c = cols[j];
-            const uint32_t cl = __float_as_uint(c.w);
-            if (cl != cur) {
-                if (cur != CLS_SKIP) flush(cur);
-                cur = cl;
-            }
-            if (cl != CLS_SKIP) {
-                const float w = j > lane ? 1.0f : (j == lane ? 0.5f : 0.0f);
-                run1<K>(acc, make_seeds<true>(ri.x, ri.y, ri.z, c, cc, h_rev, dq, w));
-            }
-        }
-        if (cur != CLS_SKIP) flush(cur);
-    }
-    // main loop: pairs of columns of one class share one recurrence pass; the
-    // running sums are flushed into X when the class changes
+    const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
+    // one weighted pair step per iteration: the second column is the next one
+    // when it has the same class, otherwise a zero-weight copy.  Outside the
+    // diagonal tile the weights are wave-uniform (0 for skipped records).
     uint32_t cur = CLS_SKIP;
+    int j = 0;
     while (j < ncol) {
         const float4 c0 = cols[j];
         const uint32_t cl0 = __float_as_uint(c0.w);
@@ -231,34 +217,183 @@ __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
             if (cur != CLS_SKIP) flush(cur);
             cur = cl0;
         }
-        if (j + 1 < ncol) {
-            const float4 c1 = cols[j + 1];
-            if (__float_as_uint(c1.w) == cl0) {
-                if (cl0 != CLS_SKIP)
-                    run2<K>(acc, make_seeds<false>(ri.x, ri.y, ri.z, c0, cc, h_rev, dq, 1.0f),
-                            make_seeds<false>(ri.x, ri.y, ri.z, c1, cc, h_rev, dq, 1.0f));
-                j += 2;
-                continue;
-            }
+        const bool pair = j + 1 < ncol && __float_as_uint(cols[j + 1].w) == cl0;
+        const float4 c1 = pair ? cols[j + 1] : c0;
+        const float live = cl0 != CLS_SKIP ? 1.0f : 0.0f;
+        float w0 = live, w1 = pair ? live : 0.0f;
+        if (j < jd) {
+            // diagonal tile: weights 1 (j > i), 1/2 (j == i), 0 (j < i)
+            w0 *= j > lane ? 1.0f : (j == lane ? 0.5f : 0.0f);
+            w1 *= j + 1 > lane ? 1.0f : (j + 1 == lane ? 0.5f : 0.0f);
         }
-        if (cl0 != CLS_SKIP) run1<K>(acc, make_seeds<false>(ri.x, ri.y, ri.z, c0, cc, h_rev, dq, 1.0f));
-        j += 1;
+        run2<K>(acc, make_seeds<true>(ri.x, ri.y, ri.z, c0, cc, h_rev, dq, w0),
+                make_seeds<true>(ri.x, ri.y, ri.z, c1, cc, h_rev, dq, w1));
+        j += pair ? 2 : 1;
     }
     if (cur != CLS_SKIP) flush(cur);
 
-    // row factor (the extra all-zero class row serves padding lanes), 1/q, job
-    // scale; FP64 wave reduction; lane k stores q index k0 + k
+    // epilogue: v_ik = F_i(q_k) X_ik (the all-zero class row serves padding
+    // lanes), transposed through LDS so lane k sums column k over the 64 rows in
+    // FP64; then 1/q_k and the job scale.  Rows of T are padded to 65 words so
+    // both the row writes and the column reads are free of bank conflicts.
+    __shared__ float T[K][65];
     const uint32_t ci = cls_i == CLS_SKIP ? (uint32_t)a.zero_class : cls_i;
     const float* __restrict__ fi = ftab + (size_t)ci * ldf + k0;
-    const double* __restrict__ invq = a.invq + k0;
-    const double scale = (double)jb.scale;
-    double mine = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) T[k][lane] = fi[k] * X[k];
+    __syncthreads();
+    if (lane < K) {
+        double sum = 0.0;
+#pragma unroll 16
+        for (int t = 0; t < 64; ++t) sum += (double)T[lane][t];
+        a.partials[(size_t)jb.slot * ldf + k0 + lane] = sum * a.invq[k0 + lane] * (double)jb.scale;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// uniform-grid path, workgroup variant: the waves of one workgroup are the
+// q chunks of one job (up to 4).  The pair geometry that does not depend on q
+// (r, 1/r and the FP64 sin(dq r / 2)) is computed once per (row, column) by one
+// of the waves and shared through LDS in tiles of SH_TC columns, so each wave
+// only forms its chunk's seeds and runs the recurrence.
+// ---------------------------------------------------------------------------
+constexpr int SH_TC = 32;   // columns per shared tile
+constexpr int SH_MAXW = 4;  // waves (q chunks) per workgroup
+
+__device__ __forceinline__ void pair_geometry(float xi, float yi, float zi, const float4 c, double half_dq,
+                                              double& rd, double& sh, float& rinv)
+{
+    const float dx = xi - c.x, dy = yi - c.y, dz = zi - c.z;
+    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const float nzf = r2 > 1e-30f ? 1.0f : 0.0f;
+    rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f)) * nzf;
+    rd = (double)(r2 * rinv);
+    sh = sin_poly(rd * half_dq);
+}
+
+template <bool WEIGHTED>
+__device__ __forceinline__ Seeds chunk_seeds(double rd, double sh, float rinv, const ChunkConst& cc, float dq,
+                                             float w)
+{
+    const float sa = sin_rev(cc.a_rev * rd);
+    const float cm = cos_rev(cc.mid_rev * rd);
+    const float nzf = rinv > 0.0f ? 1.0f : 0.0f;
+    const float wr = WEIGHTED ? w * rinv : rinv;
+    const float wz = (WEIGHTED ? w : 1.0f) * (1.0f - nzf);
+    const float f0 = fmaf(sa, wr, wz * cc.a);
+    const double fd = fma(2.0 * (double)(cm * wr), sh, (double)(wz * dq));
+    Seeds s;
+    s.c2 = fma(-4.0 * sh, sh, 2.0);
+    s.u0 = (double)f0;
+    s.u1 = s.u0 + fd;
+    return s;
+}
+
+template <int K>
+__global__ __launch_bounds__(64 * SH_MAXW, 2) void debye_shared_kernel(const DebyeArgs a)
+{
+    union Shared {
+        struct {
+            double rd[SH_TC][64];
+            double sh[SH_TC][64];
+            float ri[SH_TC][64];
+        } p;
+        float T[SH_MAXW][K][65];
+    };
+    __shared__ Shared sm;
+
+    const int nw = blockDim.x >> 6;                 // waves in the workgroup
+    const int ng = (a.n_chunks + nw - 1) / nw;      // workgroups per job
+    const int job_id = blockIdx.x / ng;
+    const int grp = blockIdx.x - job_id * ng;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int chunk = grp * nw + wave;
+    const bool active = chunk < a.n_chunks;         // wave-uniform
+    const PairJob jb = a.jobs[job_id];
+    const bool row_ok = lane < jb.nrow;
+    const float4 ri = row_ok ? jb.rows[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t cls_i = row_ok ? __float_as_uint(ri.w) : CLS_SKIP;
+    const int k0 = (active ? chunk : 0) * K;
+    const ChunkConst cc = a.chunks[active ? chunk : 0];
+    const double half_dq = a.half_dq;
+    const float dq = a.dq;
+    const float* __restrict__ ftab = a.ftab;
+    const int ldf = a.ldf;
+
+    double acc[K];
+    float X[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const double v = wave_sum((double)fi[k] * (double)X[k]) * invq[k] * scale;
-        mine = lane == k ? v : mine;
+        acc[k] = 0.0;
+        X[k] = 0.0f;
     }
-    if (lane < K) a.partials[(size_t)jb.slot * ldf + k0 + lane] = mine;
+    auto flush = [&](uint32_t cls) {
+        const float* __restrict__ fr = ftab + (size_t)cls * ldf + k0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            X[k] = fmaf(fr[k], (float)acc[k], X[k]);
+            acc[k] = 0.0;
+        }
+    };
+
+    const float4* __restrict__ cols = jb.cols;
+    const int ncol = jb.ncol;
+    const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
+    uint32_t cur = CLS_SKIP;
+    for (int t0 = 0; t0 < ncol; t0 += SH_TC) {
+        const int tn = ncol - t0 < SH_TC ? ncol - t0 : SH_TC;
+        for (int t = wave; t < tn; t += nw) {
+            double rd, sh;
+            float rinv;
+            pair_geometry(ri.x, ri.y, ri.z, cols[t0 + t], half_dq, rd, sh, rinv);
+            sm.p.rd[t][lane] = rd;
+            sm.p.sh[t][lane] = sh;
+            sm.p.ri[t][lane] = rinv;
+        }
+        __syncthreads();
+        if (active) {
+            // one weighted pair step per iteration: the second column is the next
+            // one when it is in this tile and of the same class, otherwise a
+            // zero-weight copy; skipped records and the lower half of the
+            // diagonal tile also carry weight 0
+            int t = 0;
+            while (t < tn) {
+                const int j = t0 + t;
+                const uint32_t cl0 = __float_as_uint(cols[j].w);
+                if (cl0 != cur) {
+                    if (cur != CLS_SKIP) flush(cur);
+                    cur = cl0;
+                }
+                const bool pair = t + 1 < tn && __float_as_uint(cols[j + 1].w) == cl0;
+                const int t1 = pair ? t + 1 : t;
+                const float live = cl0 != CLS_SKIP ? 1.0f : 0.0f;
+                float w0 = live, w1 = pair ? live : 0.0f;
+                if (j < jd) {
+                    w0 *= j > lane ? 1.0f : (j == lane ? 0.5f : 0.0f);
+                    w1 *= j + 1 > lane ? 1.0f : (j + 1 == lane ? 0.5f : 0.0f);
+                }
+                run2<K>(acc, chunk_seeds<true>(sm.p.rd[t][lane], sm.p.sh[t][lane], sm.p.ri[t][lane], cc, dq, w0),
+                        chunk_seeds<true>(sm.p.rd[t1][lane], sm.p.sh[t1][lane], sm.p.ri[t1][lane], cc, dq, w1));
+                t += pair ? 2 : 1;
+            }
+        }
+        __syncthreads();
+    }
+    if (active && cur != CLS_SKIP) flush(cur);
+
+    // epilogue as in debye_uniform_kernel, through this wave's slice of the union
+    const uint32_t ci = cls_i == CLS_SKIP ? (uint32_t)a.zero_class : cls_i;
+    const float* __restrict__ fi = ftab + (size_t)ci * ldf + k0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) sm.T[wave][k][lane] = fi[k] * X[k];
+    __syncthreads();
+    if (active && lane < K) {
+        double sum = 0.0;
+#pragma unroll 16
+        for (int t = 0; t < 64; ++t) sum += (double)sm.T[wave][lane][t];
+        a.partials[(size_t)jb.slot * ldf + k0 + lane] = sum * a.invq[k0 + lane] * (double)jb.scale;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -710,6 +845,23 @@ hipError_t launch_debye(const DebyeArgs& a, int n_jobs, int K, bool uniform, hip
 {
     if (n_jobs <= 0) return hipSuccess;
     const dim3 grid((unsigned)(n_jobs * a.n_chunks)), block(64);
+    if (uniform && a.shared_geometry && a.n_chunks > 1) {
+        const int nw = a.n_chunks < SH_MAXW ? a.n_chunks : SH_MAXW;
+        const int ng = (a.n_chunks + nw - 1) / nw;
+        const dim3 g2((unsigned)(n_jobs * ng)), b2((unsigned)(64 * nw));
+        switch (K) {
+        case 8: hipLaunchKernelGGL(debye_shared_kernel<8>, g2, b2, 0, s, a); break;
+        case 16: hipLaunchKernelGGL(debye_shared_kernel<16>, g2, b2, 0, s, a); break;
+        case 24: hipLaunchKernelGGL(debye_shared_kernel<24>, g2, b2, 0, s, a); break;
+        case 32: hipLaunchKernelGGL(debye_shared_kernel<32>, g2, b2, 0, s, a); break;
+        case 40: hipLaunchKernelGGL(debye_shared_kernel<40>, g2, b2, 0, s, a); break;
+        case 48: hipLaunchKernelGGL(debye_shared_kernel<48>, g2, b2, 0, s, a); break;
+        case 56: hipLaunchKernelGGL(debye_shared_kernel<56>, g2, b2, 0, s, a); break;
+        case 64: hipLaunchKernelGGL(debye_shared_kernel<64>, g2, b2, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     if (uniform) {
         switch (K) {
         case 8: hipLaunchKernelGGL(debye_uniform_kernel<8>, grid, block, 0, s, a); break;

@@ -98,6 +98,19 @@ class Engine:
     def set_tuning(self, k=0, col_range=0):
         self._check(self._L.rsx_set_tuning(self.h, int(k), int(col_range)))
 
+    def set_variant(self, shared_geometry):
+        self._check(self._L.rsx_set_variant(self.h, int(bool(shared_geometry))))
+
+    def get_tuning(self):
+        k, L, rec = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.rsx_get_tuning(self.h, ctypes.byref(k), ctypes.byref(L), ctypes.byref(rec)))
+        return {"k": k.value, "col_range": L.value, "recurrence": bool(rec.value)}
+
+    def autotune(self, cache_path=None):
+        p = cache_path.encode() if cache_path else None
+        self._check(self._L.rsx_autotune(self.h, p))
+        return self.get_tuning()
+
     def initialize(self, n_bodies):
         self._check(self._L.rsx_initialize(self.h, int(n_bodies)))
         self.n = int(n_bodies)
