@@ -26,6 +26,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "debye_pair_kernel_sasa-hydrated-20000x256_counters.txt")
+# autotuner cache (one line per arch / path / n_q / size bucket / hydration key); a missing key is
+# tuned on the spot and appended
+TUNING_CACHE = os.path.join(ROOT, "resaxs_amd", "tuning_gfx950.txt")
 
 
 def log(msg):
@@ -61,6 +64,9 @@ def forward_bench(steps, warmup):
     e = make_engine(w)
     log(f"bench: {w.name}: host calc_curve (layout + first result)")
     I_host = e.calc_curve(w.bodies)
+    tuning = e.autotune(TUNING_CACHE)
+    log(f"bench: tuning {tuning}")
+    I_host = e.calc_curve(w.bodies)
     d_b = e.dev_alloc(n * 16)
     d_out = e.dev_alloc(nq * 8)
     e.h2d(d_b, w.bodies)
@@ -83,7 +89,7 @@ def forward_bench(steps, warmup):
     e.dev_free(d_b)
     e.dev_free(d_out)
     e.close()
-    return w, I_host, ms_per_step, pair_ms
+    return w, I_host, ms_per_step, pair_ms, tuning
 
 
 def adjoint_bench(reps=3):
@@ -161,7 +167,8 @@ def main():
         log(f"bench: copy probe {args.copy_probe} MiB: {ms:.4f} ms/launch, "
             f"{2 * (args.copy_probe << 20) / ms / 1e6:.1f} GB/s (read + write)")
         e.close()
-    w, I_gpu, ms_per_step, pair_ms = forward_bench(args.steps, args.warmup)
+    w, I_gpu, ms_per_step, pair_ms, tuning = forward_bench(args.steps, args.warmup)
+    kernel = "debye_shared_kernel" if tuning["shared_geometry"] else "debye_uniform_kernel"
     n, nq = len(w.pos), len(w.q)
     terms = configs.pair_terms(n, nq)
     flops = configs.forward_flops(n, nq)
@@ -184,8 +191,8 @@ def main():
         "roofline": {"bound": "mfma", "pipe": "VALU (FP32 + FP64 vector; no MFMA)",
                      "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": read_traffic(),
-                     "kernel": "debye_uniform_kernel", "kernel_ms": round(pair_ms, 4),
-                     "flop_per_launch": flops},
+                     "kernel": f"{kernel}<{tuning['k']}>", "col_range": tuning["col_range"],
+                     "kernel_ms": round(pair_ms, 4), "flop_per_launch": flops},
     }
     if not args.no_adjoint:
         log("bench: forward + adjoint on the fit configuration")

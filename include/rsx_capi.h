@@ -123,8 +123,9 @@ int rsx_set_tuning(rsx_engine *h, int k, int col_range);
 /* pair-kernel variant: 1 = one workgroup per job shares the pair geometry across its q chunks
  * through LDS (default), 0 = one wave per (job, chunk) */
 int rsx_set_variant(rsx_engine *h, int shared_geometry);
-/* current q per thread, column range of the forward layout, and 1 if the recurrence path is active */
-int rsx_get_tuning(rsx_engine *h, int *k, int *col_range, int *recurrence);
+/* current q per thread, column range of the forward layout, 1 if the recurrence path is active,
+ * and 1 if the shared-geometry pair kernel runs (any pointer may be NULL) */
+int rsx_get_tuning(rsx_engine *h, int *k, int *col_range, int *recurrence, int *shared_geometry);
 /* average device time (ms) of the last timed region, measured with HIP events on the engine stream */
 int rsx_time_forward(rsx_engine *h, int reps, float *ms_total, float *ms_pair_kernel);
 /* plain device-to-device copy of `bytes` (float4 grid-stride kernel), average ms per launch;

@@ -460,9 +460,13 @@ static const float EXV[7] = {5.15f, 16.44f, 2.49f, 9.13f, 5.73f, 19.86f, 9.85f};
 static const int CPLX[9][2] = {{1, 1}, {1, 2}, {1, 3}, {2, 1}, {2, 2}, {2, 3}, {3, 1}, {3, 2}, {5, 1}};
 #define ORC_RHO0 0.334f
 
+/* The reference evaluates exp and pow through the C library's double
+ * functions (float arguments promoted), rounding once to float at the end of
+ * each expression; restated in that order so the table agrees bit for bit. */
 static float excl_ff(int t, float k_sq)
 {
-    return ORC_RHO0 * EXV[t] * expf(-(float)ORC_PI * powf(EXV[t], 2.0f / 3.0f) * k_sq);
+    const double e = exp((double)(-(float)ORC_PI) * pow((double)EXV[t], (double)(2.0f / 3.0f)) * (double)k_sq);
+    return (float)((double)(ORC_RHO0 * EXV[t]) * e);
 }
 
 /* radii of the 16 atom types from the excluded-volume factor at q = 0. */
@@ -482,7 +486,7 @@ void orc_atomic_radii(float *out16)
  */
 void orc_atomic_ff(const float *q, int nq, float avg_radius, float c, float *out)
 {
-    const float k43 = powf(4.0f * (float)ORC_PI / 3.0f, 2.0f / 3.0f);
+    const float k43 = (float)pow((double)(4.0f * (float)ORC_PI / 3.0f), (double)(2.0f / 3.0f));
     for (int i = 0; i < nq; ++i) {
         float *page = out + (size_t)i * 16;
         const float k = q[i] / (4.0f * (float)ORC_PI);
@@ -490,11 +494,12 @@ void orc_atomic_ff(const float *q, int nq, float avg_radius, float c, float *out
         float E = 1.0f;
         if (c != 1.0f) {
             const float c_sq = c * c;
-            E = c_sq * c * expf(-(float)ORC_PI * k43 * k_sq * avg_radius * avg_radius * (c_sq - 1.0f));
+            const float arg = -(float)ORC_PI * k43 * k_sq * avg_radius * avg_radius * (c_sq - 1.0f);
+            E = (float)((double)(c_sq * c) * exp((double)arg));
         }
         for (int t = 0; t < 7; ++t) {
             float f = CM[t][4];
-            for (int a = 0; a < 4; ++a) f += CM[t][a] * expf(-CM[t][5 + a] * k_sq);
+            for (int a = 0; a < 4; ++a) f = (float)((double)f + (double)CM[t][a] * exp((double)(-CM[t][5 + a] * k_sq)));
             page[t] = f - excl_ff(t, k_sq) * E;
         }
         for (int t = 7; t < 16; ++t)

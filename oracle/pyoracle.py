@@ -202,3 +202,99 @@ def chi2_fd(q, xyz, F, Iexp, sigma, comps, h=1e-3, nthreads=0):
                       _ptr(F, ctypes.c_double), _ptr(Iexp, ctypes.c_double), _ptr(sigma, ctypes.c_double),
                       _ptr(comps, ctypes.c_int), len(comps), float(h), _ptr(fd, ctypes.c_double), int(nthreads))
     return fd
+
+
+# ---------------------------------------------------------------------------
+# The reference's own CPU code, compiled from its sources by `make -C oracle ref`
+# into oracle/_ref/libresaxs_ref.so (see oracle/ref_bridge.cpp).  Optional:
+# every caller checks ref_lib() for None (the reference tree is only present
+# where the library was built).
+# ---------------------------------------------------------------------------
+REF_TREE = "/root/reference"
+REF_LIB_PATH = os.path.join(HERE, "_ref", "libresaxs_ref.so")
+_ref = None
+
+
+def build_ref():
+    """Compile oracle/_ref/libresaxs_ref.so when the reference tree is present."""
+    if os.path.isdir(os.path.join(REF_TREE, "resaxslib")):
+        subprocess.run(["make", "-s", "-C", HERE, "ref", f"REF={REF_TREE}"], check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+
+
+def ref_lib():
+    """The compiled reference subset, or None where it was not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB_PATH):
+        L = ctypes.CDLL(REF_LIB_PATH)
+        i, f = ctypes.c_int, ctypes.c_float
+        L.ref_host_debye.argtypes = [_fp, i, _fp, i, _fp, i, _fp]
+        L.ref_atomic_ff.argtypes = [_fp, i, f, f, _fp]
+        L.ref_atomic_radii.argtypes = [_fp]
+        L.ref_avg_radius.argtypes = [_fp, i]
+        L.ref_avg_radius.restype = f
+        for name in ("ref_fibonacci_dots", "ref_dodeca_icosa_dots"):
+            getattr(L, name).argtypes = [f, f, _fp, i]
+        L.ref_density_dots.argtypes = [f, f, f, _fp, i]
+        L.ref_rsasa_cpu.argtypes = [_fp, i, f, f, _dp]
+        _ref = L
+    return _ref
+
+
+def ref_host_debye(q, pos, types, table):
+    """host_debye<double, float, cl_float4>::calc_curve: FP32 I(q)."""
+    b = bodies(pos, types)
+    q = _c(q, np.float32)
+    table = _c(table, np.float32)
+    out = np.zeros(len(q), dtype=np.float32)
+    ref_lib().ref_host_debye(_ptr(q, ctypes.c_float), len(q), _ptr(b, ctypes.c_float), len(b),
+                             _ptr(table, ctypes.c_float), table.shape[1], _ptr(out, ctypes.c_float))
+    return out
+
+
+def ref_atomic_ff(q, avg_radius=1.62, c=1.0):
+    q = _c(q, np.float32)
+    out = np.zeros((len(q), 16), dtype=np.float32)
+    ref_lib().ref_atomic_ff(_ptr(q, ctypes.c_float), len(q), float(avg_radius), float(c), _ptr(out, ctypes.c_float))
+    return out
+
+
+def ref_atomic_radii():
+    out = np.zeros(16, dtype=np.float32)
+    ref_lib().ref_atomic_radii(_ptr(out, ctypes.c_float))
+    return out
+
+
+def ref_avg_radius(b):
+    b = _c(b, np.float32).reshape(-1, 4)
+    return np.float32(ref_lib().ref_avg_radius(_ptr(b, ctypes.c_float), len(b)))
+
+
+def _ref_dots(fn, *args):
+    cap = 4096
+    out = np.zeros((cap, 4), dtype=np.float32)
+    n = fn(*[float(a) for a in args], _ptr(out, ctypes.c_float), cap)
+    assert 0 < n <= cap
+    return out[:n].copy()
+
+
+def ref_fibonacci_dots(radius, probe):
+    return _ref_dots(ref_lib().ref_fibonacci_dots, radius, probe)
+
+
+def ref_dodeca_icosa_dots(radius, probe):
+    return _ref_dots(ref_lib().ref_dodeca_icosa_dots, radius, probe)
+
+
+def ref_density_dots(density, radius, probe):
+    return _ref_dots(ref_lib().ref_density_dots, density, radius, probe)
+
+
+def ref_rsasa_cpu(b, density, probe):
+    """rsasa_cpu_alg::calc_rsasa exposed fractions (FP64) for bodies b [N, 4]."""
+    b = _c(b, np.float32).reshape(-1, 4)
+    out = np.zeros(len(b), dtype=np.float64)
+    rc = ref_lib().ref_rsasa_cpu(_ptr(b, ctypes.c_float), len(b), float(density), float(probe),
+                                 _ptr(out, ctypes.c_double))
+    assert rc == 0
+    return out

@@ -65,7 +65,7 @@ SIGNATURES = {
     "rsx_autotune": (_i, [_vp, _c.c_char_p]),
     "rsx_set_tuning": (_i, [_vp, _i, _i]),
     "rsx_set_variant": (_i, [_vp, _i]),
-    "rsx_get_tuning": (_i, [_vp, _ip, _ip, _ip]),
+    "rsx_get_tuning": (_i, [_vp, _ip, _ip, _ip, _ip]),
     "rsx_time_forward": (_i, [_vp, _i, _fp, _fp]),
     "rsx_time_copy": (_i, [_vp, _ll, _i, _fp]),
     "rsx_dev_alloc": (_i, [_vp, _c.POINTER(_vp), _ll]),

@@ -162,9 +162,13 @@ const float kExclVol[7] = {5.15f, 16.44f, 2.49f, 9.13f, 5.73f, 19.86f, 9.85f};
 const int kComplex[9][2] = {{1, 1}, {1, 2}, {1, 3}, {2, 1}, {2, 2}, {2, 3}, {3, 1}, {3, 2}, {5, 1}};
 constexpr float kRho0 = 0.334f;
 
+// The exponentials and powers are evaluated in double from float arguments and
+// rounded to float once per expression, the evaluation order of ff_coef.hpp:86-102
+// under the C library's double exp/pow, so the table matches it bit for bit.
 float excl_ff(int t, float k_sq)
 {
-    return kRho0 * kExclVol[t] * std::exp(-(float)PI * std::pow(kExclVol[t], 2.0f / 3.0f) * k_sq);
+    const double e = std::exp((double)(-(float)PI) * std::pow((double)kExclVol[t], (double)(2.0f / 3.0f)) * (double)k_sq);
+    return (float)((double)(kRho0 * kExclVol[t]) * e);
 }
 
 void atomic_radii(float* out16)
@@ -178,7 +182,7 @@ void atomic_radii(float* out16)
 
 void atomic_ff(const float* q, int nq, float avg_radius, float c, float* out)
 {
-    const float k43 = std::pow(4.0f * (float)PI / 3.0f, 2.0f / 3.0f);
+    const float k43 = (float)std::pow((double)(4.0f * (float)PI / 3.0f), (double)(2.0f / 3.0f));
     for (int i = 0; i < nq; ++i) {
         float* page = out + (size_t)i * 16;
         const float k = q[i] / (4.0f * (float)PI);
@@ -186,11 +190,12 @@ void atomic_ff(const float* q, int nq, float avg_radius, float c, float* out)
         float E = 1.0f;
         if (c != 1.0f) {
             const float c_sq = c * c;
-            E = c_sq * c * std::exp(-(float)PI * k43 * k_sq * avg_radius * avg_radius * (c_sq - 1.0f));
+            const float arg = -(float)PI * k43 * k_sq * avg_radius * avg_radius * (c_sq - 1.0f);
+            E = (float)((double)(c_sq * c) * std::exp((double)arg));
         }
         for (int t = 0; t < 7; ++t) {
             float f = kCM[t][4];
-            for (int a = 0; a < 4; ++a) f += kCM[t][a] * std::exp(-kCM[t][5 + a] * k_sq);
+            for (int a = 0; a < 4; ++a) f = (float)((double)f + (double)kCM[t][a] * std::exp((double)(-kCM[t][5 + a] * k_sq)));
             page[t] = f - excl_ff(t, k_sq) * E;
         }
         for (int t = 7; t < 16; ++t) page[t] = page[kComplex[t - 7][0]] + (float)kComplex[t - 7][1] * page[0];
@@ -1730,12 +1735,13 @@ int rsx_set_variant(rsx_engine* h, int shared_geometry)
     return guarded(h, [&] { h->shared_geometry = shared_geometry != 0; });
 }
 
-int rsx_get_tuning(rsx_engine* h, int* k, int* col_range, int* recurrence)
+int rsx_get_tuning(rsx_engine* h, int* k, int* col_range, int* recurrence, int* shared_geometry)
 {
     return guarded(h, [&] {
         if (k) *k = h->K;
         if (col_range) *col_range = h->last_L;
         if (recurrence) *recurrence = h->uniform ? 1 : 0;
+        if (shared_geometry) *shared_geometry = (h->uniform && h->shared_geometry && h->n_chunks > 1) ? 1 : 0;
     });
 }
 

@@ -102,9 +102,10 @@ class Engine:
         self._check(self._L.rsx_set_variant(self.h, int(bool(shared_geometry))))
 
     def get_tuning(self):
-        k, L, rec = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        self._check(self._L.rsx_get_tuning(self.h, ctypes.byref(k), ctypes.byref(L), ctypes.byref(rec)))
-        return {"k": k.value, "col_range": L.value, "recurrence": bool(rec.value)}
+        k, L, rec, sg = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.rsx_get_tuning(self.h, ctypes.byref(k), ctypes.byref(L), ctypes.byref(rec),
+                                           ctypes.byref(sg)))
+        return {"k": k.value, "col_range": L.value, "recurrence": bool(rec.value), "shared_geometry": bool(sg.value)}
 
     def autotune(self, cache_path=None):
         p = cache_path.encode() if cache_path else None

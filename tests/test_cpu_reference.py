@@ -1,0 +1,117 @@
+"""The oracle and the library's host-side generators against the reference's
+own CPU code (oracle/_ref/libresaxs_ref.so, compiled from the reference's
+sources by `make -C oracle ref`).  Skipped where that library was not built.
+
+What this pins, bit for bit:
+  host_debye<double, float, cl_float4>::calc_curve (host_debye.cpp:54-109)
+      == the oracle's restatement orc_debye_ref_rows, rounded to FP32;
+  atomic_form_factors<float>::generate / calc_radii / calc_avg_radius
+      (ff_coef.hpp:112-196) == orc_atomic_ff / orc_atomic_radii and the
+      library's rsx_atomic_form_factors / rsx_atomic_radii, expansion c != 1 included;
+  create_fibonacci_dots / create_dodeca_icosa_dots (rsasa_algorithm.cpp:96-208)
+      == orc_*_dots and the library's generators;
+  rsasa_cpu_alg::calc_rsasa (rsasa_cpu_alg.cpp:112-185, voxel-pruned) exposed
+      counts == the oracle's brute-force counts over the same dot spheres.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden", "test1888.npz")
+
+from oracle import pyoracle as po  # noqa: E402
+
+pytestmark = pytest.mark.skipif(po.ref_lib() is None, reason="oracle/_ref/libresaxs_ref.so not built "
+                                                               "(needs the reference tree)")
+
+
+def test_host_debye_equals_restatement():
+    g = np.load(GOLDEN)
+    ref = po.ref_host_debye(g["q"], g["pos"], g["types"], g["table"])
+    mine = po.debye_ref_rows(g["q"], g["pos"], g["types"], g["table"], nthreads=4)
+    assert np.array_equal(ref, mine.astype(np.float32))
+    # and against the FP64 oracle (the reference rounds its FP64 sum to FP32)
+    F = po.body_factors(g["pos"], g["types"], g["q"], g["table"])
+    I = po.debye(g["q"], g["pos"], F)
+    assert np.max(np.abs(ref - I) / I) < 1.2e-7
+    # the fixture (positions rounded to 1e-3 A) to its stated accuracy
+    assert np.max(np.abs(ref - g["Iq"]) / g["Iq"]) < 5e-5
+
+
+def test_host_debye_random_and_edge_sizes():
+    rng = np.random.default_rng(31)
+    q = np.linspace(0.0, 0.5, 33).astype(np.float32)
+    table = rng.uniform(1, 9, size=(33, 4)).astype(np.float32)
+    for n in (1, 2, 3, 64, 65, 300):
+        pos = rng.uniform(-9, 9, size=(n, 3)).astype(np.float32)
+        types = rng.integers(0, 4, size=n).astype(np.uint32)
+        ref = po.ref_host_debye(q, pos, types, table)
+        mine = po.debye_ref_rows(q, pos, types, table, nthreads=2).astype(np.float32)
+        assert np.array_equal(ref, mine), n
+
+
+def test_form_factors_and_radii_bit_exact():
+    import resaxs_amd as ra
+    q = np.linspace(0.0, 0.75, 101).astype(np.float32)
+    for r, c in ((1.62, 1.0), (1.62, 0.95), (1.58, 1.08), (1.71, 1.02)):
+        ref = po.ref_atomic_ff(q, r, c)
+        assert np.array_equal(ref, po.atomic_ff(q, r, c)), (r, c)
+        assert np.array_equal(ref, ra.atomic_form_factors(q, r, c)), (r, c)
+    radii = po.ref_atomic_radii()
+    assert np.array_equal(radii, po.atomic_radii())
+    assert np.array_equal(radii, ra.atomic_radii())
+    # average radius over bodies of mixed types (ff_coef.hpp:186-194)
+    rng = np.random.default_rng(4)
+    types = rng.integers(0, 16, size=777).astype(np.uint32)
+    b = po.bodies(rng.normal(size=(777, 3)), types)
+    acc = np.float32(0)
+    for t in types:
+        acc = np.float32(acc + radii[t])
+    assert po.ref_avg_radius(b) == np.float32(acc / np.float32(777))
+
+
+def test_dot_generators_bit_exact():
+    import resaxs_amd as ra
+    for r in (1.07, 1.2, 1.58, 1.7, 1.97):
+        for p in (0.0, 1.4, 1.8):
+            fib = po.ref_fibonacci_dots(r, p)
+            assert fib.shape == (64, 4)
+            assert np.array_equal(fib, po.fibonacci_dots(32, r, p))
+            assert np.array_equal(fib, ra.fibonacci_dots(32, r, p))
+            ico = po.ref_dodeca_icosa_dots(r, p)
+            assert ico.shape == (32, 4)
+            assert np.array_equal(ico, po.dodeca_icosa_dots(r, p))
+            assert np.array_equal(ico, ra.dodeca_icosa_dots(r, p))
+
+
+def padded_density_dots(density, radii, probe):
+    """Per-type density dot spheres (rsasa_algorithm.cpp:58-86) padded to one length with
+    dots at the atom centre: such a dot is always buried by its own atom, so the
+    exposed count is unchanged."""
+    dl = [po.ref_density_dots(density, r, probe) for r in radii]
+    D = max(len(d) for d in dl)
+    dots = np.zeros((len(radii), D, 4), dtype=np.float32)
+    for t, d in enumerate(dl):
+        dots[t, :len(d)] = d
+        dots[t, len(d):, 3] = radii[t]
+    return dots, np.array([len(d) for d in dl])
+
+
+def test_rsasa_cpu_counts_equal_oracle():
+    radii = po.ref_atomic_radii()
+    probe = 1.8
+    rng = np.random.default_rng(3)
+    for density in (0.5, 2.0):
+        dots, nd = padded_density_dots(density, radii, probe)
+        R2 = ((radii + np.float32(probe)) ** 2).astype(np.float32)
+        for n, half in ((1, 1.0), (60, 8.0), (400, 14.0), (1500, 22.0)):
+            pos = rng.uniform(-half, half, size=(n, 3)).astype(np.float32)
+            types = rng.integers(0, 16, size=n).astype(np.uint32)
+            frac = po.ref_rsasa_cpu(po.bodies(pos, types), density, probe)
+            ref_counts = np.rint(frac * nd[types]).astype(np.int64)
+            assert np.allclose(ref_counts / nd[types], frac, rtol=0, atol=1e-12)
+            got = po.sasa_counts(pos, types, R2, dots, cells=False).astype(np.int64)
+            assert np.array_equal(ref_counts, got), (density, n)
