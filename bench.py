@@ -118,16 +118,30 @@ def adjoint_bench(reps=3):
             "timing": "host wall clock around the C-ABI call (host buffers in and out)"}
 
 
-def cpu_baseline(w, I_gpu, rows=2048):
-    """The reference CPU loop (host_debye.cpp:54-109 restated in oracle/) on a bounded
-    row sample, 1 thread; plus the FP64 oracle's full I(q) as the accuracy check."""
+def cpu_baseline(w, I_gpu, rows=2560):
+    """The reference's CPU path timed on this host, 1 thread, on a bounded sample: the
+    pairs among the first `rows` atoms (the first rows of the lower triangle) at all
+    n_q.  kind "reference": host_debye<double, float, cl_float4>::calc_curve compiled
+    from the reference's sources (oracle/_ref); kind "port": its restatement in the
+    oracle where that library was not built.  Plus the FP64 oracle's full I(q) as the
+    accuracy check of the GPU result."""
     from oracle import pyoracle as po
     nq = len(w.q)
-    log(f"bench: CPU baseline, reference loop over rows 0..{rows - 1}, 1 thread")
-    t0 = time.perf_counter()
-    po.debye_ref_rows(w.q, w.pos, w.types, w.table, 0, rows, nthreads=1)
-    t = time.perf_counter() - t0
     terms = rows * (rows - 1) / 2.0 * nq
+    pos, types = w.pos[:rows], w.types[:rows]
+    if po.ref_lib() is not None:
+        log(f"bench: CPU baseline, reference host_debye over the first {rows} atoms, 1 thread")
+        t0 = time.perf_counter()
+        po.ref_host_debye(w.q, pos, types, w.table)
+        t = time.perf_counter() - t0
+        kind, what = "reference", "host_debye<double,float,cl_float4>::calc_curve (host_debye.cpp:54-109, " \
+                                  "compiled from the reference's sources)"
+    else:
+        log(f"bench: CPU baseline, restated reference loop over the first {rows} atoms, 1 thread")
+        t0 = time.perf_counter()
+        po.debye_ref_rows(w.q, pos, types, w.table, 0, rows, nthreads=1)
+        t = time.perf_counter() - t0
+        kind, what = "port", "the oracle's restatement of host_debye.cpp:54-109"
     log("bench: FP64 oracle of the full workload (accuracy check)")
     dots = np.stack([po.fibonacci_dots(w.n_half, r, w.probe) for r in w.radii])
     rr = (w.radii.astype(np.float32) + np.float32(w.probe)).astype(np.float32)
@@ -136,9 +150,9 @@ def cpu_baseline(w, I_gpu, rows=2048):
     F = po.body_factors(w.pos, w.types, w.q, w.table, w.water_index, w.w, s)
     I_ref = po.debye(w.q, w.pos, F)
     err = np.abs(I_gpu - I_ref) / np.abs(I_ref)
-    base = {"value": terms / t, "unit": "pair.q terms/s", "cores": 1, "kind": "port",
-            "sample": f"reference loop (host_debye.cpp:54-109) over rows 0..{rows - 1} of the "
-                      f"{len(w.pos)}-atom lower triangle x {nq} q = {terms:.3e} terms, {t:.1f} s",
+    base = {"value": terms / t, "unit": "pair.q terms/s", "cores": 1, "kind": kind,
+            "sample": f"{what} on the first {rows} atoms of the {len(w.pos)}-atom workload (plain factors) x "
+                      f"{nq} q = {terms:.3e} pair.q terms in {t:.1f} s",
             "cpu": platform.processor() or platform.machine()}
     acc = {"max": float(err.max()), "median": float(np.median(err)), "reference": "FP64 oracle, full I(q)"}
     return base, acc

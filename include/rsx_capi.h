@@ -96,6 +96,12 @@ int rsx_calc_rsasa(rsx_engine *h, const float *bodies, int n, const float *radii
 int rsx_fibonacci_dots(int n_half, float radius, float probe, float *out);
 /* rsasa_algorithm.cpp:157-208: 32 dots {x,y,z,radius} */
 int rsx_dodeca_icosa_dots(float radius, float probe, float *out);
+/* rsasa_algorithm.cpp:58-80 (create_sphere_dots): rings of dots for n_sect sectors; writes at most
+ * cap dots {x,y,z,radius} to out and the full count to *n_dots.  The reference's create_dots uses
+ * n_sect = 2 pi radius sqrt(density), create_32_dots n_sect = 10.5. */
+int rsx_sphere_dots(float radius, float n_sect, float probe, float *out, int cap, int *n_dots);
+/* name, architecture, CU count and memory of a HIP device (report_opencl_caps, utils.hpp:38) */
+int rsx_device_name(int device, char *buf, int len);
 /* ff_coef.hpp:112-211: default table [n_q][16] and the 16 type radii */
 int rsx_atomic_form_factors(const float *q, int n_q, float avg_radius, float expansion, float *out);
 int rsx_atomic_radii(float *out16);

@@ -55,6 +55,8 @@ SIGNATURES = {
     "rsx_calc_rsasa": (_i, [_vp, _fp, _i, _fp, _i, _f, _fp, _i, _up, _fp]),
     "rsx_fibonacci_dots": (_i, [_i, _f, _f, _fp]),
     "rsx_dodeca_icosa_dots": (_i, [_f, _f, _fp]),
+    "rsx_sphere_dots": (_i, [_f, _f, _f, _fp, _i, _ip]),
+    "rsx_device_name": (_i, [_i, _c.c_char_p, _i]),
     "rsx_atomic_form_factors": (_i, [_fp, _i, _f, _f, _fp]),
     "rsx_atomic_radii": (_i, [_fp]),
     "rsx_batch_set_frames": (_i, [_vp, _fp, _ip, _i]),

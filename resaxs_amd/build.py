@@ -53,5 +53,25 @@ def build(verbose=False):
     return LIB
 
 
+COMPAT_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "compat_test.cpp")
+COMPAT_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "compat_test")
+
+
+def build_compat_test(verbose=False):
+    """Compile the C++ program that drives include/resaxs/ (the resaxslib-compatible
+    surface) against libresaxs_amd.so, as a C++14 host program (the reference's
+    language level); g++ only, no device code."""
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-std=c++14", "-O2", "-Wall", "-Wextra", "-Wno-unused-parameter", "-I" + os.path.join(ROOT, "include"),
+           COMPAT_TEST_SRC, "-o", COMPAT_TEST_BIN + ".tmp", "-L" + HERE, "-lresaxs_amd",
+           "-Wl,-rpath,$ORIGIN/../../resaxs_amd", "-Wl,--allow-shlib-undefined"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(COMPAT_TEST_BIN + ".tmp", COMPAT_TEST_BIN)
+    return COMPAT_TEST_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_compat_test(verbose=True))

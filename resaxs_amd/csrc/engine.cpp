@@ -110,6 +110,33 @@ void fibonacci_dots(int nhalf, float radius, float probe, float* out)
     }
 }
 
+// rsasa_algorithm.cpp:58-80: rings of dots at polar angles pi i / (n_sect / 2), each
+// ring with about sin(phi) n_sect dots; float arithmetic throughout, as the
+// reference's float instantiation evaluates it.  Returns the dot count; writes at
+// most cap dots.
+int sphere_dots(float radius, float n_sect, float probe, float* out, int cap)
+{
+    const float n_vert = 0.5f * n_sect;
+    const float ext = radius + probe;
+    int d = 0;
+    for (unsigned i = 0; (float)i < n_vert; ++i) {
+        const float phi = ((float)PI * (float)i) / n_vert;
+        const float z = std::cos(phi) * ext;
+        const float xy = std::sin(phi);
+        const float ring = xy * n_sect;
+        for (unsigned j = 0; (float)j < ring - 1.0f; ++j, ++d) {
+            const float theta = (2.0f * (float)PI * (float)j) / ring;
+            if (d < cap) {
+                out[4 * d + 0] = xy * std::cos(theta) * ext;
+                out[4 * d + 1] = xy * std::sin(theta) * ext;
+                out[4 * d + 2] = z;
+                out[4 * d + 3] = radius;
+            }
+        }
+    }
+    return d;
+}
+
 // rsasa_algorithm.cpp:157-208: dodecahedron (20) + icosahedron (12) vertices
 void dodeca_icosa_dots(float radius, float probe, float* out)
 {
@@ -1635,6 +1662,25 @@ int rsx_dodeca_icosa_dots(float radius, float probe, float* out)
 {
     if (!out) return RSX_INVALID_ARG;
     dodeca_icosa_dots(radius, probe, out);
+    return RSX_OK;
+}
+
+int rsx_sphere_dots(float radius, float n_sect, float probe, float* out, int cap, int* n_dots)
+{
+    if (!n_dots || cap < 0 || (cap > 0 && !out) || !(n_sect >= 0.0f) || n_sect > 1e5f) return RSX_INVALID_ARG;
+    *n_dots = sphere_dots(radius, n_sect, probe, out, cap);
+    return RSX_OK;
+}
+
+int rsx_device_name(int device, char* buf, int len)
+{
+    if (!buf || len <= 0) return RSX_INVALID_ARG;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || device < 0 || device >= c) return RSX_INVALID_DEVICE;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) return RSX_HIP_ERROR;
+    std::snprintf(buf, (size_t)len, "%s (%s, %d CUs, %.0f GiB)", p.name, p.gcnArchName, p.multiProcessorCount,
+                  (double)p.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
     return RSX_OK;
 }
 

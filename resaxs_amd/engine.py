@@ -256,6 +256,36 @@ def dodeca_icosa_dots(radius, probe):
     return out
 
 
+def sphere_dots(radius, n_sect, probe):
+    """Rings-of-dots sphere (the reference's create_sphere_dots); [n, 4] float32."""
+    n = ctypes.c_int()
+    L = _lib.lib()
+    rc = L.rsx_sphere_dots(float(radius), float(n_sect), float(probe), None, 0, ctypes.byref(n))
+    if rc:
+        raise RsxError(rc, "sphere_dots")
+    out = np.zeros((max(n.value, 1), 4), dtype=np.float32)
+    rc = L.rsx_sphere_dots(float(radius), float(n_sect), float(probe), _ptr(out, ctypes.c_float), n.value,
+                           ctypes.byref(n))
+    if rc:
+        raise RsxError(rc, "sphere_dots")
+    return out[:n.value]
+
+
+def density_dots(density, radius, probe):
+    """The reference's create_dots: n_sect = 2 pi radius sqrt(density) (float arithmetic)."""
+    s = np.sqrt(np.float32(density), dtype=np.float32)
+    n_sect = np.float32(np.float32(np.float32(2.0) * np.float32(np.pi)) * np.float32(radius)) * s
+    return sphere_dots(radius, n_sect, probe)
+
+
+def device_name(device=0):
+    buf = ctypes.create_string_buffer(256)
+    rc = _lib.lib().rsx_device_name(int(device), buf, 256)
+    if rc:
+        raise RsxError(rc, "device_name")
+    return buf.value.decode()
+
+
 def atomic_form_factors(q, avg_radius=1.62, expansion=1.0):
     q = _f32(q)
     out = np.zeros((len(q), 16), dtype=np.float32)

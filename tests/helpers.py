@@ -41,3 +41,17 @@ def oracle_iq(q, pos, types, table, water_index=-1, w=0.0, counts=None, n_dots=N
         s = (counts.astype(np.float32) / np.float32(n_dots)).astype(np.float32)
     F = po.body_factors(pos, types, q, table, water_index, w, s)
     return po.debye(q, pos, F)
+
+
+def padded_density_dots(density, radii, probe):
+    """The reference's per-type density dot spheres (rsasa_algorithm.cpp:58-86, from the
+    compiled reference) padded to one length with dots at the atom centre; such a dot is
+    always buried by its own atom, so exposed counts are unchanged.  Returns
+    (dots [n_types, D, 4], real dot count per type)."""
+    dl = [po.ref_density_dots(density, r, probe) for r in radii]
+    D = max(len(d) for d in dl)
+    dots = np.zeros((len(radii), D, 4), dtype=np.float32)
+    for t, d in enumerate(dl):
+        dots[t, :len(d)] = d
+        dots[t, len(d):, 3] = radii[t]
+    return dots, np.array([len(d) for d in dl])

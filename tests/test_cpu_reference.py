@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden", "test1888.npz")
 
 from oracle import pyoracle as po  # noqa: E402
+from helpers import padded_density_dots  # noqa: E402
 
 pytestmark = pytest.mark.skipif(po.ref_lib() is None, reason="oracle/_ref/libresaxs_ref.so not built "
                                                                "(needs the reference tree)")
@@ -87,17 +88,14 @@ def test_dot_generators_bit_exact():
             assert np.array_equal(ico, ra.dodeca_icosa_dots(r, p))
 
 
-def padded_density_dots(density, radii, probe):
-    """Per-type density dot spheres (rsasa_algorithm.cpp:58-86) padded to one length with
-    dots at the atom centre: such a dot is always buried by its own atom, so the
-    exposed count is unchanged."""
-    dl = [po.ref_density_dots(density, r, probe) for r in radii]
-    D = max(len(d) for d in dl)
-    dots = np.zeros((len(radii), D, 4), dtype=np.float32)
-    for t, d in enumerate(dl):
-        dots[t, :len(d)] = d
-        dots[t, len(d):, 3] = radii[t]
-    return dots, np.array([len(d) for d in dl])
+def test_sphere_dot_generator_bit_exact():
+    """The library's rings-of-dots generator equals create_dots / create_32_dots."""
+    import resaxs_amd as ra
+    for r in (1.07, 1.2, 1.58, 1.7, 1.97, 2.3):
+        for p in (0.0, 1.4, 1.8):
+            for dens in (0.25, 0.5, 2.0, 5.0):
+                assert np.array_equal(ra.density_dots(dens, r, p), po.ref_density_dots(dens, r, p)), (r, p, dens)
+    assert len(ra.sphere_dots(1.7, 10.5, 1.4)) > 0
 
 
 def test_rsasa_cpu_counts_equal_oracle():

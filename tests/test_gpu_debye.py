@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from helpers import engine_for, oracle_iq, rel_err
+from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
@@ -37,6 +38,26 @@ def test_golden_test1888(oracle):
     # q = 0: (sum F)^2 exactly (FP64 rounding only)
     F0 = g["table"][0][g["types"]].astype(np.float64).sum()
     assert abs(I[0] - F0 * F0) <= 1e-12 * F0 * F0
+
+
+@pytest.mark.skipif(po.ref_lib() is None, reason="reference CPU subset (oracle/_ref) not built")
+def test_golden_vs_reference_host_debye():
+    """GPU I(q) against the reference's own CPU path (host_debye.cpp:54-109 compiled from
+    its sources; FP64 sums rounded to FP32) on the golden inputs and a random cluster."""
+    from resaxs_amd import configs
+    g = np.load(GOLDEN)
+    _say("case: reference host_debye, test1888 + 2500-atom cluster")
+    e = engine_for(g["q"], g["table"], n=len(g["pos"]))
+    I = e.calc_curve(po.bodies(g["pos"], g["types"]))
+    assert rel_err(I, po.ref_host_debye(g["q"], g["pos"], g["types"], g["table"])).max() < TOL
+    rng = np.random.default_rng(17)
+    pos = configs.in_ball(rng, 2500, 22.0).astype(np.float32)
+    types = configs.seeded_types(rng, 2500)
+    q = np.linspace(0.0, 0.5, 101).astype(np.float32)
+    table = configs.factor_table(rng, q)
+    e = engine_for(q, table, n=2500)
+    I = e.calc_curve(configs.bodies_of(pos, types))
+    assert rel_err(I, po.ref_host_debye(q, pos, types, table)).max() < TOL
 
 
 def test_config1_plain(oracle):

@@ -9,7 +9,8 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import engine_for, fib_dot_table, oracle_iq, r2_table, rel_err
+from helpers import engine_for, fib_dot_table, oracle_iq, padded_density_dots, r2_table, rel_err
+from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
@@ -94,3 +95,26 @@ def test_hydrated_full_size(oracle):
     k = int(np.argmax(err))
     _say(f"  max rel err {err.max():.3e} at q={w.q[k]:.4f}, median {np.median(err):.3e}")
     assert err.max() < 1e-5
+
+
+@pytest.mark.skipif(po.ref_lib() is None, reason="reference CPU subset (oracle/_ref) not built")
+def test_sasa_counts_vs_reference_cpu_algorithm():
+    """GPU exposed-dot counts equal the reference's own CPU SASA (rsasa_cpu_alg.cpp:112-185,
+    voxel-pruned, compiled from its sources) on its density dot spheres, the 16 atomic
+    radii (ff_coef.hpp:161-184) and its 1.8 A probe."""
+    from resaxs_amd import configs
+    radii = po.ref_atomic_radii()
+    probe = 1.8
+    rng = np.random.default_rng(41)
+    e = _engine()
+    for density, n in ((0.5, 2500), (2.0, 6000)):
+        _say(f"case: reference CPU SASA, density {density}, {n} atoms")
+        dots, nd = padded_density_dots(density, radii, probe)
+        pos = configs.in_ball(rng, n, (n / 0.045 / (4.0 / 3.0 * np.pi)) ** (1.0 / 3.0)).astype(np.float32)
+        types = rng.integers(0, 16, size=n).astype(np.uint32)
+        b = po.bodies(pos, types)
+        counts, _ = e.calc_rsasa(b, radii, probe, dots)
+        frac = po.ref_rsasa_cpu(b, density, probe)
+        ref = np.rint(frac * nd[types]).astype(np.int64)
+        assert np.array_equal(counts.astype(np.int64), ref)
+        assert 0 < ref.sum() < nd[types].sum()
