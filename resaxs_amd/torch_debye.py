@@ -63,7 +63,7 @@ def chi2_loss(pos, engine, types, i_exp, sigma):
     return Chi2Loss.apply(pos, engine, types, i_exp, sigma)
 
 
-def refine(engine, types, start, i_exp, sigma, steps=100, lr=0.02):
+def refine(engine, types, start, i_exp, sigma, steps=100, lr=0.01):
     """Gradient-descent refinement of positions against a profile (Adam on chi^2).
 
     Returns (positions [N, 3] float32, chi^2 per step)."""

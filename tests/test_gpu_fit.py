@@ -7,7 +7,7 @@ start displaced by N(0, 0.5 A)):
     coordinates: relative error of the 192-vector <= 1e-4;
   - the torch.autograd wrappers (DebyeIntensity, Chi2Loss) give the adjoint's
     numbers through backward();
-  - 100 gradient-descent (Adam) refinement steps lower chi^2.
+  - 100 gradient-descent (Adam) refinement steps bring chi^2 to the noise floor.
 """
 import sys
 
@@ -98,8 +98,9 @@ def test_refinement_100_steps():
     _say("case: 100 Adam steps on chi2, 10000 atoms x 200 q")
     e = engine_for(w.q, w.table, n=len(w.pos))
     i_exp, sigma = _target(w, e)
-    pos, hist = refine(e, w.types, w.start, i_exp, sigma, steps=100, lr=0.02)
+    pos, hist = refine(e, w.types, w.start, i_exp, sigma, steps=100, lr=0.01)
     _say(f"  chi2: start {hist[0]:.4e}, after 10 {hist[9]:.4e}, after 100 {hist[-1]:.4e}")
-    assert np.all(np.isfinite(hist))
-    assert hist[-1] < 0.2 * hist[0]
-    assert np.all(np.isfinite(pos))
+    assert np.all(np.isfinite(hist)) and np.all(np.isfinite(pos))
+    # the start (0.5 A random displacements) already fits to chi^2 / n_q ~ 2; 1 % noise puts
+    # the floor near chi^2 = n_q: the refinement must get there
+    assert hist[-1] < hist[0] and hist[-1] < 1.25 * len(w.q)
