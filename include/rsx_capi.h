@@ -115,23 +115,30 @@ int rsx_batch_calc(rsx_engine *h, double *out_frames, double *out_mean);
 int rsx_batch_calc_dev_async(rsx_engine *h, double *out_frames_dev, double *out_mean_dev);
 
 /* ---- differentiable fit (no reference counterpart; uses saxs_profile.hpp:137-166 scale/chi) ---- */
-/* chi2 = sum_k (I_exp - c I)^2 / sigma^2 at the least-squares c; grad = dchi2/dx [n][3] */
+/* chi2 = sum_k (I_exp - c I)^2 / sigma^2 at the least-squares c; grad = dchi2/dx [n][3].
+ * The chi^2 weights amplify forward errors by I / sigma, so this and rsx_debye_vjp run the FP64
+ * recurrence: an engine on RSX_VARIANT_FP32 moves to RSX_VARIANT_WAVE on the first call. */
 int rsx_chi2_grad(rsx_engine *h, const float *bodies, int n, const float *i_exp, const float *sigma,
                   double *out_chi2, double *out_scale, float *out_grad, double *out_iq);
 /* adjoint only: grad_i = sum_k w_k dI_k/dx_i for caller weights w [n_q] (backward of I(q)) */
 int rsx_debye_vjp(rsx_engine *h, const float *bodies, int n, const double *w, float *out_grad);
 
 /* ---- tuning and measurement ---- */
-/* choose q-per-thread and column-range length for the current n_q / size; cache in a text file */
+/* choose variant, q-per-thread and column-range length for the current n_q / size; cache in a text file */
 int rsx_autotune(rsx_engine *h, const char *cache_path);
 /* tuning knobs: k (q per thread, 0 = automatic), col_range (atoms per job, 0 = automatic) */
 int rsx_set_tuning(rsx_engine *h, int k, int col_range);
-/* pair-kernel variant: 1 = one workgroup per job shares the pair geometry across its q chunks
- * through LDS (default), 0 = one wave per (job, chunk) */
-int rsx_set_variant(rsx_engine *h, int shared_geometry);
+/* recurrence kernel for uniform q grids (the next calc_curve lays out again):
+ *   RSX_VARIANT_WAVE   FP64 recurrence, one wave per (job, q chunk)
+ *   RSX_VARIANT_SHARED FP64 recurrence, the q chunks of a job share the pair geometry through LDS
+ *   RSX_VARIANT_FP32   FP32 difference-form recurrence (default) */
+#define RSX_VARIANT_WAVE    0
+#define RSX_VARIANT_SHARED  1
+#define RSX_VARIANT_FP32    2
+int rsx_set_variant(rsx_engine *h, int variant);
 /* current q per thread, column range of the forward layout, 1 if the recurrence path is active,
- * and 1 if the shared-geometry pair kernel runs (any pointer may be NULL) */
-int rsx_get_tuning(rsx_engine *h, int *k, int *col_range, int *recurrence, int *shared_geometry);
+ * and the recurrence variant (any pointer may be NULL) */
+int rsx_get_tuning(rsx_engine *h, int *k, int *col_range, int *recurrence, int *variant);
 /* average device time (ms) of the last timed region, measured with HIP events on the engine stream */
 int rsx_time_forward(rsx_engine *h, int reps, float *ms_total, float *ms_pair_kernel);
 /* plain device-to-device copy of `bytes` (float4 grid-stride kernel), average ms per launch;

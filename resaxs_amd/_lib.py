@@ -24,6 +24,10 @@ RSX_NOT_SUPPORTED = -102
 RSX_DOTS_DODECA_ICOSA = 0
 RSX_DOTS_FIBONACCI = 1
 
+RSX_VARIANT_WAVE = 0
+RSX_VARIANT_SHARED = 1
+RSX_VARIANT_FP32 = 2
+
 # every symbol include/rsx_capi.h declares, with (restype, argtypes)
 _c = ctypes
 _vp, _fp, _dp, _ip, _up = _c.c_void_p, _c.POINTER(_c.c_float), _c.POINTER(_c.c_double), _c.POINTER(_c.c_int), \

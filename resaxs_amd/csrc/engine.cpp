@@ -272,8 +272,9 @@ struct rsx_engine {
 
     // tuning
     int k_req = 0, range_req = 0;
-    bool shared_geometry = true;  // workgroup-shared pair geometry (uniform path)
+    int variant = rsx::VARIANT_FP32;  // recurrence kernel (rsx::VARIANT_*)
     int K = 0, n_chunks = 0, ldf = 0;
+    double path_radius = 0.0;  // bounding radius the current path and series length were chosen for
     int rebase_every = 200;
 
     // bodies (host mirror, caller order)
@@ -348,19 +349,30 @@ void set_device(rsx_engine* h)
 }
 
 // ---- q grid and chunking -------------------------------------------------
+// K list of the active pair kernel
+int pair_path(const rsx_engine* h)
+{
+    return !h->uniform ? rsx::PATH_DIRECT : h->variant == rsx::VARIANT_FP32 ? rsx::PATH_REC32 : rsx::PATH_REC64;
+}
+
 void choose_chunks(rsx_engine* h)
 {
     const bool uni = h->uniform;
-    const int nk = rsx::debye_k_count(uni);
+    const int path = pair_path(h);
+    const int nk = rsx::debye_k_count(path);
     int bestK = 0, bestC = 0;
     double best = 1e300;
     for (int i = 0; i < nk; ++i) {
-        const int K = rsx::debye_k_value(uni, i);
+        const int K = rsx::debye_k_value(path, i);
         if (h->k_req > 0 && K != h->k_req) continue;
         if (h->k_req <= 0 && !uni && K > 48) continue;  // the direct kernel spills at K = 64
         const int C = (h->nq + K - 1) / K;
-        // per (pair, chunk): 2K recurrence ops + ~44 for distance and seeds
-        const double cost = uni ? C * (2.0 * K + 44.0) : C * (6.0 * K + 12.0);
+        // cost per (pair, chunk) in FP64-op units: the FP64 recurrence 2K + ~44 for distance
+        // and seeds; the FP32 one 3K FP32 ops (~0.6 of an FP64 op each) + ~25; K above
+        // 64 runs at 2 instead of 3 waves per SIMD (~10 % slower per op)
+        const double cost = !uni ? C * (6.0 * K + 12.0)
+                            : path == rsx::PATH_REC32 ? C * (1.8 * K + 25.0) * (K > 64 ? 1.1 : 1.0)
+                                                      : C * (2.0 * K + 44.0);
         if (cost < best) {
             best = cost;
             bestK = K;
@@ -470,6 +482,7 @@ double bounding_radius(const float* bodies, int n, double* centre)
 // choose the path for a structure of the given bounding radius; true if it changed
 bool select_path(rsx_engine* h, double radius)
 {
+    h->path_radius = radius;
     const bool want = fits_recurrence(h, radius);
     if (want == h->uniform) return false;
     apply_path(h, want);
@@ -702,7 +715,9 @@ rsx::DebyeArgs debye_args(rsx_engine* h, const PairJob* d_jobs, double* partials
     a.qrev = h->d_qrev.p;
     a.qv = h->d_qv.p;
     a.partials = partials;
-    a.shared_geometry = h->shared_geometry ? 1 : 0;
+    a.variant = h->variant;
+    // pairs satisfy r <= 2 R about the centre of the layout, so dq r / 2 <= dq R
+    a.sin_terms = rsx::sin_series_terms(h->dq * h->path_radius);
     return a;
 }
 
@@ -1323,38 +1338,41 @@ void autotune_impl(rsx_engine* h, const char* cache_path)
             std::istringstream ls(line);
             std::string k;
             int K = 0, L = 0, V = 1;
-            if ((ls >> k >> K >> L >> V) && k == key) {
-                h->shared_geometry = V != 0;
+            if ((ls >> k >> K >> L >> V) && k == key && V >= 0 && V < rsx::VARIANT_COUNT) {
+                h->variant = V;
                 retune(h, K, L);
                 full_forward(h);
                 return;
             }
         }
     }
-    std::vector<int> Ks;
-    for (int i = 0; i < rsx::debye_k_count(h->uniform); ++i) {
-        const int k = rsx::debye_k_value(h->uniform, i);
-        if (k >= 16 && !(k == 64 && !h->uniform)) Ks.push_back(k);
-    }
     const int Ls[] = {256, 512, 768, 1024, 1536, 2048, 3072, 4096};
     int bestK = 0, bestL = 0;
     float best = 1e30f;
-    int bestV = 1;
-    for (int V = 0; V <= 1; ++V)
+    int bestV = h->variant;
+    const int n_var = h->uniform ? rsx::VARIANT_COUNT : 1;  // the direct path has one kernel
+    for (int V = 0; V < n_var; ++V) {
+        if (h->uniform) h->variant = V;
+        const int path = pair_path(h);
+        std::vector<int> Ks;
+        for (int i = 0; i < rsx::debye_k_count(path); ++i) {
+            const int k = rsx::debye_k_value(path, i);
+            if (k >= 16 && !(k == 64 && !h->uniform)) Ks.push_back(k);
+        }
         for (int K : Ks)
             for (int L : Ls) {
                 if (L > std::max(256, h->n)) continue;
-                h->shared_geometry = V != 0;
                 retune(h, K, L);
                 const float ms = time_pair_kernel(h, 3);
                 if (ms < best) {
                     best = ms;
                     bestK = K;
                     bestL = L;
-                    bestV = V;
+                    bestV = h->variant;
                 }
             }
-    h->shared_geometry = bestV != 0;
+    }
+    h->variant = bestV;
     retune(h, bestK, bestL);
     full_forward(h);
     if (cache_path && *cache_path) {
@@ -1379,6 +1397,20 @@ int guarded(rsx_engine* h, F&& f)
         h->err = e.what();
         return RSX_HIP_ERROR;
     }
+}
+
+// The chi^2 weights w_k = -2 c (I_exp - c I) / sigma^2 amplify forward errors by
+// I / sigma (~100x at 1 % errors), so the gradient entry points run the FP64
+// recurrence: an engine on the FP32 variant moves to the one-wave FP64 kernel (and
+// stays there) on its first gradient call.
+void use_fp64_forward(rsx_engine* h)
+{
+    if (h->variant != rsx::VARIANT_FP32 || h->nq <= 0) return;
+    h->variant = rsx::VARIANT_WAVE;
+    setup_q(h);
+    h->params_dirty = true;
+    batch_release_graph(h);
+    h->b_frames = 0;
 }
 
 }  // namespace
@@ -1736,6 +1768,7 @@ int rsx_chi2_grad(rsx_engine* h, const float* bodies, int n, const float* i_exp,
     return guarded(h, [&] {
         if (!i_exp || !sigma) fail(RSX_INVALID_ARG, "chi2_grad: null profile");
         if (!h->uniform) fail(RSX_NOT_SUPPORTED, "chi2_grad needs a uniform q grid");
+        use_fp64_forward(h);
         std::vector<double> I(h->nq);
         calc_curve_impl(h, bodies, n, nullptr, I.data());
         double num = 0.0, den = 0.0;
@@ -1765,6 +1798,7 @@ int rsx_debye_vjp(rsx_engine* h, const float* bodies, int n, const double* w, fl
     return guarded(h, [&] {
         if (!w || !out_grad) fail(RSX_INVALID_ARG, "vjp: null argument");
         if (!h->uniform) fail(RSX_NOT_SUPPORTED, "vjp needs a uniform q grid");
+        use_fp64_forward(h);
         calc_curve_impl(h, bodies, n, nullptr, nullptr);
         std::vector<double> wk(w, w + h->nq);
         adjoint_impl(h, wk, out_grad);
@@ -1776,18 +1810,30 @@ int rsx_autotune(rsx_engine* h, const char* cache_path)
     return guarded(h, [&] { autotune_impl(h, cache_path); });
 }
 
-int rsx_set_variant(rsx_engine* h, int shared_geometry)
+int rsx_set_variant(rsx_engine* h, int variant)
 {
-    return guarded(h, [&] { h->shared_geometry = shared_geometry != 0; });
+    return guarded(h, [&] {
+        if (variant < 0 || variant >= rsx::VARIANT_COUNT) fail(RSX_INVALID_ARG, "variant must be 0, 1 or 2");
+        if (variant == h->variant) return;
+        h->variant = variant;
+        if (h->nq > 0) {  // the K list may differ: lay out again on the next calc_curve
+            set_device(h);
+            setup_q(h);
+            h->params_dirty = true;
+            if (h->state > 3) h->state = 3;
+            batch_release_graph(h);
+            h->b_frames = 0;
+        }
+    });
 }
 
-int rsx_get_tuning(rsx_engine* h, int* k, int* col_range, int* recurrence, int* shared_geometry)
+int rsx_get_tuning(rsx_engine* h, int* k, int* col_range, int* recurrence, int* variant)
 {
     return guarded(h, [&] {
         if (k) *k = h->K;
         if (col_range) *col_range = h->last_L;
         if (recurrence) *recurrence = h->uniform ? 1 : 0;
-        if (shared_geometry) *shared_geometry = (h->uniform && h->shared_geometry && h->n_chunks > 1) ? 1 : 0;
+        if (variant) *variant = h->variant;
     });
 }
 

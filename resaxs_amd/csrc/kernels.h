@@ -38,6 +38,33 @@ struct ChunkConst {
     float pad;
 };
 
+// highest power of the FP64 sin(dq r / 2) series is x^(2 SIN_MAX_TERMS + 1)
+constexpr int SIN_MAX_TERMS = 8;
+
+// series length for a layout whose pairs satisfy dq r / 2 <= x_max: the first
+// omitted term (x^(2m+2) / (2m+3)! relative) stays below 1e-16 at 1.25 x_max
+// (margin for device-resident positions that drift from the laid-out ones)
+inline int sin_series_terms(double x_max)
+{
+    const double x = 1.25 * x_max;
+    double term = x * x / 6.0;  // m = 0: x^2 / 3!
+    int m = 0;
+    while (m < SIN_MAX_TERMS && term > 1e-16) {
+        ++m;
+        term *= x * x / ((2.0 * m + 2.0) * (2.0 * m + 3.0));
+    }
+    return m < 2 ? 2 : m;
+}
+
+// pair-kernel paths (K lists differ) and recurrence variants
+enum { PATH_DIRECT = 0, PATH_REC64 = 1, PATH_REC32 = 2 };
+enum {
+    VARIANT_WAVE = 0,    // FP64 recurrence, one wave per (job, q chunk)
+    VARIANT_SHARED = 1,  // FP64 recurrence, the q chunks of a job share the pair geometry through LDS
+    VARIANT_FP32 = 2,    // FP32 difference-form recurrence, one wave per (job, q chunk)
+    VARIANT_COUNT = 3
+};
+
 struct DebyeArgs {
     const PairJob* jobs;
     int n_chunks;         // q chunks per job (one wave each)
@@ -52,15 +79,17 @@ struct DebyeArgs {
     const double* qrev;   // [ldf] q / (2 pi)  (direct path)
     const float* qv;      // [ldf] q           (direct path)
     double* partials;     // [n_slots][ldf]
-    int shared_geometry;  // 1: one workgroup per job shares the pair geometry across its q chunks
+    int variant;          // recurrence kernel: VARIANT_WAVE, VARIANT_SHARED or VARIANT_FP32
+    int sin_terms;        // terms the sin(dq r / 2) series needs (sin_series_terms); the launcher
+                          // picks the shortest instantiated length >= it
 };
 
 // Debye pair sums.  uniform: q_k = a + k dq within each chunk (FP64
 // three-term recurrence); otherwise direct FP32 sin per term.
 hipError_t launch_debye(const DebyeArgs& a, int n_jobs, int K, bool uniform, hipStream_t s);
 // K values the launcher accepts, ascending.
-int debye_k_count(bool uniform);
-int debye_k_value(bool uniform, int i);
+int debye_k_count(int path);
+int debye_k_value(int path, int i);
 
 // out[o*out_ld + k] (+)= sum_{s in [slot_begin[o], slot_begin[o+1])} partials[s*ldf + k]
 // in a fixed order.  q0_index >= 0: that k is set to q0_val[o] instead (the

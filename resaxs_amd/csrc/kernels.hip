@@ -103,48 +103,74 @@ __device__ __forceinline__ double sin_poly(double x)
     return x * p;
 }
 
-template <bool WEIGHTED>
-__device__ __forceinline__ Seeds make_seeds(float xi, float yi, float zi, const float4 c,
-                                            const ChunkConst& cc, double half_dq, float dq, float w)
+// The same series cut after x^(2M+1) (M a template argument, so the Horner chain
+// is unrolled and branch-free); the host picks M so that the first omitted term
+// stays below 1e-16 relative for the layout's largest dq r / 2.
+template <int M>
+__device__ __forceinline__ double sin_series(double x)
 {
-    // branch-free: every value stays finite and r == 0 is blended in with a 0/1 mask,
-    // so the seeds of the next column pair can be scheduled into the recurrence block
+    constexpr double c[9] = {1.0,
+                             -1.6666666666666666e-01,
+                             8.333333333333333e-03,
+                             -1.984126984126984e-04,
+                             2.7557319223985893e-06,
+                             -2.505210838544172e-08,
+                             1.6059043836821613e-10,
+                             -7.647163731819816e-13,
+                             2.8114572543455206e-15};
+    static_assert(M >= 2 && M <= 8, "series length");
+    const double x2 = x * x;
+    double p = c[M];
+#pragma unroll
+    for (int n = M - 1; n >= 1; --n) p = fma(p, x2, c[n]);
+    return x * fma(p, x2, 1.0);
+}
+
+__device__ __forceinline__ float pair_r2(float xi, float yi, float zi, const float4 c)
+{
     const float dx = xi - c.x, dy = yi - c.y, dz = zi - c.z;
-    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+// Seeds of a pair, branch-free: every value stays finite and r == 0 (the i == j
+// diagonal or coincident atoms) is blended in with a 0/1 mask (u_k -> w q_k, linear
+// in k), so the seed computation of the next column pair can be scheduled into the
+// recurrence block.  w is the pair weight (diagonal weights, 0 for padding).
+template <int M>
+__device__ __forceinline__ Seeds make_seeds(float r2, const ChunkConst& cc, double half_dq, float dq, float w)
+{
     const float nzf = r2 > 1e-30f ? 1.0f : 0.0f;
     const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f)) * nzf;
-    const float r = r2 * rinv;
-    const double rd = (double)r;
-    const double sh = sin_poly(rd * half_dq);  // sin(dq r / 2)
-    const float sa = sin_rev(cc.a_rev * rd);   // sin(a r)
-    const float cm = cos_rev(cc.mid_rev * rd); // cos((a + dq/2) r)
-    const float wr = WEIGHTED ? w * rinv : rinv;
-    const float wz = (WEIGHTED ? w : 1.0f) * (1.0f - nzf);
-    // r == 0 (the i == j diagonal or coincident atoms): u_k -> q_k, linear in k.
-    const float f0 = fmaf(sa, wr, wz * cc.a);
-    const double fd = fma(2.0 * (double)(cm * wr), sh, (double)(wz * dq));
+    const double rd = (double)(r2 * rinv);
+    const double sh = sin_series<M>(rd * half_dq);  // sin(dq r / 2)
+    const float sa = sin_rev(cc.a_rev * rd);        // sin(a r)
+    const float cm = cos_rev(cc.mid_rev * rd);      // cos((a + dq/2) r)
+    const float wr = w * rinv;
+    const float wz = w * (1.0f - nzf);
     Seeds s;
     s.c2 = fma(-4.0 * sh, sh, 2.0);
-    s.u0 = (double)f0;
-    s.u1 = s.u0 + fd;
+    s.u0 = (double)fmaf(sa, wr, wz * cc.a);
+    s.u1 = fma((double)((cm + cm) * wr), sh, s.u0 + (double)(wz * dq));
     return s;
 }
 
-template <int K>
-__device__ __forceinline__ void run1(double (&acc)[K], const Seeds& s)
+// the same from shared geometry (r, 1/r with 1/r = 0 for r == 0, sin(dq r / 2))
+__device__ __forceinline__ Seeds chunk_seeds(float r, float rinv, double sh, const ChunkConst& cc, float dq, float w)
 {
-    acc[0] += s.u0;
-    acc[1] += s.u1;
-    double up = s.u0, uc = s.u1;
-#pragma unroll
-    for (int k = 2; k < K; ++k) {
-        const double un = fma(s.c2, uc, -up);
-        acc[k] += un;
-        up = uc;
-        uc = un;
-    }
+    const double rd = (double)r;
+    const float sa = sin_rev(cc.a_rev * rd);
+    const float cm = cos_rev(cc.mid_rev * rd);
+    const float nzf = rinv > 0.0f ? 1.0f : 0.0f;
+    const float wr = w * rinv;
+    const float wz = w * (1.0f - nzf);
+    Seeds s;
+    s.c2 = fma(-4.0 * sh, sh, 2.0);
+    s.u0 = (double)fmaf(sa, wr, wz * cc.a);
+    s.u1 = fma((double)((cm + cm) * wr), sh, s.u0 + (double)(wz * dq));
+    return s;
 }
 
+// two interleaved chains (two columns) added into the running sums
 template <int K>
 __device__ __forceinline__ void run2(double (&acc)[K], const Seeds& a, const Seeds& b)
 {
@@ -166,7 +192,13 @@ __device__ __forceinline__ void run2(double (&acc)[K], const Seeds& a, const See
     }
 }
 
-template <int K>
+// diagonal-tile weights: 1 above the diagonal (j > i), 1/2 on it, 0 below
+__device__ __forceinline__ float diag_weight(int j, int lane)
+{
+    return j > lane ? 1.0f : (j == lane ? 0.5f : 0.0f);
+}
+
+template <int K, int M>
 __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
 {
     const int job_id = blockIdx.x / a.n_chunks;
@@ -194,7 +226,9 @@ __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
     }
 
     auto flush = [&](uint32_t cls) {
-        const float* __restrict__ fr = ftab + (size_t)cls * ldf + k0;
+        // the class is wave-uniform: a scalar offset lets the row come in through
+        // scalar loads instead of per-lane vector loads
+        const float* __restrict__ fr = ftab + __builtin_amdgcn_readfirstlane((int)cls * ldf + k0);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             X[k] = fmaf(fr[k], (float)acc[k], X[k]);
@@ -205,9 +239,9 @@ __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
     const float4* __restrict__ cols = jb.cols;
     const int ncol = jb.ncol;
     const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
-    // one weighted pair step per iteration: the second column is the next one
-    // when it has the same class, otherwise a zero-weight copy.  Outside the
-    // diagonal tile the weights are wave-uniform (0 for skipped records).
+    // one weighted pair step per iteration: the second column is the next one when
+    // it has the same class, otherwise a zero-weight copy.  Outside the diagonal tile
+    // the weights are wave-uniform (0 for skipped records).
     uint32_t cur = CLS_SKIP;
     int j = 0;
     while (j < ncol) {
@@ -222,12 +256,11 @@ __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
         const float live = cl0 != CLS_SKIP ? 1.0f : 0.0f;
         float w0 = live, w1 = pair ? live : 0.0f;
         if (j < jd) {
-            // diagonal tile: weights 1 (j > i), 1/2 (j == i), 0 (j < i)
-            w0 *= j > lane ? 1.0f : (j == lane ? 0.5f : 0.0f);
-            w1 *= j + 1 > lane ? 1.0f : (j + 1 == lane ? 0.5f : 0.0f);
+            w0 *= diag_weight(j, lane);
+            w1 *= diag_weight(j + 1, lane);
         }
-        run2<K>(acc, make_seeds<true>(ri.x, ri.y, ri.z, c0, cc, h_rev, dq, w0),
-                make_seeds<true>(ri.x, ri.y, ri.z, c1, cc, h_rev, dq, w1));
+        run2<K>(acc, make_seeds<M>(pair_r2(ri.x, ri.y, ri.z, c0), cc, h_rev, dq, w0),
+                make_seeds<M>(pair_r2(ri.x, ri.y, ri.z, c1), cc, h_rev, dq, w1));
         j += pair ? 2 : 1;
     }
     if (cur != CLS_SKIP) flush(cur);
@@ -253,51 +286,24 @@ __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
 // ---------------------------------------------------------------------------
 // uniform-grid path, workgroup variant: the waves of one workgroup are the
 // q chunks of one job (up to 4).  The pair geometry that does not depend on q
-// (r, 1/r and the FP64 sin(dq r / 2)) is computed once per (row, column) by one
-// of the waves and shared through LDS in tiles of SH_TC columns, so each wave
-// only forms its chunk's seeds and runs the recurrence.
+// (r, 1/r and the FP64 sin(dq r / 2)) is computed once per (row, column) for
+// tiles of SH_TC columns, each wave doing every nw-th column, and shared through
+// LDS.  Tiles are double-buffered: the geometry of tile t+1 is written while
+// tile t is read, so one barrier per tile separates the phases.
 // ---------------------------------------------------------------------------
 constexpr int SH_TC = 32;   // columns per shared tile
 constexpr int SH_MAXW = 4;  // waves (q chunks) per workgroup
 
-__device__ __forceinline__ void pair_geometry(float xi, float yi, float zi, const float4 c, double half_dq,
-                                              double& rd, double& sh, float& rinv)
-{
-    const float dx = xi - c.x, dy = yi - c.y, dz = zi - c.z;
-    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    const float nzf = r2 > 1e-30f ? 1.0f : 0.0f;
-    rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f)) * nzf;
-    rd = (double)(r2 * rinv);
-    sh = sin_poly(rd * half_dq);
-}
-
-template <bool WEIGHTED>
-__device__ __forceinline__ Seeds chunk_seeds(double rd, double sh, float rinv, const ChunkConst& cc, float dq,
-                                             float w)
-{
-    const float sa = sin_rev(cc.a_rev * rd);
-    const float cm = cos_rev(cc.mid_rev * rd);
-    const float nzf = rinv > 0.0f ? 1.0f : 0.0f;
-    const float wr = WEIGHTED ? w * rinv : rinv;
-    const float wz = (WEIGHTED ? w : 1.0f) * (1.0f - nzf);
-    const float f0 = fmaf(sa, wr, wz * cc.a);
-    const double fd = fma(2.0 * (double)(cm * wr), sh, (double)(wz * dq));
-    Seeds s;
-    s.c2 = fma(-4.0 * sh, sh, 2.0);
-    s.u0 = (double)f0;
-    s.u1 = s.u0 + fd;
-    return s;
-}
-
-template <int K>
+template <int K, int M>
 __global__ __launch_bounds__(64 * SH_MAXW, 2) void debye_shared_kernel(const DebyeArgs a)
 {
+    struct Tile {
+        float r[SH_TC][64];
+        float ri[SH_TC][64];  // 1/r, 0 for r == 0
+        double sh[SH_TC][64];
+    };
     union Shared {
-        struct {
-            double rd[SH_TC][64];
-            double sh[SH_TC][64];
-            float ri[SH_TC][64];
-        } p;
+        Tile t[2];
         float T[SH_MAXW][K][65];
     };
     __shared__ Shared sm;
@@ -329,7 +335,9 @@ __global__ __launch_bounds__(64 * SH_MAXW, 2) void debye_shared_kernel(const Deb
         X[k] = 0.0f;
     }
     auto flush = [&](uint32_t cls) {
-        const float* __restrict__ fr = ftab + (size_t)cls * ldf + k0;
+        // the class is wave-uniform: a scalar offset lets the row come in through
+        // scalar loads instead of per-lane vector loads
+        const float* __restrict__ fr = ftab + __builtin_amdgcn_readfirstlane((int)cls * ldf + k0);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             X[k] = fmaf(fr[k], (float)acc[k], X[k]);
@@ -340,23 +348,35 @@ __global__ __launch_bounds__(64 * SH_MAXW, 2) void debye_shared_kernel(const Deb
     const float4* __restrict__ cols = jb.cols;
     const int ncol = jb.ncol;
     const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
-    uint32_t cur = CLS_SKIP;
-    for (int t0 = 0; t0 < ncol; t0 += SH_TC) {
+    const int n_tiles = (ncol + SH_TC - 1) / SH_TC;
+
+    // geometry of one tile into buffer b
+    auto geometry = [&](int tile, Tile& b) {
+        const int t0 = tile * SH_TC;
         const int tn = ncol - t0 < SH_TC ? ncol - t0 : SH_TC;
         for (int t = wave; t < tn; t += nw) {
-            double rd, sh;
-            float rinv;
-            pair_geometry(ri.x, ri.y, ri.z, cols[t0 + t], half_dq, rd, sh, rinv);
-            sm.p.rd[t][lane] = rd;
-            sm.p.sh[t][lane] = sh;
-            sm.p.ri[t][lane] = rinv;
+            const float r2 = pair_r2(ri.x, ri.y, ri.z, cols[t0 + t]);
+            const float nzf = r2 > 1e-30f ? 1.0f : 0.0f;
+            const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f)) * nzf;
+            const float r = r2 * rinv;
+            b.r[t][lane] = r;
+            b.ri[t][lane] = rinv;
+            b.sh[t][lane] = sin_series<M>((double)r * half_dq);
         }
-        __syncthreads();
+    };
+
+    if (n_tiles > 0) geometry(0, sm.t[0]);
+    uint32_t cur = CLS_SKIP;
+    for (int tile = 0; tile < n_tiles; ++tile) {
+        __syncthreads();  // tile's geometry is complete; every wave is done with the other buffer
+        if (tile + 1 < n_tiles) geometry(tile + 1, sm.t[(tile + 1) & 1]);
+        const Tile& b = sm.t[tile & 1];
         if (active) {
-            // one weighted pair step per iteration: the second column is the next
-            // one when it is in this tile and of the same class, otherwise a
-            // zero-weight copy; skipped records and the lower half of the
-            // diagonal tile also carry weight 0
+            const int t0 = tile * SH_TC;
+            const int tn = ncol - t0 < SH_TC ? ncol - t0 : SH_TC;
+            // one weighted pair step per iteration (the second column is the next one
+            // when it is in this tile and of the same class, otherwise a zero-weight
+            // copy); skipped records and the lower half of the diagonal tile weigh 0
             int t = 0;
             while (t < tn) {
                 const int j = t0 + t;
@@ -370,17 +390,17 @@ __global__ __launch_bounds__(64 * SH_MAXW, 2) void debye_shared_kernel(const Deb
                 const float live = cl0 != CLS_SKIP ? 1.0f : 0.0f;
                 float w0 = live, w1 = pair ? live : 0.0f;
                 if (j < jd) {
-                    w0 *= j > lane ? 1.0f : (j == lane ? 0.5f : 0.0f);
-                    w1 *= j + 1 > lane ? 1.0f : (j + 1 == lane ? 0.5f : 0.0f);
+                    w0 *= diag_weight(j, lane);
+                    w1 *= diag_weight(j + 1, lane);
                 }
-                run2<K>(acc, chunk_seeds<true>(sm.p.rd[t][lane], sm.p.sh[t][lane], sm.p.ri[t][lane], cc, dq, w0),
-                        chunk_seeds<true>(sm.p.rd[t1][lane], sm.p.sh[t1][lane], sm.p.ri[t1][lane], cc, dq, w1));
+                run2<K>(acc, chunk_seeds(b.r[t][lane], b.ri[t][lane], b.sh[t][lane], cc, dq, w0),
+                        chunk_seeds(b.r[t1][lane], b.ri[t1][lane], b.sh[t1][lane], cc, dq, w1));
                 t += pair ? 2 : 1;
             }
         }
-        __syncthreads();
     }
     if (active && cur != CLS_SKIP) flush(cur);
+    __syncthreads();  // the tile buffers are reused for the epilogue
 
     // epilogue as in debye_uniform_kernel, through this wave's slice of the union
     const uint32_t ci = cls_i == CLS_SKIP ? (uint32_t)a.zero_class : cls_i;
@@ -393,6 +413,162 @@ __global__ __launch_bounds__(64 * SH_MAXW, 2) void debye_shared_kernel(const Deb
 #pragma unroll 16
         for (int t = 0; t < 64; ++t) sum += (double)sm.T[wave][lane][t];
         a.partials[(size_t)jb.slot * ldf + k0 + lane] = sum * a.invq[k0 + lane] * (double)jb.scale;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// uniform-grid path, FP32 variant: the difference form of the recurrence
+//     d_{k+1} = d_k - delta u_k,   u_{k+1} = u_k + d_{k+1},   delta = 4 sin^2(dq r / 2)
+// (u_{k+1} = (2 - delta) u_k - u_{k-1}, the same three-term recurrence).  Unlike
+// u_{k+1} = c u_k - u_{k-1} with c = 2 cos(dq r) rounded near 2, the small step
+// delta keeps its full relative precision, so FP32 rounding grows only linearly in k
+// for any dq r (~6e-7 absolute over 64 steps for 1e-4 <= dq r <= 1 in an FP32
+// emulation).  FP32 VALU instructions issue ~1.7x as fast as FP64 ones on gfx950 at
+// 2-3 waves per SIMD (measured), so three FP32 ops per term (FMA, add, add into the
+// sum) cost less than the FP64 form's two FP64 ops.
+// Seeds keep FP64 phases (the phase a r reaches ~80 revolutions, beyond FP32's
+// resolution); sin(dq r / 2) only needs FP32 relative accuracy here (delta and d_1 are
+// FP32), so it comes from an FP32 odd polynomial -- not v_sin, whose absolute error
+// is large relative to small dq r.
+// ---------------------------------------------------------------------------
+struct Seeds32 {
+    float u0, d1, delta;
+};
+
+// sin(x) for 0 <= x <= x_max in FP32: through x^7 when the layout's series length is
+// the short one (x_max <= ~0.2, truncation < 1e-9 relative), otherwise through x^11
+// (x <= 1.5, truncation < 3e-8)
+template <int M>
+__device__ __forceinline__ float sin_small_f32(float x)
+{
+    const float x2 = x * x;
+    float p;
+    if (M <= 5) {
+        p = fmaf(x2, -1.98412698e-04f, 8.33333333e-03f);
+    } else {
+        p = fmaf(x2, -2.50521084e-08f, 2.75573192e-06f);
+        p = fmaf(p, x2, -1.98412698e-04f);
+        p = fmaf(p, x2, 8.33333333e-03f);
+    }
+    p = fmaf(p, x2, -1.66666667e-01f);
+    return fmaf(x * x2, p, x);
+}
+
+template <int M>
+__device__ __forceinline__ Seeds32 make_seeds32(float r2, const ChunkConst& cc, float half_dq, float dq, float w)
+{
+    const float nzf = r2 > 1e-30f ? 1.0f : 0.0f;
+    const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f)) * nzf;
+    const float r = r2 * rinv;
+    const double rd = (double)r;
+    const float sh = sin_small_f32<M>(r * half_dq);  // sin(dq r / 2)
+    const float sa = sin_rev(cc.a_rev * rd);         // sin(a r)
+    const float cm = cos_rev(cc.mid_rev * rd);       // cos((a + dq/2) r)
+    const float wr = w * rinv;
+    const float wz = w * (1.0f - nzf);
+    Seeds32 s;
+    s.delta = 4.0f * sh * sh;
+    s.u0 = fmaf(sa, wr, wz * cc.a);  // r == 0: u_k = w q_k, linear in k
+    s.d1 = fmaf((cm + cm) * wr, sh, wz * dq);
+    return s;
+}
+
+template <int K>
+__device__ __forceinline__ void run2_32(float (&acc)[K], const Seeds32& a, const Seeds32& b)
+{
+    float ua = a.u0, da = a.d1, ub = b.u0, db = b.d1;
+    acc[0] += ua + ub;
+    ua += da;
+    ub += db;
+    acc[1] += ua + ub;
+#pragma unroll
+    for (int k = 2; k < K; ++k) {
+        da = fmaf(-a.delta, ua, da);
+        db = fmaf(-b.delta, ub, db);
+        ua += da;
+        ub += db;
+        acc[k] += ua + ub;
+    }
+}
+
+template <int K, int M>
+__global__ __launch_bounds__(64, 2) void debye_rec32_kernel(const DebyeArgs a)
+{
+    const int job_id = blockIdx.x / a.n_chunks;
+    const int chunk = blockIdx.x - job_id * a.n_chunks;
+    const PairJob jb = a.jobs[job_id];
+    const int lane = threadIdx.x;
+    const bool row_ok = lane < jb.nrow;
+    const float4 ri = row_ok ? jb.rows[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t cls_i = row_ok ? __float_as_uint(ri.w) : CLS_SKIP;
+    const int k0 = chunk * K;
+    const ChunkConst cc = a.chunks[chunk];
+    const float h_dq = (float)a.half_dq;
+    const float dq = a.dq;
+    const float* __restrict__ ftab = a.ftab;
+    const int ldf = a.ldf;
+
+    // acc: FP32 sums of the current class run; X: per-lane sums over flushed runs
+    float acc[K], X[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        acc[k] = 0.0f;
+        X[k] = 0.0f;
+    }
+    auto flush = [&](uint32_t cls) {
+        const float* __restrict__ fr = ftab + __builtin_amdgcn_readfirstlane((int)cls * ldf + k0);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            X[k] = fmaf(fr[k], acc[k], X[k]);
+            acc[k] = 0.0f;
+        }
+    };
+
+    const float4* __restrict__ cols = jb.cols;
+    const int ncol = jb.ncol;
+    const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
+    uint32_t cur = CLS_SKIP;
+    int j = 0;
+    while (j < ncol) {
+        const float4 c0 = cols[j];
+        const uint32_t cl0 = __float_as_uint(c0.w);
+        if (cl0 != cur) {
+            if (cur != CLS_SKIP) flush(cur);
+            cur = cl0;
+        }
+        const bool pair = j + 1 < ncol && __float_as_uint(cols[j + 1].w) == cl0;
+        const float4 c1 = pair ? cols[j + 1] : c0;
+        const float live = cl0 != CLS_SKIP ? 1.0f : 0.0f;
+        float w0 = live, w1 = pair ? live : 0.0f;
+        if (j < jd) {
+            w0 *= diag_weight(j, lane);
+            w1 *= diag_weight(j + 1, lane);
+        }
+        run2_32<K>(acc, make_seeds32<M>(pair_r2(ri.x, ri.y, ri.z, c0), cc, h_dq, dq, w0),
+                   make_seeds32<M>(pair_r2(ri.x, ri.y, ri.z, c1), cc, h_dq, dq, w1));
+        j += pair ? 2 : 1;
+    }
+    if (cur != CLS_SKIP) flush(cur);
+
+    // epilogue as in debye_uniform_kernel, in slices of up to 64 q so the LDS
+    // transpose stays 16.6 KB for any K
+    constexpr int KS = K < 64 ? K : 64;
+    __shared__ float T[KS][65];
+    const uint32_t ci = cls_i == CLS_SKIP ? (uint32_t)a.zero_class : cls_i;
+    const float* __restrict__ fi = ftab + (size_t)ci * ldf + k0;
+#pragma unroll
+    for (int kb = 0; kb < K; kb += KS) {
+#pragma unroll
+        for (int k = 0; k < KS; ++k)
+            if (kb + k < K) T[k][lane] = fi[kb + k] * X[kb + k];
+        __syncthreads();
+        if (lane < KS && kb + lane < K) {
+            double sum = 0.0;
+#pragma unroll 16
+            for (int t = 0; t < 64; ++t) sum += (double)T[lane][t];
+            a.partials[(size_t)jb.slot * ldf + k0 + kb + lane] = sum * a.invq[k0 + kb + lane] * (double)jb.scale;
+        }
+        __syncthreads();
     }
 }
 
@@ -441,7 +617,9 @@ __global__ __launch_bounds__(64, 2) void debye_direct_kernel(const DebyeArgs a)
         X[k] = 0.0;
     }
     auto flush = [&](uint32_t cls) {
-        const float* __restrict__ fr = ftab + (size_t)cls * ldf + k0;
+        // the class is wave-uniform: a scalar offset lets the row come in through
+        // scalar loads instead of per-lane vector loads
+        const float* __restrict__ fr = ftab + __builtin_amdgcn_readfirstlane((int)cls * ldf + k0);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             X[k] = fma((double)fr[k], (double)fmaf(zsum, qv[k], acc[k]), X[k]);
@@ -816,19 +994,24 @@ template <int... Ks>
 struct KList {};
 
 constexpr int UNIFORM_KS[] = {8, 16, 24, 32, 40, 48, 56, 64};
+// besides powers of two: K that split common grids into equal chunks with little
+// padding (256 = 3 x 86 - 2, 200 = 2 x 100, 101 <= 2 x 51, 200 <= 3 x 67)
+constexpr int REC32_KS[] = {16, 32, 43, 48, 51, 64, 67, 80, 86, 96, 100};
 constexpr int DIRECT_KS[] = {8, 16, 32, 48, 64};
 constexpr int ADJOINT_KS[] = {8, 16, 32, 48, 64};
 
 }  // namespace
 
-int debye_k_count(bool uniform)
+int debye_k_count(int path)
 {
-    return uniform ? (int)(sizeof(UNIFORM_KS) / sizeof(int)) : (int)(sizeof(DIRECT_KS) / sizeof(int));
+    return path == PATH_DIRECT  ? (int)(sizeof(DIRECT_KS) / sizeof(int))
+           : path == PATH_REC32 ? (int)(sizeof(REC32_KS) / sizeof(int))
+                                : (int)(sizeof(UNIFORM_KS) / sizeof(int));
 }
 
-int debye_k_value(bool uniform, int i)
+int debye_k_value(int path, int i)
 {
-    return uniform ? UNIFORM_KS[i] : DIRECT_KS[i];
+    return path == PATH_DIRECT ? DIRECT_KS[i] : path == PATH_REC32 ? REC32_KS[i] : UNIFORM_KS[i];
 }
 
 int adjoint_k_count()
@@ -841,40 +1024,79 @@ int adjoint_k_value(int i)
     return ADJOINT_KS[i];
 }
 
+namespace {
+
+// series length instantiated for the recurrence kernels: the short one covers
+// layouts with dq R up to ~0.2 (most proteins at q <= 0.6), the long one the
+// rest of the recurrence range
+constexpr int SIN_SHORT = 5;
+
+template <int K, int M>
+void launch_recurrence(const DebyeArgs& a, int n_jobs, hipStream_t s)
+{
+    if (a.variant == VARIANT_SHARED && a.n_chunks > 1) {
+        const int nw = a.n_chunks < SH_MAXW ? a.n_chunks : SH_MAXW;
+        const int ng = (a.n_chunks + nw - 1) / nw;
+        hipLaunchKernelGGL((debye_shared_kernel<K, M>), dim3((unsigned)(n_jobs * ng)), dim3((unsigned)(64 * nw)), 0, s,
+                           a);
+    } else {
+        hipLaunchKernelGGL((debye_uniform_kernel<K, M>), dim3((unsigned)(n_jobs * a.n_chunks)), dim3(64), 0, s, a);
+    }
+}
+
+template <int K>
+void launch_recurrence_k(const DebyeArgs& a, int n_jobs, hipStream_t s)
+{
+    if (a.sin_terms <= SIN_SHORT)
+        launch_recurrence<K, SIN_SHORT>(a, n_jobs, s);
+    else
+        launch_recurrence<K, SIN_MAX_TERMS>(a, n_jobs, s);
+}
+
+template <int K>
+void launch_rec32_k(const DebyeArgs& a, int n_jobs, hipStream_t s)
+{
+    const dim3 grid((unsigned)(n_jobs * a.n_chunks)), block(64);
+    if (a.sin_terms <= SIN_SHORT)
+        hipLaunchKernelGGL((debye_rec32_kernel<K, SIN_SHORT>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((debye_rec32_kernel<K, SIN_MAX_TERMS>), grid, block, 0, s, a);
+}
+
+}  // namespace
+
 hipError_t launch_debye(const DebyeArgs& a, int n_jobs, int K, bool uniform, hipStream_t s)
 {
     if (n_jobs <= 0) return hipSuccess;
-    const dim3 grid((unsigned)(n_jobs * a.n_chunks)), block(64);
-    if (uniform && a.shared_geometry && a.n_chunks > 1) {
-        const int nw = a.n_chunks < SH_MAXW ? a.n_chunks : SH_MAXW;
-        const int ng = (a.n_chunks + nw - 1) / nw;
-        const dim3 g2((unsigned)(n_jobs * ng)), b2((unsigned)(64 * nw));
+    if (uniform && a.variant == VARIANT_FP32) {
         switch (K) {
-        case 8: hipLaunchKernelGGL(debye_shared_kernel<8>, g2, b2, 0, s, a); break;
-        case 16: hipLaunchKernelGGL(debye_shared_kernel<16>, g2, b2, 0, s, a); break;
-        case 24: hipLaunchKernelGGL(debye_shared_kernel<24>, g2, b2, 0, s, a); break;
-        case 32: hipLaunchKernelGGL(debye_shared_kernel<32>, g2, b2, 0, s, a); break;
-        case 40: hipLaunchKernelGGL(debye_shared_kernel<40>, g2, b2, 0, s, a); break;
-        case 48: hipLaunchKernelGGL(debye_shared_kernel<48>, g2, b2, 0, s, a); break;
-        case 56: hipLaunchKernelGGL(debye_shared_kernel<56>, g2, b2, 0, s, a); break;
-        case 64: hipLaunchKernelGGL(debye_shared_kernel<64>, g2, b2, 0, s, a); break;
+        case 16: launch_rec32_k<16>(a, n_jobs, s); break;
+        case 32: launch_rec32_k<32>(a, n_jobs, s); break;
+        case 48: launch_rec32_k<48>(a, n_jobs, s); break;
+        case 64: launch_rec32_k<64>(a, n_jobs, s); break;
+        case 43: launch_rec32_k<43>(a, n_jobs, s); break;
+        case 51: launch_rec32_k<51>(a, n_jobs, s); break;
+        case 67: launch_rec32_k<67>(a, n_jobs, s); break;
+        case 80: launch_rec32_k<80>(a, n_jobs, s); break;
+        case 86: launch_rec32_k<86>(a, n_jobs, s); break;
+        case 96: launch_rec32_k<96>(a, n_jobs, s); break;
+        case 100: launch_rec32_k<100>(a, n_jobs, s); break;
         default: return hipErrorInvalidValue;
         }
-        return hipGetLastError();
-    }
-    if (uniform) {
+    } else if (uniform) {
         switch (K) {
-        case 8: hipLaunchKernelGGL(debye_uniform_kernel<8>, grid, block, 0, s, a); break;
-        case 16: hipLaunchKernelGGL(debye_uniform_kernel<16>, grid, block, 0, s, a); break;
-        case 24: hipLaunchKernelGGL(debye_uniform_kernel<24>, grid, block, 0, s, a); break;
-        case 32: hipLaunchKernelGGL(debye_uniform_kernel<32>, grid, block, 0, s, a); break;
-        case 40: hipLaunchKernelGGL(debye_uniform_kernel<40>, grid, block, 0, s, a); break;
-        case 48: hipLaunchKernelGGL(debye_uniform_kernel<48>, grid, block, 0, s, a); break;
-        case 56: hipLaunchKernelGGL(debye_uniform_kernel<56>, grid, block, 0, s, a); break;
-        case 64: hipLaunchKernelGGL(debye_uniform_kernel<64>, grid, block, 0, s, a); break;
+        case 8: launch_recurrence_k<8>(a, n_jobs, s); break;
+        case 16: launch_recurrence_k<16>(a, n_jobs, s); break;
+        case 24: launch_recurrence_k<24>(a, n_jobs, s); break;
+        case 32: launch_recurrence_k<32>(a, n_jobs, s); break;
+        case 40: launch_recurrence_k<40>(a, n_jobs, s); break;
+        case 48: launch_recurrence_k<48>(a, n_jobs, s); break;
+        case 56: launch_recurrence_k<56>(a, n_jobs, s); break;
+        case 64: launch_recurrence_k<64>(a, n_jobs, s); break;
         default: return hipErrorInvalidValue;
         }
     } else {
+        const dim3 grid((unsigned)(n_jobs * a.n_chunks)), block(64);
         switch (K) {
         case 8: hipLaunchKernelGGL(debye_direct_kernel<8>, grid, block, 0, s, a); break;
         case 16: hipLaunchKernelGGL(debye_direct_kernel<16>, grid, block, 0, s, a); break;

@@ -98,14 +98,16 @@ class Engine:
     def set_tuning(self, k=0, col_range=0):
         self._check(self._L.rsx_set_tuning(self.h, int(k), int(col_range)))
 
-    def set_variant(self, shared_geometry):
-        self._check(self._L.rsx_set_variant(self.h, int(bool(shared_geometry))))
+    def set_variant(self, variant):
+        """0: FP64 recurrence, one wave per job x q chunk; 1: FP64, shared pair geometry;
+        2: FP32 difference-form recurrence (default)."""
+        self._check(self._L.rsx_set_variant(self.h, int(variant)))
 
     def get_tuning(self):
         k, L, rec, sg = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self._check(self._L.rsx_get_tuning(self.h, ctypes.byref(k), ctypes.byref(L), ctypes.byref(rec),
                                            ctypes.byref(sg)))
-        return {"k": k.value, "col_range": L.value, "recurrence": bool(rec.value), "shared_geometry": bool(sg.value)}
+        return {"k": k.value, "col_range": L.value, "recurrence": bool(rec.value), "variant": sg.value}
 
     def autotune(self, cache_path=None):
         p = cache_path.encode() if cache_path else None

@@ -72,18 +72,24 @@ def test_config1_plain(oracle):
     assert err.max() < TOL
 
 
-@pytest.mark.parametrize("k", [8, 16, 24, 32, 40, 48, 56, 64])
-def test_k_variants_agree(oracle, k):
-    """Every q-per-thread instantiation gives the oracle's I(q)."""
+KS_FP64 = [8, 16, 24, 32, 40, 48, 56, 64]
+KS_FP32 = [16, 32, 48, 64, 80, 96]
+
+
+@pytest.mark.parametrize("variant,k", [(0, k) for k in KS_FP64] + [(1, 32), (1, 64)] + [(2, k) for k in KS_FP32])
+def test_k_variants_agree(oracle, variant, k):
+    """Every kernel variant and q-per-thread instantiation gives the oracle's I(q)."""
     from resaxs_amd import configs
-    _say(f"case: K={k}, 1500 atoms x 120 q")
+    _say(f"case: variant {variant} K={k}, 1500 atoms x 120 q")
     rng = np.random.default_rng(11)
     n = 1500
     pos = configs.in_ball(rng, n, 16.0).astype(np.float32)
     types = configs.seeded_types(rng, n)
     q = np.linspace(0.0, 0.7, 120).astype(np.float32)
     table = configs.factor_table(rng, q)
-    e = engine_for(q, table, n=n, k=k, col_range=128)
+    e = engine_for(q, table, n=n)
+    e.set_variant(variant)
+    e.set_tuning(k, 128)
     I = e.calc_curve(configs.bodies_of(pos, types))
     ref = oracle_iq(q, pos, types, table)
     assert rel_err(I, ref).max() < TOL
@@ -130,6 +136,27 @@ def test_coincident_atoms_and_tiny_n(oracle):
     x = q.astype(np.float64) * 3.5
     sinc = np.where(x == 0, 1.0, np.sin(x) / np.where(x == 0, 1, x))
     assert rel_err(I, f1 * f1 + f2 * f2 + 2 * f1 * f2 * sinc).max() < TOL
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_coincident_atoms_off_diagonal(oracle, variant):
+    """Coincident atoms far apart in index (off the diagonal tiles) in both pair-kernel
+    variants: the zero-distance pair contributes F_i F_j at every q."""
+    from resaxs_amd import configs
+    _say(f"case: coincident pairs off the diagonal, 3000 atoms x 256 q, variant {variant}")
+    rng = np.random.default_rng(19)
+    n = 3000
+    pos = configs.in_ball(rng, n, 25.0).astype(np.float32)
+    for a, b in ((10, 2500), (900, 1700), (64, 65), (1234, 2999), (3, 2047)):
+        pos[b] = pos[a]
+    types = configs.seeded_types(rng, n)
+    q = np.linspace(0.0, 0.6, 256).astype(np.float32)
+    table = configs.factor_table(rng, q)
+    e = engine_for(q, table, n=n)
+    e.set_variant(variant)
+    I = e.calc_curve(configs.bodies_of(pos, types))
+    ref = oracle_iq(q, pos, types, table)
+    assert rel_err(I, ref).max() < TOL
 
 
 def test_invariances(oracle):
