@@ -92,6 +92,27 @@ def forward_bench(steps, warmup):
     return w, I_host, ms_per_step, pair_ms, tuning
 
 
+def forward_50k_bench():
+    """The north-star scale: plain forward of 50,000 atoms x 256 q (the incremental
+    workload's structure), pair-kernel time from HIP events on the engine stream."""
+    from resaxs_amd import configs
+    w = configs.config3()
+    n, nq = len(w.bodies), len(w.q)
+    e = make_engine(w, hydrate=False)
+    log(f"bench: {n} x {nq} plain forward (autotuned layout)")
+    e.calc_curve(w.bodies)
+    tuning = e.autotune(TUNING_CACHE)
+    e.calc_curve(w.bodies)
+    _, pair_ms = e.time_forward(5)
+    e.close()
+    flops = configs.forward_flops(n, nq)
+    return {"workload": f"plain-{n}x{nq}", "kernel_ms": round(pair_ms, 3),
+            "terms_per_s": configs.pair_terms(n, nq) / (pair_ms * 1e-3),
+            "achieved_tflops": round(flops / (pair_ms * 1e-3) / 1e12, 2),
+            "frac": round(flops / (pair_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+            "kernel": f"variant {tuning['variant']} K={tuning['k']} L={tuning['col_range']}"}
+
+
 def adjoint_bench(reps=3):
     """Forward + adjoint (chi^2 gradient) on the fit configuration (10,000 x 200)."""
     from resaxs_amd import configs
@@ -209,6 +230,7 @@ def main():
                      "kernel_ms": round(pair_ms, 4), "flop_per_launch": flops},
     }
     if not args.no_adjoint:
+        line["forward_50k"] = forward_50k_bench()
         log("bench: forward + adjoint on the fit configuration")
         line["fwd_adjoint"] = adjoint_bench()
     if not args.no_cpu_baseline:

@@ -368,10 +368,9 @@ void choose_chunks(rsx_engine* h)
         if (h->k_req <= 0 && !uni && K > 48) continue;  // the direct kernel spills at K = 64
         const int C = (h->nq + K - 1) / K;
         // cost per (pair, chunk) in FP64-op units: the FP64 recurrence 2K + ~44 for distance
-        // and seeds; the FP32 one 3K FP32 ops (~0.6 of an FP64 op each) + ~25; K above
-        // 64 runs at 2 instead of 3 waves per SIMD (~10 % slower per op)
+        // and seeds; the FP32 one 3K FP32 ops (~0.6 of an FP64 op each) + ~25
         const double cost = !uni ? C * (6.0 * K + 12.0)
-                            : path == rsx::PATH_REC32 ? C * (1.8 * K + 25.0) * (K > 64 ? 1.1 : 1.0)
+                            : path == rsx::PATH_REC32 ? C * (1.8 * K + 25.0)
                                                       : C * (2.0 * K + 44.0);
         if (cost < best) {
             best = cost;
@@ -649,7 +648,9 @@ int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles)
     // column_atoms_total counts pairs; a job covers 64 rows
     double L = column_atoms_total / 64.0 / std::max(1.0, jobs);
     (void)row_tiles;
-    L = std::max(64.0, std::min(4096.0, L));
+    // measured optimum on gfx950: 256..1024 columns (shorter jobs pay the epilogue and
+    // seeds more often; longer ones leave a tail)
+    L = std::max(256.0, std::min(1024.0, L));
     return ((int)L + 63) / 64 * 64;
 }
 
