@@ -122,6 +122,9 @@ int rsx_chi2_grad(rsx_engine *h, const float *bodies, int n, const float *i_exp,
                   double *out_chi2, double *out_scale, float *out_grad, double *out_iq);
 /* adjoint only: grad_i = sum_k w_k dI_k/dx_i for caller weights w [n_q] (backward of I(q)) */
 int rsx_debye_vjp(rsx_engine *h, const float *bodies, int n, const double *w, float *out_grad);
+/* adjoint kernel: fp32 1 = FP32 difference-form recurrence (default), 0 = FP64 recurrence;
+ * k q values per thread (0 = automatic); col_range columns per job (0 = 256) */
+int rsx_set_adjoint_tuning(rsx_engine *h, int fp32, int k, int col_range);
 
 /* ---- tuning and measurement ---- */
 /* choose variant, q-per-thread and column-range length for the current n_q / size; cache in a text file */
@@ -141,6 +144,9 @@ int rsx_set_variant(rsx_engine *h, int variant);
 int rsx_get_tuning(rsx_engine *h, int *k, int *col_range, int *recurrence, int *variant);
 /* average device time (ms) of the last timed region, measured with HIP events on the engine stream */
 int rsx_time_forward(rsx_engine *h, int reps, float *ms_total, float *ms_pair_kernel);
+/* average device time (ms) of the adjoint pass (kernel + fixed-order reduction) on the weights of
+ * the last rsx_chi2_grad / rsx_debye_vjp call, HIP events on the engine stream */
+int rsx_time_adjoint(rsx_engine *h, int reps, float *ms);
 /* plain device-to-device copy of `bytes` (float4 grid-stride kernel), average ms per launch;
  * the known-size reference for HBM counter calibration */
 int rsx_time_copy(rsx_engine *h, long long bytes, int reps, float *ms);

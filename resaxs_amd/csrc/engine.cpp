@@ -303,8 +303,21 @@ struct rsx_engine {
     DBuf<int> d_slot_begin, d_idx, d_bucket_of, d_bucket_count, d_bucket_start, d_bucket_fill, d_sorted, d_s2c;
     DBuf<uint32_t> d_counts;
     DBuf<float4> d_dots;
-    DBuf<double> d_gpart;
     DBuf<float> d_grad;
+    // adjoint layout (adjoint_layout), rebuilt when layout_stamp changes
+    DBuf<float> a_ftab, a_rowpart, a_colpart, a_invq32;
+    bool adj_fp32 = true;  // adjoint kernel precision (rsx_set_adjoint_tuning)
+    int adj_k_req = 0, adj_range_req = 0;
+    bool a_fp32 = false;   // precision the cached adjoint layout was built for
+    DBuf<ChunkConst> a_chunks;
+    DBuf<double> a_invq;
+    DBuf<PairJob> a_jobs;
+    DBuf<int> a_tri;
+    int a_K = 0, a_nch = 0, a_ldf = 0, a_L = 0, a_njobs = 0, a_stride = 0, a_n = -1;
+    uint64_t a_stamp = ~0ull;
+    const float4* a_cols = nullptr;  // the column buffer the adjoint jobs point into
+    bool a_have_w = false;           // d_wq holds the weights of the last gradient call
+    uint64_t layout_stamp = 0;  // bumped whenever the class table or the slot layout changes
     size_t ftab_rows = 0;
 
     std::string tune_report;
@@ -387,6 +400,7 @@ void choose_chunks(rsx_engine* h)
 // Upload the per-chunk constants of the active path (uniform recurrence or direct).
 void apply_path(rsx_engine* h, bool uniform)
 {
+    h->layout_stamp++;
     const int nq = h->nq;
     h->uniform = uniform;
     choose_chunks(h);
@@ -496,6 +510,7 @@ bool hydration_on(const rsx_engine* h)
 
 void build_class_table(rsx_engine* h)
 {
+    h->layout_stamp++;
     h->hydrate = hydration_on(h);
     h->d1 = h->hydrate ? h->n_dots + 1 : 1;
     h->n_class = h->n_types * h->d1;
@@ -1170,28 +1185,33 @@ void batch_run_dev(rsx_engine* h, double* out_frames, double* out_mean)
 }
 
 // ---- adjoint -----------------------------------------------------------------
-void adjoint_impl(rsx_engine* h, const std::vector<double>& wk, float* out_grad)
+// Layout of the adjoint (its own q chunking; FP64 recurrence, so its own K list):
+// rebuilt when the forward layout or the class table changes, kept otherwise, so a
+// fitting loop allocates nothing per call.
+void adjoint_layout(rsx_engine* h)
 {
     const int n = h->n;
-    set_device(h);
-    // q per thread for the adjoint: per (pair, chunk) 4K ops (two recurrences,
-    // combine, accumulate) plus ~60 for distance and seeds
+    const uint64_t stamp = h->layout_stamp;
+    const bool fp32 = h->adj_fp32;
+    if (h->a_stamp == stamp && h->a_n == n && h->a_cols == h->d_cols.p && h->a_fp32 == fp32) return;
+    // q per thread, in FP64-op units per (pair, chunk): FP64 kernel 4K ops (two
+    // recurrences, combine, accumulate) + ~60 for distance and seeds; FP32 kernel 4K
+    // FP32 ops (~0.6 of an FP64 op each) + ~30
     int K = 0;
     double best = 1e300;
-    for (int i = 0; i < rsx::adjoint_k_count(); ++i) {
-        const int k = rsx::adjoint_k_value(i);
+    for (int i = 0; i < rsx::adjoint_k_count(fp32); ++i) {
+        const int k = rsx::adjoint_k_value(fp32, i);
+        if (h->adj_k_req > 0 && k != h->adj_k_req) continue;
         const int c = (h->nq + k - 1) / k;
-        const double cost = c * (4.0 * k + 60.0);
+        // (the FP32 kernel drops from 3 to 2 waves per SIMD from K = 50 on: ~10 % per op)
+        const double cost = fp32 ? c * (2.4 * k + 30.0) * (k >= 50 ? 1.1 : 1.0) : c * (4.0 * k + 60.0);
         if (cost < best) {
             best = cost;
             K = k;
         }
     }
     const int nch = (h->nq + K - 1) / K;
-    const int ldf = h->ldf;
-    std::vector<double> wq(std::max(ldf, nch * K), 0.0);
-    for (int k = 0; k < h->nq; ++k) wq[k] = h->q[k] == 0.0f ? 0.0 : wk[k];
-    // chunk constants for the adjoint's chunking
+    const int ldfa = nch * K;
     std::vector<ChunkConst> cc(nch);
     for (int c = 0; c < nch; ++c) {
         const int k0 = c * K;
@@ -1201,57 +1221,118 @@ void adjoint_impl(rsx_engine* h, const std::vector<double>& wk, float* out_grad)
         cc[c].mid_rev = (a + 0.5 * h->dq) / (2.0 * PI);
         cc[c].pad = 0.0f;
     }
-    // class table and 1/q with the adjoint's padding
-    const int ldfa = nch * K;
-    std::vector<float> ft((size_t)h->n_class * ldfa, 0.0f);
+    std::vector<float> ft((size_t)std::max(1, h->n_class) * ldfa, 0.0f);
     for (int c = 0; c < h->n_class; ++c)
         for (int k = 0; k < h->nq; ++k) ft[(size_t)c * ldfa + k] = (float)h->f64[(size_t)c * h->nq + k];
     std::vector<double> invq(ldfa, 0.0);
-    for (int k = 0; k < h->nq; ++k) invq[k] = h->q[k] != 0.0f ? 1.0 / (double)h->q[k] : 0.0;
-    DBuf<float> ftab;
-    DBuf<ChunkConst> dcc;
-    DBuf<double> dinv;
-    ftab.ensure(ft.size());
-    dcc.ensure(nch);
-    dinv.ensure(ldfa);
-    h->d_wq.ensure(ldfa);
-    try {
-        hipchk(hipMemcpy(ftab.p, ft.data(), ft.size() * sizeof(float), hipMemcpyHostToDevice), "upload ftab");
-        hipchk(hipMemcpy(dcc.p, cc.data(), nch * sizeof(ChunkConst), hipMemcpyHostToDevice), "upload chunks");
-        hipchk(hipMemcpy(dinv.p, invq.data(), ldfa * sizeof(double), hipMemcpyHostToDevice), "upload invq");
-        hipchk(hipMemcpy(h->d_wq.p, wq.data(), ldfa * sizeof(double), hipMemcpyHostToDevice), "upload w");
-        rsx::AdjointArgs a;
-        a.rows = h->d_cols.p;
-        a.cols = h->d_cols.p;
-        a.n = n;
-        a.col_range = 2048;
-        a.n_colblocks = (n + a.col_range - 1) / a.col_range;
-        a.n_chunks = nch;
-        a.ftab = ftab.p;
-        a.ldf = ldfa;
-        a.chunks = dcc.p;
-        a.half_dq = 0.5 * h->dq;
-        a.dq = (float)h->dq;
-        a.wq = h->d_wq.p;
-        a.invq = dinv.p;
-        const int parts = a.n_colblocks * nch;
-        h->d_gpart.ensure((size_t)parts * n * 4);
-        h->d_grad.ensure((size_t)n * 3);
-        a.grad_partial = h->d_gpart.p;
-        hipchk(rsx::launch_adjoint(a, K, h->stream), "adjoint kernel");
-        hipchk(rsx::launch_grad_reduce(h->d_gpart.p, parts, n, h->d_s2c.p, h->d_grad.p, h->stream), "grad reduce");
-        hipchk(hipMemcpyAsync(out_grad, h->d_grad.p, (size_t)n * 3 * sizeof(float), hipMemcpyDeviceToHost, h->stream),
-               "download grad");
-        hipchk(hipStreamSynchronize(h->stream), "sync adjoint");
-    } catch (...) {
-        ftab.release();
-        dcc.release();
-        dinv.release();
-        throw;
+    std::vector<float> invq32(ldfa, 0.0f);
+    for (int k = 0; k < h->nq; ++k) {
+        invq[k] = h->q[k] != 0.0f ? 1.0 / (double)h->q[k] : 0.0;
+        invq32[k] = (float)invq[k];
     }
-    ftab.release();
-    dcc.release();
-    dinv.release();
+    // triangle jobs over the slot-ordered atoms; remember each job's (row block, range)
+    const int L = h->adj_range_req > 0 ? h->adj_range_req : 256;
+    std::vector<PairJob> jobs;
+    triangle_jobs(jobs, h->d_cols.p, n, L, 1.0f);
+    for (size_t j = 0; j < jobs.size(); ++j) jobs[j].pad = (int)j;
+    const int T = (n + 63) / 64;
+    const int stride = (n + L - 1) / L;
+    std::vector<int> orig_block(jobs.size()), orig_m(jobs.size());
+    {
+        size_t j = 0;
+        for (int I = 0; I < T; ++I)
+            for (int c0 = I * 64, m = 0; c0 < n; c0 += L, ++m, ++j) {
+                orig_block[j] = I;
+                orig_m[j] = m;
+            }
+    }
+    finalize_jobs(jobs);
+    std::vector<int> tri((size_t)T * stride, 0);
+    for (size_t s = 0; s < jobs.size(); ++s) {
+        const int o = jobs[s].pad;
+        tri[(size_t)orig_block[o] * stride + orig_m[o]] = (int)s;
+    }
+    const int nj = (int)jobs.size();
+    h->a_ftab.ensure(ft.size());
+    h->a_chunks.ensure(nch);
+    h->a_invq.ensure(ldfa);
+    h->a_invq32.ensure(ldfa);
+    h->d_wq.ensure(ldfa);
+    h->a_jobs.ensure(std::max(1, nj));
+    h->a_tri.ensure(tri.size());
+    h->a_rowpart.ensure((size_t)std::max(1, nj) * nch * 64 * 3);
+    h->a_colpart.ensure((size_t)std::max(1, nj) * nch * L * 3);
+    h->d_grad.ensure((size_t)n * 3);
+    hipchk(hipMemcpy(h->a_ftab.p, ft.data(), ft.size() * sizeof(float), hipMemcpyHostToDevice), "upload ftab");
+    hipchk(hipMemcpy(h->a_chunks.p, cc.data(), nch * sizeof(ChunkConst), hipMemcpyHostToDevice), "upload chunks");
+    hipchk(hipMemcpy(h->a_invq.p, invq.data(), ldfa * sizeof(double), hipMemcpyHostToDevice), "upload invq");
+    hipchk(hipMemcpy(h->a_invq32.p, invq32.data(), ldfa * sizeof(float), hipMemcpyHostToDevice), "upload invq32");
+    hipchk(hipMemcpy(h->a_jobs.p, jobs.data(), nj * sizeof(PairJob), hipMemcpyHostToDevice), "upload jobs");
+    hipchk(hipMemcpy(h->a_tri.p, tri.data(), tri.size() * sizeof(int), hipMemcpyHostToDevice), "upload tri");
+    h->a_K = K;
+    h->a_nch = nch;
+    h->a_ldf = ldfa;
+    h->a_L = L;
+    h->a_njobs = nj;
+    h->a_stride = stride;
+    h->a_n = n;
+    h->a_stamp = stamp;
+    h->a_cols = h->d_cols.p;
+    h->a_fp32 = fp32;
+}
+
+// adjoint kernel + reduction on the weights already in d_wq; grad [n][3] (caller order) -> d_grad
+void launch_adjoint_pass(rsx_engine* h)
+{
+    rsx::AdjointArgs a;
+    a.jobs = h->a_jobs.p;
+    a.n_chunks = h->a_nch;
+    a.ftab = h->a_ftab.p;
+    a.ldf = h->a_ldf;
+    a.chunks = h->a_chunks.p;
+    a.half_dq = 0.5 * h->dq;
+    a.wq = h->d_wq.p;
+    a.invq = h->a_invq.p;
+    a.invq32 = h->a_invq32.p;
+    a.dq_rev = h->dq / (2.0 * PI);
+    a.fp32 = h->a_fp32 ? 1 : 0;
+    a.sin_terms = rsx::sin_series_terms(h->dq * h->path_radius);
+    a.col_range = h->a_L;
+    a.row_partial = h->a_rowpart.p;
+    a.col_partial = h->a_colpart.p;
+    hipchk(rsx::launch_adjoint(a, h->a_njobs, h->a_K, h->stream), "adjoint kernel");
+    rsx::AdjointReduceArgs r;
+    r.row_partial = h->a_rowpart.p;
+    r.col_partial = h->a_colpart.p;
+    r.tri_slot = h->a_tri.p;
+    r.tri_stride = h->a_stride;
+    r.n = h->n;
+    r.n_chunks = h->a_nch;
+    r.col_range = h->a_L;
+    r.slot_to_caller = h->d_s2c.p;
+    r.grad = h->d_grad.p;
+    hipchk(rsx::launch_adjoint_reduce(r, h->stream), "adjoint reduce");
+}
+
+// queue the adjoint for weights wk (host) and leave grad [n][3] (caller order) in d_grad
+void enqueue_adjoint(rsx_engine* h, const std::vector<double>& wk)
+{
+    adjoint_layout(h);
+    std::vector<double> wq(h->a_ldf, 0.0);
+    for (int k = 0; k < h->nq; ++k) wq[k] = h->q[k] == 0.0f ? 0.0 : wk[k];
+    hipchk(hipMemcpyAsync(h->d_wq.p, wq.data(), h->a_ldf * sizeof(double), hipMemcpyHostToDevice, h->stream),
+           "upload w");
+    launch_adjoint_pass(h);
+    h->a_have_w = true;
+}
+
+void adjoint_impl(rsx_engine* h, const std::vector<double>& wk, float* out_grad)
+{
+    set_device(h);
+    enqueue_adjoint(h, wk);
+    hipchk(hipMemcpyAsync(out_grad, h->d_grad.p, (size_t)h->n * 3 * sizeof(float), hipMemcpyDeviceToHost, h->stream),
+           "download grad");
+    hipchk(hipStreamSynchronize(h->stream), "sync adjoint");
 }
 
 // Device-resident forward step: (SASA) -> class records in slot order -> pair
@@ -1408,6 +1489,7 @@ void use_fp64_forward(rsx_engine* h)
 {
     if (h->variant != rsx::VARIANT_FP32 || h->nq <= 0) return;
     h->variant = rsx::VARIANT_WAVE;
+    h->k_req = 0;  // the FP64 kernels have their own K list
     setup_q(h);
     h->params_dirty = true;
     batch_release_graph(h);
@@ -1458,8 +1540,8 @@ int rsx_destroy(rsx_engine* h)
     DBuf<float4>* f4[] = {&h->d_cols, &h->d_saved_cols, &h->d_rows, &h->d_vals, &h->d_sasa_atoms, &h->d_dots, &h->b_cols};
     for (auto* b : f4) b->release();
     h->d_qrev.release();
-    h->d_gpart.release();
-    DBuf<float>* ff[] = {&h->d_ftab, &h->d_qv, &h->d_r2, &h->d_rext, &h->d_grad};
+    DBuf<float>* ff[] = {&h->d_ftab, &h->d_qv, &h->d_r2, &h->d_rext, &h->d_grad,
+                         &h->a_ftab, &h->a_rowpart, &h->a_colpart, &h->a_invq32};
     for (auto* b : ff) b->release();
     DBuf<double>* fd[] = {&h->d_invq, &h->d_partials, &h->d_iq, &h->d_saved_iq, &h->d_out, &h->d_wq, &h->d_q0,
                           &h->b_partials, &h->b_out, &h->b_mean, &h->b_q0};
@@ -1468,6 +1550,10 @@ int rsx_destroy(rsx_engine* h)
                        &h->d_bucket_fill, &h->d_sorted, &h->d_s2c, &h->b_slot_begin};
     for (auto* b : fi) b->release();
     h->d_chunks.release();
+    h->a_chunks.release();
+    h->a_invq.release();
+    h->a_jobs.release();
+    h->a_tri.release();
     h->d_jobs.release();
     h->d_fjobs.release();
     h->d_f0col.release();
@@ -1867,6 +1953,41 @@ int rsx_time_forward(rsx_engine* h, int reps, float* ms_total, float* ms_pair)
         hipchk(hipEventElapsedTime(&tot, h->ev0, h->ev1), "elapsed");
         if (ms_total) *ms_total = tot / reps;
         if (ms_pair) *ms_pair = pair_ms / reps;
+    });
+}
+
+int rsx_set_adjoint_tuning(rsx_engine* h, int fp32, int k, int col_range)
+{
+    return guarded(h, [&] {
+        if (k < 0 || col_range < 0 || (col_range > 0 && col_range % 64 != 0))
+            fail(RSX_INVALID_ARG, "adjoint tuning: k >= 0, col_range a multiple of 64");
+        if (k > 0) {
+            bool ok = false;
+            for (int i = 0; i < rsx::adjoint_k_count(fp32 != 0); ++i) ok = ok || rsx::adjoint_k_value(fp32 != 0, i) == k;
+            if (!ok) fail(RSX_INVALID_ARG, "unsupported adjoint K for this precision");
+        }
+        h->adj_fp32 = fp32 != 0;
+        h->adj_k_req = k;
+        h->adj_range_req = col_range;
+        h->a_stamp = ~0ull;  // lay out again on the next gradient call
+    });
+}
+
+int rsx_time_adjoint(rsx_engine* h, int reps, float* ms)
+{
+    return guarded(h, [&] {
+        if (!h->a_have_w || h->a_n != h->n || h->state != 4)
+            fail(RSX_NO_INITIAL_CALC, "time_adjoint needs a prior chi2_grad or vjp call");
+        if (reps <= 0 || !ms) fail(RSX_INVALID_ARG, "reps must be > 0");
+        set_device(h);
+        launch_adjoint_pass(h);  // warm-up
+        hipchk(hipEventRecord(h->ev0, h->stream), "event");
+        for (int r = 0; r < reps; ++r) launch_adjoint_pass(h);
+        hipchk(hipEventRecord(h->ev1, h->stream), "event");
+        hipchk(hipEventSynchronize(h->ev1), "event sync");
+        float t = 0.0f;
+        hipchk(hipEventElapsedTime(&t, h->ev0, h->ev1), "elapsed");
+        *ms = t / reps;
     });
 }
 

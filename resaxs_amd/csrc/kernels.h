@@ -129,28 +129,42 @@ struct SasaArgs {
 hipError_t launch_sasa(const SasaArgs& a, hipStream_t s);
 
 // ---- adjoint ----
+// Adjoint over the forward's triangle pair jobs (slot-ordered atoms).
 struct AdjointArgs {
-    const float4* rows;      // all atoms (slot order)
-    const float4* cols;
-    int n;                   // atoms
-    int col_range;           // columns per job
+    const PairJob* jobs;     // triangle jobs, slot = execution index
     int n_chunks;
-    const float* ftab;       // [n_class][ldf]
-    int ldf;
+    const float* ftab;       // [n_class][ldf] FP32 class factors
+    int ldf;                 // n_chunks * K
     const ChunkConst* chunks;
     double half_dq;
-    float dq;
-    const double* wq;        // [ldf] w_k / 1 (0 padding)  -- weights dchi2/dI_k
+    const double* wq;        // [ldf] w_k = dchi2/dI_k (0 for q == 0 and padding)
     const double* invq;      // [ldf]
-    double* grad_partial;    // [n_colblocks * n_chunks][n][4] FP64 partial gradients
-    int n_colblocks;
+    const float* invq32;     // [ldf] the same in FP32 (FP32 kernel)
+    double dq_rev;           // dq / (2 pi)
+    int fp32;                // 1: FP32 difference-form kernel, 0: FP64 recurrence
+    int sin_terms;           // series length (sin_series_terms)
+    int col_range;           // columns per job (L)
+    float* row_partial;      // [n_jobs * n_chunks][64][3] FP32 row-side partial gradients
+    float* col_partial;      // [n_jobs * n_chunks][col_range][3] FP32 column-side partials
 };
-hipError_t launch_adjoint(const AdjointArgs& a, int K, hipStream_t s);
-int adjoint_k_count();
-int adjoint_k_value(int i);
-// grad[i] = sum over partial blocks (fixed order), written in caller order via slot map
-hipError_t launch_grad_reduce(const double* grad_partial, int n_parts, int n, const int* slot_to_caller,
-                              float* grad_out, hipStream_t s);
+// one wave per (job, q chunk); n_jobs = number of jobs
+hipError_t launch_adjoint(const AdjointArgs& a, int n_jobs, int K, hipStream_t s);
+int adjoint_k_count(bool fp32);
+int adjoint_k_value(bool fp32, int i);
+
+struct AdjointReduceArgs {
+    const float* row_partial;
+    const float* col_partial;
+    const int* tri_slot;     // [n_row_blocks][tri_stride]: slot of row block I's m-th column range
+    int tri_stride;
+    int n;                   // atoms
+    int n_chunks;
+    int col_range;
+    const int* slot_to_caller;
+    float* grad;             // [n][3] in caller order
+};
+// grad of every atom: its row and column partials over all chunks, fixed order
+hipError_t launch_adjoint_reduce(const AdjointReduceArgs& a, hipStream_t s);
 
 // out[0] = (sum_s f0col[class(cols[s])])^2, fixed-order FP64 sum (the q = 0 value)
 hipError_t launch_q0(const float4* cols, int n, const double* f0col, double* out, hipStream_t s);

@@ -269,17 +269,24 @@ __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
     // lanes), transposed through LDS so lane k sums column k over the 64 rows in
     // FP64; then 1/q_k and the job scale.  Rows of T are padded to 65 words so
     // both the row writes and the column reads are free of bank conflicts.
-    __shared__ float T[K][65];
+    // K > 64 goes through the transpose in slices of 64 q
+    constexpr int KS = K < 64 ? K : 64;
+    __shared__ float T[KS][65];
     const uint32_t ci = cls_i == CLS_SKIP ? (uint32_t)a.zero_class : cls_i;
     const float* __restrict__ fi = ftab + (size_t)ci * ldf + k0;
 #pragma unroll
-    for (int k = 0; k < K; ++k) T[k][lane] = fi[k] * X[k];
-    __syncthreads();
-    if (lane < K) {
-        double sum = 0.0;
+    for (int kb = 0; kb < K; kb += KS) {
+#pragma unroll
+        for (int k = 0; k < KS; ++k)
+            if (kb + k < K) T[k][lane] = fi[kb + k] * X[kb + k];
+        __syncthreads();
+        if (lane < KS && kb + lane < K) {
+            double sum = 0.0;
 #pragma unroll 16
-        for (int t = 0; t < 64; ++t) sum += (double)T[lane][t];
-        a.partials[(size_t)jb.slot * ldf + k0 + lane] = sum * a.invq[k0 + lane] * (double)jb.scale;
+            for (int t = 0; t < 64; ++t) sum += (double)T[lane][t];
+            a.partials[(size_t)jb.slot * ldf + k0 + kb + lane] = sum * a.invq[k0 + kb + lane] * (double)jb.scale;
+        }
+        __syncthreads();
     }
 }
 
@@ -887,118 +894,276 @@ __global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
 // ---------------------------------------------------------------------------
 // adjoint: grad_i = sum_k w_k dI_k/dx_i
 //   dI_k/dx_i = sum_{j != i} 2 F_i F_j [cos(q r) - sinc(q r)] (x_i - x_j) / r^2
-// Full row sweep (each lane owns row i and visits every column).  For a run of
-// one column class the per-lane weights b_k = 2 w_k F_i,k F_class,k are formed
-// once; per (pair, q) the cos and sin/r sequences advance by the same FP64
-// recurrence and G += b_k (c_k - u_k / q_k).
+// Over the triangle i < j (the pair jobs of the forward): per pair the scalar
+//   G_ij = sum_k b_k [cos(q_k r) - sin(q_k r) / (q_k r)],   b_k = 2 w_k F_i,k F_j,k
+// is formed once and both ends are updated: the row (lane) accumulates
+// +G (x_i - x_j) / r^2, the column the negation, summed over the 64 lanes through
+// LDS in tiles of ADJ_TC columns.  cos(q_k r) and sin(q_k r) / r advance by the
+// same FP64 recurrence; b_k is formed once per column class run.  Each wave
+// (job, q chunk) writes FP32 partial gradients for its 64 rows and its columns;
+// adjoint_reduce_kernel sums them per atom in a fixed order.
 // ---------------------------------------------------------------------------
-template <int K>
-__global__ __launch_bounds__(64, 2) void adjoint_kernel(const AdjointArgs a)
+constexpr int ADJ_TC = 8;  // columns per LDS reduction tile
+
+template <int K, int M>
+__global__ __launch_bounds__(64, 2) void adjoint_tri_kernel(const AdjointArgs a)
 {
-    const int n_rt = (a.n + 63) / 64;
-    int bid = blockIdx.x;
-    const int chunk = bid % a.n_chunks;
-    bid /= a.n_chunks;
-    const int cb = bid % a.n_colblocks;
-    const int rt = bid / a.n_colblocks;
-    if (rt >= n_rt) return;
+    const int job_id = blockIdx.x / a.n_chunks;
+    const int chunk = blockIdx.x - job_id * a.n_chunks;
+    const PairJob jb = a.jobs[job_id];
     const int lane = threadIdx.x;
-    const int i = rt * 64 + lane;
-    const bool row_ok = i < a.n;
-    const float4 ri = row_ok ? a.rows[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool row_ok = lane < jb.nrow;
+    const float4 ri = row_ok ? jb.rows[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t cls_i = row_ok ? __float_as_uint(ri.w) : CLS_SKIP;
     const uint32_t ci = cls_i == CLS_SKIP ? 0u : cls_i;
+    const float row_w = row_ok ? 1.0f : 0.0f;
     const int k0 = chunk * K;
     const ChunkConst cc = a.chunks[chunk];
     const float* __restrict__ ftab = a.ftab;
-    const float* __restrict__ fi = ftab + (size_t)ci * a.ldf + k0;
+    const int ldf = a.ldf;
+    const float* __restrict__ fi = ftab + (size_t)ci * ldf + k0;
     const double* __restrict__ wq = a.wq + k0;
     const double* __restrict__ invq = a.invq + k0;
+    const double half_dq = a.half_dq;
 
+    __shared__ float T[ADJ_TC][3][65];  // rows padded: the column sums read across banks
     double b[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) b[k] = 0.0;
     double gx = 0.0, gy = 0.0, gz = 0.0;
-    const int j0 = cb * a.col_range;
-    const int j1 = min(a.n, j0 + a.col_range);
+    const float4* __restrict__ cols = jb.cols;
+    const int ncol = jb.ncol;
+    const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
+    const size_t part = (size_t)jb.slot * a.n_chunks + chunk;
+    float* __restrict__ colpart = a.col_partial + part * (size_t)a.col_range * 3;
     uint32_t cur = CLS_SKIP;
-    for (int j = j0; j < j1; ++j) {
-        const float4 c = a.cols[j];
-        const uint32_t cl = __float_as_uint(c.w);
-        if (cl == CLS_SKIP) continue;
-        if (cl != cur) {
-            cur = cl;
-            const float* __restrict__ fr = ftab + (size_t)cl * a.ldf + k0;
+    for (int t0 = 0; t0 < ncol; t0 += ADJ_TC) {
+        const int tn = ncol - t0 < ADJ_TC ? ncol - t0 : ADJ_TC;
+        for (int t = 0; t < tn; ++t) {
+            const int j = t0 + t;
+            const float4 c = cols[j];
+            const uint32_t cl = __float_as_uint(c.w);
+            if (cl != cur) {
+                cur = cl;
+                const float* __restrict__ fr = ftab + __builtin_amdgcn_readfirstlane((int)cl * ldf + k0);
 #pragma unroll
-            for (int k = 0; k < K; ++k)
-                b[k] = cls_i == CLS_SKIP ? 0.0 : 2.0 * wq[k] * (double)fi[k] * (double)fr[k];
-        }
-        const float dx = ri.x - c.x, dy = ri.y - c.y, dz = ri.z - c.z;
-        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        const bool nz = r2 > 1e-30f;
-        const float rinv = nz ? __builtin_amdgcn_rsqf(r2) : 0.0f;
-        const float r = r2 * rinv;
-        const double rd = (double)r;
-        const double sh = sin_poly(rd * a.half_dq);
-        const float sa = sin_rev(cc.a_rev * rd), ca = cos_rev(cc.a_rev * rd);
-        const float sm = sin_rev(cc.mid_rev * rd), cm = cos_rev(cc.mid_rev * rd);
-        const double c2 = fma(-4.0 * sh, sh, 2.0);
-        // u_k = sin(q_k r) / r and v_k = cos(q_k r), both obey x_{k+1} = c2 x_k - x_{k-1}
-        double up = (double)(sa * rinv), uc = up + 2.0 * (double)(cm * rinv) * sh;
-        double vp = (double)ca, vc = vp - 2.0 * (double)sm * sh;
-        double G = 0.0;
-        G = fma(b[0], fma(-up, invq[0], vp), G);
-        G = fma(b[1], fma(-uc, invq[1], vc), G);
+                for (int k = 0; k < K; ++k) b[k] = 2.0 * wq[k] * (double)fi[k] * (double)fr[k];
+            }
+            // weight: padding rows and the lower half and diagonal of the diagonal tile give 0
+            const float w = j < jd ? (j > lane ? row_w : 0.0f) : row_w;
+            const float dx = ri.x - c.x, dy = ri.y - c.y, dz = ri.z - c.z;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const float nzf = r2 > 1e-30f ? 1.0f : 0.0f;
+            const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f)) * nzf;
+            const double rd = (double)(r2 * rinv);
+            const double sh = sin_series<M>(rd * half_dq);
+            const float sa = sin_rev(cc.a_rev * rd), ca = cos_rev(cc.a_rev * rd);
+            const float sm = sin_rev(cc.mid_rev * rd), cm = cos_rev(cc.mid_rev * rd);
+            const double c2 = fma(-4.0 * sh, sh, 2.0);
+            // u_k = sin(q_k r) / r and v_k = cos(q_k r), both obey x_{k+1} = c2 x_k - x_{k-1}
+            double up = (double)(sa * rinv), uc = fma(2.0 * (double)(cm * rinv), sh, up);
+            double vp = (double)ca, vc = fma(-2.0 * (double)sm, sh, vp);
+            double G = fma(b[0], fma(-up, invq[0], vp), 0.0);
+            G = fma(b[1], fma(-uc, invq[1], vc), G);
 #pragma unroll
-        for (int k = 2; k < K; ++k) {
-            const double un = fma(c2, uc, -up);
-            const double vn = fma(c2, vc, -vp);
-            G = fma(b[k], fma(-un, invq[k], vn), G);
-            up = uc;
-            uc = un;
-            vp = vc;
-            vc = vn;
+            for (int k = 2; k < K; ++k) {
+                const double un = fma(c2, uc, -up);
+                const double vn = fma(c2, vc, -vp);
+                G = fma(b[k], fma(-un, invq[k], vn), G);
+                up = uc;
+                uc = un;
+                vp = vc;
+                vc = vn;
+            }
+            const double gs = G * (double)(rinv * rinv * w);
+            const double ex = gs * (double)dx, ey = gs * (double)dy, ez = gs * (double)dz;
+            gx += ex;
+            gy += ey;
+            gz += ez;
+            T[t][0][lane] = (float)-ex;
+            T[t][1][lane] = (float)-ey;
+            T[t][2][lane] = (float)-ez;
         }
-        const double gs = nz ? G * (double)(rinv * rinv) : 0.0;
-        gx = fma(gs, (double)dx, gx);
-        gy = fma(gs, (double)dy, gy);
-        gz = fma(gs, (double)dz, gz);
+        __syncthreads();
+        // lane (t, comp) sums its column component over the 64 rows
+        if (lane < 3 * tn) {
+            const int t = lane / 3, comp = lane - 3 * (lane / 3);
+            double s = 0.0;
+#pragma unroll 16
+            for (int l = 0; l < 64; ++l) s += (double)T[t][comp][l];
+            colpart[(size_t)(t0 + t) * 3 + comp] = (float)s;
+        }
+        __syncthreads();
     }
-    if (row_ok) {
-        const size_t part = (size_t)cb * a.n_chunks + chunk;
-        double* dst = a.grad_partial + (part * a.n + i) * 4;
-        dst[0] = gx;
-        dst[1] = gy;
-        dst[2] = gz;
-        dst[3] = 0.0;
-    }
+    float* __restrict__ rowpart = a.row_partial + (part * 64 + lane) * 3;
+    rowpart[0] = (float)gx;
+    rowpart[1] = (float)gy;
+    rowpart[2] = (float)gz;
 }
 
-__global__ void grad_reduce_kernel(const double* __restrict__ gp, int n_parts, int n, const int* __restrict__ s2c,
-                                   float* out)
+// FP32 variant of the adjoint.  The difference-form recurrence of the forward's FP32
+// kernel gives u_k = sin(q_k r) / r and the differences d_k = u_k - u_{k-1}
+//   = 2 cos((q_k - dq/2) r) sin(dq r / 2) / r,
+// so cos(q_k r) = (d_k + d_{k+1}) r / (2 sin(dq r)) comes with the sine chain:
+//   G = sum_k b_k [cos(q_k r) - u_k / q_k] = R sum_{k=0..K} beta_k d_k - sum_k gamma_k u_k
+// with R = r / (2 sin(dq r)), beta_k = b_k + b_{k-1}, gamma_k = b_k / q_k (formed once
+// per column class run).  Four FP32 ops per (pair, q): the d and u updates and two
+// weighted sums.  G and the per-row and per-column gradient sums are FP32 too; the
+// fixed-order reduction over jobs runs in FP64.
+template <int K, int M>
+__global__ __launch_bounds__(64, 2) void adjoint32_tri_kernel(const AdjointArgs a)
+{
+    const int job_id = blockIdx.x / a.n_chunks;
+    const int chunk = blockIdx.x - job_id * a.n_chunks;
+    const PairJob jb = a.jobs[job_id];
+    const int lane = threadIdx.x;
+    const bool row_ok = lane < jb.nrow;
+    const float4 ri = row_ok ? jb.rows[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t cls_i = row_ok ? __float_as_uint(ri.w) : CLS_SKIP;
+    const uint32_t ci = cls_i == CLS_SKIP ? 0u : cls_i;
+    const float row_w = row_ok ? 1.0f : 0.0f;
+    const int k0 = chunk * K;
+    const ChunkConst cc = a.chunks[chunk];
+    const double lo_rev = cc.mid_rev - a.dq_rev;  // (a - dq/2) / (2 pi)
+    const float* __restrict__ ftab = a.ftab;
+    const int ldf = a.ldf;
+    const float h_dq = (float)a.half_dq;
+    const float dq = 2.0f * h_dq;
+
+    __shared__ float T[ADJ_TC][3][65];
+    // beta, gamma per column class run, from b_k = 2 w_k F_i,k F_class,k
+    const float* __restrict__ fi = ftab + (size_t)ci * ldf + k0;
+    float beta[K + 1], gamma[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        beta[k] = 0.0f;
+        gamma[k] = 0.0f;
+    }
+    beta[K] = 0.0f;
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;  // per-row sums of ~N terms: FP32 is ample for the 1e-4 bar
+    const float4* __restrict__ cols = jb.cols;
+    const int ncol = jb.ncol;
+    const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
+    const size_t part = (size_t)jb.slot * a.n_chunks + chunk;
+    float* __restrict__ colpart = a.col_partial + part * (size_t)a.col_range * 3;
+    uint32_t cur = CLS_SKIP;
+    for (int t0 = 0; t0 < ncol; t0 += ADJ_TC) {
+        const int tn = ncol - t0 < ADJ_TC ? ncol - t0 : ADJ_TC;
+        for (int t = 0; t < tn; ++t) {
+            const int j = t0 + t;
+            const float4 c = cols[j];
+            const uint32_t cl = __float_as_uint(c.w);
+            if (cl != cur) {
+                cur = cl;
+                const float* __restrict__ fr = ftab + __builtin_amdgcn_readfirstlane((int)cl * ldf + k0);
+                const float* __restrict__ iq = a.invq32 + k0;
+                float prev = 0.0f;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const float bk = (float)(2.0 * a.wq[k0 + k]) * fi[k] * fr[k];
+                    beta[k] = bk + prev;
+                    gamma[k] = bk * iq[k];
+                    prev = bk;
+                }
+                beta[K] = prev;
+            }
+            const float w = j < jd ? (j > lane ? row_w : 0.0f) : row_w;
+            const float dx = ri.x - c.x, dy = ri.y - c.y, dz = ri.z - c.z;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const float nzf = r2 > 1e-30f ? 1.0f : 0.0f;
+            const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f)) * nzf;
+            const float r = r2 * rinv;
+            const double rd = (double)r;
+            const float sh = sin_small_f32<M>(r * h_dq);             // sin(dq r / 2)
+            const float sa = sin_rev(cc.a_rev * rd);                 // sin(a r)
+            const float clo = cos_rev(lo_rev * rd);                  // cos((a - dq/2) r)
+            const float delta = 4.0f * sh * sh;
+            // R = r / (2 sin(dq r)) = r / (4 sin(dq r/2) cos(dq r/2)); 0 for r == 0
+            const float ch = __builtin_sqrtf(fmaxf(1.0f - sh * sh, 0.0f));
+            const float R = nzf * r / fmaxf(4.0f * sh * ch, 1e-30f);
+            float u = sa * rinv;
+            float d = (clo + clo) * sh * rinv;
+            float D = beta[0] * d, U = gamma[0] * u;
+#pragma unroll
+            for (int k = 1; k < K; ++k) {
+                d = fmaf(-delta, u, d);
+                u += d;
+                D = fmaf(beta[k], d, D);
+                U = fmaf(gamma[k], u, U);
+            }
+            d = fmaf(-delta, u, d);
+            D = fmaf(beta[K], d, D);
+            const float gs = fmaf(R, D, -U) * (rinv * rinv * w);
+            const float ex = gs * dx, ey = gs * dy, ez = gs * dz;
+            gx += ex;
+            gy += ey;
+            gz += ez;
+            T[t][0][lane] = -ex;
+            T[t][1][lane] = -ey;
+            T[t][2][lane] = -ez;
+        }
+        __syncthreads();
+        if (lane < 3 * tn) {
+            const int t = lane / 3, comp = lane - 3 * (lane / 3);
+            float s = 0.0f;
+#pragma unroll 16
+            for (int l = 0; l < 64; ++l) s += T[t][comp][l];
+            colpart[(size_t)(t0 + t) * 3 + comp] = s;
+        }
+        __syncthreads();
+    }
+    float* __restrict__ rowpart = a.row_partial + (part * 64 + lane) * 3;
+    rowpart[0] = gx;
+    rowpart[1] = gy;
+    rowpart[2] = gz;
+}
+
+// grad of slot s = its row partials (jobs of its row block) + its column partials
+// (one job per row block at or before its own), over every q chunk, in a fixed order
+__global__ void adjoint_reduce_kernel(AdjointReduceArgs a)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    double x = 0, y = 0, z = 0;
-    for (int p = 0; p < n_parts; ++p) {
-        const double* v = gp + ((size_t)p * n + s) * 4;
-        x += v[0];
-        y += v[1];
-        z += v[2];
+    if (s >= a.n) return;
+    const int I = s >> 6, ln = s & 63;
+    double g[3] = {0.0, 0.0, 0.0};
+    const int nm = (a.n - 64 * I + a.col_range - 1) / a.col_range;
+    for (int m = 0; m < nm; ++m) {
+        const int slot = a.tri_slot[I * a.tri_stride + m];
+        for (int c = 0; c < a.n_chunks; ++c) {
+            const float* v = a.row_partial + (((size_t)slot * a.n_chunks + c) * 64 + ln) * 3;
+            g[0] += v[0];
+            g[1] += v[1];
+            g[2] += v[2];
+        }
     }
-    const int c = s2c[s];
-    out[3 * c] = (float)x;
-    out[3 * c + 1] = (float)y;
-    out[3 * c + 2] = (float)z;
+    for (int J = 0; J <= I; ++J) {
+        const int rel = s - 64 * J;
+        const int m = rel / a.col_range;
+        const int off = rel - m * a.col_range;
+        const int slot = a.tri_slot[J * a.tri_stride + m];
+        for (int c = 0; c < a.n_chunks; ++c) {
+            const float* v = a.col_partial + (((size_t)slot * a.n_chunks + c) * a.col_range + off) * 3;
+            g[0] += v[0];
+            g[1] += v[1];
+            g[2] += v[2];
+        }
+    }
+    const int o = a.slot_to_caller[s];
+    a.grad[3 * o] = (float)g[0];
+    a.grad[3 * o + 1] = (float)g[1];
+    a.grad[3 * o + 2] = (float)g[2];
 }
 
 template <int... Ks>
 struct KList {};
 
-constexpr int UNIFORM_KS[] = {8, 16, 24, 32, 40, 48, 56, 64};
+constexpr int UNIFORM_KS[] = {8, 16, 24, 32, 40, 48, 50, 56, 64, 67};
 // besides powers of two: K that split common grids into equal chunks with little
 // padding (256 = 3 x 86 - 2, 200 = 2 x 100, 101 <= 2 x 51, 200 <= 3 x 67)
 constexpr int REC32_KS[] = {16, 32, 43, 48, 51, 64, 67, 80, 86, 96, 100};
 constexpr int DIRECT_KS[] = {8, 16, 32, 48, 64};
 constexpr int ADJOINT_KS[] = {8, 16, 32, 48, 64};
+constexpr int ADJOINT32_KS[] = {16, 32, 40, 50, 64};
 
 }  // namespace
 
@@ -1014,14 +1179,14 @@ int debye_k_value(int path, int i)
     return path == PATH_DIRECT ? DIRECT_KS[i] : path == PATH_REC32 ? REC32_KS[i] : UNIFORM_KS[i];
 }
 
-int adjoint_k_count()
+int adjoint_k_count(bool fp32)
 {
-    return (int)(sizeof(ADJOINT_KS) / sizeof(int));
+    return fp32 ? (int)(sizeof(ADJOINT32_KS) / sizeof(int)) : (int)(sizeof(ADJOINT_KS) / sizeof(int));
 }
 
-int adjoint_k_value(int i)
+int adjoint_k_value(bool fp32, int i)
 {
-    return ADJOINT_KS[i];
+    return fp32 ? ADJOINT32_KS[i] : ADJOINT_KS[i];
 }
 
 namespace {
@@ -1031,14 +1196,37 @@ namespace {
 // rest of the recurrence range
 constexpr int SIN_SHORT = 5;
 
+// the shared-geometry kernel is instantiated for K <= 64 and K % 8 == 0
+template <int K>
+constexpr bool has_shared_kernel()
+{
+    return K <= 64 && K % 8 == 0;
+}
+
+template <int K, int M, bool SHARED>
+struct SharedLaunch {
+    static void go(const DebyeArgs& a, int n_jobs, int nw, int ng, hipStream_t s)
+    {
+        hipLaunchKernelGGL((debye_shared_kernel<K, M>), dim3((unsigned)(n_jobs * ng)), dim3((unsigned)(64 * nw)), 0, s,
+                           a);
+    }
+};
+
+template <int K, int M>
+struct SharedLaunch<K, M, false> {
+    static void go(const DebyeArgs& a, int n_jobs, int, int, hipStream_t s)
+    {
+        hipLaunchKernelGGL((debye_uniform_kernel<K, M>), dim3((unsigned)(n_jobs * a.n_chunks)), dim3(64), 0, s, a);
+    }
+};
+
 template <int K, int M>
 void launch_recurrence(const DebyeArgs& a, int n_jobs, hipStream_t s)
 {
     if (a.variant == VARIANT_SHARED && a.n_chunks > 1) {
         const int nw = a.n_chunks < SH_MAXW ? a.n_chunks : SH_MAXW;
         const int ng = (a.n_chunks + nw - 1) / nw;
-        hipLaunchKernelGGL((debye_shared_kernel<K, M>), dim3((unsigned)(n_jobs * ng)), dim3((unsigned)(64 * nw)), 0, s,
-                           a);
+        SharedLaunch<K, M, has_shared_kernel<K>()>::go(a, n_jobs, nw, ng, s);
     } else {
         hipLaunchKernelGGL((debye_uniform_kernel<K, M>), dim3((unsigned)(n_jobs * a.n_chunks)), dim3(64), 0, s, a);
     }
@@ -1091,8 +1279,10 @@ hipError_t launch_debye(const DebyeArgs& a, int n_jobs, int K, bool uniform, hip
         case 32: launch_recurrence_k<32>(a, n_jobs, s); break;
         case 40: launch_recurrence_k<40>(a, n_jobs, s); break;
         case 48: launch_recurrence_k<48>(a, n_jobs, s); break;
+        case 50: launch_recurrence_k<50>(a, n_jobs, s); break;
         case 56: launch_recurrence_k<56>(a, n_jobs, s); break;
         case 64: launch_recurrence_k<64>(a, n_jobs, s); break;
+        case 67: launch_recurrence_k<67>(a, n_jobs, s); break;
         default: return hipErrorInvalidValue;
         }
     } else {
@@ -1177,27 +1367,59 @@ hipError_t launch_sasa(const SasaArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_adjoint(const AdjointArgs& a, int K, hipStream_t s)
+namespace {
+
+template <int K>
+void launch_adjoint_k(const AdjointArgs& a, int n_jobs, hipStream_t s)
 {
-    const int n_rt = (a.n + 63) / 64;
-    const dim3 grid((unsigned)(n_rt * a.n_colblocks * a.n_chunks)), block(64);
+    const dim3 grid((unsigned)(n_jobs * a.n_chunks)), block(64);
+    if (a.sin_terms <= SIN_SHORT)
+        hipLaunchKernelGGL((adjoint_tri_kernel<K, SIN_SHORT>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((adjoint_tri_kernel<K, SIN_MAX_TERMS>), grid, block, 0, s, a);
+}
+
+template <int K>
+void launch_adjoint32_k(const AdjointArgs& a, int n_jobs, hipStream_t s)
+{
+    const dim3 grid((unsigned)(n_jobs * a.n_chunks)), block(64);
+    if (a.sin_terms <= SIN_SHORT)
+        hipLaunchKernelGGL((adjoint32_tri_kernel<K, SIN_SHORT>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((adjoint32_tri_kernel<K, SIN_MAX_TERMS>), grid, block, 0, s, a);
+}
+
+}  // namespace
+
+hipError_t launch_adjoint(const AdjointArgs& a, int n_jobs, int K, hipStream_t s)
+{
+    if (n_jobs <= 0) return hipSuccess;
+    if (a.fp32) {
+        switch (K) {
+        case 16: launch_adjoint32_k<16>(a, n_jobs, s); break;
+        case 32: launch_adjoint32_k<32>(a, n_jobs, s); break;
+        case 40: launch_adjoint32_k<40>(a, n_jobs, s); break;
+        case 50: launch_adjoint32_k<50>(a, n_jobs, s); break;
+        case 64: launch_adjoint32_k<64>(a, n_jobs, s); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (K) {
-    case 8: hipLaunchKernelGGL(adjoint_kernel<8>, grid, block, 0, s, a); break;
-    case 16: hipLaunchKernelGGL(adjoint_kernel<16>, grid, block, 0, s, a); break;
-    case 32: hipLaunchKernelGGL(adjoint_kernel<32>, grid, block, 0, s, a); break;
-    case 48: hipLaunchKernelGGL(adjoint_kernel<48>, grid, block, 0, s, a); break;
-    case 64: hipLaunchKernelGGL(adjoint_kernel<64>, grid, block, 0, s, a); break;
+    case 8: launch_adjoint_k<8>(a, n_jobs, s); break;
+    case 16: launch_adjoint_k<16>(a, n_jobs, s); break;
+    case 32: launch_adjoint_k<32>(a, n_jobs, s); break;
+    case 48: launch_adjoint_k<48>(a, n_jobs, s); break;
+    case 64: launch_adjoint_k<64>(a, n_jobs, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_grad_reduce(const double* grad_partial, int n_parts, int n, const int* slot_to_caller,
-                              float* grad_out, hipStream_t s)
+hipError_t launch_adjoint_reduce(const AdjointReduceArgs& a, hipStream_t s)
 {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(grad_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, grad_partial, n_parts,
-                       n, slot_to_caller, grad_out);
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -203,6 +203,16 @@ class Engine:
         self._check(self._L.rsx_time_forward(self.h, int(reps), ctypes.byref(tot), ctypes.byref(pair)))
         return tot.value, pair.value
 
+    def set_adjoint_tuning(self, fp32=True, k=0, col_range=0):
+        """fp32: FP32 difference-form adjoint kernel (default) or FP64 recurrence; k q per
+        thread (0 = automatic); col_range columns per job (0 = 256)."""
+        self._check(self._L.rsx_set_adjoint_tuning(self.h, int(bool(fp32)), int(k), int(col_range)))
+
+    def time_adjoint(self, reps=5):
+        ms = ctypes.c_float()
+        self._check(self._L.rsx_time_adjoint(self.h, int(reps), ctypes.byref(ms)))
+        return ms.value
+
     def time_copy(self, nbytes, reps=3):
         ms = ctypes.c_float()
         self._check(self._L.rsx_time_copy(self.h, int(nbytes), int(reps), ctypes.byref(ms)))

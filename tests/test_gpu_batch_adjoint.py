@@ -50,9 +50,10 @@ def _fit_case(seed=41, n=1200, nq=80):
     return rng, q, pos, types, table
 
 
-def test_adjoint_vs_analytic_and_fd(oracle):
+@pytest.mark.parametrize("fp32", [True, False])
+def test_adjoint_vs_analytic_and_fd(oracle, fp32):
     from resaxs_amd import configs
-    _say("case: chi2 gradient, 1200 atoms x 80 q")
+    _say(f"case: chi2 gradient, 1200 atoms x 80 q, {'FP32' if fp32 else 'FP64'} adjoint")
     rng, q, pos, types, table = _fit_case()
     F = oracle.body_factors(pos, types, q, table)
     Iref = oracle.debye(q, pos, F)
@@ -60,6 +61,7 @@ def test_adjoint_vs_analytic_and_fd(oracle):
     sigma = (0.01 * Iref).astype(np.float32)
     start = (pos + rng.normal(scale=0.5, size=pos.shape)).astype(np.float32)
     e = engine_for(q, table, n=len(pos))
+    e.set_adjoint_tuning(fp32)
     chi2, scale, grad, iq = e.chi2_grad(configs.bodies_of(start, types), i_exp, sigma)
     xyz = start.astype(np.float64)
     Fs = oracle.body_factors(start, types, q, table)

@@ -72,8 +72,8 @@ def test_config1_plain(oracle):
     assert err.max() < TOL
 
 
-KS_FP64 = [8, 16, 24, 32, 40, 48, 56, 64]
-KS_FP32 = [16, 32, 48, 64, 80, 96]
+KS_FP64 = [8, 16, 24, 32, 40, 48, 50, 56, 64, 67]
+KS_FP32 = [16, 32, 43, 48, 51, 64, 67, 80, 86, 96, 100]
 
 
 @pytest.mark.parametrize("variant,k", [(0, k) for k in KS_FP64] + [(1, 32), (1, 64)] + [(2, k) for k in KS_FP32])

@@ -32,11 +32,14 @@ def _target(w, e):
     return i_exp, sigma
 
 
-def test_fit_gradient_vs_fd_full_size(oracle):
+@pytest.mark.parametrize("fp32", [True, False])
+def test_fit_gradient_vs_fd_full_size(oracle, fp32):
     from resaxs_amd import configs
     w = configs.config5()
-    _say("case: chi2 gradient at full size, 10000 atoms x 200 q, FD on 64 atoms x 3")
+    _say(f"case: chi2 gradient at full size, 10000 atoms x 200 q, FD on 64 atoms x 3, "
+         f"{'FP32' if fp32 else 'FP64'} adjoint")
     e = engine_for(w.q, w.table, n=len(w.pos))
+    e.set_adjoint_tuning(fp32)
     i_exp, sigma = _target(w, e)
     chi2, scale, grad, iq = e.chi2_grad(w.start_bodies, i_exp, sigma)
     rng = np.random.default_rng(64)
