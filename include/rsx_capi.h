@@ -113,6 +113,9 @@ int rsx_batch_set_frames(rsx_engine *h, const float *bodies, const int *offsets,
 int rsx_batch_calc(rsx_engine *h, double *out_frames, double *out_mean);
 /* same on resident frames, results left on the device (no host copy); graph-captured */
 int rsx_batch_calc_dev_async(rsx_engine *h, double *out_frames_dev, double *out_mean_dev);
+/* batched launch form (FP32 variant): 1 = one persistent grid of resident waves pulling
+ * (frame job, q chunk) items from a work counter, longest first (default); 0 = one wave per item */
+int rsx_set_persistent(rsx_engine *h, int on);
 
 /* ---- differentiable fit (no reference counterpart; uses saxs_profile.hpp:137-166 scale/chi) ---- */
 /* chi2 = sum_k (I_exp - c I)^2 / sigma^2 at the least-squares c; grad = dchi2/dx [n][3].

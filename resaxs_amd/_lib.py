@@ -66,6 +66,7 @@ SIGNATURES = {
     "rsx_batch_set_frames": (_i, [_vp, _fp, _ip, _i]),
     "rsx_batch_calc": (_i, [_vp, _dp, _dp]),
     "rsx_batch_calc_dev_async": (_i, [_vp, _vp, _vp]),
+    "rsx_set_persistent": (_i, [_vp, _i]),
     "rsx_chi2_grad": (_i, [_vp, _fp, _i, _fp, _fp, _dp, _dp, _fp, _dp]),
     "rsx_debye_vjp": (_i, [_vp, _fp, _i, _dp, _fp]),
     "rsx_autotune": (_i, [_vp, _c.c_char_p]),

@@ -317,6 +317,9 @@ struct rsx_engine {
     uint64_t a_stamp = ~0ull;
     const float4* a_cols = nullptr;  // the column buffer the adjoint jobs point into
     bool a_have_w = false;           // d_wq holds the weights of the last gradient call
+    bool persistent = true;          // batched launch: persistent work-queue kernel (FP32 variant)
+    int n_cu = 256;                  // compute units of the device
+    DBuf<int> d_queue;               // work counter of the persistent launch
     uint64_t layout_stamp = 0;  // bumped whenever the class table or the slot layout changes
     size_t ftab_rows = 0;
 
@@ -732,6 +735,8 @@ rsx::DebyeArgs debye_args(rsx_engine* h, const PairJob* d_jobs, double* partials
     a.qv = h->d_qv.p;
     a.partials = partials;
     a.variant = h->variant;
+    a.queue = nullptr;
+    a.persistent_waves = 0;
     // pairs satisfy r <= 2 R about the centre of the layout, so dq r / 2 <= dq R
     a.sin_terms = rsx::sin_series_terms(h->dq * h->path_radius);
     return a;
@@ -1151,9 +1156,20 @@ void batch_set_frames_impl(rsx_engine* h, const float* bodies, const int* offset
     h->b_offsets.assign(offsets, offsets + nf + 1);
 }
 
+// waves a persistent launch keeps resident: 4 SIMDs per CU at the kernel's occupancy
+// (3 waves per SIMD up to K = 67, 2 above)
+int persistent_waves(const rsx_engine* h)
+{
+    return h->n_cu * 4 * (h->K <= 67 ? 3 : 2);
+}
+
 void batch_enqueue(rsx_engine* h, double* out_frames, double* out_mean)
 {
-    const rsx::DebyeArgs a = debye_args(h, h->b_jobs.p, h->b_partials.p);
+    rsx::DebyeArgs a = debye_args(h, h->b_jobs.p, h->b_partials.p);
+    if (h->persistent && h->variant == rsx::VARIANT_FP32) {
+        a.queue = h->d_queue.p;
+        a.persistent_waves = persistent_waves(h);
+    }
     hipchk(rsx::launch_debye(a, h->b_njobs, h->K, h->uniform, h->stream), "batch debye");
     hipchk(rsx::launch_reduce(h->b_partials.p, h->ldf, h->nq, h->b_slot_begin.p, h->b_frames, out_frames, h->nq, 0,
                               h->b_q0.p, h->q0_index, h->stream),
@@ -1526,6 +1542,15 @@ int rsx_create(int device, rsx_engine** out)
         delete h;
         return RSX_HIP_ERROR;
     }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        h->n_cu = prop.multiProcessorCount;
+    try {
+        h->d_queue.ensure(1);
+    } catch (...) {
+        delete h;
+        return RSX_OUT_OF_MEMORY;
+    }
     h->state = 1;
     *out = h;
     return RSX_OK;
@@ -1560,6 +1585,7 @@ int rsx_destroy(rsx_engine* h)
     h->d_slot_begin_f.release();
     h->b_jobs.release();
     h->d_counts.release();
+    h->d_queue.release();
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->ev2) hipEventDestroy(h->ev2);
@@ -1953,6 +1979,14 @@ int rsx_time_forward(rsx_engine* h, int reps, float* ms_total, float* ms_pair)
         hipchk(hipEventElapsedTime(&tot, h->ev0, h->ev1), "elapsed");
         if (ms_total) *ms_total = tot / reps;
         if (ms_pair) *ms_pair = pair_ms / reps;
+    });
+}
+
+int rsx_set_persistent(rsx_engine* h, int on)
+{
+    return guarded(h, [&] {
+        h->persistent = on != 0;
+        batch_release_graph(h);  // the captured launch changes
     });
 }
 

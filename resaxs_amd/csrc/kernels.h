@@ -80,6 +80,8 @@ struct DebyeArgs {
     const float* qv;      // [ldf] q           (direct path)
     double* partials;     // [n_slots][ldf]
     int variant;          // recurrence kernel: VARIANT_WAVE, VARIANT_SHARED or VARIANT_FP32
+    int* queue;           // work counter of the persistent FP32 launch (device int), or null
+    int persistent_waves; // > 0: persistent launch of this many waves pulling items from queue
     int sin_terms;        // terms the sin(dq r / 2) series needs (sin_series_terms); the launcher
                           // picks the shortest instantiated length >= it
 };

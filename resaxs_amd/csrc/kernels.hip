@@ -498,11 +498,10 @@ __device__ __forceinline__ void run2_32(float (&acc)[K], const Seeds32& a, const
     }
 }
 
-template <int K, int M>
-__global__ __launch_bounds__(64, 2) void debye_rec32_kernel(const DebyeArgs a)
+// one (job, q chunk) work item of the FP32 recurrence; T is the wave's LDS transpose
+template <int K, int M, int KS>
+__device__ __forceinline__ void rec32_item(const DebyeArgs& a, int job_id, int chunk, float (&T)[KS][65])
 {
-    const int job_id = blockIdx.x / a.n_chunks;
-    const int chunk = blockIdx.x - job_id * a.n_chunks;
     const PairJob jb = a.jobs[job_id];
     const int lane = threadIdx.x;
     const bool row_ok = lane < jb.nrow;
@@ -559,8 +558,6 @@ __global__ __launch_bounds__(64, 2) void debye_rec32_kernel(const DebyeArgs a)
 
     // epilogue as in debye_uniform_kernel, in slices of up to 64 q so the LDS
     // transpose stays 16.6 KB for any K
-    constexpr int KS = K < 64 ? K : 64;
-    __shared__ float T[KS][65];
     const uint32_t ci = cls_i == CLS_SKIP ? (uint32_t)a.zero_class : cls_i;
     const float* __restrict__ fi = ftab + (size_t)ci * ldf + k0;
 #pragma unroll
@@ -576,6 +573,34 @@ __global__ __launch_bounds__(64, 2) void debye_rec32_kernel(const DebyeArgs a)
             a.partials[(size_t)jb.slot * ldf + k0 + kb + lane] = sum * a.invq[k0 + kb + lane] * (double)jb.scale;
         }
         __syncthreads();
+    }
+}
+
+template <int K, int M>
+__global__ __launch_bounds__(64, 2) void debye_rec32_kernel(const DebyeArgs a)
+{
+    constexpr int KS = K < 64 ? K : 64;
+    __shared__ float T[KS][65];
+    const int job_id = blockIdx.x / a.n_chunks;
+    rec32_item<K, M, KS>(a, job_id, blockIdx.x - job_id * a.n_chunks, T);
+}
+
+// Persistent form: a grid sized to the resident waves pulls (job, chunk) items from
+// a work counter in job order (longest first) until the list is exhausted; every
+// wave exits once the counter passes the item count.  The counter is zeroed by a
+// fill kernel on the same stream before the launch (graph-capturable).
+template <int K, int M>
+__global__ __launch_bounds__(64, 2) void debye_rec32_persistent(const DebyeArgs a, int n_items)
+{
+    constexpr int KS = K < 64 ? K : 64;
+    __shared__ float T[KS][65];
+    for (;;) {
+        int item = 0;
+        if (threadIdx.x == 0) item = atomicAdd(a.queue, 1);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
+        if (item >= n_items) break;
+        const int job_id = item / a.n_chunks;
+        rec32_item<K, M, KS>(a, job_id, item - job_id * a.n_chunks, T);
     }
 }
 
@@ -1244,7 +1269,17 @@ void launch_recurrence_k(const DebyeArgs& a, int n_jobs, hipStream_t s)
 template <int K>
 void launch_rec32_k(const DebyeArgs& a, int n_jobs, hipStream_t s)
 {
-    const dim3 grid((unsigned)(n_jobs * a.n_chunks)), block(64);
+    const int n_items = n_jobs * a.n_chunks;
+    if (a.queue && a.persistent_waves > 0 && a.persistent_waves < n_items) {
+        hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, s, a.queue, 0, 1LL);
+        const dim3 grid((unsigned)a.persistent_waves), block(64);
+        if (a.sin_terms <= SIN_SHORT)
+            hipLaunchKernelGGL((debye_rec32_persistent<K, SIN_SHORT>), grid, block, 0, s, a, n_items);
+        else
+            hipLaunchKernelGGL((debye_rec32_persistent<K, SIN_MAX_TERMS>), grid, block, 0, s, a, n_items);
+        return;
+    }
+    const dim3 grid((unsigned)n_items), block(64);
     if (a.sin_terms <= SIN_SHORT)
         hipLaunchKernelGGL((debye_rec32_kernel<K, SIN_SHORT>), grid, block, 0, s, a);
     else

@@ -171,6 +171,10 @@ class Engine:
                                                  len(off) - 1))
         self.n_frames = len(off) - 1
 
+    def set_persistent(self, on):
+        """Batched launch: persistent work-queue grid (True, default) or one wave per item."""
+        self._check(self._L.rsx_set_persistent(self.h, int(bool(on))))
+
     def batch_calc(self):
         frames = np.zeros((self.n_frames, self.nq), dtype=np.float64)
         mean = np.zeros(self.nq, dtype=np.float64)
