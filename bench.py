@@ -147,6 +147,86 @@ def adjoint_bench(reps=3):
             "timing": "device: HIP events on the engine stream; wall: host clock around rsx_chi2_grad"}
 
 
+def incremental_bench(hydrate, n_moves=2000):
+    """The incremental workload: 50,000 atoms x 256 q, n_moves committed rigid moves of
+    500-atom segments, running FP64 I(q) re-baselined every 200 moves.  Host wall clock
+    around recalc_curve + commit (the synchronous calls a Monte Carlo caller makes),
+    work per move 2 P_move (5 n_q + 9) with P_move = m (N - m) + m (m - 1) / 2.  The
+    running I(q) after move 1999 (199 moves since the last re-baseline) is compared
+    with a fresh full recompute by the FP64 pair kernel."""
+    from resaxs_amd import configs
+    w = configs.config3()
+    n, nq, m = len(w.pos), len(w.q), w.seg
+    e = make_engine(w, hydrate=hydrate)
+    e.set_rebaseline(200)
+    pos = w.pos.copy()
+    b = configs.bodies_of(pos, w.types)
+    e.calc_curve(b)
+    rng = w.move_rng
+    log(f"bench: incremental, {n_moves} moves of {m} atoms on {n} atoms, hydration={hydrate}")
+    ts, err_1999 = [], None
+    for k in range(n_moves):
+        start, newp = configs.random_move(rng, pos, m)
+        pos[start:start + m] = newp
+        b[start:start + m, :3] = newp
+        t0 = time.perf_counter()
+        I = e.recalc_curve(b, start, m)
+        e.commit()
+        ts.append(time.perf_counter() - t0)
+        if k == n_moves - 2 and n_moves >= 2:
+            f = make_engine(w, hydrate=hydrate)
+            f.set_variant(0)
+            I_full = f.calc_curve(b)
+            err_1999 = float(np.max(np.abs(I - I_full) / np.abs(I_full)))
+            f.close()
+    e.close()
+    p_move = m * (n - m) + m * (m - 1) / 2.0
+    ms = float(np.mean(ts)) * 1e3
+    return {"workload": f"{w.name}{'-sasa' if hydrate else ''}", "moves": n_moves, "segment": m,
+            "rebaseline_every": 200, "ms_per_move_mean": round(ms, 4),
+            "ms_per_move_median": round(float(np.median(ts)) * 1e3, 4),
+            "terms_per_s": 2 * p_move * nq / (ms * 1e-3),
+            "achieved_tflops": round(2 * p_move * (5 * nq + 9) / (ms * 1e-3) / 1e12, 2),
+            "rel_err_after_199_moves_vs_fp64_full": err_1999,
+            "timing": "host wall clock around rsx_recalc_curve_f64 + rsx_commit (host bodies in, I(q) out)"
+                      + ("; every move recounts SASA for all atoms" if hydrate else "")}
+
+
+def batched_bench(reps=5):
+    """The batched ensemble: 512 frames of 1,000-15,000 atoms x 128 q, one graph-captured
+    persistent launch per replay; per-frame I(q) and the uniform mean to device buffers."""
+    from resaxs_amd import Engine, configs
+    w = configs.config4()
+    nq = len(w.q)
+    e = Engine(0)
+    e.set_q(w.q)
+    e.set_factors(w.table, w.water_index)
+    e.initialize(1)
+    t0 = time.perf_counter()
+    e.batch_set_frames(w.bodies, w.offsets)
+    layout_s = time.perf_counter() - t0
+    fr = e.dev_alloc(len(w.sizes) * nq * 8)
+    mn = e.dev_alloc(nq * 8)
+    e.batch_calc_dev_async(fr, mn)
+    e.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e.batch_calc_dev_async(fr, mn)
+    e.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    tuning = e.get_tuning()
+    e.dev_free(fr)
+    e.dev_free(mn)
+    e.close()
+    terms = float(sum(s * (s - 1) / 2.0 for s in w.sizes)) * nq
+    flops = float(sum(configs.forward_flops(int(s), nq) for s in w.sizes))
+    return {"workload": f"ensemble-{len(w.sizes)}-frames-1000..15000x{nq}", "ms_per_batch": round(ms, 3),
+            "terms_per_s": terms / (ms * 1e-3), "achieved_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+            "frac": round(flops / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4), "layout_s": round(layout_s, 3),
+            "kernel": f"variant {tuning['variant']} K={tuning['k']} (persistent work queue)",
+            "timing": "host wall clock over graph replays on device-resident frames, outputs to device"}
+
+
 def cpu_baseline(w, I_gpu, rows=2560):
     """The reference's CPU path timed on this host, 1 thread, on a bounded sample: the
     pairs among the first `rows` atoms (the first rows of the lower triangle) at all
@@ -194,6 +274,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the batched-ensemble and incremental-move measurements")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -241,6 +323,10 @@ def main():
         line["forward_50k"] = forward_50k_bench()
         log("bench: forward + adjoint on the fit configuration")
         line["fwd_adjoint"] = adjoint_bench()
+    if not args.no_workloads:
+        log("bench: batched ensemble")
+        line["batched"] = batched_bench()
+        line["incremental"] = [incremental_bench(False), incremental_bench(True)]
     if not args.no_cpu_baseline:
         base, acc = cpu_baseline(w, I_gpu)
         line["cpu_baseline"] = base
