@@ -293,7 +293,8 @@ def main():
             f"{2 * (args.copy_probe << 20) / ms / 1e6:.1f} GB/s (read + write)")
         e.close()
     w, I_gpu, ms_per_step, pair_ms, tuning = forward_bench(args.steps, args.warmup)
-    kernel = ("debye_uniform_kernel", "debye_shared_kernel", "debye_rec32_kernel")[tuning["variant"]]
+    kernel = ("debye_uniform_kernel", "debye_shared_kernel", "debye_rec32_kernel",
+              "debye_rec32r_kernel")[tuning["variant"]]
     n, nq = len(w.pos), len(w.q)
     terms = configs.pair_terms(n, nq)
     flops = configs.forward_flops(n, nq)

@@ -113,14 +113,15 @@ int rsx_batch_set_frames(rsx_engine *h, const float *bodies, const int *offsets,
 int rsx_batch_calc(rsx_engine *h, double *out_frames, double *out_mean);
 /* same on resident frames, results left on the device (no host copy); graph-captured */
 int rsx_batch_calc_dev_async(rsx_engine *h, double *out_frames_dev, double *out_mean_dev);
-/* batched launch form (FP32 variant): 1 = one persistent grid of resident waves pulling
- * (frame job, q chunk) items from a work counter, longest first (default); 0 = one wave per item */
+/* batched launch form (FP32 variants): 1 = one persistent grid of resident waves pulling
+ * (frame job, q chunk) items from a work counter, longest first (default; taken only where
+ * the work-queue kernel keeps the per-item kernel's occupancy); 0 = one wave per item */
 int rsx_set_persistent(rsx_engine *h, int on);
 
 /* ---- differentiable fit (no reference counterpart; uses saxs_profile.hpp:137-166 scale/chi) ---- */
 /* chi2 = sum_k (I_exp - c I)^2 / sigma^2 at the least-squares c; grad = dchi2/dx [n][3].
  * The chi^2 weights amplify forward errors by I / sigma, so this and rsx_debye_vjp run the FP64
- * recurrence: an engine on RSX_VARIANT_FP32 moves to RSX_VARIANT_WAVE on the first call. */
+ * recurrence: an engine on an FP32 variant moves to RSX_VARIANT_WAVE on the first call. */
 int rsx_chi2_grad(rsx_engine *h, const float *bodies, int n, const float *i_exp, const float *sigma,
                   double *out_chi2, double *out_scale, float *out_grad, double *out_iq);
 /* adjoint only: grad_i = sum_k w_k dI_k/dx_i for caller weights w [n_q] (backward of I(q)) */
@@ -137,10 +138,13 @@ int rsx_set_tuning(rsx_engine *h, int k, int col_range);
 /* recurrence kernel for uniform q grids (the next calc_curve lays out again):
  *   RSX_VARIANT_WAVE   FP64 recurrence, one wave per (job, q chunk)
  *   RSX_VARIANT_SHARED FP64 recurrence, the q chunks of a job share the pair geometry through LDS
- *   RSX_VARIANT_FP32   FP32 difference-form recurrence (default) */
+ *   RSX_VARIANT_FP32   FP32 difference-form recurrence, per-lane sums over class runs
+ *   RSX_VARIANT_FP32R  the FP32 recurrence with each column class run reduced over the
+ *                      rows when it ends (fewer registers, K up to 128; default) */
 #define RSX_VARIANT_WAVE    0
 #define RSX_VARIANT_SHARED  1
 #define RSX_VARIANT_FP32    2
+#define RSX_VARIANT_FP32R   3
 int rsx_set_variant(rsx_engine *h, int variant);
 /* current q per thread, column range of the forward layout, 1 if the recurrence path is active,
  * and the recurrence variant (any pointer may be NULL) */

@@ -27,6 +27,7 @@ RSX_DOTS_FIBONACCI = 1
 RSX_VARIANT_WAVE = 0
 RSX_VARIANT_SHARED = 1
 RSX_VARIANT_FP32 = 2
+RSX_VARIANT_FP32R = 3
 
 # every symbol include/rsx_capi.h declares, with (restype, argtypes)
 _c = ctypes

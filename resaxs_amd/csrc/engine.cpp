@@ -272,7 +272,7 @@ struct rsx_engine {
 
     // tuning
     int k_req = 0, range_req = 0;
-    int variant = rsx::VARIANT_FP32;  // recurrence kernel (rsx::VARIANT_*)
+    int variant = rsx::VARIANT_FP32R;  // recurrence kernel (rsx::VARIANT_*)
     int K = 0, n_chunks = 0, ldf = 0;
     double path_radius = 0.0;  // bounding radius the current path and series length were chosen for
     int rebase_every = 200;
@@ -297,6 +297,7 @@ struct rsx_engine {
     DBuf<double> d_qrev;
     DBuf<ChunkConst> d_chunks;
     DBuf<double> d_invq, d_partials, d_iq, d_saved_iq, d_out, d_wq, d_q0;
+    DBuf<double> d_red;              // chunk sums of the two-stage reduction (single structure)
     DBuf<PairJob> d_jobs, d_fjobs;  // incremental jobs; forward (triangle) jobs
     DBuf<double> d_f0col;           // FP64 F_class(q = 0) per class
     DBuf<int> d_slot_begin_f;  // [0, n forward jobs]
@@ -305,7 +306,7 @@ struct rsx_engine {
     DBuf<float4> d_dots;
     DBuf<float> d_grad;
     // adjoint layout (adjoint_layout), rebuilt when layout_stamp changes
-    DBuf<float> a_ftab, a_rowpart, a_colpart, a_invq32;
+    DBuf<float> a_ftab, a_rowpart, a_colpart, a_invq32, a_cb, a_cg;
     bool adj_fp32 = true;  // adjoint kernel precision (rsx_set_adjoint_tuning)
     int adj_k_req = 0, adj_range_req = 0;
     bool a_fp32 = false;   // precision the cached adjoint layout was built for
@@ -348,9 +349,9 @@ struct rsx_engine {
     std::vector<int> b_offsets;
     DBuf<float4> b_cols;
     DBuf<PairJob> b_jobs;
-    DBuf<double> b_partials, b_out, b_mean, b_q0;
+    DBuf<double> b_partials, b_out, b_mean, b_q0, b_red;
     DBuf<int> b_slot_begin;
-    int b_njobs = 0, b_slots = 0;
+    int b_njobs = 0, b_slots = 0, b_max_slots = 0;
     hipGraph_t b_graph = nullptr;
     hipGraphExec_t b_exec = nullptr;
     double* b_graph_out = nullptr;
@@ -368,7 +369,10 @@ void set_device(rsx_engine* h)
 // K list of the active pair kernel
 int pair_path(const rsx_engine* h)
 {
-    return !h->uniform ? rsx::PATH_DIRECT : h->variant == rsx::VARIANT_FP32 ? rsx::PATH_REC32 : rsx::PATH_REC64;
+    return !h->uniform                          ? rsx::PATH_DIRECT
+           : h->variant == rsx::VARIANT_FP32  ? rsx::PATH_REC32
+           : h->variant == rsx::VARIANT_FP32R ? rsx::PATH_REC32R
+                                              : rsx::PATH_REC64;
 }
 
 void choose_chunks(rsx_engine* h)
@@ -386,7 +390,7 @@ void choose_chunks(rsx_engine* h)
         // cost per (pair, chunk) in FP64-op units: the FP64 recurrence 2K + ~44 for distance
         // and seeds; the FP32 one 3K FP32 ops (~0.6 of an FP64 op each) + ~25
         const double cost = !uni ? C * (6.0 * K + 12.0)
-                            : path == rsx::PATH_REC32 ? C * (1.8 * K + 25.0)
+                            : path == rsx::PATH_REC32 || path == rsx::PATH_REC32R ? C * (1.8 * K + 25.0)
                                                       : C * (2.0 * K + 44.0);
         if (cost < best) {
             best = cost;
@@ -660,15 +664,20 @@ void compute_sasa(rsx_engine* h)
 int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles)
 {
     if (h->range_req > 0) return h->range_req;
-    // aim for ~16384 waves (8 per wave slot at 2 waves/SIMD over 1024 SIMDs): short
-    // enough jobs that the longest-first order leaves little tail
-    const double jobs = 16384.0 / std::max(1, h->n_chunks);
-    // column_atoms_total counts pairs; a job covers 64 rows
-    double L = column_atoms_total / 64.0 / std::max(1.0, jobs);
+    // resident waves of the pair kernel (the runtime's occupancy for the FP32 kernels,
+    // two per SIMD otherwise)
+    int per_cu = 0;
+    if (h->uniform && (h->variant == rsx::VARIANT_FP32 || h->variant == rsx::VARIANT_FP32R))
+        per_cu = rsx::rec32_blocks_per_cu(h->variant, h->K, rsx::sin_series_terms(h->dq * h->path_radius), false);
+    if (per_cu <= 0) per_cu = 8;
+    const double slots = (double)std::max(1, h->n_cu) * per_cu;
+    // about four rounds of waves over the resident slots: fewer leave a partly filled last
+    // round (measured on incremental updates: L = 128-192 beats 512 by 1.3-1.9x), more
+    // pay the per-job set-up and epilogue more often.  128..1024 columns.
+    const double jobs = std::max(1.0, 4.0 * slots / std::max(1, h->n_chunks));
+    double L = column_atoms_total / 64.0 / jobs;
     (void)row_tiles;
-    // measured optimum on gfx950: 256..1024 columns (shorter jobs pay the epilogue and
-    // seeds more often; longer ones leave a tail)
-    L = std::max(256.0, std::min(1024.0, L));
+    L = std::max(128.0, std::min(1024.0, L));
     return ((int)L + 63) / 64 * 64;
 }
 
@@ -749,6 +758,7 @@ void run_jobs(rsx_engine* h, DBuf<PairJob>& jbuf, const std::vector<PairJob>& jo
     const int nj = (int)jobs.size();
     jbuf.ensure(nj);
     h->d_partials.ensure((size_t)std::max(nj, 1) * h->ldf);
+    h->d_red.ensure(std::max<size_t>(1, rsx::reduce_scratch_doubles(h->nq, 1, nj)));
     h->d_slot_begin.ensure(2);
     if (nj > 0)
         hipchk(hipMemcpyAsync(jbuf.p, jobs.data(), nj * sizeof(PairJob), hipMemcpyHostToDevice, h->stream),
@@ -760,7 +770,7 @@ void run_jobs(rsx_engine* h, DBuf<PairJob>& jbuf, const std::vector<PairJob>& jo
     hipchk(rsx::launch_debye(a, nj, h->K, h->uniform, h->stream), "debye kernel");
     if (nj > 0) {
         hipchk(rsx::launch_reduce(h->d_partials.p, h->ldf, h->nq, h->d_slot_begin.p, 1, iq, h->nq, accumulate,
-                                  h->d_q0.p, h->q0_index, h->stream),
+                                  h->d_q0.p, h->q0_index, nj, h->d_red.p, h->stream),
                "reduce kernel");
     } else if (!accumulate) {
         std::vector<double> z(h->nq, 0.0);
@@ -1122,11 +1132,13 @@ void batch_set_frames_impl(rsx_engine* h, const float* bodies, const int* offset
     hipchk(hipMemcpy(h->b_cols.p, cols.data(), cols.size() * sizeof(float4), hipMemcpyHostToDevice), "upload frames");
     // jobs grouped by frame: slots of frame f are [sb[f], sb[f+1])
     double work = 0.0;
+    int row_tiles = 0;
     for (int f = 0; f < nf; ++f) {
         const double m = offsets[f + 1] - offsets[f];
         work += m * m / 2.0;
+        row_tiles += (offsets[f + 1] - offsets[f] + 63) / 64;
     }
-    const int L = pick_range(h, work, 0);
+    const int L = pick_range(h, work, row_tiles);
     std::vector<PairJob> jobs;
     std::vector<int> sb(nf + 1, 0);
     for (int f = 0; f < nf; ++f) {
@@ -1143,8 +1155,11 @@ void batch_set_frames_impl(rsx_engine* h, const float* bodies, const int* offset
                      [](const PairJob& a, const PairJob& b) { return (long)a.ncol * a.nrow > (long)b.ncol * b.nrow; });
     h->b_njobs = (int)jobs.size();
     h->b_slots = h->b_njobs;
+    h->b_max_slots = 0;
+    for (int f = 0; f < nf; ++f) h->b_max_slots = std::max(h->b_max_slots, sb[f + 1] - sb[f]);
     h->b_jobs.ensure(jobs.size());
     h->b_partials.ensure((size_t)std::max(1, h->b_slots) * h->ldf);
+    h->b_red.ensure(std::max<size_t>(1, rsx::reduce_scratch_doubles(h->nq, nf, h->b_max_slots)));
     h->b_slot_begin.ensure(nf + 1);
     h->b_out.ensure((size_t)nf * h->nq);
     h->b_mean.ensure(h->nq);
@@ -1156,23 +1171,27 @@ void batch_set_frames_impl(rsx_engine* h, const float* bodies, const int* offset
     h->b_offsets.assign(offsets, offsets + nf + 1);
 }
 
-// waves a persistent launch keeps resident: 4 SIMDs per CU at the kernel's occupancy
-// (3 waves per SIMD up to K = 67, 2 above)
+// waves a persistent launch keeps resident: the CUs times the kernel's resident
+// one-wave blocks per CU (from the runtime's occupancy calculator); 0 (one wave per
+// item instead) where the work-queue loop would cost the kernel occupancy
 int persistent_waves(const rsx_engine* h)
 {
-    return h->n_cu * 4 * (h->K <= 67 ? 3 : 2);
+    const int st = rsx::sin_series_terms(h->dq * h->path_radius);
+    const int pb = rsx::rec32_blocks_per_cu(h->variant, h->K, st, true);
+    const int ib = rsx::rec32_blocks_per_cu(h->variant, h->K, st, false);
+    return pb >= ib ? h->n_cu * pb : 0;
 }
 
-void batch_enqueue(rsx_engine* h, double* out_frames, double* out_mean)
+void batch_enqueue(rsx_engine* h, double* out_frames, double* out_mean, int pers_waves)
 {
     rsx::DebyeArgs a = debye_args(h, h->b_jobs.p, h->b_partials.p);
-    if (h->persistent && h->variant == rsx::VARIANT_FP32) {
+    if (pers_waves > 0) {
         a.queue = h->d_queue.p;
-        a.persistent_waves = persistent_waves(h);
+        a.persistent_waves = pers_waves;
     }
     hipchk(rsx::launch_debye(a, h->b_njobs, h->K, h->uniform, h->stream), "batch debye");
     hipchk(rsx::launch_reduce(h->b_partials.p, h->ldf, h->nq, h->b_slot_begin.p, h->b_frames, out_frames, h->nq, 0,
-                              h->b_q0.p, h->q0_index, h->stream),
+                              h->b_q0.p, h->q0_index, h->b_max_slots, h->b_red.p, h->stream),
            "batch reduce");
     hipchk(rsx::launch_frame_mean(out_frames, h->b_frames, h->nq, out_mean, h->stream), "frame mean");
 }
@@ -1183,9 +1202,11 @@ void batch_run_dev(rsx_engine* h, double* out_frames, double* out_mean)
     set_device(h);
     if (!h->b_exec || h->b_graph_out != out_frames || h->b_graph_mean != out_mean) {
         batch_release_graph(h);
+        const bool pers = h->persistent && (h->variant == rsx::VARIANT_FP32 || h->variant == rsx::VARIANT_FP32R);
+        const int pw = pers ? persistent_waves(h) : 0;
         hipchk(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture");
         try {
-            batch_enqueue(h, out_frames, out_mean);
+            batch_enqueue(h, out_frames, out_mean, pw);
         } catch (...) {
             hipGraph_t g;
             hipStreamEndCapture(h->stream, &g);
@@ -1219,8 +1240,8 @@ void adjoint_layout(rsx_engine* h)
         const int k = rsx::adjoint_k_value(fp32, i);
         if (h->adj_k_req > 0 && k != h->adj_k_req) continue;
         const int c = (h->nq + k - 1) / k;
-        // (the FP32 kernel drops from 3 to 2 waves per SIMD from K = 50 on: ~10 % per op)
-        const double cost = fp32 ? c * (2.4 * k + 30.0) * (k >= 50 ? 1.1 : 1.0) : c * (4.0 * k + 60.0);
+        // (the FP32 kernel keeps 3 waves per SIMD up to K = 67)
+        const double cost = fp32 ? c * (2.4 * k + 30.0) * (k > 67 ? 1.1 : 1.0) : c * (4.0 * k + 60.0);
         if (cost < best) {
             best = cost;
             K = k;
@@ -1270,6 +1291,8 @@ void adjoint_layout(rsx_engine* h)
     }
     const int nj = (int)jobs.size();
     h->a_ftab.ensure(ft.size());
+    h->a_cb.ensure(ft.size());
+    h->a_cg.ensure(ft.size());
     h->a_chunks.ensure(nch);
     h->a_invq.ensure(ldfa);
     h->a_invq32.ensure(ldfa);
@@ -1310,12 +1333,18 @@ void launch_adjoint_pass(rsx_engine* h)
     a.wq = h->d_wq.p;
     a.invq = h->a_invq.p;
     a.invq32 = h->a_invq32.p;
+    a.cb = h->a_cb.p;
+    a.cg = h->a_cg.p;
     a.dq_rev = h->dq / (2.0 * PI);
     a.fp32 = h->a_fp32 ? 1 : 0;
     a.sin_terms = rsx::sin_series_terms(h->dq * h->path_radius);
     a.col_range = h->a_L;
     a.row_partial = h->a_rowpart.p;
     a.col_partial = h->a_colpart.p;
+    if (h->a_fp32)
+        hipchk(rsx::launch_adjoint_coef(h->a_ftab.p, h->d_wq.p, h->a_invq.p, std::max(1, h->n_class), h->a_ldf, h->a_cb.p,
+                                        h->a_cg.p, h->stream),
+               "adjoint coefficients");
     hipchk(rsx::launch_adjoint(a, h->a_njobs, h->a_K, h->stream), "adjoint kernel");
     rsx::AdjointReduceArgs r;
     r.row_partial = h->a_rowpart.p;
@@ -1370,7 +1399,7 @@ void enqueue_device_forward(rsx_engine* h, const float4* bodies_dev, double* out
     const rsx::DebyeArgs a = debye_args(h, h->d_fjobs.p, h->d_partials.p);
     hipchk(rsx::launch_debye(a, nj, h->K, h->uniform, h->stream), "debye kernel");
     hipchk(rsx::launch_reduce(h->d_partials.p, h->ldf, h->nq, h->d_slot_begin_f.p, 1, out, h->nq, 0, h->d_q0.p,
-                              h->q0_index, h->stream),
+                              h->q0_index, nj, h->d_red.p, h->stream),
            "reduce kernel");
 }
 
@@ -1448,9 +1477,12 @@ void autotune_impl(rsx_engine* h, const char* cache_path)
     int bestK = 0, bestL = 0;
     float best = 1e30f;
     int bestV = h->variant;
-    const int n_var = h->uniform ? rsx::VARIANT_COUNT : 1;  // the direct path has one kernel
-    for (int V = 0; V < n_var; ++V) {
-        if (h->uniform) h->variant = V;
+    // uniform grids: the two FP32 recurrence kernels (the FP64 ones stay selectable by
+    // rsx_set_variant but are slower on every layout measured); the direct path has one
+    const int uni_variants[] = {rsx::VARIANT_FP32R, rsx::VARIANT_FP32};
+    const int n_var = h->uniform ? 2 : 1;
+    for (int vi = 0; vi < n_var; ++vi) {
+        if (h->uniform) h->variant = uni_variants[vi];
         const int path = pair_path(h);
         std::vector<int> Ks;
         for (int i = 0; i < rsx::debye_k_count(path); ++i) {
@@ -1503,7 +1535,7 @@ int guarded(rsx_engine* h, F&& f)
 // stays there) on its first gradient call.
 void use_fp64_forward(rsx_engine* h)
 {
-    if (h->variant != rsx::VARIANT_FP32 || h->nq <= 0) return;
+    if ((h->variant != rsx::VARIANT_FP32 && h->variant != rsx::VARIANT_FP32R) || h->nq <= 0) return;
     h->variant = rsx::VARIANT_WAVE;
     h->k_req = 0;  // the FP64 kernels have their own K list
     setup_q(h);
@@ -1566,10 +1598,10 @@ int rsx_destroy(rsx_engine* h)
     for (auto* b : f4) b->release();
     h->d_qrev.release();
     DBuf<float>* ff[] = {&h->d_ftab, &h->d_qv, &h->d_r2, &h->d_rext, &h->d_grad,
-                         &h->a_ftab, &h->a_rowpart, &h->a_colpart, &h->a_invq32};
+                         &h->a_ftab, &h->a_rowpart, &h->a_colpart, &h->a_invq32, &h->a_cb, &h->a_cg};
     for (auto* b : ff) b->release();
     DBuf<double>* fd[] = {&h->d_invq, &h->d_partials, &h->d_iq, &h->d_saved_iq, &h->d_out, &h->d_wq, &h->d_q0,
-                          &h->b_partials, &h->b_out, &h->b_mean, &h->b_q0};
+                          &h->d_red, &h->b_partials, &h->b_out, &h->b_mean, &h->b_q0, &h->b_red};
     for (auto* b : fd) b->release();
     DBuf<int>* fi[] = {&h->d_slot_begin, &h->d_idx, &h->d_bucket_of, &h->d_bucket_count, &h->d_bucket_start,
                        &h->d_bucket_fill, &h->d_sorted, &h->d_s2c, &h->b_slot_begin};
@@ -1926,7 +1958,7 @@ int rsx_autotune(rsx_engine* h, const char* cache_path)
 int rsx_set_variant(rsx_engine* h, int variant)
 {
     return guarded(h, [&] {
-        if (variant < 0 || variant >= rsx::VARIANT_COUNT) fail(RSX_INVALID_ARG, "variant must be 0, 1 or 2");
+        if (variant < 0 || variant >= rsx::VARIANT_COUNT) fail(RSX_INVALID_ARG, "variant must be 0, 1, 2 or 3");
         if (variant == h->variant) return;
         h->variant = variant;
         if (h->nq > 0) {  // the K list may differ: lay out again on the next calc_curve
@@ -1966,7 +1998,7 @@ int rsx_time_forward(rsx_engine* h, int reps, float* ms_total, float* ms_pair)
             hipchk(rsx::launch_debye(a, nj, h->K, h->uniform, h->stream), "debye kernel");
             hipchk(hipEventRecord(h->ev3, h->stream), "event");
             hipchk(rsx::launch_reduce(h->d_partials.p, h->ldf, h->nq, h->d_slot_begin_f.p, 1, h->d_iq.p, h->nq, 0,
-                                      h->d_q0.p, h->q0_index, h->stream),
+                                      h->d_q0.p, h->q0_index, nj, h->d_red.p, h->stream),
                    "reduce kernel");
             hipchk(hipEventSynchronize(h->ev3), "event sync");
             float t = 0.0f;

@@ -57,12 +57,13 @@ inline int sin_series_terms(double x_max)
 }
 
 // pair-kernel paths (K lists differ) and recurrence variants
-enum { PATH_DIRECT = 0, PATH_REC64 = 1, PATH_REC32 = 2 };
+enum { PATH_DIRECT = 0, PATH_REC64 = 1, PATH_REC32 = 2, PATH_REC32R = 3 };
 enum {
     VARIANT_WAVE = 0,    // FP64 recurrence, one wave per (job, q chunk)
     VARIANT_SHARED = 1,  // FP64 recurrence, the q chunks of a job share the pair geometry through LDS
     VARIANT_FP32 = 2,    // FP32 difference-form recurrence, one wave per (job, q chunk)
-    VARIANT_COUNT = 3
+    VARIANT_FP32R = 3,   // the same recurrence, each class run reduced over the rows when it ends
+    VARIANT_COUNT = 4
 };
 
 struct DebyeArgs {
@@ -79,7 +80,7 @@ struct DebyeArgs {
     const double* qrev;   // [ldf] q / (2 pi)  (direct path)
     const float* qv;      // [ldf] q           (direct path)
     double* partials;     // [n_slots][ldf]
-    int variant;          // recurrence kernel: VARIANT_WAVE, VARIANT_SHARED or VARIANT_FP32
+    int variant;          // recurrence kernel: VARIANT_WAVE, VARIANT_SHARED, VARIANT_FP32 or VARIANT_FP32R
     int* queue;           // work counter of the persistent FP32 launch (device int), or null
     int persistent_waves; // > 0: persistent launch of this many waves pulling items from queue
     int sin_terms;        // terms the sin(dq r / 2) series needs (sin_series_terms); the launcher
@@ -89,16 +90,23 @@ struct DebyeArgs {
 // Debye pair sums.  uniform: q_k = a + k dq within each chunk (FP64
 // three-term recurrence); otherwise direct FP32 sin per term.
 hipError_t launch_debye(const DebyeArgs& a, int n_jobs, int K, bool uniform, hipStream_t s);
+// Resident one-wave blocks per CU of an FP32 pair kernel (variant VARIANT_FP32 or
+// VARIANT_FP32R, the given K and series length; the persistent work-queue form or the
+// one-wave-per-item form); 0 if none.
+int rec32_blocks_per_cu(int variant, int K, int sin_terms, bool persistent);
 // K values the launcher accepts, ascending.
 int debye_k_count(int path);
 int debye_k_value(int path, int i);
 
 // out[o*out_ld + k] (+)= sum_{s in [slot_begin[o], slot_begin[o+1])} partials[s*ldf + k]
 // in a fixed order.  q0_index >= 0: that k is set to q0_val[o] instead (the
-// exact (sum F)^2 rule for q == 0, added when accumulate != 0).
+// exact (sum F)^2 rule for q == 0, added when accumulate != 0).  max_slots bounds the
+// slots of any output; with scratch (reduce_scratch_doubles of them, null if 0) the
+// slots are summed in chunks by more blocks, then the chunks in order.
 hipError_t launch_reduce(const double* partials, int ldf, int n_q, const int* slot_begin, int n_out,
                          double* out, int out_ld, int accumulate, const double* q0_val, int q0_index,
-                         hipStream_t s);
+                         int max_slots, double* scratch, hipStream_t s);
+size_t reduce_scratch_doubles(int n_q, int n_out, int max_slots);
 
 // out[i] = mean over frames (uniform weights) of in[f*n_q + i]
 hipError_t launch_frame_mean(const double* in, int n_frames, int n_q, double* out, hipStream_t s);
@@ -142,6 +150,8 @@ struct AdjointArgs {
     const double* wq;        // [ldf] w_k = dchi2/dI_k (0 for q == 0 and padding)
     const double* invq;      // [ldf]
     const float* invq32;     // [ldf] the same in FP32 (FP32 kernel)
+    const float* cb;         // [n_class][ldf] 2 w_k F_c,k (FP32 kernel; launch_adjoint_coef)
+    const float* cg;         // [n_class][ldf] 2 w_k F_c,k / q_k
     double dq_rev;           // dq / (2 pi)
     int fp32;                // 1: FP32 difference-form kernel, 0: FP64 recurrence
     int sin_terms;           // series length (sin_series_terms)
@@ -151,6 +161,9 @@ struct AdjointArgs {
 };
 // one wave per (job, q chunk); n_jobs = number of jobs
 hipError_t launch_adjoint(const AdjointArgs& a, int n_jobs, int K, hipStream_t s);
+// cb, cg of the FP32 adjoint from the class factors and the weights
+hipError_t launch_adjoint_coef(const float* ftab, const double* wq, const double* invq, int n_class, int ldf, float* cb,
+                               float* cg, hipStream_t s);
 int adjoint_k_count(bool fp32);
 int adjoint_k_value(bool fp32, int i);
 

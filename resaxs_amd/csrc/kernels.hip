@@ -498,6 +498,43 @@ __device__ __forceinline__ void run2_32(float (&acc)[K], const Seeds32& a, const
     }
 }
 
+// The column sweep of one (job, q chunk) item of the FP32 recurrence: columns in
+// pairs, acc[k] += sin(q_k r) / r over the current class run; flush(cls) is called when
+// the run of class cls ends.  Skip records (the slots of moved atoms during an
+// incremental update) join whatever run they sit in with weight 0 instead of ending it.
+template <int K, int M, typename Flush>
+__device__ __forceinline__ void rec32_columns(const PairJob& jb, const float4 ri, const ChunkConst& cc, float h_dq,
+                                              float dq, float (&acc)[K], Flush&& flush)
+{
+    const int lane = threadIdx.x;
+    const float4* __restrict__ cols = jb.cols;
+    const int ncol = jb.ncol;
+    const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
+    uint32_t cur = CLS_SKIP;
+    int j = 0;
+    while (j < ncol) {
+        const float4 c0 = cols[j];
+        const uint32_t cl0 = __float_as_uint(c0.w);
+        const uint32_t cl1 = j + 1 < ncol ? __float_as_uint(cols[j + 1].w) : CLS_SKIP;
+        const uint32_t cl = cl0 != CLS_SKIP ? cl0 : cl1;  // class of this step (skip if both are)
+        if (cl != CLS_SKIP && cl != cur) {
+            if (cur != CLS_SKIP) flush(cur);
+            cur = cl;
+        }
+        const bool pair = j + 1 < ncol && (cl1 == cl || cl1 == CLS_SKIP);
+        const float4 c1 = pair ? cols[j + 1] : c0;
+        float w0 = cl0 != CLS_SKIP ? 1.0f : 0.0f, w1 = pair && cl1 != CLS_SKIP ? 1.0f : 0.0f;
+        if (j < jd) {
+            w0 *= diag_weight(j, lane);
+            w1 *= diag_weight(j + 1, lane);
+        }
+        run2_32<K>(acc, make_seeds32<M>(pair_r2(ri.x, ri.y, ri.z, c0), cc, h_dq, dq, w0),
+                   make_seeds32<M>(pair_r2(ri.x, ri.y, ri.z, c1), cc, h_dq, dq, w1));
+        j += pair ? 2 : 1;
+    }
+    if (cur != CLS_SKIP) flush(cur);
+}
+
 // one (job, q chunk) work item of the FP32 recurrence; T is the wave's LDS transpose
 template <int K, int M, int KS>
 __device__ __forceinline__ void rec32_item(const DebyeArgs& a, int job_id, int chunk, float (&T)[KS][65])
@@ -509,8 +546,6 @@ __device__ __forceinline__ void rec32_item(const DebyeArgs& a, int job_id, int c
     const uint32_t cls_i = row_ok ? __float_as_uint(ri.w) : CLS_SKIP;
     const int k0 = chunk * K;
     const ChunkConst cc = a.chunks[chunk];
-    const float h_dq = (float)a.half_dq;
-    const float dq = a.dq;
     const float* __restrict__ ftab = a.ftab;
     const int ldf = a.ldf;
 
@@ -521,40 +556,14 @@ __device__ __forceinline__ void rec32_item(const DebyeArgs& a, int job_id, int c
         acc[k] = 0.0f;
         X[k] = 0.0f;
     }
-    auto flush = [&](uint32_t cls) {
+    rec32_columns<K, M>(jb, ri, cc, (float)a.half_dq, a.dq, acc, [&](uint32_t cls) {
         const float* __restrict__ fr = ftab + __builtin_amdgcn_readfirstlane((int)cls * ldf + k0);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             X[k] = fmaf(fr[k], acc[k], X[k]);
             acc[k] = 0.0f;
         }
-    };
-
-    const float4* __restrict__ cols = jb.cols;
-    const int ncol = jb.ncol;
-    const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
-    uint32_t cur = CLS_SKIP;
-    int j = 0;
-    while (j < ncol) {
-        const float4 c0 = cols[j];
-        const uint32_t cl0 = __float_as_uint(c0.w);
-        if (cl0 != cur) {
-            if (cur != CLS_SKIP) flush(cur);
-            cur = cl0;
-        }
-        const bool pair = j + 1 < ncol && __float_as_uint(cols[j + 1].w) == cl0;
-        const float4 c1 = pair ? cols[j + 1] : c0;
-        const float live = cl0 != CLS_SKIP ? 1.0f : 0.0f;
-        float w0 = live, w1 = pair ? live : 0.0f;
-        if (j < jd) {
-            w0 *= diag_weight(j, lane);
-            w1 *= diag_weight(j + 1, lane);
-        }
-        run2_32<K>(acc, make_seeds32<M>(pair_r2(ri.x, ri.y, ri.z, c0), cc, h_dq, dq, w0),
-                   make_seeds32<M>(pair_r2(ri.x, ri.y, ri.z, c1), cc, h_dq, dq, w1));
-        j += pair ? 2 : 1;
-    }
-    if (cur != CLS_SKIP) flush(cur);
+    });
 
     // epilogue as in debye_uniform_kernel, in slices of up to 64 q so the LDS
     // transpose stays 16.6 KB for any K
@@ -573,6 +582,71 @@ __device__ __forceinline__ void rec32_item(const DebyeArgs& a, int job_id, int c
             a.partials[(size_t)jb.slot * ldf + k0 + kb + lane] = sum * a.invq[k0 + kb + lane] * (double)jb.scale;
         }
         __syncthreads();
+    }
+}
+
+// Run-flush form of the FP32 recurrence: no per-lane sums over runs.  Each class run
+// is reduced over the 64 rows when it ends: the lane's row factor times its run sum
+// goes through an LDS transpose in slices of RSLICE q, lanes (k, half) sum 32 rows
+// each, a cross-half shuffle completes the column, and the run's class factor scales
+// it into an FP64 per-q total.  Only the K run accumulators stay in registers, so K
+// reaches 128 at three waves per SIMD; the flush costs ~4K lane operations, which
+// pays off when a column range holds few class runs (type classes).
+constexpr int RSLICE = 32;
+constexpr int FGROUP = 4;
+
+template <int K, int M>
+__device__ __forceinline__ void rec32r_item(const DebyeArgs& a, int job_id, int chunk, float (&T)[RSLICE][65])
+{
+    constexpr int NS = (K + RSLICE - 1) / RSLICE;
+    const PairJob jb = a.jobs[job_id];
+    const int lane = threadIdx.x;
+    const bool row_ok = lane < jb.nrow;
+    const float4 ri = row_ok ? jb.rows[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t cls_i = row_ok ? __float_as_uint(ri.w) : CLS_SKIP;
+    const uint32_t ci = cls_i == CLS_SKIP ? (uint32_t)a.zero_class : cls_i;
+    const int k0 = chunk * K;
+    const ChunkConst cc = a.chunks[chunk];
+    const float* __restrict__ ftab = a.ftab;
+    const int ldf = a.ldf;
+    const float* __restrict__ fi = ftab + (size_t)ci * ldf + k0;
+    const int ks = lane & (RSLICE - 1), half = lane >> 5;
+
+    float acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.0f;
+    double Y[NS];  // lane (ks, half): total of q index s * RSLICE + ks over flushed runs
+#pragma unroll
+    for (int s = 0; s < NS; ++s) Y[s] = 0.0;
+    rec32_columns<K, M>(jb, ri, cc, (float)a.half_dq, a.dq, acc, [&](uint32_t cls) {
+        const float* __restrict__ fr = ftab + __builtin_amdgcn_readfirstlane((int)cls * ldf + k0);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+#pragma unroll
+            for (int k = 0; k < RSLICE; ++k) {
+                // row factors in groups of FGROUP loads: the compiler may not gather a
+                // slice's loads into live registers on top of the K accumulators
+                if (k % FGROUP == 0) asm volatile("" ::: "memory");
+                if (s * RSLICE + k < K) {
+                    T[k][lane] = fi[s * RSLICE + k] * acc[s * RSLICE + k];
+                    acc[s * RSLICE + k] = 0.0f;
+                }
+            }
+            __syncthreads();
+            float sum = 0.0f;
+#pragma unroll 8
+            for (int t = 0; t < 32; ++t) sum += T[ks][half * 32 + t];
+            sum += __shfl_xor(sum, 32, 64);
+            if (s * RSLICE + ks < K) Y[s] = fma((double)fr[s * RSLICE + ks], (double)sum, Y[s]);
+            __syncthreads();
+        }
+    });
+    if (half == 0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int k = s * RSLICE + ks;
+            if (k < K) a.partials[(size_t)jb.slot * ldf + k0 + k] = Y[s] * a.invq[k0 + k] * (double)jb.scale;
+        }
     }
 }
 
@@ -601,6 +675,30 @@ __global__ __launch_bounds__(64, 2) void debye_rec32_persistent(const DebyeArgs 
         if (item >= n_items) break;
         const int job_id = item / a.n_chunks;
         rec32_item<K, M, KS>(a, job_id, item - job_id * a.n_chunks, T);
+    }
+}
+
+// three resident waves per SIMD: K = 128 fits in 168 registers
+template <int K, int M>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void debye_rec32r_kernel(const DebyeArgs a)
+{
+    __shared__ float T[RSLICE][65];
+    const int job_id = blockIdx.x / a.n_chunks;
+    rec32r_item<K, M>(a, job_id, blockIdx.x - job_id * a.n_chunks, T);
+}
+
+// (the work-queue loop needs a few more registers: K = 128 runs at two waves per SIMD)
+template <int K, int M>
+__global__ __launch_bounds__(64, 2) void debye_rec32r_persistent(const DebyeArgs a, int n_items)
+{
+    __shared__ float T[RSLICE][65];
+    for (;;) {
+        int item = 0;
+        if (threadIdx.x == 0) item = atomicAdd(a.queue, 1);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
+        if (item >= n_items) break;
+        const int job_id = item / a.n_chunks;
+        rec32r_item<K, M>(a, job_id, item - job_id * a.n_chunks, T);
     }
 }
 
@@ -733,6 +831,47 @@ __global__ __launch_bounds__(64 * RED_GROUPS) void reduce_kernel(const double* _
         if (k == q0_index) tot = q0_val[o];
         *dst = accumulate ? *dst + tot : tot;
     }
+}
+
+// Two-stage form of the fixed-order reduction for many slots: stage 1 sums slot chunk z
+// of output o into scratch[(o * nsplit + z) * n_q + k] (16 interleaved groups of the
+// chunk, then the groups in order); stage 2 sums the chunks in order.
+__global__ __launch_bounds__(64 * RED_GROUPS) void reduce_part_kernel(const double* __restrict__ partials, int ldf,
+                                                                    int n_q, const int* __restrict__ slot_begin,
+                                                                    int chunk, int nsplit, double* scratch)
+{
+    __shared__ double part[RED_GROUPS][64];
+    const int o = blockIdx.y, z = blockIdx.z;
+    const int kl = threadIdx.x & 63;
+    const int g = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + kl;
+    const int sb = slot_begin[o], se = slot_begin[o + 1];
+    const int s0 = sb + z * chunk;
+    const int s1 = s0 + chunk < se ? s0 + chunk : se;
+    double acc = 0.0;
+    if (k < n_q)
+        for (int s = s0 + g; s < s1; s += RED_GROUPS) acc += partials[(size_t)s * ldf + k];
+    part[g][kl] = acc;
+    __syncthreads();
+    if (g == 0 && k < n_q) {
+        double tot = 0.0;
+#pragma unroll
+        for (int t = 0; t < RED_GROUPS; ++t) tot += part[t][kl];
+        scratch[((size_t)o * nsplit + z) * n_q + k] = tot;
+    }
+}
+
+__global__ void reduce_final_kernel(const double* __restrict__ scratch, int nsplit, int n_q, double* out, int out_ld,
+                                    int accumulate, const double* __restrict__ q0_val, int q0_index)
+{
+    const int o = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_q) return;
+    double tot = 0.0;
+    for (int z = 0; z < nsplit; ++z) tot += scratch[((size_t)o * nsplit + z) * n_q + k];
+    double* dst = out + (size_t)o * out_ld + k;
+    if (k == q0_index) tot = q0_val[o];
+    *dst = accumulate ? *dst + tot : tot;
 }
 
 __global__ void frame_mean_kernel(const double* __restrict__ in, int n_frames, int n_q, double* out)
@@ -930,6 +1069,9 @@ __global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
 // ---------------------------------------------------------------------------
 constexpr int ADJ_TC = 8;  // columns per LDS reduction tile
 
+// uniform tables read through the constant address space (scalar loads)
+typedef const __attribute__((address_space(4))) float cfloat;
+
 template <int K, int M>
 __global__ __launch_bounds__(64, 2) void adjoint_tri_kernel(const AdjointArgs a)
 {
@@ -1036,7 +1178,8 @@ __global__ __launch_bounds__(64, 2) void adjoint_tri_kernel(const AdjointArgs a)
 // weighted sums.  G and the per-row and per-column gradient sums are FP32 too; the
 // fixed-order reduction over jobs runs in FP64.
 template <int K, int M>
-__global__ __launch_bounds__(64, 2) void adjoint32_tri_kernel(const AdjointArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 67 ? 3 : 2))) void adjoint32_tri_kernel(
+    const AdjointArgs a)
 {
     const int job_id = blockIdx.x / a.n_chunks;
     const int chunk = blockIdx.x - job_id * a.n_chunks;
@@ -1074,20 +1217,28 @@ __global__ __launch_bounds__(64, 2) void adjoint32_tri_kernel(const AdjointArgs 
     uint32_t cur = CLS_SKIP;
     for (int t0 = 0; t0 < ncol; t0 += ADJ_TC) {
         const int tn = ncol - t0 < ADJ_TC ? ncol - t0 : ADJ_TC;
+#pragma unroll 1
         for (int t = 0; t < tn; ++t) {
             const int j = t0 + t;
             const float4 c = cols[j];
             const uint32_t cl = __float_as_uint(c.w);
             if (cl != cur) {
                 cur = cl;
-                const float* __restrict__ fr = ftab + __builtin_amdgcn_readfirstlane((int)cl * ldf + k0);
-                const float* __restrict__ iq = a.invq32 + k0;
+                // b_k = F_i,k cb_k with cb_k = 2 w_k F_class,k and gamma_k = F_i,k cg_k,
+                // cg_k = cb_k / q_k: the class tables come through scalar loads (uniform
+                // address in the constant space) and the row factors in groups of 8, so
+                // only beta and gamma stay in vector registers
+                const int off = __builtin_amdgcn_readfirstlane((int)cl * ldf + k0);
+                cfloat* cbp = (cfloat*)(a.cb + off);
+                cfloat* cgp = (cfloat*)(a.cg + off);
                 float prev = 0.0f;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const float bk = (float)(2.0 * a.wq[k0 + k]) * fi[k] * fr[k];
+                    if (k % 8 == 0) asm volatile("" ::: "memory");
+                    const float f = fi[k];
+                    const float bk = f * cbp[k];
                     beta[k] = bk + prev;
-                    gamma[k] = bk * iq[k];
+                    gamma[k] = f * cgp[k];
                     prev = bk;
                 }
                 beta[K] = prev;
@@ -1143,6 +1294,19 @@ __global__ __launch_bounds__(64, 2) void adjoint32_tri_kernel(const AdjointArgs 
     rowpart[2] = gz;
 }
 
+// per-class adjoint coefficients of the FP32 kernel: cb[c][k] = 2 w_k F_c,k and
+// cg[c][k] = cb[c][k] / q_k, formed in FP64 and rounded once
+__global__ void adjoint_coef_kernel(const float* __restrict__ ftab, const double* __restrict__ wq,
+                                    const double* __restrict__ invq, int n, float* cb, float* cg, int ldf)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int k = t % ldf;
+    const double b = 2.0 * wq[k] * (double)ftab[t];
+    cb[t] = (float)b;
+    cg[t] = (float)(b * invq[k]);
+}
+
 // grad of slot s = its row partials (jobs of its row block) + its column partials
 // (one job per row block at or before its own), over every q chunk, in a fixed order
 __global__ void adjoint_reduce_kernel(AdjointReduceArgs a)
@@ -1186,22 +1350,28 @@ constexpr int UNIFORM_KS[] = {8, 16, 24, 32, 40, 48, 50, 56, 64, 67};
 // besides powers of two: K that split common grids into equal chunks with little
 // padding (256 = 3 x 86 - 2, 200 = 2 x 100, 101 <= 2 x 51, 200 <= 3 x 67)
 constexpr int REC32_KS[] = {16, 32, 43, 48, 51, 64, 67, 80, 86, 96, 100};
+// run-flush form: only the run accumulators live in registers, so K reaches 128 (256 = 2 x 128)
+constexpr int REC32R_KS[] = {64, 86, 96, 100, 128};
 constexpr int DIRECT_KS[] = {8, 16, 32, 48, 64};
 constexpr int ADJOINT_KS[] = {8, 16, 32, 48, 64};
-constexpr int ADJOINT32_KS[] = {16, 32, 40, 50, 64};
+constexpr int ADJOINT32_KS[] = {16, 32, 40, 50, 64, 67};
 
 }  // namespace
 
 int debye_k_count(int path)
 {
-    return path == PATH_DIRECT  ? (int)(sizeof(DIRECT_KS) / sizeof(int))
-           : path == PATH_REC32 ? (int)(sizeof(REC32_KS) / sizeof(int))
-                                : (int)(sizeof(UNIFORM_KS) / sizeof(int));
+    return path == PATH_DIRECT    ? (int)(sizeof(DIRECT_KS) / sizeof(int))
+           : path == PATH_REC32   ? (int)(sizeof(REC32_KS) / sizeof(int))
+           : path == PATH_REC32R  ? (int)(sizeof(REC32R_KS) / sizeof(int))
+                                  : (int)(sizeof(UNIFORM_KS) / sizeof(int));
 }
 
 int debye_k_value(int path, int i)
 {
-    return path == PATH_DIRECT ? DIRECT_KS[i] : path == PATH_REC32 ? REC32_KS[i] : UNIFORM_KS[i];
+    return path == PATH_DIRECT    ? DIRECT_KS[i]
+           : path == PATH_REC32   ? REC32_KS[i]
+           : path == PATH_REC32R  ? REC32R_KS[i]
+                                  : UNIFORM_KS[i];
 }
 
 int adjoint_k_count(bool fp32)
@@ -1286,7 +1456,84 @@ void launch_rec32_k(const DebyeArgs& a, int n_jobs, hipStream_t s)
         hipLaunchKernelGGL((debye_rec32_kernel<K, SIN_MAX_TERMS>), grid, block, 0, s, a);
 }
 
+template <int K>
+void launch_rec32r_k(const DebyeArgs& a, int n_jobs, hipStream_t s)
+{
+    const int n_items = n_jobs * a.n_chunks;
+    if (a.queue && a.persistent_waves > 0 && a.persistent_waves < n_items) {
+        hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, s, a.queue, 0, 1LL);
+        const dim3 grid((unsigned)a.persistent_waves), block(64);
+        if (a.sin_terms <= SIN_SHORT)
+            hipLaunchKernelGGL((debye_rec32r_persistent<K, SIN_SHORT>), grid, block, 0, s, a, n_items);
+        else
+            hipLaunchKernelGGL((debye_rec32r_persistent<K, SIN_MAX_TERMS>), grid, block, 0, s, a, n_items);
+        return;
+    }
+    const dim3 grid((unsigned)n_items), block(64);
+    if (a.sin_terms <= SIN_SHORT)
+        hipLaunchKernelGGL((debye_rec32r_kernel<K, SIN_SHORT>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((debye_rec32r_kernel<K, SIN_MAX_TERMS>), grid, block, 0, s, a);
+}
+
 }  // namespace
+
+namespace {
+
+template <int K, bool RUN_FLUSH>
+int rec32_blocks_per_cu(int sin_terms, bool persistent)
+{
+    int nb = 0;
+    hipError_t e;
+    const bool sh = sin_terms <= SIN_SHORT;
+    if constexpr (RUN_FLUSH) {
+        if (persistent)
+            e = sh ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, debye_rec32r_persistent<K, SIN_SHORT>, 64, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, debye_rec32r_persistent<K, SIN_MAX_TERMS>, 64, 0);
+        else
+            e = sh ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, debye_rec32r_kernel<K, SIN_SHORT>, 64, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, debye_rec32r_kernel<K, SIN_MAX_TERMS>, 64, 0);
+    } else {
+        if (persistent)
+            e = sh ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, debye_rec32_persistent<K, SIN_SHORT>, 64, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, debye_rec32_persistent<K, SIN_MAX_TERMS>, 64, 0);
+        else
+            e = sh ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, debye_rec32_kernel<K, SIN_SHORT>, 64, 0)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, debye_rec32_kernel<K, SIN_MAX_TERMS>, 64, 0);
+    }
+    return e == hipSuccess ? nb : 0;
+}
+
+}  // namespace
+
+int rec32_blocks_per_cu(int variant, int K, int sin_terms, bool persistent)
+{
+    if (variant == VARIANT_FP32R) {
+        switch (K) {
+        case 64: return rec32_blocks_per_cu<64, true>(sin_terms, persistent);
+        case 86: return rec32_blocks_per_cu<86, true>(sin_terms, persistent);
+        case 96: return rec32_blocks_per_cu<96, true>(sin_terms, persistent);
+        case 100: return rec32_blocks_per_cu<100, true>(sin_terms, persistent);
+        case 128: return rec32_blocks_per_cu<128, true>(sin_terms, persistent);
+        default: return 0;
+        }
+    }
+    if (variant != VARIANT_FP32) return 0;
+    switch (K) {
+    case 16: return rec32_blocks_per_cu<16, false>(sin_terms, persistent);
+    case 32: return rec32_blocks_per_cu<32, false>(sin_terms, persistent);
+    case 43: return rec32_blocks_per_cu<43, false>(sin_terms, persistent);
+    case 48: return rec32_blocks_per_cu<48, false>(sin_terms, persistent);
+    case 51: return rec32_blocks_per_cu<51, false>(sin_terms, persistent);
+    case 64: return rec32_blocks_per_cu<64, false>(sin_terms, persistent);
+    case 67: return rec32_blocks_per_cu<67, false>(sin_terms, persistent);
+    case 80: return rec32_blocks_per_cu<80, false>(sin_terms, persistent);
+    case 86: return rec32_blocks_per_cu<86, false>(sin_terms, persistent);
+    case 96: return rec32_blocks_per_cu<96, false>(sin_terms, persistent);
+    case 100: return rec32_blocks_per_cu<100, false>(sin_terms, persistent);
+    default: return 0;
+    }
+}
 
 hipError_t launch_debye(const DebyeArgs& a, int n_jobs, int K, bool uniform, hipStream_t s)
 {
@@ -1304,6 +1551,15 @@ hipError_t launch_debye(const DebyeArgs& a, int n_jobs, int K, bool uniform, hip
         case 86: launch_rec32_k<86>(a, n_jobs, s); break;
         case 96: launch_rec32_k<96>(a, n_jobs, s); break;
         case 100: launch_rec32_k<100>(a, n_jobs, s); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else if (uniform && a.variant == VARIANT_FP32R) {
+        switch (K) {
+        case 64: launch_rec32r_k<64>(a, n_jobs, s); break;
+        case 86: launch_rec32r_k<86>(a, n_jobs, s); break;
+        case 96: launch_rec32r_k<96>(a, n_jobs, s); break;
+        case 100: launch_rec32r_k<100>(a, n_jobs, s); break;
+        case 128: launch_rec32r_k<128>(a, n_jobs, s); break;
         default: return hipErrorInvalidValue;
         }
     } else if (uniform) {
@@ -1334,10 +1590,41 @@ hipError_t launch_debye(const DebyeArgs& a, int n_jobs, int K, bool uniform, hip
     return hipGetLastError();
 }
 
+namespace {
+
+// slot chunks per output: enough blocks to spread the reads over the CUs, at least
+// 256 slots per chunk
+int reduce_splits(int n_q, int n_out, int max_slots)
+{
+    const int base = ((n_q + 63) / 64) * (n_out > 0 ? n_out : 1);
+    const int by_blocks = (1024 + base - 1) / base;
+    const int by_slots = (max_slots + 255) / 256;
+    const int n = by_blocks < by_slots ? by_blocks : by_slots;
+    return n > 1 ? n : 1;
+}
+
+}  // namespace
+
+size_t reduce_scratch_doubles(int n_q, int n_out, int max_slots)
+{
+    const int ns = reduce_splits(n_q, n_out, max_slots);
+    return ns > 1 ? (size_t)ns * (size_t)n_out * (size_t)n_q : 0;
+}
+
 hipError_t launch_reduce(const double* partials, int ldf, int n_q, const int* slot_begin, int n_out, double* out,
-                         int out_ld, int accumulate, const double* q0_val, int q0_index, hipStream_t s)
+                         int out_ld, int accumulate, const double* q0_val, int q0_index, int max_slots,
+                         double* scratch, hipStream_t s)
 {
     if (n_out <= 0 || n_q <= 0) return hipSuccess;
+    const int ns = reduce_splits(n_q, n_out, max_slots);
+    if (ns > 1 && scratch) {
+        const int chunk = (max_slots + ns - 1) / ns;
+        hipLaunchKernelGGL(reduce_part_kernel, dim3((unsigned)((n_q + 63) / 64), (unsigned)n_out, (unsigned)ns),
+                           dim3(64 * RED_GROUPS), 0, s, partials, ldf, n_q, slot_begin, chunk, ns, scratch);
+        hipLaunchKernelGGL(reduce_final_kernel, dim3((unsigned)((n_q + 63) / 64), (unsigned)n_out), dim3(64), 0, s,
+                           scratch, ns, n_q, out, out_ld, accumulate, q0_val, q0_index);
+        return hipGetLastError();
+    }
     const dim3 grid((unsigned)((n_q + 63) / 64), (unsigned)n_out), block(64 * RED_GROUPS);
     hipLaunchKernelGGL(reduce_kernel, grid, block, 0, s, partials, ldf, n_q, slot_begin, out, out_ld, accumulate,
                        q0_val, q0_index);
@@ -1426,6 +1713,16 @@ void launch_adjoint32_k(const AdjointArgs& a, int n_jobs, hipStream_t s)
 
 }  // namespace
 
+hipError_t launch_adjoint_coef(const float* ftab, const double* wq, const double* invq, int n_class, int ldf, float* cb,
+                               float* cg, hipStream_t s)
+{
+    const int n = n_class * ldf;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(adjoint_coef_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ftab, wq, invq, n, cb, cg,
+                       ldf);
+    return hipGetLastError();
+}
+
 hipError_t launch_adjoint(const AdjointArgs& a, int n_jobs, int K, hipStream_t s)
 {
     if (n_jobs <= 0) return hipSuccess;
@@ -1436,6 +1733,7 @@ hipError_t launch_adjoint(const AdjointArgs& a, int n_jobs, int K, hipStream_t s
         case 40: launch_adjoint32_k<40>(a, n_jobs, s); break;
         case 50: launch_adjoint32_k<50>(a, n_jobs, s); break;
         case 64: launch_adjoint32_k<64>(a, n_jobs, s); break;
+        case 67: launch_adjoint32_k<67>(a, n_jobs, s); break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();

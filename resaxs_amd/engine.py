@@ -100,7 +100,9 @@ class Engine:
 
     def set_variant(self, variant):
         """0: FP64 recurrence, one wave per job x q chunk; 1: FP64, shared pair geometry;
-        2: FP32 difference-form recurrence (default)."""
+        2: FP32 difference-form recurrence with per-lane sums over class runs; 3: the FP32
+        recurrence with each column class run reduced over the rows as it ends (K up to
+        128; default)."""
         self._check(self._L.rsx_set_variant(self.h, int(variant)))
 
     def get_tuning(self):

@@ -11,7 +11,7 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import engine_for, oracle_iq, rel_err
+from helpers import engine_for, fib_dot_table, oracle_iq, r2_table, rel_err
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
@@ -74,9 +74,11 @@ def test_config1_plain(oracle):
 
 KS_FP64 = [8, 16, 24, 32, 40, 48, 50, 56, 64, 67]
 KS_FP32 = [16, 32, 43, 48, 51, 64, 67, 80, 86, 96, 100]
+KS_FP32R = [64, 86, 96, 100, 128]
 
 
-@pytest.mark.parametrize("variant,k", [(0, k) for k in KS_FP64] + [(1, 32), (1, 64)] + [(2, k) for k in KS_FP32])
+@pytest.mark.parametrize("variant,k", [(0, k) for k in KS_FP64] + [(1, 32), (1, 64)] + [(2, k) for k in KS_FP32]
+                         + [(3, k) for k in KS_FP32R])
 def test_k_variants_agree(oracle, variant, k):
     """Every kernel variant and q-per-thread instantiation gives the oracle's I(q)."""
     from resaxs_amd import configs
@@ -138,7 +140,7 @@ def test_coincident_atoms_and_tiny_n(oracle):
     assert rel_err(I, f1 * f1 + f2 * f2 + 2 * f1 * f2 * sinc).max() < TOL
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_coincident_atoms_off_diagonal(oracle, variant):
     """Coincident atoms far apart in index (off the diagonal tiles) in both pair-kernel
     variants: the zero-distance pair contributes F_i F_j at every q."""
@@ -157,6 +159,26 @@ def test_coincident_atoms_off_diagonal(oracle, variant):
     I = e.calc_curve(configs.bodies_of(pos, types))
     ref = oracle_iq(q, pos, types, table)
     assert rel_err(I, ref).max() < TOL
+
+
+@pytest.mark.parametrize("variant,k", [(2, 86), (3, 86), (3, 128)])
+def test_fp32_variants_hydrated(oracle, variant, k):
+    """Hydration classes (type x exposed-dot count) give many class runs per column
+    range: every run flush of both FP32 kernels against the FP64 oracle."""
+    from resaxs_amd import configs
+    n = 4000
+    w = configs.config2(n=n)
+    _say(f"case: hydrated 4000 x 256, variant {variant} K={k}")
+    e = engine_for(w.q, w.table, w.water_index, w.w, n=n, radii=w.radii, probe=w.probe, n_half=w.n_half)
+    e.set_variant(variant)
+    e.set_tuning(k, 512)
+    I = e.calc_curve(w.bodies)
+    dots = fib_dot_table(w.radii, w.probe, w.n_half)
+    counts = oracle.sasa_counts(w.pos, w.types, r2_table(w.radii, w.probe), dots)
+    ref = oracle_iq(w.q, w.pos, w.types, w.table, w.water_index, w.w, counts, 256)
+    err = rel_err(I, ref)
+    _say(f"  max rel err {err.max():.3e}")
+    assert err.max() < TOL
 
 
 def test_invariances(oracle):
