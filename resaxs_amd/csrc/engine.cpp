@@ -292,7 +292,9 @@ struct rsx_engine {
     bool hydrate = false;
 
     // device buffers
-    DBuf<float4> d_cols, d_saved_cols, d_rows, d_vals, d_sasa_atoms;
+    DBuf<float4> d_cols, d_saved_cols, d_rows, d_vals, d_sasa_atoms, d_sorted_atoms;
+    bool sasa_mirror = false;        // d_sasa_atoms holds exactly the host bodies
+    DBuf<int> d_subset;              // atom ids of a local SASA recount
     DBuf<float> d_ftab, d_qv, d_r2, d_rext;
     DBuf<double> d_qrev;
     DBuf<ChunkConst> d_chunks;
@@ -609,8 +611,9 @@ void upload_sasa_tables(rsx_engine* h)
            "upload dots");
 }
 
-// counts for n atoms already on the device in d_sasa_atoms
-void run_sasa_device(rsx_engine* h, int n)
+// counts for n atoms already on the device in d_sasa_atoms; with a subset (device list
+// of n_subset atom ids) only those atoms' counts are written
+void run_sasa_device(rsx_engine* h, int n, const int* subset = nullptr, int n_subset = 0)
 {
     float rmax = 0.0f;
     for (float v : h->rext) rmax = std::max(rmax, v);
@@ -621,6 +624,7 @@ void run_sasa_device(rsx_engine* h, int n)
     h->d_bucket_start.ensure(nb + 1);
     h->d_bucket_fill.ensure(nb);
     h->d_sorted.ensure(n);
+    h->d_sorted_atoms.ensure(n);
     h->d_counts.ensure(n);
     rsx::SasaArgs a;
     a.atoms = h->d_sasa_atoms.p;
@@ -637,7 +641,10 @@ void run_sasa_device(rsx_engine* h, int n)
     a.bucket_start = h->d_bucket_start.p;
     a.bucket_fill = h->d_bucket_fill.p;
     a.sorted = h->d_sorted.p;
+    a.sorted_atoms = h->d_sorted_atoms.p;
     a.counts = h->d_counts.p;
+    a.subset = subset;
+    a.n_subset = n_subset;
     hipchk(rsx::launch_sasa(a, h->stream), "sasa kernels");
 }
 
@@ -650,6 +657,7 @@ void compute_sasa(rsx_engine* h)
     h->d_sasa_atoms.ensure(n);
     hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, h->bodies.data(), (size_t)n * 16, hipMemcpyHostToDevice, h->stream),
            "upload sasa atoms");
+    h->sasa_mirror = true;
     run_sasa_device(h, n);
     h->counts.resize(n);
     hipchk(hipMemcpyAsync(h->counts.data(), h->d_counts.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream),
@@ -658,6 +666,62 @@ void compute_sasa(rsx_engine* h)
     h->rsasa.resize(n);
     for (int i = 0; i < n; ++i) h->rsasa[i] = (float)h->counts[i] / (float)h->n_dots;
     h->have_rsasa = true;
+}
+
+// Recount after bodies [start, start+len) moved (old positions in old_bodies): only an
+// atom within reach (2 max(r + p) + 0.02) of a moved atom's old or new position can
+// change its count, so the atoms inside the moved set's bounding box widened by that
+// reach are recounted against the full, updated cell list; every other count is
+// unchanged by construction.  Returns false (nothing done) where the box holds more
+// than half the atoms: the full recount is then cheaper.
+bool compute_sasa_local(rsx_engine* h, int start, int len, const std::vector<float>& old_bodies)
+{
+    const int n = h->n;
+    if (len <= 0 || (int)h->counts.size() != n || !h->have_rsasa) return false;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int a = 0; a < len; ++a)
+        for (int d = 0; d < 3; ++d) {
+            const float vo = old_bodies[4 * (size_t)a + d], vn = h->bodies[4 * (size_t)(start + a) + d];
+            lo[d] = std::min(lo[d], std::min(vo, vn));
+            hi[d] = std::max(hi[d], std::max(vo, vn));
+        }
+    float rmax = 0.0f;
+    for (float v : h->rext) rmax = std::max(rmax, v);
+    const float reach = 2.0f * rmax + 0.02f;
+    std::vector<int> sub;
+    sub.reserve(4096);
+    for (int i = 0; i < n; ++i) {
+        const float* b = &h->bodies[4 * (size_t)i];
+        if ((i >= start && i < start + len) ||
+            (b[0] >= lo[0] - reach && b[0] <= hi[0] + reach && b[1] >= lo[1] - reach && b[1] <= hi[1] + reach &&
+             b[2] >= lo[2] - reach && b[2] <= hi[2] + reach))
+            sub.push_back(i);
+    }
+    if (2 * sub.size() > (size_t)n) return false;
+    for (int i : sub)
+        if (h->type[i] >= h->radii.size()) fail(RSX_INVALID_ARG, "body type index outside the SASA radius table");
+    h->d_sasa_atoms.ensure(n);
+    if (h->sasa_mirror)
+        hipchk(hipMemcpyAsync(h->d_sasa_atoms.p + start, &h->bodies[4 * (size_t)start], (size_t)len * 16,
+                              hipMemcpyHostToDevice, h->stream),
+               "upload moved sasa atoms");
+    else
+        hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, h->bodies.data(), (size_t)n * 16, hipMemcpyHostToDevice, h->stream),
+               "upload sasa atoms");
+    h->sasa_mirror = true;
+    h->d_subset.ensure(sub.size());
+    hipchk(hipMemcpyAsync(h->d_subset.p, sub.data(), sub.size() * sizeof(int), hipMemcpyHostToDevice, h->stream),
+           "upload subset");
+    run_sasa_device(h, n, h->d_subset.p, (int)sub.size());
+    std::vector<uint32_t> cnt(n);
+    hipchk(hipMemcpyAsync(cnt.data(), h->d_counts.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream),
+           "download counts");
+    hipchk(hipStreamSynchronize(h->stream), "sync sasa");
+    for (int i : sub) {
+        h->counts[i] = cnt[i];
+        h->rsasa[i] = (float)cnt[i] / (float)h->n_dots;
+    }
+    return true;
 }
 
 // ---- pair jobs ---------------------------------------------------------------
@@ -895,7 +959,7 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
     std::vector<uint32_t> old_counts;
     if (sasa_now) {
         old_counts = h->counts;
-        compute_sasa(h);
+        if (!compute_sasa_local(h, start, len, old_bodies)) compute_sasa(h);
     }
     // changed set: moved atoms plus atoms whose class changed
     std::vector<int> M;
@@ -1009,6 +1073,7 @@ void revert_impl(rsx_engine* h)
     if (!h->xaction) return;
     set_device(h);
     const bool layout_changed = h->saved.caller_of != h->caller_of;
+    h->sasa_mirror = false;  // the device copy may hold the reverted positions
     h->bodies = h->saved.bodies;
     h->counts = h->saved.counts;
     h->cls = h->saved.cls;
@@ -1056,6 +1121,7 @@ void calc_rsasa_impl(rsx_engine* h, const float* bodies, int n, const float* rad
     upload_sasa_tables(h);
     if (n > 0) {
         h->d_sasa_atoms.ensure(n);
+        h->sasa_mirror = false;
         hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, bodies, (size_t)n * 16, hipMemcpyHostToDevice, h->stream),
                "upload atoms");
         run_sasa_device(h, n);
@@ -1388,6 +1454,7 @@ void enqueue_device_forward(rsx_engine* h, const float4* bodies_dev, double* out
     const uint32_t* counts = nullptr;
     if (h->hydrate) {
         h->d_sasa_atoms.ensure(n);
+        h->sasa_mirror = false;
         hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, bodies_dev, (size_t)n * 16, hipMemcpyDeviceToDevice, h->stream),
                "copy atoms");
         run_sasa_device(h, n);
@@ -1594,7 +1661,8 @@ int rsx_destroy(rsx_engine* h)
     hipSetDevice(h->device);
     batch_release_graph(h);
     if (h->stream) hipStreamSynchronize(h->stream);
-    DBuf<float4>* f4[] = {&h->d_cols, &h->d_saved_cols, &h->d_rows, &h->d_vals, &h->d_sasa_atoms, &h->d_dots, &h->b_cols};
+    DBuf<float4>* f4[] = {&h->d_cols, &h->d_saved_cols, &h->d_rows, &h->d_vals, &h->d_sasa_atoms, &h->d_dots, &h->b_cols,
+                          &h->d_sorted_atoms};
     for (auto* b : f4) b->release();
     h->d_qrev.release();
     DBuf<float>* ff[] = {&h->d_ftab, &h->d_qv, &h->d_r2, &h->d_rext, &h->d_grad,
@@ -1604,7 +1672,7 @@ int rsx_destroy(rsx_engine* h)
                           &h->d_red, &h->b_partials, &h->b_out, &h->b_mean, &h->b_q0, &h->b_red};
     for (auto* b : fd) b->release();
     DBuf<int>* fi[] = {&h->d_slot_begin, &h->d_idx, &h->d_bucket_of, &h->d_bucket_count, &h->d_bucket_start,
-                       &h->d_bucket_fill, &h->d_sorted, &h->d_s2c, &h->b_slot_begin};
+                       &h->d_bucket_fill, &h->d_sorted, &h->d_s2c, &h->b_slot_begin, &h->d_subset};
     for (auto* b : fi) b->release();
     h->d_chunks.release();
     h->a_chunks.release();

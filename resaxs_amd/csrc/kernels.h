@@ -134,7 +134,10 @@ struct SasaArgs {
     int* bucket_start;       // [n_buckets + 1]
     int* bucket_fill;        // [n_buckets]
     int* sorted;             // [n] atom ids grouped by bucket
-    uint32_t* counts;        // [n] exposed dots
+    float4* sorted_atoms;    // [n] the atoms in bucket order
+    uint32_t* counts;        // [n] exposed dots (only the listed atoms when subset is set)
+    const int* subset;       // null: count every atom; else count atoms subset[0 .. n_subset)
+    int n_subset;
 };
 hipError_t launch_sasa(const SasaArgs& a, hipStream_t s);
 

@@ -959,38 +959,36 @@ __global__ void sasa_bin_kernel(SasaArgs a)
 }
 
 // exclusive scan of bucket_count into bucket_start (one workgroup of 1024)
+// exclusive scan of the bucket counts by one workgroup: each thread sums a contiguous
+// chunk, the 1024 chunk totals are scanned across the block, and each thread writes
+// its chunk's starts
 __global__ __launch_bounds__(1024) void sasa_scan_kernel(SasaArgs a)
 {
-    __shared__ int warp_tot[16];
-    __shared__ int carry;
-    const int tid = threadIdx.x;
-    if (tid == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < a.n_buckets; base += 1024) {
-        const int idx = base + tid;
-        const int v = idx < a.n_buckets ? a.bucket_count[idx] : 0;
-        // inclusive scan within the wave
-        int x = v;
-        const int lane = tid & 63, w = tid >> 6;
+    __shared__ int wave_tot[16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int per = (a.n_buckets + 1023) / 1024;
+    const int b0 = tid * per < a.n_buckets ? tid * per : a.n_buckets;
+    const int b1 = b0 + per < a.n_buckets ? b0 + per : a.n_buckets;
+    int sum = 0;
+    for (int b = b0; b < b1; ++b) sum += a.bucket_count[b];
+    int x = sum;  // inclusive scan within the wave
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int y = __shfl_up(x, d, 64);
-            if (lane >= d) x += y;
-        }
-        if (lane == 63) warp_tot[w] = x;
-        __syncthreads();
-        int off = 0;
-        for (int t = 0; t < w; ++t) off += warp_tot[t];
-        const int excl = carry + off + x - v;
-        if (idx < a.n_buckets) {
-            a.bucket_start[idx] = excl;
-            a.bucket_fill[idx] = excl;
-        }
-        __syncthreads();
-        if (tid == 1023) carry = excl + v;
-        __syncthreads();
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
     }
-    if (tid == 0) a.bucket_start[a.n_buckets] = carry;
+    if (lane == 63) wave_tot[w] = x;
+    __syncthreads();
+    int off = 0;
+    for (int t = 0; t < w; ++t) off += wave_tot[t];
+    int run = off + x - sum;
+    for (int b = b0; b < b1; ++b) {
+        const int c = a.bucket_count[b];
+        a.bucket_start[b] = run;
+        a.bucket_fill[b] = run;
+        run += c;
+    }
+    if (tid == 1023) a.bucket_start[a.n_buckets] = off + x;
 }
 
 __global__ void sasa_place_kernel(SasaArgs a)
@@ -999,57 +997,63 @@ __global__ void sasa_place_kernel(SasaArgs a)
     if (i >= a.n) return;
     const int pos = atomicAdd(&a.bucket_fill[a.bucket_of[i]], 1);
     a.sorted[pos] = i;
+    a.sorted_atoms[pos] = a.atoms[i];
 }
 
 constexpr int SASA_MAX_DPL = 8;  // dots per lane (n_dots <= 512)
 
-#pragma clang fp contract(off)
+// One wave per atom (or per listed atom), DPL dots per lane.  Candidates come from the
+// 27 neighbour cells in bucket order (positions copied there by the place kernel, one
+// load per candidate).  A dot once occluded stays occluded, so the wave stops scanning
+// once no lane holds a live dot (checked per cell): the count is 0 either way.
+template <int DPL>
 __global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
 {
-    const int i = blockIdx.x;
+    const int i = a.subset ? a.subset[blockIdx.x] : (int)blockIdx.x;
     const int lane = threadIdx.x;
     const float4 pi = a.atoms[i];
     const uint32_t ti = __float_as_uint(pi.w);
     const float rext_i = a.rext[ti];
-    const int dpl = (a.n_dots + 63) / 64;
-    float px[SASA_MAX_DPL], py[SASA_MAX_DPL], pz[SASA_MAX_DPL];
-    bool live[SASA_MAX_DPL];
+    float px[DPL], py[DPL], pz[DPL];
+    bool live[DPL];
 #pragma unroll
-    for (int t = 0; t < SASA_MAX_DPL; ++t) {
+    for (int t = 0; t < DPL; ++t) {
         const int d = lane + 64 * t;
-        live[t] = t < dpl && d < a.n_dots;
+        live[t] = d < a.n_dots;
         const float4 tp = live[t] ? a.dots[(size_t)ti * a.n_dots + d] : make_float4(0.f, 0.f, 0.f, 0.f);
         px[t] = tp.x + pi.x;
         py[t] = tp.y + pi.y;
         pz[t] = tp.z + pi.z;
     }
     const int cx = cell_coord(pi.x, a.inv_cell), cy = cell_coord(pi.y, a.inv_cell), cz = cell_coord(pi.z, a.inv_cell);
-    for (int oz = -1; oz <= 1; ++oz)
-        for (int oy = -1; oy <= 1; ++oy)
-            for (int ox = -1; ox <= 1; ++ox) {
-                const int b = cell_hash(cx + ox, cy + oy, cz + oz, a.n_buckets);
-                const int e = a.bucket_start[b + 1];
-                for (int p = a.bucket_start[b]; p < e; ++p) {
-                    const int jdx = a.sorted[p];
-                    const float4 pj = a.atoms[jdx];
-                    const uint32_t tj = __float_as_uint(pj.w);
-                    const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-                    const float lim = rext_i + a.rext[tj] + 0.01f;
-                    if (ex * ex + ey * ey + ez * ez >= lim * lim) continue;
-                    const float r2j = a.r2[tj];
+    for (int c = 0; c < 27; ++c) {
+        const int oz = c / 9 - 1, oy = (c / 3) % 3 - 1, ox = c % 3 - 1;
+        const int b = cell_hash(cx + ox, cy + oy, cz + oz, a.n_buckets);
+        const int e = a.bucket_start[b + 1];
+        for (int p = a.bucket_start[b]; p < e; ++p) {
+            const float4 pj = a.sorted_atoms[p];
+            const uint32_t tj = __float_as_uint(pj.w);
+            const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+            const float lim = rext_i + a.rext[tj] + 0.01f;
+            if (ex * ex + ey * ey + ez * ez >= lim * lim) continue;
+            const float r2j = a.r2[tj];
 #pragma unroll
-                    for (int t = 0; t < SASA_MAX_DPL; ++t) {
-                        const float dx = px[t] - pj.x, dy = py[t] - pj.y, dz = pz[t] - pj.z;
-                        float dist2 = dx * dx;
-                        dist2 = dist2 + dy * dy;
-                        dist2 = dist2 + dz * dz;
-                        live[t] = live[t] && !((r2j - dist2) > 1e-4f);
-                    }
-                }
+            for (int t = 0; t < DPL; ++t) {
+                const float dx = px[t] - pj.x, dy = py[t] - pj.y, dz = pz[t] - pj.z;
+                float dist2 = dx * dx;
+                dist2 = dist2 + dy * dy;
+                dist2 = dist2 + dz * dz;
+                live[t] = live[t] && !((r2j - dist2) > 1e-4f);
             }
+        }
+        bool any = false;
+#pragma unroll
+        for (int t = 0; t < DPL; ++t) any = any || live[t];
+        if (!__any(any)) break;
+    }
     int cnt = 0;
 #pragma unroll
-    for (int t = 0; t < SASA_MAX_DPL; ++t) cnt += live[t] ? 1 : 0;
+    for (int t = 0; t < DPL; ++t) cnt += live[t] ? 1 : 0;
     cnt = wave_sum_i(cnt);
     if (lane == 0) a.counts[i] = (uint32_t)cnt;
 }
@@ -1685,7 +1689,18 @@ hipError_t launch_sasa(const SasaArgs& a, hipStream_t s)
     hipLaunchKernelGGL(sasa_bin_kernel, g, b, 0, s, a);
     hipLaunchKernelGGL(sasa_scan_kernel, dim3(1), dim3(1024), 0, s, a);
     hipLaunchKernelGGL(sasa_place_kernel, g, b, 0, s, a);
-    hipLaunchKernelGGL(sasa_dots_kernel, dim3((unsigned)a.n), dim3(64), 0, s, a);
+    const int n_waves = a.subset ? a.n_subset : a.n;
+    if (n_waves <= 0) return hipGetLastError();
+    const dim3 gd((unsigned)n_waves), bd(64);
+    const int dpl = (a.n_dots + 63) / 64;
+    if (dpl <= 1)
+        hipLaunchKernelGGL(sasa_dots_kernel<1>, gd, bd, 0, s, a);
+    else if (dpl <= 2)
+        hipLaunchKernelGGL(sasa_dots_kernel<2>, gd, bd, 0, s, a);
+    else if (dpl <= 4)
+        hipLaunchKernelGGL(sasa_dots_kernel<4>, gd, bd, 0, s, a);
+    else
+        hipLaunchKernelGGL(sasa_dots_kernel<8>, gd, bd, 0, s, a);
     return hipGetLastError();
 }
 
