@@ -1334,7 +1334,7 @@ void adjoint_layout(rsx_engine* h)
         invq32[k] = (float)invq[k];
     }
     // triangle jobs over the slot-ordered atoms; remember each job's (row block, range)
-    const int L = h->adj_range_req > 0 ? h->adj_range_req : 256;
+    const int L = h->adj_range_req > 0 ? h->adj_range_req : 128;  // measured: 128 <= 256 < 512 at 10k x 200
     std::vector<PairJob> jobs;
     triangle_jobs(jobs, h->d_cols.p, n, L, 1.0f);
     for (size_t j = 0; j < jobs.size(); ++j) jobs[j].pad = (int)j;

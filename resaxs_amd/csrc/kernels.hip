@@ -1196,11 +1196,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 67 ? 3 
     const float row_w = row_ok ? 1.0f : 0.0f;
     const int k0 = chunk * K;
     const ChunkConst cc = a.chunks[chunk];
-    const double lo_rev = cc.mid_rev - a.dq_rev;  // (a - dq/2) / (2 pi)
     const float* __restrict__ ftab = a.ftab;
     const int ldf = a.ldf;
     const float h_dq = (float)a.half_dq;
-    const float dq = 2.0f * h_dq;
 
     __shared__ float T[ADJ_TC][3][65];
     // beta, gamma per column class run, from b_k = 2 w_k F_i,k F_class,k
@@ -1255,12 +1253,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 67 ? 3 
             const float r = r2 * rinv;
             const double rd = (double)r;
             const float sh = sin_small_f32<M>(r * h_dq);             // sin(dq r / 2)
-            const float sa = sin_rev(cc.a_rev * rd);                 // sin(a r)
-            const float clo = cos_rev(lo_rev * rd);                  // cos((a - dq/2) r)
+            const float ch = __builtin_sqrtf(fmaxf(1.0f - sh * sh, 0.0f));  // cos(dq r / 2)
+            const float fa = (float)__builtin_amdgcn_fract(cc.a_rev * rd);  // a r in revolutions
+            const float sa = __builtin_amdgcn_sinf(fa);              // sin(a r)
+            const float clo = fmaf(__builtin_amdgcn_cosf(fa), ch, sa * sh);  // cos((a - dq/2) r)
             const float delta = 4.0f * sh * sh;
             // R = r / (2 sin(dq r)) = r / (4 sin(dq r/2) cos(dq r/2)); 0 for r == 0
-            const float ch = __builtin_sqrtf(fmaxf(1.0f - sh * sh, 0.0f));
-            const float R = nzf * r / fmaxf(4.0f * sh * ch, 1e-30f);
+            const float R = nzf * r * __builtin_amdgcn_rcpf(fmaxf(4.0f * sh * ch, 1e-30f));
             float u = sa * rinv;
             float d = (clo + clo) * sh * rinv;
             float D = beta[0] * d, U = gamma[0] * u;
