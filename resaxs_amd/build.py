@@ -55,21 +55,28 @@ def build(verbose=False):
 
 COMPAT_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "compat_test.cpp")
 COMPAT_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "compat_test")
+FIT_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "fit_test.cpp")
+FIT_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "fit_test")
 
 
-def build_compat_test(verbose=False):
-    """Compile the C++ program that drives include/resaxs/ (the resaxslib-compatible
-    surface) against libresaxs_amd.so, as a C++14 host program (the reference's
-    language level); g++ only, no device code."""
+def _build_cpp_program(src, binary, verbose):
     cxx = os.environ.get("CXX", "g++")
     cmd = [cxx, "-std=c++14", "-O2", "-Wall", "-Wextra", "-Wno-unused-parameter", "-I" + os.path.join(ROOT, "include"),
-           COMPAT_TEST_SRC, "-o", COMPAT_TEST_BIN + ".tmp", "-L" + HERE, "-lresaxs_amd",
+           src, "-o", binary + ".tmp", "-L" + HERE, "-lresaxs_amd",
            "-Wl,-rpath,$ORIGIN/../../resaxs_amd", "-Wl,--allow-shlib-undefined"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(COMPAT_TEST_BIN + ".tmp", COMPAT_TEST_BIN)
-    return COMPAT_TEST_BIN
+    os.replace(binary + ".tmp", binary)
+    return binary
+
+
+def build_compat_test(verbose=False):
+    """Compile the C++ programs that drive include/resaxs/ (the resaxslib-compatible
+    surface and the calc_saxs fitting driver) against libresaxs_amd.so, as C++14 host
+    programs (the reference's language level); g++ only, no device code."""
+    _build_cpp_program(FIT_TEST_SRC, FIT_TEST_BIN, verbose)
+    return _build_cpp_program(COMPAT_TEST_SRC, COMPAT_TEST_BIN, verbose)
 
 
 if __name__ == "__main__":
