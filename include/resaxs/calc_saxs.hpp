@@ -27,6 +27,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "parse_pdb.hpp"
 #include "resaxs.hpp"
 #include "saxs_algorithm.hpp"
 #include "saxs_hip_gpu.hpp"
@@ -252,6 +253,24 @@ struct calc_saxs {
         }
         intensity_.resize(v_q_.size());
         if (!v_models_.empty()) v_bodies_ = v_models_[0];
+    }
+
+    // the tool's constructor for atomic models (calc_saxs.cpp:48-91, 279-303): every MODEL of
+    // every PDB file is one ensemble member; bodies are the first model.  The coarse
+    // two-body model (atomic = false) reads its tables and residues with the reference's
+    // third-party PDB library and is not provided here.
+    calc_saxs(const std::vector<std::string>& bodies_filenames, const std::string& /*exe_base_path*/, bool atomic,
+              FLT_T q_min, FLT_T q_max, unsigned int q_n, calc_params<FLT_T> params, verbose_levels verbose_lvl)
+        : calc_saxs(std::vector<std::vector<real4>>(), q_min, q_max, q_n, std::move(params), verbose_lvl)
+    {
+        verify(atomic, error::SAXS_INVALID_ARG, "RESAXS::error -- only atomic PDB models are supported");
+        for (const auto& f : bodies_filenames) {
+            parse_pdb<FLT_T> parser(f);
+            parser.set_verbose_level(typename parse_pdb<FLT_T>::verbose_levels(verbose_lvl_));
+            for (auto& m : parser.all_models_as_complex_atoms()) v_models_.push_back(std::move(m));
+        }
+        verify(!v_models_.empty(), error::SAXS_INVALID_ARG, "RESAXS::error -- no models in the PDB files");
+        v_bodies_ = v_models_[0];
     }
 
     void set_verbose_level(verbose_levels lvl = NORMAL) { verbose_lvl_ = lvl; }

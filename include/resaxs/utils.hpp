@@ -89,6 +89,19 @@ inline void set_factor_index(cl_double4& b, unsigned int index)
     std::memcpy(&b.s[3], &i, sizeof i);
 }
 
+// PDB atom typing through the library's tables (utils.cpp:292-314): the element from the
+// label's first letter (unknown: N), or given as a type
+inline atoms::atom_type map_pdb_atom_to_ff_type(const std::string& atom_label, const std::string& res_type)
+{
+    return (atoms::atom_type)rsx_pdb_atom_type(atom_label.c_str(), res_type.c_str(), nullptr, -1);
+}
+
+inline atoms::atom_type map_pdb_atom_to_ff_type(atoms::atom_type element, const std::string& atom_label,
+                                                const std::string& res_type)
+{
+    return (atoms::atom_type)rsx_pdb_atom_type(atom_label.c_str(), res_type.c_str(), nullptr, (int)element);
+}
+
 // num rounded up to a multiple of stride
 template <typename T>
 inline T aligned_num(T num, unsigned int stride)

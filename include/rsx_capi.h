@@ -34,6 +34,7 @@ extern "C" {
 #define RSX_HIP_ERROR          -100   /* a HIP runtime call failed */
 #define RSX_OUT_OF_MEMORY      -101
 #define RSX_NOT_SUPPORTED      -102
+#define RSX_PDB_FILE_NOT_FOUND -1100  /* parse_pdb.hpp SAXS_PDB_FILE_NOT_FOUND */
 
 /* dot-sphere kinds for SASA */
 #define RSX_DOTS_DODECA_ICOSA     0   /* 32 dots, rsasa_algorithm.cpp:157-208 (reference GPU default) */
@@ -105,6 +106,22 @@ int rsx_device_name(int device, char *buf, int len);
 /* ff_coef.hpp:112-211: default table [n_q][16] and the 16 type radii */
 int rsx_atomic_form_factors(const float *q, int n_q, float avg_radius, float expansion, float *out);
 int rsx_atomic_radii(float *out16);
+
+/* ---- PDB input (parse_pdb.cpp:35-364; typing tables utils.cpp:201-314) ---- */
+/* united-atom form-factor type (0..15) of a PDB atom with label atom_label ("CA") in
+ * residue res_name ("GLY"); the element is element_index when >= 0 (a type), else the
+ * symbol element ("C", "N", ..., "Zn"; NULL: the label's first letter), unknown symbols
+ * typed N.  Replaces both map_pdb_atom_to_ff_type overloads (utils.cpp:292-314). */
+int rsx_pdb_atom_type(const char *atom_label, const char *res_name, const char *element, int element_index);
+/* ATOM / HETATM records of a PDB file as bodies {x, y, z, type bits}: waters (HOH),
+ * hydrogens and alternate locations other than 'A' skipped.  all_models = 0: the first
+ * model (parse_pdb<>::first_model_as_complex_atoms); 1: every MODEL, model m is
+ * bodies [offsets[m], offsets[m+1]) (all_models_as_complex_atoms).  Call with bodies and
+ * offsets NULL for the counts, then with room for n_atoms bodies and n_models + 1
+ * offsets.  verbose >= 1 reports elements typed N on stderr.  RSX_PDB_FILE_NOT_FOUND
+ * when the file cannot be opened. */
+int rsx_pdb_read(const char *path, int all_models, int verbose, float *bodies, long long capacity,
+                 long long *n_atoms, int *offsets, int offsets_capacity, int *n_models);
 
 /* ---- batched ensemble (no reference counterpart; calc_saxs/calc_saxs.hpp:611-624 loops frames) ---- */
 /* frames: bodies concatenated, frame f is [offsets[f], offsets[f+1]); uploads and lays them out */

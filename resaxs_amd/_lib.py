@@ -20,6 +20,7 @@ RSX_NO_INITIAL_CALC = -6
 RSX_HIP_ERROR = -100
 RSX_OUT_OF_MEMORY = -101
 RSX_NOT_SUPPORTED = -102
+RSX_PDB_FILE_NOT_FOUND = -1100
 
 RSX_DOTS_DODECA_ICOSA = 0
 RSX_DOTS_FIBONACCI = 1
@@ -64,6 +65,8 @@ SIGNATURES = {
     "rsx_device_name": (_i, [_i, _c.c_char_p, _i]),
     "rsx_atomic_form_factors": (_i, [_fp, _i, _f, _f, _fp]),
     "rsx_atomic_radii": (_i, [_fp]),
+    "rsx_pdb_atom_type": (_i, [_c.c_char_p, _c.c_char_p, _c.c_char_p, _i]),
+    "rsx_pdb_read": (_i, [_c.c_char_p, _i, _i, _fp, _ll, _c.POINTER(_ll), _ip, _i, _ip]),
     "rsx_batch_set_frames": (_i, [_vp, _fp, _ip, _i]),
     "rsx_batch_calc": (_i, [_vp, _dp, _dp]),
     "rsx_batch_calc_dev_async": (_i, [_vp, _vp, _vp]),

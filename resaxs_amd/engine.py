@@ -5,6 +5,7 @@ The method names follow the reference's i_saxs / i_rsasa interface
 calc_curve, recalc_curve, commit, revert, calc_rsasa.  Arrays are numpy.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -320,3 +321,30 @@ def atomic_radii():
     if rc:
         raise RsxError(rc, "atomic_radii")
     return out
+
+
+def pdb_atom_type(atom_label, res_name, element=None):
+    """United-atom form-factor type (0..15) of a PDB atom; element: a type index, a
+    symbol, or None (the label's first letter)."""
+    if isinstance(element, int):
+        return _lib.lib().rsx_pdb_atom_type(atom_label.encode(), res_name.encode(), None, element)
+    return _lib.lib().rsx_pdb_atom_type(atom_label.encode(), res_name.encode(),
+                                        None if element is None else element.encode(), -1)
+
+
+def read_pdb(path, all_models=True, verbose=0):
+    """Bodies [n, 4] (float32, type bits in column 3) and model offsets [n_models + 1]
+    of the ATOM / HETATM records of a PDB file."""
+    L = _lib.lib()
+    n, nm = ctypes.c_longlong(), ctypes.c_int()
+    p = os.fsencode(path)
+    rc = L.rsx_pdb_read(p, int(bool(all_models)), int(verbose), None, 0, ctypes.byref(n), None, 0, ctypes.byref(nm))
+    if rc:
+        raise RsxError(rc, f"read_pdb {path}")
+    bodies = np.zeros((n.value, 4), dtype=np.float32)
+    offsets = np.zeros(nm.value + 1, dtype=np.int32)
+    rc = L.rsx_pdb_read(p, int(bool(all_models)), int(verbose), _ptr(bodies, ctypes.c_float), n.value,
+                        ctypes.byref(n), _ptr(offsets, ctypes.c_int), len(offsets), ctypes.byref(nm))
+    if rc:
+        raise RsxError(rc, f"read_pdb {path}")
+    return bodies, offsets

@@ -3,6 +3,8 @@
 // from a file, scale fitted, water weight over [-2, 4], expansion factor over
 // [0.95, 1.05], the calc_cl_saxs evaluator on the GPU.
 //
+//   fit_test pdb FILE    (no device) builds the driver from a PDB file as the tool does
+//                        and prints "models M atoms N q Q" (q: 0.01..0.5, 50 points)
 //   fit_test DIR [c w]   reads DIR/models.bin (int32 n_models, n_atoms, then float32
 //                        [n_models][n_atoms][4], atomic types in the w bits) and
 //                        DIR/profile.dat (q I sigma), runs the fit on device 0 and
@@ -50,6 +52,21 @@ int main(int argc, char** argv)
         return 2;
     }
     const std::string dir = argv[1];
+    if (dir == "pdb" && argc >= 3) {
+        try {
+            calc_params<float> params;
+            calc_saxs<float> calc(std::vector<std::string>{argv[2]}, "", true, 0.01f, 0.5f, 50, params,
+                                  calc_saxs<float>::QUIET);
+            std::printf("models %zu atoms %zu q %zu\n", calc.v_models_.size(), calc.v_bodies_.size(),
+                        calc.v_q_.size());
+            CHECK(calc.v_q_.size() == 50 && std::fabs(calc.v_q_[49] - 0.5f) < 1e-6f, "q grid");
+        } catch (const error& e) {
+            std::printf("error %d\n", e.err());
+            return 3;
+        }
+        std::printf("%d failures\n", failures);
+        return failures ? 1 : 0;
+    }
     try {
         std::ifstream in(dir + "/models.bin", std::ios::binary);
         int hdr[2] = {0, 0};
