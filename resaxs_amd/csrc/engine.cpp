@@ -735,14 +735,14 @@ int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles)
         per_cu = rsx::rec32_blocks_per_cu(h->variant, h->K, rsx::sin_series_terms(h->dq * h->path_radius), false);
     if (per_cu <= 0) per_cu = 8;
     const double slots = (double)std::max(1, h->n_cu) * per_cu;
-    // about four rounds of waves over the resident slots: fewer leave a partly filled last
-    // round (measured on incremental updates: L = 128-192 beats 512 by 1.3-1.9x), more
-    // pay the per-job set-up and epilogue more often.  128..1024 columns.
-    const double jobs = std::max(1.0, 4.0 * slots / std::max(1, h->n_chunks));
+    // about six rounds of waves over the resident slots: fewer leave a partly filled last
+    // round (measured on incremental updates: L = 64-128 beats 512 by 1.3-2.2x), more
+    // pay the per-job set-up and epilogue more often.  64..1024 columns, rounded down.
+    const double jobs = std::max(1.0, 6.0 * slots / std::max(1, h->n_chunks));
     double L = column_atoms_total / 64.0 / jobs;
     (void)row_tiles;
-    L = std::max(128.0, std::min(1024.0, L));
-    return ((int)L + 63) / 64 * 64;
+    L = std::max(64.0, std::min(1024.0, L));
+    return std::max(64, (int)L / 64 * 64);
 }
 
 // triangle jobs over records [0, m) of cols (rows == cols); appends to jobs
@@ -987,17 +987,29 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
 
     const int m = (int)M.size();
     const int mp = (m + 63) / 64 * 64;
-    // rows: [new (mp) | old (mp)], pads carry the skip class
+    // rows: [new (mp) | old (mp)], each half sorted by class (row tiles of few classes,
+    // and class runs when the rows serve as columns of the M x M blocks); pads carry the
+    // skip class
     std::vector<float4> rows(2 * (size_t)mp, make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP)));
     std::vector<uint32_t> cls_new(m), cls_old(m);
+    std::vector<float4> rec_new(m), rec_old(m);
     for (int a = 0; a < m; ++a) {
         const int i = M[a];
         cls_old[a] = h->cls[i];
         cls_new[a] = class_of(h, i);
         const float* nb = &h->bodies[4 * (size_t)i];
         const float* ob = (i >= start && i < start + len) ? &old_bodies[4 * (size_t)(i - start)] : nb;
-        rows[a] = make_float4(nb[0], nb[1], nb[2], bitsf(cls_new[a]));
-        rows[mp + a] = make_float4(ob[0], ob[1], ob[2], bitsf(cls_old[a]));
+        rec_new[a] = make_float4(nb[0], nb[1], nb[2], bitsf(cls_new[a]));
+        rec_old[a] = make_float4(ob[0], ob[1], ob[2], bitsf(cls_old[a]));
+    }
+    std::vector<int> order_new(m), order_old(m);
+    std::iota(order_new.begin(), order_new.end(), 0);
+    std::iota(order_old.begin(), order_old.end(), 0);
+    std::stable_sort(order_new.begin(), order_new.end(), [&](int x, int y) { return cls_new[x] < cls_new[y]; });
+    std::stable_sort(order_old.begin(), order_old.end(), [&](int x, int y) { return cls_old[x] < cls_old[y]; });
+    for (int t = 0; t < m; ++t) {
+        rows[t] = rec_new[order_new[t]];
+        rows[mp + t] = rec_old[order_old[t]];
     }
     // column edits: first remove M from the columns, later write the new records
     std::vector<int> slots(m);
@@ -1005,7 +1017,7 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
     for (int a = 0; a < m; ++a) {
         slots[a] = h->slot_of[M[a]];
         vals[a] = make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP));
-        vals[m + a] = rows[a];
+        vals[m + a] = rec_new[a];
     }
     h->d_rows.ensure(rows.size());
     h->d_idx.ensure(m);
