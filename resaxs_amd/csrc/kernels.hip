@@ -1311,39 +1311,47 @@ __global__ void adjoint_coef_kernel(const float* __restrict__ ftab, const double
 }
 
 // grad of slot s = its row partials (jobs of its row block) + its column partials
-// (one job per row block at or before its own), over every q chunk, in a fixed order
-__global__ void adjoint_reduce_kernel(AdjointReduceArgs a)
+// (one job per row block at or before its own), over every q chunk.  One wave per atom:
+// lanes take the (job, chunk) items in turn and a fixed shuffle tree adds the lanes, so
+// the order is fixed run to run.
+__global__ __launch_bounds__(256) void adjoint_reduce_kernel(AdjointReduceArgs a)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.n) return;
+    const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (s >= a.n) return;  // whole waves
     const int I = s >> 6, ln = s & 63;
-    double g[3] = {0.0, 0.0, 0.0};
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
     const int nm = (a.n - 64 * I + a.col_range - 1) / a.col_range;
-    for (int m = 0; m < nm; ++m) {
+    const int n_row_items = nm * a.n_chunks;
+    for (int t = lane; t < n_row_items; t += 64) {
+        const int m = t / a.n_chunks, c = t - m * a.n_chunks;
         const int slot = a.tri_slot[I * a.tri_stride + m];
-        for (int c = 0; c < a.n_chunks; ++c) {
-            const float* v = a.row_partial + (((size_t)slot * a.n_chunks + c) * 64 + ln) * 3;
-            g[0] += v[0];
-            g[1] += v[1];
-            g[2] += v[2];
-        }
+        const float* v = a.row_partial + (((size_t)slot * a.n_chunks + c) * 64 + ln) * 3;
+        g0 += v[0];
+        g1 += v[1];
+        g2 += v[2];
     }
-    for (int J = 0; J <= I; ++J) {
+    const int n_col_items = (I + 1) * a.n_chunks;
+    for (int t = lane; t < n_col_items; t += 64) {
+        const int J = t / a.n_chunks, c = t - J * a.n_chunks;
         const int rel = s - 64 * J;
         const int m = rel / a.col_range;
         const int off = rel - m * a.col_range;
         const int slot = a.tri_slot[J * a.tri_stride + m];
-        for (int c = 0; c < a.n_chunks; ++c) {
-            const float* v = a.col_partial + (((size_t)slot * a.n_chunks + c) * a.col_range + off) * 3;
-            g[0] += v[0];
-            g[1] += v[1];
-            g[2] += v[2];
-        }
+        const float* v = a.col_partial + (((size_t)slot * a.n_chunks + c) * a.col_range + off) * 3;
+        g0 += v[0];
+        g1 += v[1];
+        g2 += v[2];
     }
-    const int o = a.slot_to_caller[s];
-    a.grad[3 * o] = (float)g[0];
-    a.grad[3 * o + 1] = (float)g[1];
-    a.grad[3 * o + 2] = (float)g[2];
+    g0 = wave_sum(g0);
+    g1 = wave_sum(g1);
+    g2 = wave_sum(g2);
+    if (lane == 0) {
+        const int o = a.slot_to_caller[s];
+        a.grad[3 * o] = (float)g0;
+        a.grad[3 * o + 1] = (float)g1;
+        a.grad[3 * o + 2] = (float)g2;
+    }
 }
 
 template <int... Ks>
@@ -1766,7 +1774,7 @@ hipError_t launch_adjoint(const AdjointArgs& a, int n_jobs, int K, hipStream_t s
 hipError_t launch_adjoint_reduce(const AdjointReduceArgs& a, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)((a.n + 3) / 4)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
