@@ -107,6 +107,17 @@ int rsx_device_name(int device, char *buf, int len);
 int rsx_atomic_form_factors(const float *q, int n_q, float avg_radius, float expansion, float *out);
 int rsx_atomic_radii(float *out16);
 
+/* ---- ensemble fit (calc_saxs/calc_saxs.hpp:420-624; the C++ driver is include/resaxs/calc_saxs.hpp) ---- */
+/* Grid search of the expansion factor and water weight for the uniform mean of the
+ * models' hydrated I(q) (default atomic factors, GPU SASA), least-squares scale and chi
+ * against (q, i_exp, sigma).  Models: bodies [offsets[m], offsets[m+1]), all of one atom
+ * count.  Each range is {lo, hi}; lo == hi fixes the value (for the scale: no fit).
+ * out = {expansion factor, water weight, scale, chi, model evaluations}; out_iq (may be
+ * NULL) = the scaled best intensity [n_q]. */
+int rsx_fit_ensemble(const float *bodies, const int *offsets, int n_models, const float *q, const float *i_exp,
+                     const float *sigma, int n_q, const float *scale_range, const float *water_range,
+                     const float *expansion_range, int device, double *out, float *out_iq);
+
 /* ---- PDB input (parse_pdb.cpp:35-364; typing tables utils.cpp:201-314) ---- */
 /* united-atom form-factor type (0..15) of a PDB atom with label atom_label ("CA") in
  * residue res_name ("GLY"); the element is element_index when >= 0 (a type), else the

@@ -348,3 +348,26 @@ def read_pdb(path, all_models=True, verbose=0):
     if rc:
         raise RsxError(rc, f"read_pdb {path}")
     return bodies, offsets
+
+
+def fit_ensemble(models_bodies, offsets, q, i_exp, sigma, scale=None, water=(-2.0, 4.0), expansion=(0.95, 1.05),
+                 device=0):
+    """The calc_saxs ensemble fit on the GPU: returns (expansion factor, water weight,
+    scale, chi, evaluations, scaled best I(q)).  scale None: fitted; a number: fixed.
+    A range (lo, hi) with lo == hi fixes that parameter."""
+    b = _f32(models_bodies).reshape(-1, 4)
+    off = np.ascontiguousarray(offsets, dtype=np.int32)
+    qq, ie, sg = _f32(q), _f32(i_exp), _f32(sigma)
+    sr = np.array([1.0, 2.0] if scale is None else [scale, scale], dtype=np.float32)
+    wr = np.array(water, dtype=np.float32)
+    er = np.array(expansion, dtype=np.float32)
+    out = np.zeros(5, dtype=np.float64)
+    iq = np.zeros(len(qq), dtype=np.float32)
+    rc = _lib.lib().rsx_fit_ensemble(_ptr(b, ctypes.c_float), _ptr(off, ctypes.c_int), len(off) - 1,
+                                     _ptr(qq, ctypes.c_float), _ptr(ie, ctypes.c_float), _ptr(sg, ctypes.c_float),
+                                     len(qq), _ptr(sr, ctypes.c_float), _ptr(wr, ctypes.c_float),
+                                     _ptr(er, ctypes.c_float), int(device), _ptr(out, ctypes.c_double),
+                                     _ptr(iq, ctypes.c_float))
+    if rc:
+        raise RsxError(rc, "fit_ensemble")
+    return float(out[0]), float(out[1]), float(out[2]), float(out[3]), int(out[4]), iq

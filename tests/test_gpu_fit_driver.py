@@ -87,3 +87,37 @@ def test_fit_ensemble_recovers_known_parameters(tmp_path):
     p_iq = np.fromfile(os.path.join(d, "point_iq.bin"), dtype=np.float32)
     assert abs(p_scale / s_true - 1) < 1e-5 and p_chi < 2e-3, (p_scale, p_chi)
     assert rel_err(p_iq, target).max() < 1e-5
+
+
+def test_fit_ensemble_c_abi():
+    """The same search through the C ABI (rsx_fit_ensemble).  A fixed point (every range
+    a single value) returns the oracle's ensemble mean; a full search returns a chi no
+    worse than the best point of its own first 6 x 6 round (the best only improves,
+    calc_saxs.hpp:571-575), evaluated here point by point."""
+    from resaxs_amd import configs, fit_ensemble
+    print("case: rsx_fit_ensemble, 2 models x 900 atoms x 60 q", flush=True)
+    rng = np.random.default_rng(78)
+    n = 900
+    radius = (n / 0.045 / (4.0 / 3.0 * np.pi)) ** (1.0 / 3.0)
+    base = configs.in_ball(rng, n, radius)
+    types = rng.choice(np.array([1, 2, 3, 5, 8, 9, 13], dtype=np.uint32), size=n).astype(np.uint32)
+    models = [(base + rng.normal(scale=0.4, size=base.shape)).astype(np.float32) for _ in range(2)]
+    bodies = np.concatenate([po.bodies(p, types) for p in models])
+    offsets = np.array([0, n, 2 * n], dtype=np.int32)
+    q = np.linspace(0.02, 0.45, 60).astype(np.float32)
+    c_true, w_true = np.float32(0.985), np.float32(-0.6)
+    ones = np.ones(len(q), dtype=np.float32)
+    c, w, s, chi, ev, iq = fit_ensemble(bodies, offsets, q, ones, ones, scale=1.0, water=(w_true, w_true),
+                                        expansion=(c_true, c_true))
+    assert (c, s) == (float(c_true), 1.0) and w == float(w_true) and ev == 2
+    ref = _ensemble_oracle(q, models, types, c_true, w_true)
+    assert rel_err(iq, ref).max() < 1e-5
+    target = (3.0 * ref).astype(np.float32)
+    sigma = (0.01 * target).astype(np.float32)
+    c, w, s, chi, ev, iq = fit_ensemble(bodies, offsets, q, target, sigma)
+    print(f"  search: c {c:.5f} w {w:.4f} scale {s:.5f} chi {chi:.3g} in {ev} evaluations", flush=True)
+    first_round = min(fit_ensemble(bodies, offsets, q, target, sigma, water=(ww, ww), expansion=(ef, ef))[3]
+                      for ef in np.float32(0.95) + np.arange(6, dtype=np.float32) * np.float32(0.02)
+                      for ww in np.float32(-2.0) + np.arange(6, dtype=np.float32) * np.float32(1.2))
+    print(f"  best chi of the first 6 x 6 round: {first_round:.3g}", flush=True)
+    assert chi <= first_round * (1 + 1e-6)
