@@ -258,6 +258,28 @@ def fit_driver_bench(n_models=8, n=4000, nq=100):
             "timing": "host wall clock around rsx_fit_ensemble (engines created, laid out and SASA counted inside)"}
 
 
+def histogram_bench(bin_width=0.01, reps=3):
+    """The distance-histogram Debye (approximate) on the 50,000-atom plain workload x 256 q:
+    host wall clock of rsx_calc_curve_hist (bodies from the host, I(q) back) against the
+    exact engine's forward on the same engine, and its error relative to that forward."""
+    from resaxs_amd import configs
+    w = configs.config3()
+    n, nq = len(w.bodies), len(w.q)
+    e = make_engine(w, hydrate=False)
+    I_exact = e.calc_curve(w.bodies)
+    _, pair_ms = e.time_forward(3)
+    e.calc_curve_hist(w.bodies, bin_width)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        I_h = e.calc_curve_hist(w.bodies, bin_width)
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    e.close()
+    return {"workload": f"plain-{n}x{nq}", "bin_width_A": bin_width, "ms_wall": round(ms, 3),
+            "pairs_per_s": n * (n - 1) / 2.0 / (ms * 1e-3), "exact_pair_kernel_ms": round(pair_ms, 3),
+            "max_rel_diff_vs_exact": float(np.max(np.abs(I_h - I_exact) / np.abs(I_exact))),
+            "timing": "host wall clock around rsx_calc_curve_hist (histogram, transform, host combination)"}
+
+
 def cpu_baseline(w, I_gpu, rows=2560):
     """The reference's CPU path timed on this host, 1 thread, on a bounded sample: the
     pairs among the first `rows` atoms (the first rows of the lower triangle) at all
@@ -361,6 +383,8 @@ def main():
         line["incremental"] = [incremental_bench(False), incremental_bench(True)]
         log("bench: ensemble fit driver")
         line["fit_driver"] = fit_driver_bench()
+        log("bench: distance-histogram Debye")
+        line["histogram"] = histogram_bench()
     if not args.no_cpu_baseline:
         base, acc = cpu_baseline(w, I_gpu)
         line["cpu_baseline"] = base

@@ -31,7 +31,7 @@ class saxs_hip_gpu : public i_saxs<FLT_T, hip_base<FLT_T>> {
 public:
     typedef typename real_type<FLT_T>::real4 real4;
 
-private:
+protected:
     struct params_t;
 
     struct ff_t : form_factor_params<FLT_T> {
@@ -256,9 +256,72 @@ struct saxs_creator<FLT_T, hip_base<FLT_T>, saxs_enum::saxs_gpu_v2> {
     static i_saxs<FLT_T, hip_base<FLT_T>>* create() { return new saxs_hip_gpu<FLT_T>(); }
 };
 
+// The slot the reference reserves for a histogram engine (saxs_algorithm.hpp:188-193):
+// I(q) from pair-distance histograms at bin_width (rsx_calc_curve_hist; approximate,
+// the error falls with the square of the bin width).  Set-up as saxs_hip_gpu; every
+// calc_curve / recalc_curve recomputes the histogram of the given bodies (no incremental
+// state), commit and revert keep nothing to restore, and access_rsasa() is not updated.
+template <typename FLT_T>
+class saxs_hip_hist : public saxs_hip_gpu<FLT_T> {
+    using base = saxs_hip_gpu<FLT_T>;
+    using typename base::real4;
+
+public:
+    FLT_T bin_width = FLT_T(0.01);
+
+    void calc_curve(const std::vector<real4>& bodies, std::vector<FLT_T>& out) override
+    {
+        verify(this->args_set(), error::SAXS_ARGS_NOT_SET);
+        verify(bodies.size() == this->params_.n_bodies_, error::SAXS_INVALID_ARG,
+               "RESAXS::error -- different number of bodies expected in calc_curve()");
+        this->push_factors();
+        this->push_water();
+        this->params_.clear_all();
+        run(bodies, out);
+        this->state = this->alg_computing;
+    }
+
+    void recalc_curve(const std::vector<real4>& bodies, unsigned int upd_start, unsigned int upd_length,
+                      std::vector<FLT_T>& out) override
+    {
+        verify(this->computing(), error::SAXS_NO_INITIAL_CALC,
+               "RESAXS::error -- recalc_curve() called before an initial call to calc_curve()");
+        verify(bodies.size() >= (size_t)upd_start + upd_length && bodies.size() == this->params_.n_bodies_,
+               error::SAXS_INVALID_ARG, "RESAXS::error -- bodies vector too small in recalc_curve()");
+        if (this->params_.is_dirty()) {
+            if (this->params_.ff.is_dirty()) this->push_factors();
+            this->push_water();
+            this->params_.clear_all();
+        }
+        run(bodies, out);
+    }
+
+    void commit() override
+    {
+        verify(this->computing(), error::SAXS_NO_INITIAL_CALC,
+               "RESAXS::error -- commit() called before an initial call to calc_curve()");
+    }
+
+    void revert() override
+    {
+        verify(this->computing(), error::SAXS_NO_INITIAL_CALC,
+               "RESAXS::error -- revert() called before an initial call to calc_curve()");
+    }
+
+private:
+    void run(const std::vector<real4>& bodies, std::vector<FLT_T>& out)
+    {
+        std::vector<double> iq((size_t)this->nq_);
+        detail::check(rsx_calc_curve_hist(this->h_, this->body_ptr(bodies), (int)bodies.size(), (float)bin_width,
+                                          iq.data()),
+                      this->h_, "calc_curve_hist");
+        out.assign(iq.begin(), iq.end());
+    }
+};
+
 template <typename FLT_T>
 struct saxs_creator<FLT_T, hip_base<FLT_T>, saxs_enum::saxs_hist_cl_gpu> {
-    static i_saxs<FLT_T, hip_base<FLT_T>>* create() { return nullptr; }
+    static i_saxs<FLT_T, hip_base<FLT_T>>* create() { return new saxs_hip_hist<FLT_T>(); }
 };
 
 }  // namespace algorithm

@@ -107,6 +107,16 @@ int rsx_device_name(int device, char *buf, int len);
 int rsx_atomic_form_factors(const float *q, int n_q, float avg_radius, float expansion, float *out);
 int rsx_atomic_radii(float *out16);
 
+/* ---- distance-histogram Debye (the reference's reserved saxs_hist_cl_gpu slot,
+ * saxs_algorithm.hpp:188-193; approximate) ---- */
+/* I(q) [n_q] (FP64) of n bodies from their pair-distance histogram at bin width
+ * bin_width (A), per type pair, with the engine's q, factor table and hydration
+ * settings (SASA counts from the engine's dot spheres when the water weight is set).
+ * Exact at q = 0; elsewhere each pair's sinc is taken at its bin centre: the error
+ * falls as bin_width^2 (stated by the tests).  Needs rsx_set_q and rsx_set_factors, not
+ * rsx_initialize; the engine's incremental state is untouched. */
+int rsx_calc_curve_hist(rsx_engine *h, const float *bodies, int n, float bin_width, double *out);
+
 /* ---- ensemble fit (calc_saxs/calc_saxs.hpp:420-624; the C++ driver is include/resaxs/calc_saxs.hpp) ---- */
 /* Grid search of the expansion factor and water weight for the uniform mean of the
  * models' hydrated I(q) (default atomic factors, GPU SASA), least-squares scale and chi

@@ -65,6 +65,7 @@ SIGNATURES = {
     "rsx_device_name": (_i, [_i, _c.c_char_p, _i]),
     "rsx_atomic_form_factors": (_i, [_fp, _i, _f, _f, _fp]),
     "rsx_atomic_radii": (_i, [_fp]),
+    "rsx_calc_curve_hist": (_i, [_vp, _fp, _i, _f, _dp]),
     "rsx_fit_ensemble": (_i, [_fp, _ip, _i, _fp, _fp, _fp, _i, _fp, _fp, _fp, _i, _dp, _fp]),
     "rsx_pdb_atom_type": (_i, [_c.c_char_p, _c.c_char_p, _c.c_char_p, _i]),
     "rsx_pdb_read": (_i, [_c.c_char_p, _i, _i, _fp, _ll, _c.POINTER(_ll), _ip, _i, _ip]),

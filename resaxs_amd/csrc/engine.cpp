@@ -295,6 +295,10 @@ struct rsx_engine {
     DBuf<float4> d_cols, d_saved_cols, d_rows, d_vals, d_sasa_atoms, d_sorted_atoms;
     bool sasa_mirror = false;        // d_sasa_atoms holds exactly the host bodies
     DBuf<int> d_subset;              // atom ids of a local SASA recount
+    DBuf<unsigned int> h_u32;        // distance histograms (rsx_calc_curve_hist)
+    DBuf<unsigned long long> h_u64;
+    DBuf<double> h_dbl, h_out, h_q;
+    DBuf<int> h_pidx;
     DBuf<float> d_ftab, d_qv, d_r2, d_rext;
     DBuf<double> d_qrev;
     DBuf<ChunkConst> d_chunks;
@@ -1111,6 +1115,119 @@ void revert_impl(rsx_engine* h)
 }
 
 // ---- standalone SASA -----------------------------------------------------------
+// Distance-histogram I(q) (approximate): pair distances of the bodies binned at width dr
+// per type pair (and, with hydration, weighted by the exposed-dot counts), each histogram
+// transformed once per q; the factors enter only in the final FP64 combination
+//   I = sum_i F_i^2 + 2 sum_b [sum_{t<=t'} f_t f_t' H0 + (g/D) sum_t f_t H1_t + (g/D)^2 H2](b) sinc(q r_b)
+// with F_i = f_t + (count_i / D) g, g = w f_water.  The bins are the only approximation.
+void calc_hist_impl(rsx_engine* h, const float* bodies, int n, float dr, double* out)
+{
+    if (h->nq <= 0 || h->n_types <= 0) fail(RSX_ARGS_NOT_SET, "histogram: q and factors must be set first");
+    if (!bodies || n < 0 || !out || !(dr > 0.0f)) fail(RSX_INVALID_ARG, "histogram: invalid argument");
+    const int nt = h->n_types, nq = h->nq;
+    std::vector<int> type(n);
+    std::vector<double> cen(3, 0.0);
+    for (int i = 0; i < n; ++i) {
+        uint32_t t;
+        std::memcpy(&t, &bodies[4 * (size_t)i + 3], 4);
+        if ((int)t >= nt) fail(RSX_INVALID_ARG, "histogram: body type index outside the factor table");
+        type[i] = (int)t;
+        for (int d = 0; d < 3; ++d) cen[d] += bodies[4 * (size_t)i + d];
+    }
+    set_device(h);
+    const bool hyd = hydration_on(h);
+    double r_max = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double s2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double v = bodies[4 * (size_t)i + d] - cen[d] / std::max(1, n);
+            s2 += v * v;
+        }
+        r_max = std::max(r_max, std::sqrt(s2));
+    }
+    const int n_bins = (int)std::min(4.0e6, 2.0 * r_max / (double)dr) + 2;
+    const int n_pairs = nt * (nt + 1) / 2;
+    std::vector<int> pidx((size_t)nt * nt);
+    for (int a = 0, p = 0; a < nt; ++a)
+        for (int b = a; b < nt; ++b, ++p) pidx[(size_t)a * nt + b] = pidx[(size_t)b * nt + a] = p;
+    const int n_hist = n_pairs + (hyd ? nt + 1 : 0);
+    // device buffers (engine-owned scratch of this entry point)
+    h->d_sasa_atoms.ensure(std::max(n, 1));
+    h->sasa_mirror = false;
+    if (n > 0)
+        hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, bodies, (size_t)n * 16, hipMemcpyHostToDevice, h->stream),
+               "upload bodies");
+    std::vector<uint32_t> counts;
+    if (hyd && n > 0) {
+        run_sasa_device(h, n);
+        counts.resize(n);
+        hipchk(hipMemcpyAsync(counts.data(), h->d_counts.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream),
+               "download counts");
+    }
+    const size_t n0 = (size_t)n_pairs * n_bins, n1 = hyd ? (size_t)nt * n_bins : 0, n2 = hyd ? (size_t)n_bins : 0;
+    h->h_u32.ensure(std::max<size_t>(n0, 1));
+    h->h_u64.ensure(std::max<size_t>(n1 + n2, 1));
+    h->h_dbl.ensure((size_t)n_hist * n_bins);
+    h->h_out.ensure((size_t)n_hist * nq);
+    h->h_pidx.ensure(pidx.size());
+    h->h_q.ensure(nq);
+    std::vector<double> qd(h->q.begin(), h->q.end());
+    hipchk(hipMemcpyAsync(h->h_pidx.p, pidx.data(), pidx.size() * sizeof(int), hipMemcpyHostToDevice, h->stream),
+           "upload pair index");
+    hipchk(hipMemcpyAsync(h->h_q.p, qd.data(), nq * sizeof(double), hipMemcpyHostToDevice, h->stream), "upload q");
+    hipchk(rsx::launch_fill_i32((int*)h->h_u32.p, 0, (long long)n0, h->stream), "zero h0");
+    if (hyd) hipchk(rsx::launch_fill_i32((int*)h->h_u64.p, 0, 2LL * (long long)(n1 + n2), h->stream), "zero h1 h2");
+    rsx::HistArgs a;
+    a.atoms = h->d_sasa_atoms.p;
+    a.counts = hyd ? h->d_counts.p : nullptr;
+    a.n = n;
+    a.n_types = nt;
+    a.inv_dr = 1.0f / dr;
+    a.n_bins = n_bins;
+    a.pair_index = h->h_pidx.p;
+    a.h0 = h->h_u32.p;
+    a.h1 = h->h_u64.p;
+    a.h2 = h->h_u64.p + n1;
+    hipchk(rsx::launch_hist_pairs(a, h->stream), "histogram");
+    hipchk(rsx::launch_widen_u32(h->h_u32.p, h->h_dbl.p, (long long)n0, h->stream), "widen h0");
+    if (hyd) hipchk(rsx::launch_widen_u64(h->h_u64.p, h->h_dbl.p + n0, (long long)(n1 + n2), h->stream), "widen h1 h2");
+    hipchk(rsx::launch_hist_transform(h->h_dbl.p, n_hist, n_bins, (double)dr, h->h_q.p, nq, h->h_out.p, h->stream),
+           "histogram transform");
+    std::vector<double> A((size_t)n_hist * nq);
+    hipchk(hipMemcpyAsync(A.data(), h->h_out.p, A.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+           "download transforms");
+    hipchk(hipStreamSynchronize(h->stream), "sync histogram");
+    // self terms per type: n_t, sum of counts, sum of squared counts
+    std::vector<double> n_t(nt, 0.0), c1(nt, 0.0);
+    double c2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+        n_t[type[i]] += 1.0;
+        if (hyd) {
+            c1[type[i]] += counts[i];
+            c2 += (double)counts[i] * counts[i];
+        }
+    }
+    const double inv_d = hyd ? 1.0 / (double)h->n_dots : 0.0;
+    for (int k = 0; k < nq; ++k) {
+        const float* row = &h->table[(size_t)k * nt];
+        const double g = hyd ? (double)h->w * (double)row[h->water] * inv_d : 0.0;  // per unit count
+        double I = 0.0;
+        for (int t = 0; t < nt; ++t) I += n_t[t] * (double)row[t] * row[t];
+        if (hyd) {
+            for (int t = 0; t < nt; ++t) I += 2.0 * g * row[t] * c1[t];
+            I += g * g * c2;
+        }
+        double pair = 0.0;
+        for (int t = 0; t < nt; ++t)
+            for (int u = t; u < nt; ++u) pair += (double)row[t] * row[u] * A[(size_t)pidx[(size_t)t * nt + u] * nq + k];
+        if (hyd) {
+            for (int t = 0; t < nt; ++t) pair += g * row[t] * A[(size_t)(n_pairs + t) * nq + k];
+            pair += g * g * A[(size_t)(n_pairs + nt) * nq + k];
+        }
+        out[k] = I + 2.0 * pair;
+    }
+}
+
 void calc_rsasa_impl(rsx_engine* h, const float* bodies, int n, const float* radii, int nt, float probe,
                      const float* dots, int n_dots, unsigned* out_counts, float* out_frac)
 {
@@ -1691,6 +1808,12 @@ int rsx_destroy(rsx_engine* h)
     h->a_invq.release();
     h->a_jobs.release();
     h->a_tri.release();
+    h->h_u32.release();
+    h->h_u64.release();
+    h->h_dbl.release();
+    h->h_out.release();
+    h->h_q.release();
+    h->h_pidx.release();
     h->d_jobs.release();
     h->d_fjobs.release();
     h->d_f0col.release();
@@ -1906,6 +2029,11 @@ int rsx_calc_rsasa(rsx_engine* h, const float* bodies, int n, const float* radii
                    const float* dots, int n_dots, unsigned* out_counts, float* out_frac)
 {
     return guarded(h, [&] { calc_rsasa_impl(h, bodies, n, radii, n_types, probe, dots, n_dots, out_counts, out_frac); });
+}
+
+int rsx_calc_curve_hist(rsx_engine* h, const float* bodies, int n, float bin_width, double* out)
+{
+    return guarded(h, [&] { calc_hist_impl(h, bodies, n, bin_width, out); });
 }
 
 int rsx_fibonacci_dots(int n_half, float radius, float probe, float* out)

@@ -190,6 +190,31 @@ hipError_t launch_q0(const float4* cols, int n, const double* f0col, double* out
 // dst[i] = src[i] for n float4 (grid-stride), the HBM calibration copy
 hipError_t launch_copy(float4* dst, const float4* src, long long n, hipStream_t s);
 
+// ---- distance-histogram Debye (approximate; large N) ----
+// Pair distances binned by r_b = (b + 1/2) dr: h0[type pair][bin] counts unordered
+// pairs, and with exposed-dot counts (hydration) h1[type][bin] sums the partner's count
+// of pairs whose other atom has that type, h2[bin] the product of both counts.  Integer
+// sums: the histogram is exact and order-free.
+struct HistArgs {
+    const float4* atoms;       // [n] x, y, z, type bits
+    const uint32_t* counts;    // [n] exposed-dot counts, or null (plain)
+    int n;
+    int n_types;
+    float inv_dr;
+    int n_bins;
+    const int* pair_index;     // [n_types][n_types] -> type pair (symmetric)
+    unsigned int* h0;          // [n_pairs][n_bins]
+    unsigned long long* h1;    // [n_types][n_bins]
+    unsigned long long* h2;    // [n_bins]
+};
+hipError_t launch_hist_pairs(const HistArgs& a, hipStream_t s);
+// out[x][k] = sum_b hist[x][b] sin(q_k r_b) / (q_k r_b) (1 at q = 0), FP64; hist as doubles
+hipError_t launch_hist_transform(const double* hist, int n_hist, int n_bins, double dr, const double* q, int n_q,
+                                 double* out, hipStream_t s);
+// widen u32 / u64 histograms to doubles (dst[i] = src[i])
+hipError_t launch_widen_u32(const unsigned int* src, double* dst, long long n, hipStream_t s);
+hipError_t launch_widen_u64(const unsigned long long* src, double* dst, long long n, hipStream_t s);
+
 // workspace reset kernels (used instead of memset nodes so launches can be graph-captured)
 hipError_t launch_fill_i32(int* p, int value, long long n, hipStream_t s);
 

@@ -1679,6 +1679,134 @@ hipError_t launch_copy(float4* dst, const float4* src, long long n, hipStream_t 
     return hipGetLastError();
 }
 
+namespace {
+
+// One workgroup of 64 lanes per (row tile I, column tile J >= I) of the triangle: lane =
+// row atom, the 64 columns of tile J staged in LDS; on the diagonal tile only j > i.
+// Global atomics without return (fire and forget into L2).
+__global__ __launch_bounds__(64) void hist_pairs_kernel(HistArgs a, int n_tiles)
+{
+    // blockIdx.x -> (I, J) with I <= J, row-major over the upper triangle
+    const long long t = blockIdx.x;
+    int I = (int)((2.0 * n_tiles + 1.0 - sqrt((2.0 * n_tiles + 1.0) * (2.0 * n_tiles + 1.0) - 8.0 * (double)t)) / 2.0);
+    long long start = (long long)I * n_tiles - (long long)I * (I - 1) / 2;
+    while (I > 0 && start > t) {
+        --I;
+        start = (long long)I * n_tiles - (long long)I * (I - 1) / 2;
+    }
+    while (start + (n_tiles - I) <= t) {
+        start += n_tiles - I;
+        ++I;
+    }
+    const int J = I + (int)(t - start);
+    const int lane = threadIdx.x;
+    __shared__ float4 cj[64];
+    __shared__ unsigned int cntj[64];
+    const int j0 = J * 64;
+    const int ncol = a.n - j0 < 64 ? a.n - j0 : 64;
+    if (lane < ncol) {
+        cj[lane] = a.atoms[j0 + lane];
+        cntj[lane] = a.counts ? a.counts[j0 + lane] : 0u;
+    }
+    __syncthreads();
+    const int i = I * 64 + lane;
+    if (i >= a.n) return;
+    const float4 pi = a.atoms[i];
+    const int ti = (int)__float_as_uint(pi.w);
+    const unsigned int ci = a.counts ? a.counts[i] : 0u;
+    const int jstart = I == J ? lane + 1 : 0;
+    for (int jj = jstart; jj < ncol; ++jj) {
+        const float4 pj = cj[jj];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r = __builtin_sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+        int b = (int)(r * a.inv_dr);
+        b = b < a.n_bins - 1 ? b : a.n_bins - 1;
+        const int tj = (int)__float_as_uint(pj.w);
+        atomicAdd(&a.h0[(size_t)a.pair_index[ti * a.n_types + tj] * a.n_bins + b], 1u);
+        if (a.counts) {
+            const unsigned int cj_ = cntj[jj];
+            atomicAdd(&a.h1[(size_t)tj * a.n_bins + b], (unsigned long long)ci);
+            atomicAdd(&a.h1[(size_t)ti * a.n_bins + b], (unsigned long long)cj_);
+            atomicAdd(&a.h2[b], (unsigned long long)ci * cj_);
+        }
+    }
+}
+
+// out[x][k] = sum_b hist[x][b] sinc(q_k r_b): lane = q index, sin(q r_b) advanced along
+// b by the FP64 three-term recurrence (angle q dr per bin)
+__global__ __launch_bounds__(64) void hist_transform_kernel(const double* __restrict__ hist, int n_bins, double dr,
+                                                            const double* __restrict__ q, int n_q,
+                                                            double* __restrict__ out)
+{
+    const int x = blockIdx.y;
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k >= n_q) return;
+    const double* __restrict__ h = hist + (size_t)x * n_bins;
+    const double qk = q[k];
+    double acc = 0.0;
+    if (qk == 0.0) {
+        for (int b = 0; b < n_bins; ++b) acc += h[b];
+    } else {
+        const double th = qk * dr;
+        const double c2 = 2.0 * cos(th);
+        double sp = sin(-0.5 * th), sc = sin(0.5 * th);  // sin(q r_{-1}), sin(q r_0)
+        for (int b = 0; b < n_bins; ++b) {
+            acc = fma(h[b] / ((b + 0.5) * dr), sc, acc);
+            const double sn = fma(c2, sc, -sp);
+            sp = sc;
+            sc = sn;
+        }
+        acc /= qk;
+    }
+    out[(size_t)x * n_q + k] = acc;
+}
+
+__global__ void widen_u32_kernel(const unsigned int* src, double* dst, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (double)src[i];
+}
+
+__global__ void widen_u64_kernel(const unsigned long long* src, double* dst, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (double)src[i];
+}
+
+}  // namespace
+
+hipError_t launch_hist_pairs(const HistArgs& a, hipStream_t s)
+{
+    if (a.n <= 1) return hipSuccess;
+    const int T = (a.n + 63) / 64;
+    const long long n_blocks = (long long)T * (T + 1) / 2;
+    hipLaunchKernelGGL(hist_pairs_kernel, dim3((unsigned)n_blocks), dim3(64), 0, s, a, T);
+    return hipGetLastError();
+}
+
+hipError_t launch_hist_transform(const double* hist, int n_hist, int n_bins, double dr, const double* q, int n_q,
+                                 double* out, hipStream_t s)
+{
+    if (n_hist <= 0 || n_q <= 0) return hipSuccess;
+    hipLaunchKernelGGL(hist_transform_kernel, dim3((unsigned)((n_q + 63) / 64), (unsigned)n_hist), dim3(64), 0, s,
+                       hist, n_bins, dr, q, n_q, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_widen_u32(const unsigned int* src, double* dst, long long n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(widen_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_widen_u64(const unsigned long long* src, double* dst, long long n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(widen_u64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_i32(int* p, int value, long long n, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;

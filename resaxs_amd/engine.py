@@ -166,6 +166,14 @@ class Engine:
                                            _ptr(counts, ctypes.c_uint), _ptr(frac, ctypes.c_float)))
         return counts, frac
 
+    def calc_curve_hist(self, bodies, bin_width=0.02):
+        """Approximate I(q) (FP64) from the pair-distance histogram at bin_width (A)."""
+        b = _f32(bodies).reshape(-1, 4)
+        out = np.zeros(self.nq, dtype=np.float64)
+        self._check(self._L.rsx_calc_curve_hist(self.h, _ptr(b, ctypes.c_float), len(b), float(bin_width),
+                                                _ptr(out, ctypes.c_double)))
+        return out
+
     # -- batched ensemble ----------------------------------------------------
     def batch_set_frames(self, bodies, offsets):
         b = _f32(bodies).reshape(-1, 4)

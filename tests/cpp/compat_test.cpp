@@ -241,7 +241,21 @@ static int run_gpu(const std::string& dir)
         std::vector<double> id;
         d->calc_curve(bd, id);
         write_bin(dir + "/iq_double.bin", id);
-        CHECK(saxs<float>::create<cl_base<float>>(saxs_enum::saxs_hist_cl_gpu) == nullptr, "no histogram engine");
+    }
+
+    // ---- the histogram engine in the saxs_hist_cl_gpu slot (default 0.01 A bins, plain)
+    {
+        std::unique_ptr<i_saxs<float, cl_base<float>>> hg(
+            saxs<float>::create<cl_base<float>>(saxs_enum::saxs_hist_cl_gpu));
+        CHECK(hg != nullptr, "histogram engine created");
+        hg->access_params().initialize(q, bodies, devices);
+        hg->initialize();
+        std::vector<float> ih;
+        hg->calc_curve(bodies, ih);
+        write_bin(dir + "/iq_hist.bin", ih);
+        std::vector<float> ih2;
+        hg->recalc_curve(bodies, 0, 0, ih2);
+        CHECK(ih2 == ih, "histogram recalc of the same bodies repeats");
     }
 
     // ---- host_debye (host_debye.hpp:30-40) with the default factor table

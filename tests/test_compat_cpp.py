@@ -95,6 +95,8 @@ def test_compat_surface_gpu(tmp_path):
         return po.debye(q, p, po.body_factors(p, types, q, tab))
 
     assert rel_err(ld("iq_plain.bin", np.float32), plain(pos)).max() < TOL
+    # the histogram engine (saxs_hist_cl_gpu slot): approximate, 0.01 A bins
+    assert rel_err(ld("iq_hist.bin", np.float32), plain(pos)).max() < 1e-3
     assert rel_err(ld("iq_double.bin", np.float64), plain(pos)).max() < TOL
 
     # moves: the last committed state, the last (reverted) move, and the revert itself
