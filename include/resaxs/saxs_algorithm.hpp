@@ -136,7 +136,7 @@ struct i_saxs : public BASE {
 enum class saxs_enum {
     saxs_gpu_pt_wf,   // served by the MI355X engine
     saxs_gpu_v2,      // the same engine (the reference's older GPU decomposition of the same sum)
-    saxs_hist_cl_gpu, // distance-histogram engine (saxs_hip_hist, approximate)
+    saxs_hist_cl_gpu, // the same engine (the reference's class of that name runs the pt_wf kernels)
 };
 
 template <typename FLT_T, typename BASE, saxs_enum alg>

@@ -256,7 +256,8 @@ struct saxs_creator<FLT_T, hip_base<FLT_T>, saxs_enum::saxs_gpu_v2> {
     static i_saxs<FLT_T, hip_base<FLT_T>>* create() { return new saxs_hip_gpu<FLT_T>(); }
 };
 
-// The slot the reference reserves for a histogram engine (saxs_algorithm.hpp:188-193):
+// Distance-histogram engine (no counterpart in the reference; constructed directly,
+// `new saxs_hip_hist<float>()`, not through the factory):
 // I(q) from pair-distance histograms at bin_width (rsx_calc_curve_hist; approximate,
 // the error falls with the square of the bin width).  Set-up as saxs_hip_gpu; every
 // calc_curve / recalc_curve recomputes the histogram of the given bodies (no incremental
@@ -319,9 +320,11 @@ private:
     }
 };
 
+// The reference's saxs_hist_cl_gpu (alg_saxs_hist_cl_gpu.cpp, saxs_hist_gpu.cl:22-333)
+// runs the pt_wf block kernels unchanged, so its slot gets the exact engine.
 template <typename FLT_T>
 struct saxs_creator<FLT_T, hip_base<FLT_T>, saxs_enum::saxs_hist_cl_gpu> {
-    static i_saxs<FLT_T, hip_base<FLT_T>>* create() { return new saxs_hip_hist<FLT_T>(); }
+    static i_saxs<FLT_T, hip_base<FLT_T>>* create() { return new saxs_hip_gpu<FLT_T>(); }
 };
 
 }  // namespace algorithm

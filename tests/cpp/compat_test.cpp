@@ -243,11 +243,21 @@ static int run_gpu(const std::string& dir)
         write_bin(dir + "/iq_double.bin", id);
     }
 
-    // ---- the histogram engine in the saxs_hist_cl_gpu slot (default 0.01 A bins, plain)
+    // ---- the saxs_hist_cl_gpu slot: the exact sum, as the reference's class of that name
     {
-        std::unique_ptr<i_saxs<float, cl_base<float>>> hg(
+        std::unique_ptr<i_saxs<float, cl_base<float>>> sg(
             saxs<float>::create<cl_base<float>>(saxs_enum::saxs_hist_cl_gpu));
-        CHECK(hg != nullptr, "histogram engine created");
+        CHECK(sg != nullptr, "saxs_hist_cl_gpu engine created");
+        sg->access_params().initialize(q, bodies, devices);
+        sg->initialize();
+        std::vector<float> is;
+        sg->calc_curve(bodies, is);
+        write_bin(dir + "/iq_hist_slot.bin", is);
+    }
+
+    // ---- the distance-histogram engine, constructed directly (default 0.01 A bins, plain)
+    {
+        std::unique_ptr<i_saxs<float, cl_base<float>>> hg(new saxs_hip_hist<float>());
         hg->access_params().initialize(q, bodies, devices);
         hg->initialize();
         std::vector<float> ih;

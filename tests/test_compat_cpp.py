@@ -95,7 +95,9 @@ def test_compat_surface_gpu(tmp_path):
         return po.debye(q, p, po.body_factors(p, types, q, tab))
 
     assert rel_err(ld("iq_plain.bin", np.float32), plain(pos)).max() < TOL
-    # the histogram engine (saxs_hist_cl_gpu slot): approximate, 0.01 A bins
+    # the saxs_hist_cl_gpu factory slot is the exact sum (the reference's class of that
+    # name runs the pt_wf kernels); the distance-histogram engine is approximate (0.01 A bins)
+    assert rel_err(ld("iq_hist_slot.bin", np.float32), plain(pos)).max() < TOL
     assert rel_err(ld("iq_hist.bin", np.float32), plain(pos)).max() < 1e-3
     assert rel_err(ld("iq_double.bin", np.float64), plain(pos)).max() < TOL
 

@@ -1,6 +1,5 @@
-"""Distance-histogram Debye (rsx_calc_curve_hist; the reference reserves the
-saxs_hist_cl_gpu slot, resaxslib/include/saxs_algorithm.hpp:188-193, without an
-implementation).  Approximate by construction: each pair's sinc is taken at its bin
+"""Distance-histogram Debye (rsx_calc_curve_hist; no counterpart in the reference,
+whose saxs_hist_cl_gpu class runs the exact pt_wf kernels).  Approximate by construction: each pair's sinc is taken at its bin
 centre.  Its accuracy contract, checked against the FP64 oracle:
   * I(0) exact: (sum F)^2 to 1e-12 (the histogram counts every pair once);
   * the error falls as the square of the bin width (ratio >= 3 between 0.04 and 0.02 A);
