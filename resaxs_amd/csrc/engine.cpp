@@ -307,7 +307,8 @@ struct rsx_engine {
     DBuf<PairJob> d_jobs, d_fjobs;  // incremental jobs; forward (triangle) jobs
     DBuf<double> d_f0col;           // FP64 F_class(q = 0) per class
     DBuf<int> d_slot_begin_f;  // [0, n forward jobs]
-    DBuf<int> d_slot_begin, d_idx, d_bucket_of, d_bucket_count, d_bucket_start, d_bucket_fill, d_sorted, d_s2c;
+    DBuf<int> d_slot_begin, d_idx, d_bucket_of, d_bucket_count, d_bucket_start, d_bucket_fill, d_sorted, d_s2c,
+        d_tile_sum;
     DBuf<uint32_t> d_counts;
     DBuf<float4> d_dots;
     DBuf<float> d_grad;
@@ -627,6 +628,7 @@ void run_sasa_device(rsx_engine* h, int n, const int* subset = nullptr, int n_su
     h->d_bucket_count.ensure(nb + 1);
     h->d_bucket_start.ensure(nb + 1);
     h->d_bucket_fill.ensure(nb);
+    h->d_tile_sum.ensure(nb / 1024);
     h->d_sorted.ensure(n);
     h->d_sorted_atoms.ensure(n);
     h->d_counts.ensure(n);
@@ -644,6 +646,7 @@ void run_sasa_device(rsx_engine* h, int n, const int* subset = nullptr, int n_su
     a.bucket_count = h->d_bucket_count.p;
     a.bucket_start = h->d_bucket_start.p;
     a.bucket_fill = h->d_bucket_fill.p;
+    a.tile_sum = h->d_tile_sum.p;
     a.sorted = h->d_sorted.p;
     a.sorted_atoms = h->d_sorted_atoms.p;
     a.counts = h->d_counts.p;
@@ -1801,7 +1804,8 @@ int rsx_destroy(rsx_engine* h)
                           &h->d_red, &h->b_partials, &h->b_out, &h->b_mean, &h->b_q0, &h->b_red};
     for (auto* b : fd) b->release();
     DBuf<int>* fi[] = {&h->d_slot_begin, &h->d_idx, &h->d_bucket_of, &h->d_bucket_count, &h->d_bucket_start,
-                       &h->d_bucket_fill, &h->d_sorted, &h->d_s2c, &h->b_slot_begin, &h->d_subset};
+                       &h->d_bucket_fill, &h->d_sorted, &h->d_s2c, &h->b_slot_begin, &h->d_subset,
+                       &h->d_tile_sum};
     for (auto* b : fi) b->release();
     h->d_chunks.release();
     h->a_chunks.release();

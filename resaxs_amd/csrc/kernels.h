@@ -133,6 +133,7 @@ struct SasaArgs {
     int* bucket_count;       // [n_buckets + 1]
     int* bucket_start;       // [n_buckets + 1]
     int* bucket_fill;        // [n_buckets]
+    int* tile_sum;           // [n_buckets / 1024] scan scratch
     int* sorted;             // [n] atom ids grouped by bucket
     float4* sorted_atoms;    // [n] the atoms in bucket order
     uint32_t* counts;        // [n] exposed dots (only the listed atoms when subset is set)

@@ -958,19 +958,19 @@ __global__ void sasa_bin_kernel(SasaArgs a)
     atomicAdd(&a.bucket_count[b], 1);
 }
 
-// exclusive scan of bucket_count into bucket_start (one workgroup of 1024)
-// exclusive scan of the bucket counts by one workgroup: each thread sums a contiguous
-// chunk, the 1024 chunk totals are scanned across the block, and each thread writes
-// its chunk's starts
-__global__ __launch_bounds__(1024) void sasa_scan_kernel(SasaArgs a)
+// Exclusive scan of the bucket counts in three steps: tiles of SCAN_TILE buckets
+// (each thread one int4, coalesced) scanned locally with their totals set aside; the
+// tile totals scanned by one workgroup; the tile offsets added back.  n_buckets is a
+// power of two >= SCAN_TILE.
+constexpr int SCAN_TILE = 1024;
+
+__global__ __launch_bounds__(256) void sasa_scan_tiles_kernel(SasaArgs a)
 {
-    __shared__ int wave_tot[16];
+    __shared__ int wave_tot[4];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int per = (a.n_buckets + 1023) / 1024;
-    const int b0 = tid * per < a.n_buckets ? tid * per : a.n_buckets;
-    const int b1 = b0 + per < a.n_buckets ? b0 + per : a.n_buckets;
-    int sum = 0;
-    for (int b = b0; b < b1; ++b) sum += a.bucket_count[b];
+    const int b = blockIdx.x * SCAN_TILE + tid * 4;
+    const int4 c = *reinterpret_cast<const int4*>(a.bucket_count + b);
+    const int sum = c.x + c.y + c.z + c.w;
     int x = sum;  // inclusive scan within the wave
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -981,14 +981,48 @@ __global__ __launch_bounds__(1024) void sasa_scan_kernel(SasaArgs a)
     __syncthreads();
     int off = 0;
     for (int t = 0; t < w; ++t) off += wave_tot[t];
-    int run = off + x - sum;
-    for (int b = b0; b < b1; ++b) {
-        const int c = a.bucket_count[b];
-        a.bucket_start[b] = run;
-        a.bucket_fill[b] = run;
-        run += c;
+    const int e = off + x - sum;
+    *reinterpret_cast<int4*>(a.bucket_start + b) = make_int4(e, e + c.x, e + c.x + c.y, e + c.x + c.y + c.z);
+    if (tid == 255) a.tile_sum[blockIdx.x] = off + x;
+}
+
+// exclusive scan of the tile totals in place, in passes of 1024 with a running carry;
+// bucket_start[n_buckets] = the total
+__global__ __launch_bounds__(1024) void sasa_scan_top_kernel(SasaArgs a)
+{
+    __shared__ int wave_tot[16];
+    __shared__ int carry_s;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int n_tiles = a.n_buckets / SCAN_TILE;
+    int carry = 0;
+    for (int t0 = 0; t0 < n_tiles; t0 += 1024) {
+        const int t = t0 + tid;
+        const int v = t < n_tiles ? a.tile_sum[t] : 0;
+        int x = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wave_tot[w] = x;
+        __syncthreads();
+        int off = carry;
+        for (int u = 0; u < w; ++u) off += wave_tot[u];
+        if (t < n_tiles) a.tile_sum[t] = off + x - v;
+        if (tid == 1023) carry_s = off + x;
+        __syncthreads();
+        carry = carry_s;
+        __syncthreads();
     }
-    if (tid == 1023) a.bucket_start[a.n_buckets] = off + x;
+    if (tid == 0) a.bucket_start[a.n_buckets] = carry;
+}
+
+__global__ __launch_bounds__(256) void sasa_scan_add_kernel(SasaArgs a)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    const int v = a.bucket_start[b] + a.tile_sum[b / SCAN_TILE];
+    a.bucket_start[b] = v;
+    a.bucket_fill[b] = v;
 }
 
 __global__ void sasa_place_kernel(SasaArgs a)
@@ -1002,10 +1036,15 @@ __global__ void sasa_place_kernel(SasaArgs a)
 
 constexpr int SASA_MAX_DPL = 8;  // dots per lane (n_dots <= 512)
 
-// One wave per atom (or per listed atom), DPL dots per lane.  Candidates come from the
-// 27 neighbour cells in bucket order (positions copied there by the place kernel, one
-// load per candidate).  A dot once occluded stays occluded, so the wave stops scanning
-// once no lane holds a live dot (checked per cell): the count is 0 either way.
+// One wave per atom (or per listed atom), DPL dots per lane.  The 27 neighbour cells'
+// bucket ranges (lanes 0..26) form one candidate list; it is taken 64 candidates at a
+// time, one per lane: each lane loads its candidate (positions in bucket order, copied
+// there by the place kernel) and applies the distance prefilter, and the candidates
+// that pass are broadcast one by one (readlane) to test every lane's dots.  A dot once
+// occluded stays occluded, so the wave stops once no lane holds a live dot: the count
+// is 0 either way.  The occlusion test per (dot, candidate) pair is the same
+// expression in the same order whatever the visiting order, so the counts do not
+// depend on it.
 template <int DPL>
 __global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
 {
@@ -1025,25 +1064,60 @@ __global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
         py[t] = tp.y + pi.y;
         pz[t] = tp.z + pi.z;
     }
+    // lane c < 27: neighbour cell c's bucket range
     const int cx = cell_coord(pi.x, a.inv_cell), cy = cell_coord(pi.y, a.inv_cell), cz = cell_coord(pi.z, a.inv_cell);
-    for (int c = 0; c < 27; ++c) {
-        const int oz = c / 9 - 1, oy = (c / 3) % 3 - 1, ox = c % 3 - 1;
+    int cstart = 0, ccount = 0;
+    if (lane < 27) {
+        const int oz = lane / 9 - 1, oy = (lane / 3) % 3 - 1, ox = lane % 3 - 1;
         const int b = cell_hash(cx + ox, cy + oy, cz + oz, a.n_buckets);
-        const int e = a.bucket_start[b + 1];
-        for (int p = a.bucket_start[b]; p < e; ++p) {
-            const float4 pj = a.sorted_atoms[p];
+        cstart = a.bucket_start[b];
+        ccount = a.bucket_start[b + 1] - cstart;
+    }
+    int incl = ccount;  // inclusive prefix of the counts (lanes >= 27 add 0)
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    const int excl = incl - ccount;
+    const int total = __builtin_amdgcn_readlane(incl, 26);
+    for (int base = 0; base < total; base += 64) {
+        // this lane's candidate: position g of the list -> its cell and bucket slot
+        const int g = base + lane;
+        int p = -1;
+#pragma unroll
+        for (int c = 0; c < 27; ++c) {
+            const int e = __builtin_amdgcn_readlane(excl, c);
+            const int n_c = __builtin_amdgcn_readlane(ccount, c);
+            const int s_c = __builtin_amdgcn_readlane(cstart, c);
+            if (g >= e && g < e + n_c) p = s_c + (g - e);
+        }
+        float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+        float r2j = 0.0f;
+        bool pass = false;
+        if (p >= 0) {
+            pj = a.sorted_atoms[p];
             const uint32_t tj = __float_as_uint(pj.w);
             const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
             const float lim = rext_i + a.rext[tj] + 0.01f;
-            if (ex * ex + ey * ey + ez * ez >= lim * lim) continue;
-            const float r2j = a.r2[tj];
+            pass = ex * ex + ey * ey + ez * ez < lim * lim;
+            r2j = a.r2[tj];
+        }
+        uint64_t m = __ballot(pass);
+        while (m) {
+            const int l = __builtin_ctzll(m);
+            m &= m - 1;
+            const float jx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pj.x), l));
+            const float jy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pj.y), l));
+            const float jz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pj.z), l));
+            const float jr2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r2j), l));
 #pragma unroll
             for (int t = 0; t < DPL; ++t) {
-                const float dx = px[t] - pj.x, dy = py[t] - pj.y, dz = pz[t] - pj.z;
+                const float dx = px[t] - jx, dy = py[t] - jy, dz = pz[t] - jz;
                 float dist2 = dx * dx;
                 dist2 = dist2 + dy * dy;
                 dist2 = dist2 + dz * dz;
-                live[t] = live[t] && !((r2j - dist2) > 1e-4f);
+                live[t] = live[t] && !((jr2 - dist2) > 1e-4f);
             }
         }
         bool any = false;
@@ -1821,8 +1895,11 @@ hipError_t launch_sasa(const SasaArgs& a, hipStream_t s)
     hipError_t e = launch_fill_i32(a.bucket_count, 0, (long long)a.n_buckets + 1, s);
     if (e != hipSuccess) return e;
     const dim3 g((unsigned)((a.n + 255) / 256)), b(256);
+    if (a.n_buckets < SCAN_TILE || (a.n_buckets & (a.n_buckets - 1))) return hipErrorInvalidValue;
     hipLaunchKernelGGL(sasa_bin_kernel, g, b, 0, s, a);
-    hipLaunchKernelGGL(sasa_scan_kernel, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(sasa_scan_tiles_kernel, dim3((unsigned)(a.n_buckets / SCAN_TILE)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(sasa_scan_top_kernel, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(sasa_scan_add_kernel, dim3((unsigned)(a.n_buckets / 256)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(sasa_place_kernel, g, b, 0, s, a);
     const int n_waves = a.subset ? a.n_subset : a.n;
     if (n_waves <= 0) return hipGetLastError();
