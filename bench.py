@@ -193,7 +193,7 @@ def incremental_bench(hydrate, n_moves=2000):
             "achieved_tflops": round(2 * p_move * (5 * nq + 9) / (ms * 1e-3) / 1e12, 2),
             "rel_err_after_199_moves_vs_fp64_full": err_1999,
             "timing": "host wall clock around rsx_recalc_curve_f64 + rsx_commit (host bodies in, I(q) out)"
-                      + ("; every move recounts SASA for all atoms" if hydrate else "")}
+                      + ("; SASA recounted for the atoms within reach of the moved segment" if hydrate else "")}
 
 
 def batched_bench(reps=5):

@@ -851,6 +851,65 @@ void run_jobs(rsx_engine* h, DBuf<PairJob>& jbuf, const std::vector<PairJob>& jo
     hipchk(hipStreamSynchronize(h->stream), "sync pair kernels");
 }
 
+// Order of the forward (triangle) jobs, and a finer tail.  A job costs its columns plus
+// the flushes of the class runs among them (in column equivalents; a run flush of the
+// FP32 run-flush kernel is ~1.4 columns of work, of the other kernels less), and jobs
+// run most expensive first.  Waves are handed out in launch order as resident slots
+// free up, so with jobs of near-equal cost the last round is quantised: a partly
+// filled round still takes a whole job's time.  The cheapest splittable jobs are
+// therefore halved, twice, so the final rounds are made of ever shorter waves and the
+// idle tail shrinks to about a quarter of a job.  (Incremental launches keep the plain
+// longest-first order: their column range is already sized for about six rounds, and
+// splitting measured slower there.)
+void order_jobs(rsx_engine* h, std::vector<PairJob>& jobs)
+{
+    const int n = h->n;
+    const bool have_cls = (int)h->caller_of.size() == n && (int)h->cls.size() == n;
+    std::vector<int> P((size_t)n + 1, 0);  // P[s]: class changes at slots t < s (t >= 1)
+    for (int t = 0; t < n; ++t)
+        P[t + 1] = P[t] + (have_cls && t > 0 && h->cls[h->caller_of[t]] != h->cls[h->caller_of[t - 1]] ? 1 : 0);
+    const double flush_w = h->variant == rsx::VARIANT_FP32R ? 1.4 : 0.6;
+    const float4* base = h->d_cols.p;
+    auto cost = [&](const PairJob& j) {
+        const long c0 = (long)(j.cols - base), c1 = c0 + j.ncol;
+        const double runs = 1.0 + (c1 - 1 > c0 ? P[c1] - P[c0 + 1] : 0);
+        return ((double)j.ncol + flush_w * runs) * j.nrow;
+    };
+    auto by_cost = [&]() {
+        std::vector<std::pair<double, int>> key(jobs.size());
+        for (size_t k = 0; k < jobs.size(); ++k) key[k] = {-cost(jobs[k]), (int)k};
+        std::stable_sort(key.begin(), key.end());
+        std::vector<PairJob> out(jobs.size());
+        for (size_t k = 0; k < jobs.size(); ++k) out[k] = jobs[key[k].second];
+        jobs.swap(out);
+    };
+    by_cost();
+    int per_cu = 0;
+    if (h->uniform && (h->variant == rsx::VARIANT_FP32 || h->variant == rsx::VARIANT_FP32R))
+        per_cu = rsx::rec32_blocks_per_cu(h->variant, h->K, rsx::sin_series_terms(h->dq * h->path_radius), false);
+    if (per_cu <= 0) per_cu = 8;
+    const long round = std::max<long>(1, (long)std::max(1, h->n_cu) * per_cu / std::max(1, h->n_chunks));
+    if ((long)jobs.size() > 2 * round) {
+        for (int level = 0; level < 2; ++level) {
+            long done = 0;
+            const size_t m = jobs.size();
+            for (size_t k = m; k-- > 0 && done < round;) {
+                PairJob& j = jobs[k];
+                if (j.diag || j.ncol < 128) continue;
+                PairJob b = j;
+                const int first = j.ncol / 2;
+                j.ncol = first;
+                b.cols = j.cols + first;
+                b.ncol -= first;
+                jobs.push_back(b);
+                ++done;
+            }
+            by_cost();
+        }
+    }
+    for (size_t k = 0; k < jobs.size(); ++k) jobs[k].slot = (int)k;
+}
+
 std::vector<PairJob> forward_jobs(rsx_engine* h)
 {
     const int n = h->n;
@@ -858,7 +917,7 @@ std::vector<PairJob> forward_jobs(rsx_engine* h)
     const int L = pick_range(h, (double)n * n / 2.0, (n + 63) / 64);
     h->last_L = L;
     triangle_jobs(jobs, h->d_cols.p, n, L, 2.0f);
-    finalize_jobs(jobs);
+    order_jobs(h, jobs);
     return jobs;
 }
 
