@@ -86,10 +86,29 @@ def forward_bench(steps, warmup):
         raise RuntimeError("device-resident step differs from the host-path result")
     # pair-kernel duration from HIP events on the engine stream
     _, pair_ms = e.time_forward(10)
+    sasa_ms = e.time_sasa(10)
     e.dev_free(d_b)
     e.dev_free(d_out)
     e.close()
-    return w, I_host, ms_per_step, pair_ms, tuning
+    return w, I_host, ms_per_step, pair_ms, tuning, sasa_ms
+
+
+def sasa_summary(w, ms):
+    """SASA at its algorithmic unit: one (dot, candidate) test over every atom j != i with
+    |x_i - x_j| < r_i + r_j + 2p, 10 FP32 FLOP each (distance, subtract, compare); the
+    count is a property of the structure, not of the kernel (which stops scanning an atom
+    once all its dots are buried, so it executes fewer)."""
+    from scipy.spatial import cKDTree
+    R = (w.radii.astype(np.float64) + w.probe)[w.types]
+    pairs = cKDTree(w.pos.astype(np.float64)).query_pairs(2.0 * R.max(), output_type="ndarray")
+    d = np.linalg.norm(w.pos[pairs[:, 0]].astype(np.float64) - w.pos[pairs[:, 1]], axis=1)
+    close = int(np.count_nonzero(d < R[pairs[:, 0]] + R[pairs[:, 1]]))
+    tests = 2.0 * close * 2 * w.n_half
+    return {"workload": f"sasa-{len(w.pos)}-atoms-{2 * w.n_half}-dots", "ms": round(ms, 4),
+            "kernels": "sasa_bin + sasa_scan_tiles/top/add + sasa_place + sasa_dots (HIP events, engine stream)",
+            "tests": tests, "tests_per_s": tests / (ms * 1e-3),
+            "achieved_tflops": round(10 * tests / (ms * 1e-3) / 1e12, 3),
+            "frac": round(10 * tests / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4)}
 
 
 def forward_50k_bench():
@@ -198,7 +217,7 @@ def incremental_bench(hydrate, n_moves=2000):
 
 def batched_bench(reps=5):
     """The batched ensemble: 512 frames of 1,000-15,000 atoms x 128 q, one graph-captured
-    persistent launch per replay; per-frame I(q) and the uniform mean to device buffers."""
+    launch per replay; per-frame I(q) and the uniform mean to device buffers."""
     from resaxs_amd import Engine, configs
     w = configs.config4()
     nq = len(w.q)
@@ -219,6 +238,7 @@ def batched_bench(reps=5):
     e.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / reps
     tuning = e.get_tuning()
+    launch = e.batch_launch()
     e.dev_free(fr)
     e.dev_free(mn)
     e.close()
@@ -227,7 +247,9 @@ def batched_bench(reps=5):
     return {"workload": f"ensemble-{len(w.sizes)}-frames-1000..15000x{nq}", "ms_per_batch": round(ms, 3),
             "terms_per_s": terms / (ms * 1e-3), "achieved_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
             "frac": round(flops / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4), "layout_s": round(layout_s, 3),
-            "kernel": f"variant {tuning['variant']} K={tuning['k']} (persistent work queue)",
+            "kernel": f"variant {tuning['variant']} K={tuning['k']} " + (
+                f"(work queue, {launch['persistent_waves']} resident waves)" if launch["persistent_waves"] > 0
+                else "(one wave per (job, q chunk) item)") + f", {launch['n_jobs']} jobs",
             "timing": "host wall clock over graph replays on device-resident frames, outputs to device"}
 
 
@@ -320,6 +342,24 @@ def cpu_baseline(w, I_gpu, rows=2560):
     return base, acc
 
 
+def cpu_baseline_all_cores(w, rows=6400):
+    """The same loop on all of this host's cores (at most 16, the host share of one GPU):
+    the oracle's OpenMP restatement of host_debye.cpp:54-109 over the pairs among the
+    first `rows` atoms at all n_q (the reference's own loop is single-threaded)."""
+    from oracle import pyoracle as po
+    nq = len(w.q)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    terms = rows * (rows - 1) / 2.0 * nq
+    log(f"bench: CPU baseline on {threads} threads, restated reference loop over the first {rows} atoms")
+    t0 = time.perf_counter()
+    po.debye_ref_rows(w.q, w.pos[:rows], w.types[:rows], w.table, 0, rows, nthreads=threads)
+    t = time.perf_counter() - t0
+    return {"value": terms / t, "unit": "pair.q terms/s", "cores": threads, "kind": "port",
+            "sample": f"the oracle's restatement of host_debye.cpp:54-109 (OpenMP over rows) on the first {rows} "
+                      f"atoms of the {len(w.pos)}-atom workload (plain factors) x {nq} q = {terms:.3e} pair.q "
+                      f"terms in {t:.1f} s"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -345,7 +385,7 @@ def main():
         log(f"bench: copy probe {args.copy_probe} MiB: {ms:.4f} ms/launch, "
             f"{2 * (args.copy_probe << 20) / ms / 1e6:.1f} GB/s (read + write)")
         e.close()
-    w, I_gpu, ms_per_step, pair_ms, tuning = forward_bench(args.steps, args.warmup)
+    w, I_gpu, ms_per_step, pair_ms, tuning, sasa_ms = forward_bench(args.steps, args.warmup)
     kernel = ("debye_uniform_kernel", "debye_shared_kernel", "debye_rec32_kernel",
               "debye_rec32r_kernel")[tuning["variant"]]
     n, nq = len(w.pos), len(w.q)
@@ -372,6 +412,7 @@ def main():
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": read_traffic(),
                      "kernel": f"{kernel}<{tuning['k']}>", "col_range": tuning["col_range"],
                      "kernel_ms": round(pair_ms, 4), "flop_per_launch": flops},
+        "sasa": sasa_summary(w, sasa_ms),
     }
     if not args.no_adjoint:
         line["forward_50k"] = forward_50k_bench()
@@ -388,6 +429,7 @@ def main():
     if not args.no_cpu_baseline:
         base, acc = cpu_baseline(w, I_gpu)
         line["cpu_baseline"] = base
+        line["cpu_baseline_all_cores"] = cpu_baseline_all_cores(w)
         line["rel_err_vs_fp64"] = acc
     print(json.dumps(line), flush=True)
 

@@ -187,11 +187,19 @@ int rsx_set_variant(rsx_engine *h, int variant);
 /* current q per thread, column range of the forward layout, 1 if the recurrence path is active,
  * and the recurrence variant (any pointer may be NULL) */
 int rsx_get_tuning(rsx_engine *h, int *k, int *col_range, int *recurrence, int *variant);
+/* launch shape of the current batch graph (no reference counterpart): persistent_waves > 0 when
+ * the work-queue kernel runs with that many resident waves, 0 for one wave per (job, q chunk)
+ * item; n_jobs = pair jobs over all frames.  RSX_NO_INITIAL_CALC before the first batch launch. */
+int rsx_batch_launch(rsx_engine *h, int *persistent_waves, int *n_jobs);
 /* average device time (ms) of the last timed region, measured with HIP events on the engine stream */
 int rsx_time_forward(rsx_engine *h, int reps, float *ms_total, float *ms_pair_kernel);
 /* average device time (ms) of the adjoint pass (kernel + fixed-order reduction) on the weights of
  * the last rsx_chi2_grad / rsx_debye_vjp call, HIP events on the engine stream */
 int rsx_time_adjoint(rsx_engine *h, int reps, float *ms);
+/* average device time (ms) of one SASA count of the engine's atoms (cell list + dot occlusion
+ * kernels), HIP events on the engine stream; needs a prior hydrated calc_curve (no reference
+ * counterpart) */
+int rsx_time_sasa(rsx_engine *h, int reps, float *ms);
 /* plain device-to-device copy of `bytes` (float4 grid-stride kernel), average ms per launch;
  * the known-size reference for HBM counter calibration */
 int rsx_time_copy(rsx_engine *h, long long bytes, int reps, float *ms);

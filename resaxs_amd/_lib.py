@@ -79,10 +79,12 @@ SIGNATURES = {
     "rsx_set_tuning": (_i, [_vp, _i, _i]),
     "rsx_set_variant": (_i, [_vp, _i]),
     "rsx_get_tuning": (_i, [_vp, _ip, _ip, _ip, _ip]),
+    "rsx_batch_launch": (_i, [_vp, _ip, _ip]),
     "rsx_time_forward": (_i, [_vp, _i, _fp, _fp]),
     "rsx_set_adjoint_tuning": (_i, [_vp, _i, _i, _i]),
     "rsx_time_adjoint": (_i, [_vp, _i, _fp]),
     "rsx_time_copy": (_i, [_vp, _ll, _i, _fp]),
+    "rsx_time_sasa": (_i, [_vp, _i, _fp]),
     "rsx_dev_alloc": (_i, [_vp, _c.POINTER(_vp), _ll]),
     "rsx_dev_free": (_i, [_vp, _vp]),
     "rsx_memcpy_h2d": (_i, [_vp, _vp, _vp, _ll]),

@@ -363,6 +363,7 @@ struct rsx_engine {
     hipGraphExec_t b_exec = nullptr;
     double* b_graph_out = nullptr;
     double* b_graph_mean = nullptr;
+    int b_pers_waves = -1;  // launch shape of the captured batch graph (-1: none yet)
 };
 
 namespace {
@@ -1461,6 +1462,7 @@ void batch_run_dev(rsx_engine* h, double* out_frames, double* out_mean)
         batch_release_graph(h);
         const bool pers = h->persistent && (h->variant == rsx::VARIANT_FP32 || h->variant == rsx::VARIANT_FP32R);
         const int pw = pers ? persistent_waves(h) : 0;
+        h->b_pers_waves = pw;
         hipchk(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture");
         try {
             batch_enqueue(h, out_frames, out_mean, pw);
@@ -2232,6 +2234,15 @@ int rsx_debye_vjp(rsx_engine* h, const float* bodies, int n, const double* w, fl
     });
 }
 
+int rsx_batch_launch(rsx_engine* h, int* persistent_waves_out, int* n_jobs)
+{
+    return guarded(h, [&] {
+        if (h->b_pers_waves < 0 || !h->b_exec) fail(RSX_NO_INITIAL_CALC, "batch_launch: no batch graph yet");
+        if (persistent_waves_out) *persistent_waves_out = h->b_pers_waves;
+        if (n_jobs) *n_jobs = h->b_njobs;
+    });
+}
+
 int rsx_autotune(rsx_engine* h, const char* cache_path)
 {
     return guarded(h, [&] { autotune_impl(h, cache_path); });
@@ -2336,6 +2347,24 @@ int rsx_time_adjoint(rsx_engine* h, int reps, float* ms)
         float t = 0.0f;
         hipchk(hipEventElapsedTime(&t, h->ev0, h->ev1), "elapsed");
         *ms = t / reps;
+    });
+}
+
+int rsx_time_sasa(rsx_engine* h, int reps, float* ms)
+{
+    return guarded(h, [&] {
+        if (reps <= 0) fail(RSX_INVALID_ARG, "time_sasa: reps > 0");
+        if (!h->hydrate || !h->have_rsasa || !h->d_sasa_atoms.p || h->d_sasa_atoms.cap < (size_t)h->n)
+            fail(RSX_NO_INITIAL_CALC, "time_sasa needs a prior hydrated calc_curve");
+        set_device(h);
+        run_sasa_device(h, h->n);
+        hipchk(hipEventRecord(h->ev0, h->stream), "event");
+        for (int r = 0; r < reps; ++r) run_sasa_device(h, h->n);
+        hipchk(hipEventRecord(h->ev1, h->stream), "event");
+        hipchk(hipEventSynchronize(h->ev1), "event sync");
+        float t = 0.0f;
+        hipchk(hipEventElapsedTime(&t, h->ev0, h->ev1), "elapsed");
+        if (ms) *ms = t / reps;
     });
 }
 

@@ -112,6 +112,13 @@ class Engine:
                                            ctypes.byref(sg)))
         return {"k": k.value, "col_range": L.value, "recurrence": bool(rec.value), "variant": sg.value}
 
+    def batch_launch(self):
+        """Launch shape of the current batch graph: resident waves of the work-queue kernel
+        (0: one wave per (job, q chunk) item) and the pair-job count."""
+        pw, nj = ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.rsx_batch_launch(self.h, ctypes.byref(pw), ctypes.byref(nj)))
+        return {"persistent_waves": pw.value, "n_jobs": nj.value}
+
     def autotune(self, cache_path=None):
         p = cache_path.encode() if cache_path else None
         self._check(self._L.rsx_autotune(self.h, p))
@@ -226,6 +233,12 @@ class Engine:
     def time_adjoint(self, reps=5):
         ms = ctypes.c_float()
         self._check(self._L.rsx_time_adjoint(self.h, int(reps), ctypes.byref(ms)))
+        return ms.value
+
+    def time_sasa(self, reps=5):
+        """Average device ms of one SASA count of the engine's atoms (after a hydrated calc_curve)."""
+        ms = ctypes.c_float()
+        self._check(self._L.rsx_time_sasa(self.h, int(reps), ctypes.byref(ms)))
         return ms.value
 
     def time_copy(self, nbytes, reps=3):

@@ -22,8 +22,7 @@ def _say(msg):
 
 
 # (variant, K, persistent): the default run-flush kernel at K = 128 runs one wave per
-# item (its work-queue form would lose occupancy); at K = 64 it takes the work queue,
-# as does the per-lane-sum FP32 kernel
+# item (its work-queue form would lose occupancy); at K = 64 it takes the work queue
 @pytest.mark.parametrize("variant,k,persistent", [(3, 0, True), (3, 64, True), (3, 64, False), (2, 0, True),
                                                   (2, 0, False)])
 def test_batch_frames_and_mean(oracle, variant, k, persistent):
@@ -39,6 +38,16 @@ def test_batch_frames_and_mean(oracle, variant, k, persistent):
     frames, mean = e.batch_calc()
     frames2, mean2 = e.batch_calc()          # graph replay gives the same bits
     assert np.array_equal(frames, frames2) and np.array_equal(mean, mean2)
+    launch = e.batch_launch()
+    assert launch["n_jobs"] > 0
+    # the work queue only where asked for and where it keeps the kernel's occupancy (the
+    # run-flush kernel at K = 64 does; at K = 128 its queue form would drop to two waves per
+    # SIMD, and the engine launches one wave per item instead)
+    _say(f"  launch: {launch}")
+    if not persistent:
+        assert launch["persistent_waves"] == 0
+    if persistent and (variant, k) == (3, 64):
+        assert launch["persistent_waves"] > 0
     refs = []
     for f in range(24):
         b = w.bodies[w.offsets[f]:w.offsets[f + 1]]
@@ -80,6 +89,7 @@ def test_batch_full_size(oracle):
     e.set_persistent(True)
     e.batch_set_frames(w.bodies, w.offsets)
     frames3, _ = e.batch_calc()
+    assert e.batch_launch()["persistent_waves"] > 0
     err = rel_err(frames3, frames).max()
     _say(f"  K=64 work-queue launch vs default: max rel diff {err:.3e}")
     assert err < 5e-6

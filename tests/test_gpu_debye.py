@@ -216,3 +216,44 @@ def test_config2_plain_full_size(oracle):
     k = int(np.argmax(err))
     _say(f"  max rel err {err.max():.3e} at q={w.q[k]:.4f}, median {np.median(err):.3e}")
     assert err.max() < TOL
+
+
+def test_autotune_and_cache(oracle, tmp_path):
+    """rsx_autotune: sweeps the FP32 kernels over K and column range, keeps the fastest,
+    appends it to the cache file; a second engine with the same key takes the cached
+    entry.  I(q) after tuning still meets the FP64 bar, and the time_sasa / time_forward
+    probes report positive device times."""
+    from resaxs_amd import configs
+    _say("case: autotune + cache, 3000 atoms x 128 q, hydrated")
+    rng = np.random.default_rng(5)
+    n = 3000
+    q = np.linspace(0.0, 0.5, 128).astype(np.float32)
+    pos = configs.in_ellipsoid(rng, n, configs.ellipsoid_axes(n)).astype(np.float32)
+    types = configs.seeded_types(rng, n)
+    table = configs.factor_table(rng, q)
+    water = table.shape[1] - 1            # the table's last column serves as the water factor
+    radii = np.full(int(types.max()) + 1, 1.6, dtype=np.float32)
+    b = configs.bodies_of(pos, types)
+    cache = str(tmp_path / "tune.txt")
+    e = engine_for(q, table, water, 0.8, n=n, radii=radii, probe=1.4, n_half=64)
+    e.calc_curve(b)
+    e.autotune(cache)
+    t1 = e.get_tuning()
+    assert t1["variant"] in (2, 3) and t1["k"] > 0 and t1["col_range"] > 0
+    tok = open(cache).read().split()   # key K L variant ms
+    assert len(tok) == 5 and int(tok[1]) == t1["k"] and int(tok[2]) == t1["col_range"] and int(tok[3]) == t1["variant"]
+    I = e.calc_curve(b)
+    assert e.time_forward(3)[1] > 0 and e.time_sasa(3) > 0
+    counts, _ = e.calc_rsasa(b, radii, 1.4, fib_dot_table(radii, 1.4, 64))
+    ref = oracle_iq(q, pos, types, table, water, 0.8, counts=counts, n_dots=128)
+    err = rel_err(I, ref).max()
+    _say(f"  tuned K={t1['k']} L={t1['col_range']} variant {t1['variant']}: max rel err {err:.3e}")
+    assert err < TOL
+    e.close()
+    f = engine_for(q, table, water, 0.8, n=n, radii=radii, probe=1.4, n_half=64)
+    f.calc_curve(b)
+    f.autotune(cache)
+    assert f.get_tuning() == t1
+    assert np.array_equal(f.calc_curve(b), I)
+    f.close()
+    assert len(open(cache).read().split()) == 5
