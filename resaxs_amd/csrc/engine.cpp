@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -288,6 +289,8 @@ struct rsx_engine {
     std::vector<int> slot_of, caller_of;
     int d1 = 1;                      // classes per type
     int n_class = 0;
+    std::vector<int> cls_count;      // atoms per class (built on demand from cls)
+    bool cls_count_ok = false;
     std::vector<double> f64;         // [n_class][nq] FP64 class factors
     bool hydrate = false;
 
@@ -333,6 +336,24 @@ struct rsx_engine {
 
     std::string tune_report;
     int last_L = 0;  // column range of the last forward job layout
+    // job table of the last incremental update, reused while its shape is unchanged
+    struct IncKey {
+        int mp = -1, n = -1, L = -1, K = -1, n_chunks = -1, variant = -1, uniform = -1;
+        const float4* rows = nullptr;
+        const float4* cols = nullptr;
+        bool operator==(const IncKey& o) const
+        {
+            return mp == o.mp && n == o.n && L == o.L && K == o.K && n_chunks == o.n_chunks &&
+                   variant == o.variant && uniform == o.uniform && rows == o.rows && cols == o.cols;
+        }
+    } inc_key;
+    int inc_njobs = 0;
+    std::vector<PairJob> inc_jobs, inc_scratch;  // host tables of the incremental jobs (capacity kept)
+    std::vector<uint32_t> cnt_scratch;            // counts downloaded by a local SASA recount
+    std::vector<char> in_m;                       // membership of the changed set (all zero between calls)
+    // host sources of small uploads that may still be in flight when launch_jobs returns
+    int up_sb[2] = {0, 0};
+    double up_q0 = 0.0;
 
     // last forward launch (for timing)
     std::vector<PairJob> last_jobs;
@@ -526,6 +547,8 @@ bool hydration_on(const rsx_engine* h)
 void build_class_table(rsx_engine* h)
 {
     h->layout_stamp++;
+    h->cls_count_ok = false;
+    h->inc_njobs = 0;
     h->hydrate = hydration_on(h);
     h->d1 = h->hydrate ? h->n_dots + 1 : 1;
     h->n_class = h->n_types * h->d1;
@@ -564,6 +587,7 @@ void assign_slots(rsx_engine* h)
     const int n = h->n;
     h->cls.resize(n);
     for (int i = 0; i < n; ++i) h->cls[i] = class_of(h, i);
+    h->cls_count_ok = false;
     std::vector<int> cnt(h->n_class + 1, 0);
     for (int i = 0; i < n; ++i) cnt[h->cls[i] + 1]++;
     for (int c = 0; c < h->n_class; ++c) cnt[c + 1] += cnt[c];
@@ -588,12 +612,21 @@ void assign_slots(rsx_engine* h)
     hipchk(hipStreamSynchronize(h->stream), "sync");
 }
 
-double sum_f0(const rsx_engine* h, const std::vector<uint32_t>& cls_list)
+// sum of F(q = 0) over the atoms, as sum over classes of (atoms in the class) x F_class(0)
+// in class order: O(n_class), and the same bits for the same class populations however
+// they were reached (full layouts and incremental updates agree)
+double sum_f0(rsx_engine* h)
 {
-    double s = 0.0;
     if (h->q0_index < 0) return 0.0;
-    for (uint32_t c : cls_list)
-        if (c != CLS_SKIP) s += h->f64[(size_t)c * h->nq + h->q0_index];
+    if (!h->cls_count_ok || (int)h->cls_count.size() != h->n_class) {
+        h->cls_count.assign(h->n_class, 0);
+        for (uint32_t c : h->cls)
+            if (c != CLS_SKIP && (int)c < h->n_class) h->cls_count[c]++;
+        h->cls_count_ok = true;
+    }
+    double s = 0.0;
+    for (int c = 0; c < h->n_class; ++c)
+        if (h->cls_count[c]) s += (double)h->cls_count[c] * h->f64[(size_t)c * h->nq + h->q0_index];
     return s;
 }
 
@@ -682,8 +715,10 @@ void compute_sasa(rsx_engine* h)
 // reach are recounted against the full, updated cell list; every other count is
 // unchanged by construction.  Returns false (nothing done) where the box holds more
 // than half the atoms: the full recount is then cheaper.
-bool compute_sasa_local(rsx_engine* h, int start, int len, const std::vector<float>& old_bodies)
+bool compute_sasa_local(rsx_engine* h, int start, int len, const std::vector<float>& old_bodies,
+                        std::vector<int>& changed)
 {
+    changed.clear();
     const int n = h->n;
     if (len <= 0 || (int)h->counts.size() != n || !h->have_rsasa) return false;
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -721,11 +756,13 @@ bool compute_sasa_local(rsx_engine* h, int start, int len, const std::vector<flo
     hipchk(hipMemcpyAsync(h->d_subset.p, sub.data(), sub.size() * sizeof(int), hipMemcpyHostToDevice, h->stream),
            "upload subset");
     run_sasa_device(h, n, h->d_subset.p, (int)sub.size());
-    std::vector<uint32_t> cnt(n);
+    std::vector<uint32_t>& cnt = h->cnt_scratch;
+    cnt.resize(n);
     hipchk(hipMemcpyAsync(cnt.data(), h->d_counts.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream),
            "download counts");
     hipchk(hipStreamSynchronize(h->stream), "sync sasa");
     for (int i : sub) {
+        if (h->counts[i] != cnt[i]) changed.push_back(i);
         h->counts[i] = cnt[i];
         h->rsasa[i] = (float)cnt[i] / (float)h->n_dots;
     }
@@ -777,6 +814,7 @@ void triangle_jobs(std::vector<PairJob>& jobs, const float4* cols, int m, int L,
 // rows [0, nr) against cols [0, nc) (full rectangle)
 void rect_jobs(std::vector<PairJob>& jobs, const float4* rows, int nr, const float4* cols, int nc, int L, float scale)
 {
+    jobs.reserve(jobs.size() + (size_t)((nr + 63) / 64) * ((nc + L - 1) / L));
     for (int r0 = 0; r0 < nr; r0 += 64)
         for (int c0 = 0; c0 < nc; c0 += L) {
             PairJob j;
@@ -792,11 +830,32 @@ void rect_jobs(std::vector<PairJob>& jobs, const float4* rows, int nr, const flo
         }
 }
 
-// order by cost (longest first) and number the slots in execution order
-void finalize_jobs(std::vector<PairJob>& jobs)
+// order by cost (longest first, ties in build order) and number the slots in execution
+// order.  The tables hold few distinct job sizes (full ranges, ragged ends, the M x M
+// blocks), so a bucket pass over the sizes replaces a comparison sort.
+void finalize_jobs(std::vector<PairJob>& jobs, std::vector<PairJob>* scratch = nullptr)
 {
-    std::stable_sort(jobs.begin(), jobs.end(),
-                     [](const PairJob& a, const PairJob& b) { return (long)a.ncol * a.nrow > (long)b.ncol * b.nrow; });
+    std::vector<long> sizes;
+    for (const PairJob& j : jobs) {
+        const long c = (long)j.ncol * j.nrow;
+        if (std::find(sizes.begin(), sizes.end(), c) == sizes.end()) sizes.push_back(c);
+        if (sizes.size() > 64) break;
+    }
+    if (sizes.size() > 64) {
+        std::stable_sort(jobs.begin(), jobs.end(), [](const PairJob& a, const PairJob& b) {
+            return (long)a.ncol * a.nrow > (long)b.ncol * b.nrow;
+        });
+    } else {
+        std::sort(sizes.begin(), sizes.end(), std::greater<long>());
+        std::vector<PairJob> local;
+        std::vector<PairJob>& out = scratch ? *scratch : local;
+        out.clear();
+        out.reserve(jobs.size());
+        for (long c : sizes)
+            for (const PairJob& j : jobs)
+                if ((long)j.ncol * j.nrow == c) out.push_back(j);
+        jobs.swap(out);
+    }
     for (size_t s = 0; s < jobs.size(); ++s) jobs[s].slot = (int)s;
 }
 
@@ -824,21 +883,34 @@ rsx::DebyeArgs debye_args(rsx_engine* h, const PairJob* d_jobs, double* partials
 }
 
 // upload jobs, launch the pair kernel and reduce into iq (single output)
+void launch_jobs(rsx_engine* h, const PairJob* d_jobs, int nj, double* iq, int accumulate, double q0val,
+                 bool sync = true);
+
 void run_jobs(rsx_engine* h, DBuf<PairJob>& jbuf, const std::vector<PairJob>& jobs, double* iq, int accumulate,
               double q0val)
 {
     const int nj = (int)jobs.size();
     jbuf.ensure(nj);
-    h->d_partials.ensure((size_t)std::max(nj, 1) * h->ldf);
-    h->d_red.ensure(std::max<size_t>(1, rsx::reduce_scratch_doubles(h->nq, 1, nj)));
-    h->d_slot_begin.ensure(2);
     if (nj > 0)
         hipchk(hipMemcpyAsync(jbuf.p, jobs.data(), nj * sizeof(PairJob), hipMemcpyHostToDevice, h->stream),
                "upload jobs");
-    int sb[2] = {0, nj};
-    hipchk(hipMemcpyAsync(h->d_slot_begin.p, sb, sizeof(sb), hipMemcpyHostToDevice, h->stream), "upload slots");
-    hipchk(hipMemcpyAsync(h->d_q0.p, &q0val, sizeof(double), hipMemcpyHostToDevice, h->stream), "upload q0");
-    const rsx::DebyeArgs a = debye_args(h, jbuf.p, h->d_partials.p);
+    launch_jobs(h, jbuf.p, nj, iq, accumulate, q0val);
+}
+
+// launch nj jobs already on the device and reduce into iq (single output); without sync
+// the work stays queued on the engine stream
+void launch_jobs(rsx_engine* h, const PairJob* d_jobs, int nj, double* iq, int accumulate, double q0val, bool sync)
+{
+    h->d_partials.ensure((size_t)std::max(nj, 1) * h->ldf);
+    h->d_red.ensure(std::max<size_t>(1, rsx::reduce_scratch_doubles(h->nq, 1, nj)));
+    h->d_slot_begin.ensure(2);
+    h->up_sb[0] = 0;
+    h->up_sb[1] = nj;
+    h->up_q0 = q0val;
+    hipchk(hipMemcpyAsync(h->d_slot_begin.p, h->up_sb, sizeof(h->up_sb), hipMemcpyHostToDevice, h->stream),
+           "upload slots");
+    hipchk(hipMemcpyAsync(h->d_q0.p, &h->up_q0, sizeof(double), hipMemcpyHostToDevice, h->stream), "upload q0");
+    const rsx::DebyeArgs a = debye_args(h, d_jobs, h->d_partials.p);
     hipchk(rsx::launch_debye(a, nj, h->K, h->uniform, h->stream), "debye kernel");
     if (nj > 0) {
         hipchk(rsx::launch_reduce(h->d_partials.p, h->ldf, h->nq, h->d_slot_begin.p, 1, iq, h->nq, accumulate,
@@ -848,8 +920,9 @@ void run_jobs(rsx_engine* h, DBuf<PairJob>& jbuf, const std::vector<PairJob>& jo
         std::vector<double> z(h->nq, 0.0);
         if (h->q0_index >= 0) z[h->q0_index] = q0val;
         hipchk(hipMemcpyAsync(iq, z.data(), h->nq * sizeof(double), hipMemcpyHostToDevice, h->stream), "zero iq");
+        hipchk(hipStreamSynchronize(h->stream), "sync zero iq");  // z is a local buffer
     }
-    hipchk(hipStreamSynchronize(h->stream), "sync pair kernels");
+    if (sync) hipchk(hipStreamSynchronize(h->stream), "sync pair kernels");
 }
 
 // Order of the forward (triangle) jobs, and a finer tail.  A job costs its columns plus
@@ -925,7 +998,7 @@ std::vector<PairJob> forward_jobs(rsx_engine* h)
 void full_forward(rsx_engine* h)
 {
     const std::vector<PairJob> jobs = forward_jobs(h);
-    const double sf = sum_f0(h, h->cls);
+    const double sf = sum_f0(h);
     run_jobs(h, h->d_fjobs, jobs, h->d_iq.p, 0, sf * sf);
     h->last_jobs = jobs;
     h->last_q0 = sf * sf;
@@ -938,7 +1011,9 @@ void full_forward(rsx_engine* h)
 void copy_out(rsx_engine* h, float* out32, double* out64)
 {
     std::vector<double> tmp(h->nq);
-    hipchk(hipMemcpy(tmp.data(), h->d_iq.p, h->nq * sizeof(double), hipMemcpyDeviceToHost), "download iq");
+    hipchk(hipMemcpyAsync(tmp.data(), h->d_iq.p, h->nq * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+           "download iq");
+    hipchk(hipStreamSynchronize(h->stream), "sync download iq");
     for (int k = 0; k < h->nq; ++k) {
         if (out32) out32[k] = (float)tmp[k];
         if (out64) out64[k] = tmp[k];
@@ -1023,24 +1098,31 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
         return;
     }
 
-    std::vector<uint32_t> old_counts;
-    if (sasa_now) {
-        old_counts = h->counts;
-        if (!compute_sasa_local(h, start, len, old_bodies)) compute_sasa(h);
+    // atoms whose exposed-dot count changed: from the local recount's subset, or by
+    // comparing all counts after a full recount
+    std::vector<int> changed;
+    if (sasa_now && !compute_sasa_local(h, start, len, old_bodies, changed)) {
+        const std::vector<uint32_t> old_counts = h->counts;
+        compute_sasa(h);
+        changed.clear();
+        for (int i = 0; i < h->n; ++i)
+            if (old_counts[i] != h->counts[i]) changed.push_back(i);
     }
     // changed set: moved atoms plus atoms whose class changed
     std::vector<int> M;
-    std::vector<char> in_m(h->n, 0);
+    M.reserve((size_t)len + changed.size());
+    std::vector<char>& in_m = h->in_m;
+    in_m.resize(h->n, 0);
     for (int i = start; i < start + len; ++i) {
         M.push_back(i);
         in_m[i] = 1;
     }
-    if (sasa_now)
-        for (int i = 0; i < h->n; ++i)
-            if (!in_m[i] && old_counts[i] != h->counts[i]) {
-                M.push_back(i);
-                in_m[i] = 1;
-            }
+    for (int i : changed)
+        if (!in_m[i]) {
+            M.push_back(i);
+            in_m[i] = 1;
+        }
+    for (int i : M) in_m[i] = 0;
     if (M.empty()) {
         copy_out(h, out32, out64);
         return;
@@ -1097,29 +1179,49 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
            "upload vals");
     hipchk(rsx::launch_scatter(h->d_cols.p, h->d_idx.p, h->d_vals.p, m, h->stream), "scatter skip");
 
-    std::vector<PairJob> jobs;
+    // the job table depends only on the shape (moved-set size, N, column range, kernel) and
+    // the buffers: reused as it is on the device while those are unchanged
     const int L = pick_range(h, 2.0 * mp * (double)h->n, 2 * mp / 64);
-    rect_jobs(jobs, h->d_rows.p, mp, h->d_cols.p, h->n, L, 2.0f);            // new x rest
-    rect_jobs(jobs, h->d_rows.p + mp, mp, h->d_cols.p, h->n, L, -2.0f);      // old x rest
-    rect_jobs(jobs, h->d_rows.p, mp, h->d_rows.p, mp, L, 1.0f);             // new x new
-    rect_jobs(jobs, h->d_rows.p + mp, mp, h->d_rows.p + mp, mp, L, -1.0f);  // old x old
-    finalize_jobs(jobs);
-
-    // q = 0: (sum F_new)^2 - (sum F_old)^2 from the class lists
-    double q0d = 0.0;
-    if (h->q0_index >= 0) {
-        const double s_old = sum_f0(h, h->cls);
-        double s_new = s_old;
-        for (int a = 0; a < m; ++a) {
-            s_new -= h->f64[(size_t)cls_old[a] * h->nq + h->q0_index];
-            s_new += h->f64[(size_t)cls_new[a] * h->nq + h->q0_index];
-        }
-        q0d = s_new * s_new - s_old * s_old;
+    rsx_engine::IncKey key;
+    key.mp = mp;
+    key.n = h->n;
+    key.L = L;
+    key.K = h->K;
+    key.n_chunks = h->n_chunks;
+    key.variant = h->variant;
+    key.uniform = h->uniform ? 1 : 0;
+    key.rows = h->d_rows.p;
+    key.cols = h->d_cols.p;
+    if (!(h->inc_njobs > 0 && key == h->inc_key)) {
+        std::vector<PairJob>& jobs = h->inc_jobs;
+        jobs.clear();
+        rect_jobs(jobs, h->d_rows.p, mp, h->d_cols.p, h->n, L, 2.0f);            // new x rest
+        rect_jobs(jobs, h->d_rows.p + mp, mp, h->d_cols.p, h->n, L, -2.0f);      // old x rest
+        rect_jobs(jobs, h->d_rows.p, mp, h->d_rows.p, mp, L, 1.0f);             // new x new
+        rect_jobs(jobs, h->d_rows.p + mp, mp, h->d_rows.p + mp, mp, L, -1.0f);  // old x old
+        finalize_jobs(jobs, &h->inc_scratch);
+        h->d_jobs.ensure(jobs.size());
+        // (the host table stays alive in the engine until the copy has been consumed)
+        hipchk(hipMemcpyAsync(h->d_jobs.p, jobs.data(), jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice,
+                              h->stream),
+               "upload jobs");
+        h->inc_key = key;
+        h->inc_njobs = (int)jobs.size();
     }
-    run_jobs(h, h->d_jobs, jobs, h->d_iq.p, 1, q0d);
+
+    // q = 0: (sum F_new)^2 - (sum F_old)^2 from the class populations
+    const double s_old = sum_f0(h);
+    for (int a = 0; a < m; ++a) {
+        h->cls[M[a]] = cls_new[a];
+        if (h->cls_count_ok) {
+            h->cls_count[cls_old[a]]--;
+            h->cls_count[cls_new[a]]++;
+        }
+    }
+    const double s_new = sum_f0(h);
+    const double q0d = s_new * s_new - s_old * s_old;
+    launch_jobs(h, h->d_jobs.p, h->inc_njobs, h->d_iq.p, 1, q0d, false);
     hipchk(rsx::launch_scatter(h->d_cols.p, h->d_idx.p, h->d_vals.p + m, m, h->stream), "scatter new");
-    hipchk(hipStreamSynchronize(h->stream), "sync scatter");
-    for (int a = 0; a < m; ++a) h->cls[M[a]] = cls_new[a];
     h->since_rebase++;
     copy_out(h, out32, out64);
 }
@@ -1156,6 +1258,7 @@ void revert_impl(rsx_engine* h)
     h->bodies = h->saved.bodies;
     h->counts = h->saved.counts;
     h->cls = h->saved.cls;
+    h->cls_count_ok = false;
     h->rsasa = h->saved.rsasa;
     h->slot_of = h->saved.slot_of;
     h->caller_of = h->saved.caller_of;
@@ -1892,6 +1995,7 @@ int rsx_destroy(rsx_engine* h)
     h->h_pidx.release();
     h->d_jobs.release();
     h->d_fjobs.release();
+    h->inc_njobs = 0;
     h->d_f0col.release();
     h->d_slot_begin_f.release();
     h->b_jobs.release();
