@@ -348,6 +348,9 @@ struct rsx_engine {
         }
     } inc_key;
     int inc_njobs = 0;
+    DBuf<ChunkConst> d_chunks_inc;  // q chunks of the incremental kernel
+    int inc_chunks_K = 0, inc_chunks_n = 0;
+    uint64_t inc_chunks_stamp = ~0ull;
     std::vector<PairJob> inc_jobs, inc_scratch;  // host tables of the incremental jobs (capacity kept)
     std::vector<uint32_t> cnt_scratch;            // counts downloaded by a local SASA recount
     std::vector<char> in_m;                       // membership of the changed set (all zero between calls)
@@ -770,24 +773,74 @@ bool compute_sasa_local(rsx_engine* h, int start, int len, const std::vector<flo
 }
 
 // ---- pair jobs ---------------------------------------------------------------
-int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles)
+int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles, int variant, int K, int n_chunks)
 {
     if (h->range_req > 0) return h->range_req;
     // resident waves of the pair kernel (the runtime's occupancy for the FP32 kernels,
     // two per SIMD otherwise)
     int per_cu = 0;
-    if (h->uniform && (h->variant == rsx::VARIANT_FP32 || h->variant == rsx::VARIANT_FP32R))
-        per_cu = rsx::rec32_blocks_per_cu(h->variant, h->K, rsx::sin_series_terms(h->dq * h->path_radius), false);
+    if (h->uniform && (variant == rsx::VARIANT_FP32 || variant == rsx::VARIANT_FP32R))
+        per_cu = rsx::rec32_blocks_per_cu(variant, K, rsx::sin_series_terms(h->dq * h->path_radius), false);
     if (per_cu <= 0) per_cu = 8;
     const double slots = (double)std::max(1, h->n_cu) * per_cu;
     // about six rounds of waves over the resident slots: fewer leave a partly filled last
     // round (measured on incremental updates: L = 64-128 beats 512 by 1.3-2.2x), more
     // pay the per-job set-up and epilogue more often.  64..1024 columns, rounded down.
-    const double jobs = std::max(1.0, 6.0 * slots / std::max(1, h->n_chunks));
+    const double jobs = std::max(1.0, 6.0 * slots / std::max(1, n_chunks));
     double L = column_atoms_total / 64.0 / jobs;
     (void)row_tiles;
     L = std::max(64.0, std::min(1024.0, L));
     return std::max(64, (int)L / 64 * 64);
+}
+
+int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles)
+{
+    return pick_range(h, column_atoms_total, row_tiles, h->variant, h->K, h->n_chunks);
+}
+
+// Kernel of the incremental launches.  Their rows are the changed atoms sorted by class;
+// with hydration classes (type x exposed-dot count) a 64-row tile holds tens of classes,
+// so the run-flush kernel's per-flush loads of the 64 rows' factors miss the caches.  The
+// per-lane-sum kernel applies each lane's row factor once at the end, and the column
+// factor of a flush is one scalar load: hydrated moves take it (3.63 -> 3.10 ms per
+// 500-atom move at 50k atoms), plain ones keep the forward kernel (0.91 vs 1.15 ms).  It
+// runs on the forward's factor table (stride ldf) with its own q chunks of K_inc: the
+// largest K of that kernel at most 64 whose chunks fit in ldf.
+void incremental_kernel(rsx_engine* h, int& variant, int& K, int& n_chunks, const ChunkConst*& chunks)
+{
+    variant = h->variant;
+    K = h->K;
+    n_chunks = h->n_chunks;
+    chunks = h->d_chunks.p;
+    if (!h->uniform || h->variant != rsx::VARIANT_FP32R || !h->hydrate) return;
+    int best = 0;
+    for (int i = 0; i < rsx::debye_k_count(rsx::PATH_REC32); ++i) {
+        const int k = rsx::debye_k_value(rsx::PATH_REC32, i);
+        if (k <= 64 && k > best && (long)((h->nq + k - 1) / k) * k <= h->ldf) best = k;
+    }
+    if (best <= 0) return;
+    if (h->inc_chunks_K != best || h->inc_chunks_stamp != h->layout_stamp) {
+        const int nc = (h->nq + best - 1) / best;
+        std::vector<ChunkConst> cc(nc);
+        for (int c = 0; c < nc; ++c) {
+            const int k0 = c * best;
+            const double a = k0 < h->nq ? (double)h->q[k0] : 0.0;
+            cc[c].a = (float)a;
+            cc[c].a_rev = a / (2.0 * PI);
+            cc[c].mid_rev = (a + 0.5 * h->dq) / (2.0 * PI);
+            cc[c].pad = 0.0f;
+        }
+        h->d_chunks_inc.ensure(nc);
+        hipchk(hipMemcpy(h->d_chunks_inc.p, cc.data(), nc * sizeof(ChunkConst), hipMemcpyHostToDevice),
+               "upload incremental chunks");
+        h->inc_chunks_K = best;
+        h->inc_chunks_n = nc;
+        h->inc_chunks_stamp = h->layout_stamp;
+    }
+    variant = rsx::VARIANT_FP32;
+    K = best;
+    n_chunks = h->inc_chunks_n;
+    chunks = h->d_chunks_inc.p;
 }
 
 // triangle jobs over records [0, m) of cols (rows == cols); appends to jobs
@@ -883,8 +936,12 @@ rsx::DebyeArgs debye_args(rsx_engine* h, const PairJob* d_jobs, double* partials
 }
 
 // upload jobs, launch the pair kernel and reduce into iq (single output)
+struct KernelChoice {
+    int variant, K, n_chunks;
+    const ChunkConst* chunks;
+};
 void launch_jobs(rsx_engine* h, const PairJob* d_jobs, int nj, double* iq, int accumulate, double q0val,
-                 bool sync = true);
+                 bool sync = true, const KernelChoice* kc = nullptr);
 
 void run_jobs(rsx_engine* h, DBuf<PairJob>& jbuf, const std::vector<PairJob>& jobs, double* iq, int accumulate,
               double q0val)
@@ -899,7 +956,8 @@ void run_jobs(rsx_engine* h, DBuf<PairJob>& jbuf, const std::vector<PairJob>& jo
 
 // launch nj jobs already on the device and reduce into iq (single output); without sync
 // the work stays queued on the engine stream
-void launch_jobs(rsx_engine* h, const PairJob* d_jobs, int nj, double* iq, int accumulate, double q0val, bool sync)
+void launch_jobs(rsx_engine* h, const PairJob* d_jobs, int nj, double* iq, int accumulate, double q0val, bool sync,
+                 const KernelChoice* kc)
 {
     h->d_partials.ensure((size_t)std::max(nj, 1) * h->ldf);
     h->d_red.ensure(std::max<size_t>(1, rsx::reduce_scratch_doubles(h->nq, 1, nj)));
@@ -910,8 +968,15 @@ void launch_jobs(rsx_engine* h, const PairJob* d_jobs, int nj, double* iq, int a
     hipchk(hipMemcpyAsync(h->d_slot_begin.p, h->up_sb, sizeof(h->up_sb), hipMemcpyHostToDevice, h->stream),
            "upload slots");
     hipchk(hipMemcpyAsync(h->d_q0.p, &h->up_q0, sizeof(double), hipMemcpyHostToDevice, h->stream), "upload q0");
-    const rsx::DebyeArgs a = debye_args(h, d_jobs, h->d_partials.p);
-    hipchk(rsx::launch_debye(a, nj, h->K, h->uniform, h->stream), "debye kernel");
+    rsx::DebyeArgs a = debye_args(h, d_jobs, h->d_partials.p);
+    int K = h->K;
+    if (kc) {
+        a.variant = kc->variant;
+        a.n_chunks = kc->n_chunks;
+        a.chunks = kc->chunks;
+        K = kc->K;
+    }
+    hipchk(rsx::launch_debye(a, nj, K, h->uniform, h->stream), "debye kernel");
     if (nj > 0) {
         hipchk(rsx::launch_reduce(h->d_partials.p, h->ldf, h->nq, h->d_slot_begin.p, 1, iq, h->nq, accumulate,
                                   h->d_q0.p, h->q0_index, nj, h->d_red.p, h->stream),
@@ -1181,14 +1246,16 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
 
     // the job table depends only on the shape (moved-set size, N, column range, kernel) and
     // the buffers: reused as it is on the device while those are unchanged
-    const int L = pick_range(h, 2.0 * mp * (double)h->n, 2 * mp / 64);
+    KernelChoice kc;
+    incremental_kernel(h, kc.variant, kc.K, kc.n_chunks, kc.chunks);
+    const int L = pick_range(h, 2.0 * mp * (double)h->n, 2 * mp / 64, kc.variant, kc.K, kc.n_chunks);
     rsx_engine::IncKey key;
     key.mp = mp;
     key.n = h->n;
     key.L = L;
-    key.K = h->K;
-    key.n_chunks = h->n_chunks;
-    key.variant = h->variant;
+    key.K = kc.K;
+    key.n_chunks = kc.n_chunks;
+    key.variant = kc.variant;
     key.uniform = h->uniform ? 1 : 0;
     key.rows = h->d_rows.p;
     key.cols = h->d_cols.p;
@@ -1220,7 +1287,7 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
     }
     const double s_new = sum_f0(h);
     const double q0d = s_new * s_new - s_old * s_old;
-    launch_jobs(h, h->d_jobs.p, h->inc_njobs, h->d_iq.p, 1, q0d, false);
+    launch_jobs(h, h->d_jobs.p, h->inc_njobs, h->d_iq.p, 1, q0d, false, &kc);
     hipchk(rsx::launch_scatter(h->d_cols.p, h->d_idx.p, h->d_vals.p + m, m, h->stream), "scatter new");
     h->since_rebase++;
     copy_out(h, out32, out64);
@@ -1983,6 +2050,8 @@ int rsx_destroy(rsx_engine* h)
                        &h->d_tile_sum};
     for (auto* b : fi) b->release();
     h->d_chunks.release();
+    h->d_chunks_inc.release();
+    h->inc_chunks_K = 0;
     h->a_chunks.release();
     h->a_invq.release();
     h->a_jobs.release();
