@@ -565,8 +565,9 @@ __device__ __forceinline__ void rec32_item(const DebyeArgs& a, int job_id, int c
         }
     });
 
-    // epilogue as in debye_uniform_kernel, in slices of up to 64 q so the LDS
-    // transpose stays 16.6 KB for any K
+    // epilogue as in debye_uniform_kernel, in slices of up to KS (32) q so the LDS
+    // transpose stays 8.3 KB for any K: LDS then does not limit the waves per CU below
+    // what the registers allow (16.6 KB slices held it to nine per CU)
     const uint32_t ci = cls_i == CLS_SKIP ? (uint32_t)a.zero_class : cls_i;
     const float* __restrict__ fi = ftab + (size_t)ci * ldf + k0;
 #pragma unroll
@@ -653,7 +654,7 @@ __device__ __forceinline__ void rec32r_item(const DebyeArgs& a, int job_id, int 
 template <int K, int M>
 __global__ __launch_bounds__(64, 2) void debye_rec32_kernel(const DebyeArgs a)
 {
-    constexpr int KS = K < 64 ? K : 64;
+    constexpr int KS = K < 32 ? K : 32;
     __shared__ float T[KS][65];
     const int job_id = blockIdx.x / a.n_chunks;
     rec32_item<K, M, KS>(a, job_id, blockIdx.x - job_id * a.n_chunks, T);
@@ -666,7 +667,7 @@ __global__ __launch_bounds__(64, 2) void debye_rec32_kernel(const DebyeArgs a)
 template <int K, int M>
 __global__ __launch_bounds__(64, 2) void debye_rec32_persistent(const DebyeArgs a, int n_items)
 {
-    constexpr int KS = K < 64 ? K : 64;
+    constexpr int KS = K < 32 ? K : 32;
     __shared__ float T[KS][65];
     for (;;) {
         int item = 0;
