@@ -802,8 +802,8 @@ int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles)
 // with hydration classes (type x exposed-dot count) a 64-row tile holds tens of classes,
 // so the run-flush kernel's per-flush loads of the 64 rows' factors miss the caches.  The
 // per-lane-sum kernel applies each lane's row factor once at the end, and the column
-// factor of a flush is one scalar load: hydrated moves take it (3.63 -> 3.10 ms per
-// 500-atom move at 50k atoms), plain ones keep the forward kernel (0.91 vs 1.15 ms).  It
+// factor of a flush is one scalar load: hydrated moves take it (3.63 -> 2.70 ms per
+// 500-atom move at 50k atoms), plain ones keep the forward kernel (0.89 vs 1.01 ms).  It
 // runs on the forward's factor table (stride ldf) with its own q chunks of K_inc: the
 // largest K of that kernel at most 64 whose chunks fit in ldf.
 void incremental_kernel(rsx_engine* h, int& variant, int& K, int& n_chunks, const ChunkConst*& chunks)
