@@ -76,6 +76,32 @@ struct DBuf {
     }
 };
 
+// pinned host staging buffer: copies from it are truly asynchronous, so its contents
+// must stay unchanged until the stream has passed the copy (every user synchronizes its
+// stream before the next use of the same buffer)
+template <typename T>
+struct HBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    void ensure(size_t n)
+    {
+        if (n <= cap && p) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        if (n == 0) n = 1;
+        hipError_t e = hipHostMalloc((void**)&p, n * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) fail(RSX_OUT_OF_MEMORY, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        cap = n;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
 inline uint32_t fbits(float f)
 {
     uint32_t u;
@@ -352,7 +378,12 @@ struct rsx_engine {
     int inc_chunks_K = 0, inc_chunks_n = 0;
     uint64_t inc_chunks_stamp = ~0ull;
     std::vector<PairJob> inc_jobs, inc_scratch;  // host tables of the incremental jobs (capacity kept)
-    std::vector<uint32_t> cnt_scratch;            // counts downloaded by a local SASA recount
+    // pinned staging of the per-update transfers
+    HBuf<float4> pin_rows, pin_vals, pin_moved;
+    HBuf<int> pin_idx, pin_sub;
+    HBuf<uint32_t> pin_cnt;
+    HBuf<double> pin_iq;
+    DBuf<uint32_t> d_subcnt;                      // counts of a local recount's subset, gathered
     std::vector<char> in_m;                       // membership of the changed set (all zero between calls)
     // host sources of small uploads that may still be in flight when launch_jobs returns
     int up_sb[2] = {0, 0};
@@ -747,27 +778,37 @@ bool compute_sasa_local(rsx_engine* h, int start, int len, const std::vector<flo
     for (int i : sub)
         if (h->type[i] >= h->radii.size()) fail(RSX_INVALID_ARG, "body type index outside the SASA radius table");
     h->d_sasa_atoms.ensure(n);
-    if (h->sasa_mirror)
-        hipchk(hipMemcpyAsync(h->d_sasa_atoms.p + start, &h->bodies[4 * (size_t)start], (size_t)len * 16,
-                              hipMemcpyHostToDevice, h->stream),
+    if (h->sasa_mirror) {
+        h->pin_moved.ensure(len);
+        std::memcpy(h->pin_moved.p, &h->bodies[4 * (size_t)start], (size_t)len * 16);
+        hipchk(hipMemcpyAsync(h->d_sasa_atoms.p + start, h->pin_moved.p, (size_t)len * 16, hipMemcpyHostToDevice,
+                              h->stream),
                "upload moved sasa atoms");
-    else
+    } else {
         hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, h->bodies.data(), (size_t)n * 16, hipMemcpyHostToDevice, h->stream),
                "upload sasa atoms");
+    }
     h->sasa_mirror = true;
-    h->d_subset.ensure(sub.size());
-    hipchk(hipMemcpyAsync(h->d_subset.p, sub.data(), sub.size() * sizeof(int), hipMemcpyHostToDevice, h->stream),
+    const int ns = (int)sub.size();
+    h->d_subset.ensure(ns);
+    h->pin_sub.ensure(ns);
+    std::memcpy(h->pin_sub.p, sub.data(), ns * sizeof(int));
+    hipchk(hipMemcpyAsync(h->d_subset.p, h->pin_sub.p, ns * sizeof(int), hipMemcpyHostToDevice, h->stream),
            "upload subset");
-    run_sasa_device(h, n, h->d_subset.p, (int)sub.size());
-    std::vector<uint32_t>& cnt = h->cnt_scratch;
-    cnt.resize(n);
-    hipchk(hipMemcpyAsync(cnt.data(), h->d_counts.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream),
+    run_sasa_device(h, n, h->d_subset.p, ns);
+    // only the subset's counts come back (gathered on the device)
+    h->d_subcnt.ensure(ns);
+    h->pin_cnt.ensure(ns);
+    hipchk(rsx::launch_gather_u32(h->d_subcnt.p, h->d_counts.p, h->d_subset.p, ns, h->stream), "gather counts");
+    hipchk(hipMemcpyAsync(h->pin_cnt.p, h->d_subcnt.p, ns * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream),
            "download counts");
     hipchk(hipStreamSynchronize(h->stream), "sync sasa");
-    for (int i : sub) {
-        if (h->counts[i] != cnt[i]) changed.push_back(i);
-        h->counts[i] = cnt[i];
-        h->rsasa[i] = (float)cnt[i] / (float)h->n_dots;
+    for (int t = 0; t < ns; ++t) {
+        const int i = sub[t];
+        const uint32_t c = h->pin_cnt.p[t];
+        if (h->counts[i] != c) changed.push_back(i);
+        h->counts[i] = c;
+        h->rsasa[i] = (float)c / (float)h->n_dots;
     }
     return true;
 }
@@ -1075,9 +1116,9 @@ void full_forward(rsx_engine* h)
 
 void copy_out(rsx_engine* h, float* out32, double* out64)
 {
-    std::vector<double> tmp(h->nq);
-    hipchk(hipMemcpyAsync(tmp.data(), h->d_iq.p, h->nq * sizeof(double), hipMemcpyDeviceToHost, h->stream),
-           "download iq");
+    h->pin_iq.ensure(h->nq);
+    double* tmp = h->pin_iq.p;
+    hipchk(hipMemcpyAsync(tmp, h->d_iq.p, h->nq * sizeof(double), hipMemcpyDeviceToHost, h->stream), "download iq");
     hipchk(hipStreamSynchronize(h->stream), "sync download iq");
     for (int k = 0; k < h->nq; ++k) {
         if (out32) out32[k] = (float)tmp[k];
@@ -1236,11 +1277,18 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
     h->d_rows.ensure(rows.size());
     h->d_idx.ensure(m);
     h->d_vals.ensure(2 * (size_t)m);
-    hipchk(hipMemcpyAsync(h->d_rows.p, rows.data(), rows.size() * sizeof(float4), hipMemcpyHostToDevice, h->stream),
+    h->pin_rows.ensure(rows.size());
+    h->pin_idx.ensure(m);
+    h->pin_vals.ensure(vals.size());
+    std::memcpy(h->pin_rows.p, rows.data(), rows.size() * sizeof(float4));
+    std::memcpy(h->pin_idx.p, slots.data(), m * sizeof(int));
+    std::memcpy(h->pin_vals.p, vals.data(), vals.size() * sizeof(float4));
+    hipchk(hipMemcpyAsync(h->d_rows.p, h->pin_rows.p, rows.size() * sizeof(float4), hipMemcpyHostToDevice, h->stream),
            "upload rows");
-    hipchk(hipMemcpyAsync(h->d_idx.p, slots.data(), m * sizeof(int), hipMemcpyHostToDevice, h->stream),
+    hipchk(hipMemcpyAsync(h->d_idx.p, h->pin_idx.p, m * sizeof(int), hipMemcpyHostToDevice, h->stream),
            "upload idx");
-    hipchk(hipMemcpyAsync(h->d_vals.p, vals.data(), vals.size() * sizeof(float4), hipMemcpyHostToDevice, h->stream),
+    hipchk(hipMemcpyAsync(h->d_vals.p, h->pin_vals.p, vals.size() * sizeof(float4), hipMemcpyHostToDevice,
+                          h->stream),
            "upload vals");
     hipchk(rsx::launch_scatter(h->d_cols.p, h->d_idx.p, h->d_vals.p, m, h->stream), "scatter skip");
 
@@ -2052,6 +2100,14 @@ int rsx_destroy(rsx_engine* h)
     h->d_chunks.release();
     h->d_chunks_inc.release();
     h->inc_chunks_K = 0;
+    h->d_subcnt.release();
+    h->pin_rows.release();
+    h->pin_vals.release();
+    h->pin_moved.release();
+    h->pin_idx.release();
+    h->pin_sub.release();
+    h->pin_cnt.release();
+    h->pin_iq.release();
     h->a_chunks.release();
     h->a_invq.release();
     h->a_jobs.release();

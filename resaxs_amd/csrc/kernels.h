@@ -113,6 +113,8 @@ hipError_t launch_frame_mean(const double* in, int n_frames, int n_q, double* ou
 
 // cols[slot[t]] = vals[t]
 hipError_t launch_scatter(float4* cols, const int* slot, const float4* vals, int m, hipStream_t s);
+// dst[t] = src[idx[t]] for t < m
+hipError_t launch_gather_u32(uint32_t* dst, const uint32_t* src, const int* idx, int m, hipStream_t s);
 
 // cols[s] = {bodies[src[s]].xyz, class}, class = type(bodies[src[s]]) * d1 + (counts ? counts[src[s]] : 0)
 hipError_t launch_gather_cols(float4* cols, const float4* bodies, const int* src, const uint32_t* counts,

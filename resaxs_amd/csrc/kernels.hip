@@ -890,6 +890,12 @@ __global__ void scatter_kernel(float4* cols, const int* __restrict__ slot, const
     if (t < m) cols[slot[t]] = vals[t];
 }
 
+__global__ void gather_u32_kernel(uint32_t* dst, const uint32_t* __restrict__ src, const int* __restrict__ idx, int m)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) dst[t] = src[idx[t]];
+}
+
 __global__ void gather_cols_kernel(float4* cols, const float4* __restrict__ bodies, const int* __restrict__ src,
                                    const uint32_t* __restrict__ counts, int d1, int n)
 {
@@ -1729,6 +1735,13 @@ hipError_t launch_scatter(float4* cols, const int* slot, const float4* vals, int
 {
     if (m <= 0) return hipSuccess;
     hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, cols, slot, vals, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_u32(uint32_t* dst, const uint32_t* src, const int* idx, int m, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_u32_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, dst, src, idx, m);
     return hipGetLastError();
 }
 
