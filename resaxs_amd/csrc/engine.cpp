@@ -364,12 +364,12 @@ struct rsx_engine {
     int last_L = 0;  // column range of the last forward job layout
     // job table of the last incremental update, reused while its shape is unchanged
     struct IncKey {
-        int mp = -1, n = -1, L = -1, K = -1, n_chunks = -1, variant = -1, uniform = -1;
+        int mp = -1, p1 = -1, p2 = -1, n = -1, L = -1, K = -1, n_chunks = -1, variant = -1, uniform = -1;
         const float4* rows = nullptr;
         const float4* cols = nullptr;
         bool operator==(const IncKey& o) const
         {
-            return mp == o.mp && n == o.n && L == o.L && K == o.K && n_chunks == o.n_chunks &&
+            return mp == o.mp && p1 == o.p1 && p2 == o.p2 && n == o.n && L == o.L && K == o.K && n_chunks == o.n_chunks &&
                    variant == o.variant && uniform == o.uniform && rows == o.rows && cols == o.cols;
         }
     } inc_key;
@@ -588,7 +588,9 @@ void build_class_table(rsx_engine* h)
     h->n_class = h->n_types * h->d1;
     const int nq = h->nq, ldf = h->ldf;
     h->f64.assign((size_t)h->n_class * nq, 0.0);
-    std::vector<float> ft((size_t)(h->n_class + 1) * ldf, 0.0f);  // + an all-zero row
+    // + an all-zero row; with hydration + 2 D + 1 delta rows (delta_class)
+    const int n_delta = h->hydrate ? 2 * h->n_dots + 1 : 0;
+    std::vector<float> ft((size_t)(h->n_class + 1 + n_delta) * ldf, 0.0f);
     for (int t = 0; t < h->n_types; ++t)
         for (int c = 0; c < h->d1; ++c) {
             const int id = t * h->d1 + c;
@@ -601,6 +603,14 @@ void build_class_table(rsx_engine* h)
                 ft[(size_t)id * ldf + k] = (float)f;
             }
         }
+    // delta rows: the factor change of an atom whose exposed-dot count changed by d,
+    // (d / D) w f_water(q), for d in [-D, D]
+    for (int d = -h->n_dots; h->hydrate && d <= h->n_dots; ++d) {
+        const size_t id = (size_t)h->n_class + 1 + (size_t)(d + h->n_dots);
+        for (int k = 0; k < nq; ++k)
+            ft[id * ldf + k] = (float)((double)d / (double)h->n_dots *
+                                       ((double)h->w * (double)h->table[(size_t)k * h->n_types + h->water]));
+    }
     h->d_ftab.ensure(ft.size());
     hipchk(hipMemcpy(h->d_ftab.p, ft.data(), ft.size() * sizeof(float), hipMemcpyHostToDevice), "upload ftab");
     std::vector<double> f0(h->n_class, 0.0);
@@ -608,6 +618,12 @@ void build_class_table(rsx_engine* h)
         for (int c = 0; c < h->n_class; ++c) f0[c] = h->f64[(size_t)c * nq + h->q0_index];
     h->d_f0col.ensure(h->n_class);
     hipchk(hipMemcpy(h->d_f0col.p, f0.data(), f0.size() * sizeof(double), hipMemcpyHostToDevice), "upload f0");
+}
+
+// class id of a delta row: an exposed-dot count change of d (hydration layouts only)
+uint32_t delta_class(const rsx_engine* h, int d)
+{
+    return (uint32_t)(h->n_class + 1 + d + h->n_dots);
 }
 
 uint32_t class_of(const rsx_engine* h, int i)
@@ -1242,10 +1258,6 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
 
     const int m = (int)M.size();
     const int mp = (m + 63) / 64 * 64;
-    // rows: [new (mp) | old (mp)], each half sorted by class (row tiles of few classes,
-    // and class runs when the rows serve as columns of the M x M blocks); pads carry the
-    // skip class
-    std::vector<float4> rows(2 * (size_t)mp, make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP)));
     std::vector<uint32_t> cls_new(m), cls_old(m);
     std::vector<float4> rec_new(m), rec_old(m);
     for (int a = 0; a < m; ++a) {
@@ -1257,14 +1269,45 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
         rec_new[a] = make_float4(nb[0], nb[1], nb[2], bitsf(cls_new[a]));
         rec_old[a] = make_float4(ob[0], ob[1], ob[2], bitsf(cls_old[a]));
     }
-    std::vector<int> order_new(m), order_old(m);
-    std::iota(order_new.begin(), order_new.end(), 0);
-    std::iota(order_old.begin(), order_old.end(), 0);
-    std::stable_sort(order_new.begin(), order_new.end(), [&](int x, int y) { return cls_new[x] < cls_new[y]; });
-    std::stable_sort(order_old.begin(), order_old.end(), [&](int x, int y) { return cls_old[x] < cls_old[y]; });
-    for (int t = 0; t < m; ++t) {
-        rows[t] = rec_new[order_new[t]];
-        rows[mp + t] = rec_old[order_old[t]];
+    // Rows, four segments, each sorted by class (row tiles of few classes, and class runs
+    // where rows serve as columns) and padded with skip records to a multiple of 64:
+    //   [M new (mp) | M old (mp)]  the M x M blocks (+1 / -1);
+    //   [P1 (p1) | P2 (p2)]        against the unchanged atoms: P1 = moved atoms' new
+    //                              records and, for atoms whose exposed-dot count changed
+    //                              without moving (u'_ij = u_ij there), one delta-class
+    //                              record carrying F' - F (+2); P2 = the moved atoms' old
+    //                              records (-2).
+    std::vector<int> p1_of, p2_of;  // indices into M
+    std::vector<float4> rec_p1;
+    for (int a = 0; a < m; ++a) {
+        const int i = M[a];
+        if (i >= start && i < start + len) {
+            p1_of.push_back(a);
+            p2_of.push_back(a);
+            rec_p1.push_back(rec_new[a]);
+        } else {
+            const int d = (int)cls_new[a] - (int)cls_old[a];  // same type: the count change
+            p1_of.push_back(a);
+            rec_p1.push_back(make_float4(rec_new[a].x, rec_new[a].y, rec_new[a].z, bitsf(delta_class(h, d))));
+        }
+    }
+    const int n1 = (int)p1_of.size(), n2 = (int)p2_of.size();
+    // every changed atom moved (always so without hydration): P1, P2 are the M segments
+    const bool p_is_m = n2 == m;
+    const int p1 = p_is_m ? 0 : (n1 + 63) / 64 * 64, p2 = p_is_m ? 0 : (n2 + 63) / 64 * 64;
+    std::vector<float4> rows(2 * (size_t)mp + p1 + p2, make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP)));
+    auto place = [&](size_t at, int cnt, auto&& rec, auto&& cls) {
+        std::vector<int> order(cnt);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cls(x) < cls(y); });
+        for (int t = 0; t < cnt; ++t) rows[at + t] = rec(order[t]);
+    };
+    place(0, m, [&](int a) { return rec_new[a]; }, [&](int a) { return cls_new[a]; });
+    place(mp, m, [&](int a) { return rec_old[a]; }, [&](int a) { return cls_old[a]; });
+    if (!p_is_m) {
+        place(2 * (size_t)mp, n1, [&](int t) { return rec_p1[t]; }, [&](int t) { return fbits(rec_p1[t].w); });
+        place(2 * (size_t)mp + p1, n2, [&](int t) { return rec_old[p2_of[t]]; },
+              [&](int t) { return cls_old[p2_of[t]]; });
     }
     // column edits: first remove M from the columns, later write the new records
     std::vector<int> slots(m);
@@ -1296,9 +1339,12 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
     // the buffers: reused as it is on the device while those are unchanged
     KernelChoice kc;
     incremental_kernel(h, kc.variant, kc.K, kc.n_chunks, kc.chunks);
-    const int L = pick_range(h, 2.0 * mp * (double)h->n, 2 * mp / 64, kc.variant, kc.K, kc.n_chunks);
+    const int q1 = p_is_m ? mp : p1, q2 = p_is_m ? mp : p2;  // row counts of the segments against the rest
+    const int L = pick_range(h, (double)(q1 + q2) * (double)h->n, (q1 + q2) / 64, kc.variant, kc.K, kc.n_chunks);
     rsx_engine::IncKey key;
     key.mp = mp;
+    key.p1 = p1;
+    key.p2 = p2;
     key.n = h->n;
     key.L = L;
     key.K = kc.K;
@@ -1310,8 +1356,10 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
     if (!(h->inc_njobs > 0 && key == h->inc_key)) {
         std::vector<PairJob>& jobs = h->inc_jobs;
         jobs.clear();
-        rect_jobs(jobs, h->d_rows.p, mp, h->d_cols.p, h->n, L, 2.0f);            // new x rest
-        rect_jobs(jobs, h->d_rows.p + mp, mp, h->d_cols.p, h->n, L, -2.0f);      // old x rest
+        const float4* P1 = p_is_m ? h->d_rows.p : h->d_rows.p + 2 * (size_t)mp;
+        const float4* P2 = p_is_m ? h->d_rows.p + mp : P1 + p1;
+        rect_jobs(jobs, P1, q1, h->d_cols.p, h->n, L, 2.0f);                    // new (+ delta) x rest
+        rect_jobs(jobs, P2, q2, h->d_cols.p, h->n, L, -2.0f);                   // old x rest
         rect_jobs(jobs, h->d_rows.p, mp, h->d_rows.p, mp, L, 1.0f);             // new x new
         rect_jobs(jobs, h->d_rows.p + mp, mp, h->d_rows.p + mp, mp, L, -1.0f);  // old x old
         finalize_jobs(jobs, &h->inc_scratch);
