@@ -594,7 +594,12 @@ __device__ __forceinline__ void rec32_item(const DebyeArgs& a, int job_id, int c
 // reaches 128 at three waves per SIMD; the flush costs ~4K lane operations, which
 // pays off when a column range holds few class runs (type classes).
 constexpr int RSLICE = 32;
-constexpr int FGROUP = 4;
+// Row factors of a flush are loaded in groups of FGROUP between compiler memory
+// barriers.  Larger groups keep more loads in flight per flush (each group waits for its
+// loads) at the price of registers: at K = 128 and 168 VGPRs, groups of 16 spill 16
+// values around the flush and still run ~2 % faster on hydration layouts (many flushes)
+// than groups of 4 (no spills), and equal elsewhere.
+constexpr int FGROUP = 16;
 
 template <int K, int M>
 __device__ __forceinline__ void rec32r_item(const DebyeArgs& a, int job_id, int chunk, float (&T)[RSLICE][65])
