@@ -639,11 +639,13 @@ __device__ __forceinline__ void rec32r_item(const DebyeArgs& a, int job_id, int 
                 }
             }
             __syncthreads();
-            float sum = 0.0f;
+            // the 64 rows' contributions cancel deeply near minima of I(q): summed in
+            // FP64 (an FP32 sum here cost up to 2e-5 relative at such q on long runs)
+            double sum = 0.0;
 #pragma unroll 8
-            for (int t = 0; t < 32; ++t) sum += T[ks][half * 32 + t];
+            for (int t = 0; t < 32; ++t) sum += (double)T[ks][half * 32 + t];
             sum += __shfl_xor(sum, 32, 64);
-            if (s * RSLICE + ks < K) Y[s] = fma((double)fr[s * RSLICE + ks], (double)sum, Y[s]);
+            if (s * RSLICE + ks < K) Y[s] = fma((double)fr[s * RSLICE + ks], sum, Y[s]);
             __syncthreads();
         }
     });

@@ -218,13 +218,10 @@ def test_config2_plain_full_size(oracle):
     assert err.max() < TOL
 
 
-def test_autotune_and_cache(oracle, tmp_path):
-    """rsx_autotune: sweeps the FP32 kernels over K and column range, keeps the fastest,
-    appends it to the cache file; a second engine with the same key takes the cached
-    entry.  I(q) after tuning still meets the FP64 bar, and the time_sasa / time_forward
-    probes report positive device times."""
+def _hydrated_dip_case():
+    """3000 atoms, 128 q over [0, 0.5], SASA hydration with uniform radii: I(q) dips
+    30x below its maximum near q index 20, where the pair terms cancel deeply."""
     from resaxs_amd import configs
-    _say("case: autotune + cache, 3000 atoms x 128 q, hydrated")
     rng = np.random.default_rng(5)
     n = 3000
     q = np.linspace(0.0, 0.5, 128).astype(np.float32)
@@ -233,7 +230,37 @@ def test_autotune_and_cache(oracle, tmp_path):
     table = configs.factor_table(rng, q)
     water = table.shape[1] - 1            # the table's last column serves as the water factor
     radii = np.full(int(types.max()) + 1, 1.6, dtype=np.float32)
-    b = configs.bodies_of(pos, types)
+    return q, pos, types, table, water, radii, configs.bodies_of(pos, types)
+
+
+@pytest.mark.parametrize("variant,k,col_range", [(3, 128, 256), (3, 128, 1024), (2, 64, 256), (2, 64, 1024)])
+def test_deep_minimum_accuracy(oracle, variant, k, col_range):
+    """The FP32 kernels at the bottom of a deep dip of I(q): the 64 rows' run sums cancel
+    there, so they are summed in FP64 (an FP32 row sum of the run-flush kernel reached
+    2.3e-5 at L = 1024 on this case)."""
+    _say(f"case: deep minimum, hydrated 3000 x 128, variant {variant} K={k} L={col_range}")
+    q, pos, types, table, water, radii, b = _hydrated_dip_case()
+    e = engine_for(q, table, water, 0.8, n=len(pos), radii=radii, probe=1.4, n_half=64)
+    e.set_variant(variant)
+    e.set_tuning(k, col_range)
+    I = e.calc_curve(b)
+    counts, _ = e.calc_rsasa(b, radii, 1.4, fib_dot_table(radii, 1.4, 64))
+    e.close()
+    ref = oracle_iq(q, pos, types, table, water, 0.8, counts=counts, n_dots=128)
+    err = rel_err(I, ref)
+    _say(f"  max rel err {err.max():.3e} at q index {int(np.argmax(err))} (I min at {int(np.argmin(ref))})")
+    assert err.max() < 8e-6
+
+
+def test_autotune_and_cache(oracle, tmp_path):
+    """rsx_autotune: sweeps the FP32 kernels over K and column range, keeps the fastest,
+    appends it to the cache file; a second engine with the same key takes the cached
+    entry.  I(q) after tuning still meets the FP64 bar, and the time_sasa / time_forward
+    probes report positive device times."""
+    from resaxs_amd import configs
+    _say("case: autotune + cache, 3000 atoms x 128 q, hydrated")
+    q, pos, types, table, water, radii, b = _hydrated_dip_case()
+    n = len(pos)
     cache = str(tmp_path / "tune.txt")
     e = engine_for(q, table, water, 0.8, n=n, radii=radii, probe=1.4, n_half=64)
     e.calc_curve(b)
