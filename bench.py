@@ -175,8 +175,9 @@ def incremental_bench(hydrate, n_moves=2000):
     500-atom segments, running FP64 I(q) re-baselined every 200 moves.  Host wall clock
     around recalc_curve + commit (the synchronous calls a Monte Carlo caller makes),
     work per move 2 P_move (5 n_q + 9) with P_move = m (N - m) + m (m - 1) / 2.  The
-    running I(q) after move 1999 (199 moves since the last re-baseline) is compared
-    with a fresh full recompute by the FP64 pair kernel."""
+    running I(q) after the next-to-last move (with 2000 moves: move 1999, 199 moves
+    since the last re-baseline) is compared with a fresh full recompute by the FP64
+    pair kernel."""
     from resaxs_amd import configs
     w = configs.config3()
     n, nq, m = len(w.pos), len(w.q), w.seg
@@ -187,7 +188,7 @@ def incremental_bench(hydrate, n_moves=2000):
     e.calc_curve(b)
     rng = w.move_rng
     log(f"bench: incremental, {n_moves} moves of {m} atoms on {n} atoms, hydration={hydrate}")
-    ts, err_1999 = [], None
+    ts, err_last = [], None
     for k in range(n_moves):
         start, newp = configs.random_move(rng, pos, m)
         pos[start:start + m] = newp
@@ -200,7 +201,7 @@ def incremental_bench(hydrate, n_moves=2000):
             f = make_engine(w, hydrate=hydrate)
             f.set_variant(0)
             I_full = f.calc_curve(b)
-            err_1999 = float(np.max(np.abs(I - I_full) / np.abs(I_full)))
+            err_last = float(np.max(np.abs(I - I_full) / np.abs(I_full)))
             f.close()
     e.close()
     p_move = m * (n - m) + m * (m - 1) / 2.0
@@ -210,7 +211,8 @@ def incremental_bench(hydrate, n_moves=2000):
             "ms_per_move_median": round(float(np.median(ts)) * 1e3, 4),
             "terms_per_s": 2 * p_move * nq / (ms * 1e-3),
             "achieved_tflops": round(2 * p_move * (5 * nq + 9) / (ms * 1e-3) / 1e12, 2),
-            "rel_err_after_199_moves_vs_fp64_full": err_1999,
+            "rel_err_vs_fp64_full": err_last,
+            "moves_since_rebaseline_at_check": (n_moves - 2) % 200 + 1,
             "timing": "host wall clock around rsx_recalc_curve_f64 + rsx_commit (host bodies in, I(q) out)"
                       + ("; SASA recounted for the atoms within reach of the moved segment" if hydrate else "")}
 
@@ -371,12 +373,21 @@ def main():
                     help="skip the batched-ensemble and incremental-move measurements")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--section", choices=("fwd_adjoint", "incremental", "incremental_sasa", "batched"),
+                    help="run only this workload section, shortened for profiling (incremental: 100 moves; "
+                         "batched: 2 replays), and print its JSON")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding plan is in "
-              "DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.section:
+        run = {"fwd_adjoint": lambda: adjoint_bench(),
+               "incremental": lambda: incremental_bench(False, n_moves=100),
+               "incremental_sasa": lambda: incremental_bench(True, n_moves=100),
+               "batched": lambda: batched_bench(reps=2)}[args.section]
+        print(json.dumps({"section": args.section, "result": run()}), flush=True)
+        return
     from resaxs_amd import configs
     if args.copy_probe:
         from resaxs_amd import Engine
