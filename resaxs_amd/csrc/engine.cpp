@@ -326,8 +326,9 @@ struct rsx_engine {
     DBuf<int> d_subset;              // atom ids of a local SASA recount
     DBuf<unsigned int> h_u32;        // distance histograms (rsx_calc_curve_hist)
     DBuf<unsigned long long> h_u64;
-    DBuf<double> h_dbl, h_out, h_q;
+    DBuf<double> h_dbl, h_out, h_q, h_part;
     DBuf<int> h_pidx;
+    DBuf<rsx::HistTile> h_tiles;
     DBuf<float> d_ftab, d_qv, d_r2, d_rext;
     DBuf<double> d_qrev;
     DBuf<ChunkConst> d_chunks;
@@ -1506,21 +1507,60 @@ void calc_hist_impl(rsx_engine* h, const float* bodies, int n, float dr, double*
     hipchk(hipMemcpyAsync(h->h_q.p, qd.data(), nq * sizeof(double), hipMemcpyHostToDevice, h->stream), "upload q");
     hipchk(rsx::launch_fill_i32((int*)h->h_u32.p, 0, (long long)n0, h->stream), "zero h0");
     if (hyd) hipchk(rsx::launch_fill_i32((int*)h->h_u64.p, 0, 2LL * (long long)(n1 + n2), h->stream), "zero h1 h2");
-    rsx::HistArgs a;
-    a.atoms = h->d_sasa_atoms.p;
-    a.counts = hyd ? h->d_counts.p : nullptr;
-    a.n = n;
-    a.n_types = nt;
-    a.inv_dr = 1.0f / dr;
-    a.n_bins = n_bins;
-    a.pair_index = h->h_pidx.p;
-    a.h0 = h->h_u32.p;
-    a.h1 = h->h_u64.p;
-    a.h2 = h->h_u64.p + n1;
-    hipchk(rsx::launch_hist_pairs(a, h->stream), "histogram");
-    hipchk(rsx::launch_widen_u32(h->h_u32.p, h->h_dbl.p, (long long)n0, h->stream), "widen h0");
-    if (hyd) hipchk(rsx::launch_widen_u64(h->h_u64.p, h->h_dbl.p + n0, (long long)(n1 + n2), h->stream), "widen h1 h2");
-    hipchk(rsx::launch_hist_transform(h->h_dbl.p, n_hist, n_bins, (double)dr, h->h_q.p, nq, h->h_out.p, h->stream),
+    const int n_win = (n_bins + rsx::HIST_WINDOW - 1) / rsx::HIST_WINDOW;
+    if (!hyd && n > 1 && n_win <= 16) {
+        // plain: atoms sorted by type, tiles of one type pair, LDS bin windows
+        std::vector<int> seg(nt + 1, 0);
+        for (int i = 0; i < n; ++i) seg[type[i] + 1]++;
+        for (int t = 0; t < nt; ++t) seg[t + 1] += seg[t];
+        std::vector<float4> sorted(n);
+        std::vector<int> fill(seg.begin(), seg.end() - 1);
+        for (int i = 0; i < n; ++i) {
+            const float* b = &bodies[4 * (size_t)i];
+            sorted[fill[type[i]]++] = make_float4(b[0], b[1], b[2], b[3]);
+        }
+        constexpr int CB = 4096;  // columns per tile
+        std::vector<rsx::HistTile> tiles;
+        for (int ta = 0; ta < nt; ++ta)
+            for (int tb = ta; tb < nt; ++tb)
+                for (int r0 = seg[ta]; r0 < seg[ta + 1]; r0 += rsx::HIST_ROWS) {
+                    const int nr = std::min(rsx::HIST_ROWS, seg[ta + 1] - r0);
+                    const int cs = ta == tb ? r0 : seg[tb];  // same type: columns from the block start on
+                    for (int c0 = cs; c0 < seg[tb + 1]; c0 += CB)
+                        tiles.push_back({r0, nr, c0, std::min(CB, seg[tb + 1] - c0), pidx[(size_t)ta * nt + tb],
+                                         ta == tb ? 1 : 0});
+                }
+        hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, sorted.data(), (size_t)n * 16, hipMemcpyHostToDevice, h->stream),
+               "upload sorted bodies");
+        h->h_tiles.ensure(tiles.size());
+        hipchk(hipMemcpyAsync(h->h_tiles.p, tiles.data(), tiles.size() * sizeof(rsx::HistTile), hipMemcpyHostToDevice,
+                              h->stream),
+               "upload tiles");
+        hipchk(rsx::launch_hist_tiles(h->d_sasa_atoms.p, h->h_tiles.p, (int)tiles.size(), 1.0f / dr, n_bins,
+                                      h->h_u32.p, h->stream),
+               "histogram tiles");
+        hipchk(hipStreamSynchronize(h->stream), "sync histogram tiles");  // host tables above
+    } else {
+        rsx::HistArgs a;
+        a.atoms = h->d_sasa_atoms.p;
+        a.counts = hyd ? h->d_counts.p : nullptr;
+        a.n = n;
+        a.n_types = nt;
+        a.inv_dr = 1.0f / dr;
+        a.n_bins = n_bins;
+        a.pair_index = h->h_pidx.p;
+        a.h0 = h->h_u32.p;
+        a.h1 = h->h_u64.p;
+        a.h2 = h->h_u64.p + n1;
+        hipchk(rsx::launch_hist_pairs(a, h->stream), "histogram");
+    }
+    hipchk(rsx::launch_widen_u32(h->h_u32.p, h->h_dbl.p, (long long)n0, n_bins, (double)dr, h->stream), "widen h0");
+    if (hyd)
+        hipchk(rsx::launch_widen_u64(h->h_u64.p, h->h_dbl.p + n0, (long long)(n1 + n2), n_bins, (double)dr, h->stream),
+               "widen h1 h2");
+    h->h_part.ensure(std::max<size_t>(1, rsx::hist_transform_scratch(n_hist, n_bins, nq)));
+    hipchk(rsx::launch_hist_transform(h->h_dbl.p, n_hist, n_bins, (double)dr, h->h_q.p, nq, h->h_part.p, h->h_out.p,
+                                      h->stream),
            "histogram transform");
     std::vector<double> A((size_t)n_hist * nq);
     hipchk(hipMemcpyAsync(A.data(), h->h_out.p, A.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream),
@@ -2164,8 +2204,10 @@ int rsx_destroy(rsx_engine* h)
     h->h_u64.release();
     h->h_dbl.release();
     h->h_out.release();
+    h->h_part.release();
     h->h_q.release();
     h->h_pidx.release();
+    h->h_tiles.release();
     h->d_jobs.release();
     h->d_fjobs.release();
     h->inc_njobs = 0;

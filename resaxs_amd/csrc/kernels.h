@@ -211,12 +211,30 @@ struct HistArgs {
     unsigned long long* h2;    // [n_bins]
 };
 hipError_t launch_hist_pairs(const HistArgs& a, hipStream_t s);
-// out[x][k] = sum_b hist[x][b] sin(q_k r_b) / (q_k r_b) (1 at q = 0), FP64; hist as doubles
-hipError_t launch_hist_transform(const double* hist, int n_hist, int n_bins, double dr, const double* q, int n_q,
-                                 double* out, hipStream_t s);
-// widen u32 / u64 histograms to doubles (dst[i] = src[i])
-hipError_t launch_widen_u32(const unsigned int* src, double* dst, long long n, hipStream_t s);
-hipError_t launch_widen_u64(const unsigned long long* src, double* dst, long long n, hipStream_t s);
+
+// Plain (unweighted) pair-distance histograms through workgroup-private LDS bins.  Atoms
+// sorted by type; a tile = rows [r0, r0 + nr) of one type x columns [c0, c0 + nc) of
+// one type (same type: only pairs with column index > row index).  Bins are counted in
+// windows of HIST_WINDOW (grid y = window), each flushed with one global add per
+// nonzero bin.
+constexpr int HIST_WINDOW = 16384;
+constexpr int HIST_ROWS = 256;
+struct HistTile {
+    int r0, nr, c0, nc;
+    int pair;  // histogram row (type pair)
+    int diag;  // rows and columns from the same type segment
+};
+hipError_t launch_hist_tiles(const float4* sorted_atoms, const HistTile* tiles, int n_tiles, float inv_dr,
+                             int n_bins, unsigned int* h0, hipStream_t s);
+// out[x][k] = sum_b hist[x][b] sin(q_k r_b) / (q_k r_b) (sum_b hist[x][b] at q = 0), FP64, from
+// hw = hist / r_b (the widening below); partial: hist_transform_scratch doubles
+size_t hist_transform_scratch(int n_hist, int n_bins, int n_q);
+hipError_t launch_hist_transform(const double* hw, int n_hist, int n_bins, double dr, const double* q, int n_q,
+                                 double* partial, double* out, hipStream_t s);
+// widen u32 / u64 histograms to doubles divided by the bin centre: dst[i] = src[i] / ((i mod n_bins + 0.5) dr)
+hipError_t launch_widen_u32(const unsigned int* src, double* dst, long long n, int n_bins, double dr, hipStream_t s);
+hipError_t launch_widen_u64(const unsigned long long* src, double* dst, long long n, int n_bins, double dr,
+                            hipStream_t s);
 
 // workspace reset kernels (used instead of memset nodes so launches can be graph-captured)
 hipError_t launch_fill_i32(int* p, int value, long long n, hipStream_t s);

@@ -1827,45 +1827,102 @@ __global__ __launch_bounds__(64) void hist_pairs_kernel(HistArgs a, int n_tiles)
     }
 }
 
-// out[x][k] = sum_b hist[x][b] sinc(q_k r_b): lane = q index, sin(q r_b) advanced along
-// b by the FP64 three-term recurrence (angle q dr per bin)
-__global__ __launch_bounds__(64) void hist_transform_kernel(const double* __restrict__ hist, int n_bins, double dr,
-                                                            const double* __restrict__ q, int n_q,
-                                                            double* __restrict__ out)
+// Plain histograms through LDS: one workgroup of HIST_ROWS lanes per (tile, bin window),
+// one row per lane, the tile's columns read in turn by every lane (uniform loads); each
+// pair's bin (the same expression as hist_pairs_kernel, so the counts are identical)
+// is counted into the workgroup's LDS window with an LDS atomic, and the window's
+// nonzero bins are added to the global histogram at the end.
+__global__ __launch_bounds__(HIST_ROWS) void hist_tiles_kernel(const float4* __restrict__ atoms,
+                                                               const HistTile* __restrict__ tiles, float inv_dr,
+                                                               int n_bins, unsigned int* __restrict__ h0)
 {
-    const int x = blockIdx.y;
+    __shared__ unsigned int win[HIST_WINDOW];
+    const HistTile tl = tiles[blockIdx.x];
+    const int b0 = blockIdx.y * HIST_WINDOW;
+    const int tid = threadIdx.x;
+    for (int b = tid; b < HIST_WINDOW; b += HIST_ROWS) win[b] = 0u;
+    __syncthreads();
+    const bool row_ok = tid < tl.nr;
+    const int i = tl.r0 + tid;
+    const float4 pi = row_ok ? atoms[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int jmin = tl.diag ? i + 1 : 0;  // same segment: only j > i
+    const int c1 = tl.c0 + tl.nc;
+    for (int j = tl.c0; j < c1; ++j) {
+        const float4 pj = atoms[j];
+        const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        const float r = __builtin_sqrtf(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+        int b = (int)(r * inv_dr);
+        b = b < n_bins - 1 ? b : n_bins - 1;
+        const int bw = b - b0;
+        if (row_ok && j >= jmin && bw >= 0 && bw < HIST_WINDOW) atomicAdd(&win[bw], 1u);
+    }
+    __syncthreads();
+    unsigned int* __restrict__ dst = h0 + (size_t)tl.pair * n_bins + b0;
+    const int nb = n_bins - b0 < HIST_WINDOW ? n_bins - b0 : HIST_WINDOW;
+    for (int b = tid; b < nb; b += HIST_ROWS) {
+        const unsigned int v = win[b];
+        if (v) atomicAdd(&dst[b], v);
+    }
+}
+
+// Transform of the histograms over bin ranges of HIST_RANGE bins: grid (q blocks,
+// histograms, ranges), lane = q index.  hw holds hist[x][b] / r_b (r_b the bin centre);
+// partial[z][x][k] = sum over the range of hw[x][b] sin(q_k r_b), the sine advanced
+// along b by the FP64 three-term recurrence (angle q dr per bin) from seeds taken
+// directly at the range start; at q = 0 the sum is of hw[x][b] r_b = hist[x][b].
+constexpr int HIST_RANGE = 512;
+
+__global__ __launch_bounds__(64) void hist_transform_kernel(const double* __restrict__ hw, int n_bins, double dr,
+                                                            const double* __restrict__ q, int n_q, int n_hist,
+                                                            double* __restrict__ partial)
+{
+    const int x = blockIdx.y, z = blockIdx.z;
     const int k = blockIdx.x * 64 + threadIdx.x;
     if (k >= n_q) return;
-    const double* __restrict__ h = hist + (size_t)x * n_bins;
+    const int b0 = z * HIST_RANGE;
+    const int b1 = b0 + HIST_RANGE < n_bins ? b0 + HIST_RANGE : n_bins;
+    const double* __restrict__ h = hw + (size_t)x * n_bins;
     const double qk = q[k];
     double acc = 0.0;
     if (qk == 0.0) {
-        for (int b = 0; b < n_bins; ++b) acc += h[b];
+        for (int b = b0; b < b1; ++b) acc = fma(h[b], (b + 0.5) * dr, acc);
     } else {
         const double th = qk * dr;
         const double c2 = 2.0 * cos(th);
-        double sp = sin(-0.5 * th), sc = sin(0.5 * th);  // sin(q r_{-1}), sin(q r_0)
-        for (int b = 0; b < n_bins; ++b) {
-            acc = fma(h[b] / ((b + 0.5) * dr), sc, acc);
+        double sp = sin(qk * (b0 - 0.5) * dr), sc = sin(qk * (b0 + 0.5) * dr);  // sin(q r_{b0-1}), sin(q r_b0)
+        for (int b = b0; b < b1; ++b) {
+            acc = fma(h[b], sc, acc);
             const double sn = fma(c2, sc, -sp);
             sp = sc;
             sc = sn;
         }
-        acc /= qk;
     }
-    out[(size_t)x * n_q + k] = acc;
+    partial[((size_t)z * n_hist + x) * n_q + k] = acc;
 }
 
-__global__ void widen_u32_kernel(const unsigned int* src, double* dst, long long n)
+// out[x][k] = sum over ranges of partial (fixed order) / q_k (1 / 1 at q = 0)
+__global__ void hist_range_sum_kernel(const double* __restrict__ partial, int n_ranges, int n_hist,
+                                      const double* __restrict__ q, int n_q, double* __restrict__ out)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = (double)src[i];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_hist * n_q) return;
+    const int k = t % n_q;
+    double s = 0.0;
+    for (int z = 0; z < n_ranges; ++z) s += partial[(size_t)z * n_hist * n_q + t];
+    out[t] = q[k] == 0.0 ? s : s / q[k];
 }
 
-__global__ void widen_u64_kernel(const unsigned long long* src, double* dst, long long n)
+// dst[i] = src[i] / r_b with r_b = (b + 0.5) dr the centre of bin b = i mod n_bins
+__global__ void widen_u32_kernel(const unsigned int* src, double* dst, long long n, int n_bins, double dr)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = (double)src[i];
+    if (i < n) dst[i] = (double)src[i] / (((double)(i % n_bins) + 0.5) * dr);
+}
+
+__global__ void widen_u64_kernel(const unsigned long long* src, double* dst, long long n, int n_bins, double dr)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (double)src[i] / (((double)(i % n_bins) + 0.5) * dr);
 }
 
 }  // namespace
@@ -1879,26 +1936,46 @@ hipError_t launch_hist_pairs(const HistArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_hist_transform(const double* hist, int n_hist, int n_bins, double dr, const double* q, int n_q,
-                                 double* out, hipStream_t s)
+hipError_t launch_hist_tiles(const float4* sorted_atoms, const HistTile* tiles, int n_tiles, float inv_dr,
+                             int n_bins, unsigned int* h0, hipStream_t s)
 {
-    if (n_hist <= 0 || n_q <= 0) return hipSuccess;
-    hipLaunchKernelGGL(hist_transform_kernel, dim3((unsigned)((n_q + 63) / 64), (unsigned)n_hist), dim3(64), 0, s,
-                       hist, n_bins, dr, q, n_q, out);
+    if (n_tiles <= 0 || n_bins <= 0) return hipSuccess;
+    const unsigned n_win = (unsigned)((n_bins + HIST_WINDOW - 1) / HIST_WINDOW);
+    hipLaunchKernelGGL(hist_tiles_kernel, dim3((unsigned)n_tiles, n_win), dim3(HIST_ROWS), 0, s, sorted_atoms, tiles,
+                       inv_dr, n_bins, h0);
     return hipGetLastError();
 }
 
-hipError_t launch_widen_u32(const unsigned int* src, double* dst, long long n, hipStream_t s)
+size_t hist_transform_scratch(int n_hist, int n_bins, int n_q)
 {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(widen_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+    return (size_t)((n_bins + HIST_RANGE - 1) / HIST_RANGE) * n_hist * n_q;
+}
+
+hipError_t launch_hist_transform(const double* hw, int n_hist, int n_bins, double dr, const double* q, int n_q,
+                                 double* partial, double* out, hipStream_t s)
+{
+    if (n_hist <= 0 || n_q <= 0 || n_bins <= 0) return hipSuccess;
+    const int n_ranges = (n_bins + HIST_RANGE - 1) / HIST_RANGE;
+    hipLaunchKernelGGL(hist_transform_kernel, dim3((unsigned)((n_q + 63) / 64), (unsigned)n_hist, (unsigned)n_ranges),
+                       dim3(64), 0, s, hw, n_bins, dr, q, n_q, n_hist, partial);
+    const int nt = n_hist * n_q;
+    hipLaunchKernelGGL(hist_range_sum_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, partial, n_ranges,
+                       n_hist, q, n_q, out);
     return hipGetLastError();
 }
 
-hipError_t launch_widen_u64(const unsigned long long* src, double* dst, long long n, hipStream_t s)
+hipError_t launch_widen_u32(const unsigned int* src, double* dst, long long n, int n_bins, double dr, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(widen_u64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+    hipLaunchKernelGGL(widen_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n, n_bins, dr);
+    return hipGetLastError();
+}
+
+hipError_t launch_widen_u64(const unsigned long long* src, double* dst, long long n, int n_bins, double dr,
+                            hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(widen_u64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n, n_bins, dr);
     return hipGetLastError();
 }
 
