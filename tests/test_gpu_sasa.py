@@ -63,6 +63,25 @@ def test_sasa_small_cases_bruteforce(oracle):
         assert np.array_equal(c, ref), n
 
 
+@pytest.mark.parametrize("n_half", [16, 50, 100, 256])
+def test_sasa_dots_per_lane(oracle, n_half):
+    """Dot counts of 32, 100, 200 and 512 dots per sphere (one, two, four and eight dots per
+    lane, the last one ragged over the lanes) on a dense 1500-atom cluster, bit-exact
+    against the brute-force oracle."""
+    _say(f"case: SASA {2 * n_half} dots, 1500 atoms")
+    radii = np.array([1.7, 1.55, 1.52, 1.8, 1.2], dtype=np.float32)
+    probe = 1.4
+    dots = fib_dot_table(radii, probe, n_half)
+    rng = np.random.default_rng(90 + n_half)
+    pos = rng.uniform(-14, 14, size=(1500, 3)).astype(np.float32)
+    types = rng.integers(0, 5, size=1500).astype(np.uint32)
+    e = _engine()
+    c, _ = e.calc_rsasa(oracle.bodies(pos, types), radii, probe, dots)
+    ref = oracle.sasa_counts(pos, types, r2_table(radii, probe), dots, cells=False)
+    assert np.array_equal(c, ref)
+    assert 0 < int(np.count_nonzero(c)) < 1500  # both buried and exposed atoms
+
+
 def test_hydrated_debye_vs_oracle(oracle):
     """Hydration factors F = f_t + (count/D) w f_w from the GPU's own counts;
     I(q) against the FP64 oracle using the oracle's (identical) counts."""
