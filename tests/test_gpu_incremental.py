@@ -159,6 +159,41 @@ def test_hydrated_recalc(oracle):
     assert err.max() < 1e-5
 
 
+@pytest.mark.parametrize("variant,uniform_q", [(3, False), (0, True), (2, True)])
+def test_hydrated_recalc_kernels(oracle, variant, uniform_q):
+    """Hydrated moves through the other pair kernels: the direct kernel of a non-uniform
+    q grid, the FP64 recurrence and the per-lane-sum FP32 kernel.  Atoms whose count
+    changed without moving enter as delta-class rows in each of them; after 6 moves the
+    running I(q) equals the FP64 oracle on the final positions and counts."""
+    from resaxs_amd import configs
+    _say(f"case: hydrated recalc, variant {variant}, {'uniform' if uniform_q else 'non-uniform'} q, "
+         f"6 moves of 150 atoms on 2500 atoms")
+    n = 2500
+    w = configs.config3(n=n)
+    q = w.q
+    if not uniform_q:
+        rng_q = np.random.default_rng(3)
+        q = np.sort(np.concatenate([[0.0], rng_q.uniform(0.005, 0.6, size=len(w.q) - 1)])).astype(np.float32)
+    table = configs.factor_table(np.random.default_rng(4), q)
+    pos = w.pos.copy()
+    e = engine_for(q, table, w.water_index, w.w, n=n, radii=w.radii, probe=w.probe, n_half=w.n_half)
+    e.set_variant(variant)
+    e.set_rebaseline(0)
+    e.calc_curve(configs.bodies_of(pos, w.types))
+    rng = np.random.default_rng(37)
+    for m in range(6):
+        start, newp = configs.random_move(rng, pos, 150)
+        pos[start:start + 150] = newp
+        I = e.recalc_curve(configs.bodies_of(pos, w.types), start, 150)
+        e.commit()
+    dots = fib_dot_table(w.radii, w.probe, w.n_half)
+    counts = oracle.sasa_counts(pos, w.types, r2_table(w.radii, w.probe), dots)
+    ref = oracle_iq(q, pos, w.types, table, w.water_index, w.w, counts, 256)
+    err = rel_err(I, ref)
+    _say(f"  max rel err {err.max():.3e}")
+    assert err.max() < 1e-5
+
+
 def _full_recompute(w, pos, hydrate, variant):
     """Fresh engine, full calc_curve with the given pair-kernel variant."""
     from resaxs_amd import configs
