@@ -1096,16 +1096,16 @@ __global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
     const int excl = incl - ccount;
     const int total = __builtin_amdgcn_readlane(incl, 26);
     for (int base = 0; base < total; base += 64) {
-        // this lane's candidate: position g of the list -> its cell and bucket slot
+        // this lane's candidate: position g of the list -> its cell (the first c with
+        // incl[c] > g, by a binary search over lanes 0..31: incl does not decrease and
+        // lanes 27..31 hold the total) and its bucket slot
         const int g = base + lane;
-        int p = -1;
+        int c = 0;
 #pragma unroll
-        for (int c = 0; c < 27; ++c) {
-            const int e = __builtin_amdgcn_readlane(excl, c);
-            const int n_c = __builtin_amdgcn_readlane(ccount, c);
-            const int s_c = __builtin_amdgcn_readlane(cstart, c);
-            if (g >= e && g < e + n_c) p = s_c + (g - e);
-        }
+        for (int step = 16; step >= 1; step >>= 1)
+            if (__shfl(incl, c + step - 1, 64) <= g) c += step;
+        const int e = __shfl(excl, c, 64), s_c = __shfl(cstart, c, 64);
+        const int p = g < total ? s_c + (g - e) : -1;
         float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
         float r2j = 0.0f;
         bool pass = false;
