@@ -56,6 +56,24 @@ __device__ __forceinline__ float cos_rev(double x)
     return __builtin_amdgcn_cosf((float)__builtin_amdgcn_fract(x));
 }
 
+// Column records of a pair job are read through the constant address space: the index
+// is uniform over the wave, so they come as scalar loads into scalar registers and the
+// column loops are uniform control flow.  (Read as ordinary global loads, the loaded
+// class bits made the loops divergent: vector loads whose latency each step waited for,
+// and exec-mask bookkeeping; 3.5 % of the run-flush kernel's time.)
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) v4f cv4f;
+
+struct ConstCols {
+    cv4f* p;
+    __device__ __forceinline__ float4 operator[](int j) const
+    {
+        const v4f v = p[j];
+        return make_float4(v.x, v.y, v.z, v.w);
+    }
+};
+
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -236,7 +254,7 @@ __global__ __launch_bounds__(64, 2) void debye_uniform_kernel(const DebyeArgs a)
         }
     };
 
-    const float4* __restrict__ cols = jb.cols;
+    const ConstCols cols{(cv4f*)jb.cols};
     const int ncol = jb.ncol;
     const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
     // one weighted pair step per iteration: the second column is the next one when
@@ -352,7 +370,7 @@ __global__ __launch_bounds__(64 * SH_MAXW, 2) void debye_shared_kernel(const Deb
         }
     };
 
-    const float4* __restrict__ cols = jb.cols;
+    const ConstCols cols{(cv4f*)jb.cols};
     const int ncol = jb.ncol;
     const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
     const int n_tiles = (ncol + SH_TC - 1) / SH_TC;
@@ -507,7 +525,7 @@ __device__ __forceinline__ void rec32_columns(const PairJob& jb, const float4 ri
                                               float dq, float (&acc)[K], Flush&& flush)
 {
     const int lane = threadIdx.x;
-    const float4* __restrict__ cols = jb.cols;
+    const ConstCols cols{(cv4f*)jb.cols};
     const int ncol = jb.ncol;
     const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
     uint32_t cur = CLS_SKIP;
@@ -766,7 +784,7 @@ __global__ __launch_bounds__(64, 2) void debye_direct_kernel(const DebyeArgs a)
         zsum = 0.0f;
     };
 
-    const float4* __restrict__ cols = jb.cols;
+    const ConstCols cols{(cv4f*)jb.cols};
     const int ncol = jb.ncol;
     uint32_t cur = CLS_SKIP;
     int j = 0;
@@ -1190,7 +1208,7 @@ __global__ __launch_bounds__(64, 2) void adjoint_tri_kernel(const AdjointArgs a)
 #pragma unroll
     for (int k = 0; k < K; ++k) b[k] = 0.0;
     double gx = 0.0, gy = 0.0, gz = 0.0;
-    const float4* __restrict__ cols = jb.cols;
+    const ConstCols cols{(cv4f*)jb.cols};
     const int ncol = jb.ncol;
     const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
     const size_t part = (size_t)jb.slot * a.n_chunks + chunk;
@@ -1299,7 +1317,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 67 ? 3 
     }
     beta[K] = 0.0f;
     float gx = 0.0f, gy = 0.0f, gz = 0.0f;  // per-row sums of ~N terms: FP32 is ample for the 1e-4 bar
-    const float4* __restrict__ cols = jb.cols;
+    const ConstCols cols{(cv4f*)jb.cols};
     const int ncol = jb.ncol;
     const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
     const size_t part = (size_t)jb.slot * a.n_chunks + chunk;
