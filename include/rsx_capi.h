@@ -107,8 +107,8 @@ int rsx_device_name(int device, char *buf, int len);
 int rsx_atomic_form_factors(const float *q, int n_q, float avg_radius, float expansion, float *out);
 int rsx_atomic_radii(float *out16);
 
-/* ---- distance-histogram Debye (the reference's reserved saxs_hist_cl_gpu slot,
- * saxs_algorithm.hpp:188-193; approximate) ---- */
+/* ---- distance-histogram Debye (approximate; no reference counterpart: the reference's
+ * saxs_hist_cl_gpu class, saxs_algorithm.hpp:188-193, runs the exact pt_wf kernels) ---- */
 /* I(q) [n_q] (FP64) of n bodies from their pair-distance histogram at bin width
  * bin_width (A), per type pair, with the engine's q, factor table and hydration
  * settings (SASA counts from the engine's dot spheres when the water weight is set).
