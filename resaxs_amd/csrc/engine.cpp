@@ -378,7 +378,7 @@ struct rsx_engine {
     DBuf<ChunkConst> d_chunks_inc;  // q chunks of the incremental kernel
     int inc_chunks_K = 0, inc_chunks_n = 0;
     uint64_t inc_chunks_stamp = ~0ull;
-    std::vector<PairJob> inc_jobs, inc_scratch;  // host tables of the incremental jobs (capacity kept)
+    std::vector<PairJob> inc_jobs;  // host table of the incremental jobs (capacity kept)
     // pinned staging of the per-update transfers
     HBuf<float4> pin_rows, pin_vals, pin_moved;
     HBuf<int> pin_idx, pin_sub;
@@ -831,7 +831,8 @@ bool compute_sasa_local(rsx_engine* h, int start, int len, const std::vector<flo
 }
 
 // ---- pair jobs ---------------------------------------------------------------
-int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles, int variant, int K, int n_chunks)
+int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles, int variant, int K, int n_chunks,
+               double rounds = 6.0)
 {
     if (h->range_req > 0) return h->range_req;
     // resident waves of the pair kernel (the runtime's occupancy for the FP32 kernels,
@@ -841,10 +842,11 @@ int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles, in
         per_cu = rsx::rec32_blocks_per_cu(variant, K, rsx::sin_series_terms(h->dq * h->path_radius), false);
     if (per_cu <= 0) per_cu = 8;
     const double slots = (double)std::max(1, h->n_cu) * per_cu;
-    // about six rounds of waves over the resident slots: fewer leave a partly filled last
-    // round (measured on incremental updates: L = 64-128 beats 512 by 1.3-2.2x), more
-    // pay the per-job set-up and epilogue more often.  64..1024 columns, rounded down.
-    const double jobs = std::max(1.0, 6.0 * slots / std::max(1, n_chunks));
+    // about `rounds` rounds of waves over the resident slots (default six): fewer leave a
+    // partly filled last round unless the tail is split (measured on incremental updates
+    // without tail splitting: L = 64-128 beats 512 by 1.3-2.2x), more pay the per-job
+    // set-up and epilogue more often.  64..1024 columns, rounded down.
+    const double jobs = std::max(1.0, rounds * slots / std::max(1, n_chunks));
     double L = column_atoms_total / 64.0 / jobs;
     (void)row_tiles;
     L = std::max(64.0, std::min(1024.0, L));
@@ -944,7 +946,7 @@ void rect_jobs(std::vector<PairJob>& jobs, const float4* rows, int nr, const flo
 // order by cost (longest first, ties in build order) and number the slots in execution
 // order.  The tables hold few distinct job sizes (full ranges, ragged ends, the M x M
 // blocks), so a bucket pass over the sizes replaces a comparison sort.
-void finalize_jobs(std::vector<PairJob>& jobs, std::vector<PairJob>* scratch = nullptr)
+void finalize_jobs(std::vector<PairJob>& jobs)
 {
     std::vector<long> sizes;
     for (const PairJob& j : jobs) {
@@ -958,9 +960,7 @@ void finalize_jobs(std::vector<PairJob>& jobs, std::vector<PairJob>* scratch = n
         });
     } else {
         std::sort(sizes.begin(), sizes.end(), std::greater<long>());
-        std::vector<PairJob> local;
-        std::vector<PairJob>& out = scratch ? *scratch : local;
-        out.clear();
+        std::vector<PairJob> out;
         out.reserve(jobs.size());
         for (long c : sizes)
             for (const PairJob& j : jobs)
@@ -1048,6 +1048,51 @@ void launch_jobs(rsx_engine* h, const PairJob* d_jobs, int nj, double* iq, int a
     if (sync) hipchk(hipStreamSynchronize(h->stream), "sync pair kernels");
 }
 
+// Sort jobs by cost (most expensive first) and halve the cheapest splittable ones (at
+// least 128 columns, not diagonal), `round` of them (0: none), twice, re-sorting after
+// each pass: the final rounds of waves then consist of ever shorter jobs.
+template <typename Cost>
+void split_tail(std::vector<PairJob>& jobs, long round, Cost&& cost)
+{
+    auto by_cost = [&]() {
+        std::vector<std::pair<double, int>> key(jobs.size());
+        for (size_t k = 0; k < jobs.size(); ++k) key[k] = {-cost(jobs[k]), (int)k};
+        std::stable_sort(key.begin(), key.end());
+        std::vector<PairJob> out(jobs.size());
+        for (size_t k = 0; k < jobs.size(); ++k) out[k] = jobs[key[k].second];
+        jobs.swap(out);
+    };
+    by_cost();
+    for (int level = 0; level < 2 && round > 0 && (long)jobs.size() > round; ++level) {
+        long done = 0;
+        const size_t m = jobs.size();
+        for (size_t k = m; k-- > 0 && done < round;) {
+            PairJob& j = jobs[k];
+            if (j.diag || j.ncol < 128) continue;
+            PairJob b = j;
+            const int first = j.ncol / 2;
+            j.ncol = first;
+            b.cols = j.cols + first;
+            b.ncol -= first;
+            jobs.push_back(b);
+            ++done;
+        }
+        by_cost();
+    }
+    for (size_t k = 0; k < jobs.size(); ++k) jobs[k].slot = (int)k;
+}
+
+// waves of the pair kernel resident at once, in jobs (the runtime's occupancy for the FP32
+// kernels, eight one-wave blocks per CU otherwise), over the q chunks
+long resident_jobs(const rsx_engine* h, int variant, int K, int n_chunks)
+{
+    int per_cu = 0;
+    if (h->uniform && (variant == rsx::VARIANT_FP32 || variant == rsx::VARIANT_FP32R))
+        per_cu = rsx::rec32_blocks_per_cu(variant, K, rsx::sin_series_terms(h->dq * h->path_radius), false);
+    if (per_cu <= 0) per_cu = 8;
+    return std::max<long>(1, (long)std::max(1, h->n_cu) * per_cu / std::max(1, n_chunks));
+}
+
 // Order of the forward (triangle) jobs, and a finer tail.  A job costs its columns plus
 // the flushes of the class runs among them (in column equivalents; a run flush of the
 // FP32 run-flush kernel is ~1.4 columns of work, of the other kernels less), and jobs
@@ -1072,39 +1117,8 @@ void order_jobs(rsx_engine* h, std::vector<PairJob>& jobs)
         const double runs = 1.0 + (c1 - 1 > c0 ? P[c1] - P[c0 + 1] : 0);
         return ((double)j.ncol + flush_w * runs) * j.nrow;
     };
-    auto by_cost = [&]() {
-        std::vector<std::pair<double, int>> key(jobs.size());
-        for (size_t k = 0; k < jobs.size(); ++k) key[k] = {-cost(jobs[k]), (int)k};
-        std::stable_sort(key.begin(), key.end());
-        std::vector<PairJob> out(jobs.size());
-        for (size_t k = 0; k < jobs.size(); ++k) out[k] = jobs[key[k].second];
-        jobs.swap(out);
-    };
-    by_cost();
-    int per_cu = 0;
-    if (h->uniform && (h->variant == rsx::VARIANT_FP32 || h->variant == rsx::VARIANT_FP32R))
-        per_cu = rsx::rec32_blocks_per_cu(h->variant, h->K, rsx::sin_series_terms(h->dq * h->path_radius), false);
-    if (per_cu <= 0) per_cu = 8;
-    const long round = std::max<long>(1, (long)std::max(1, h->n_cu) * per_cu / std::max(1, h->n_chunks));
-    if ((long)jobs.size() > 2 * round) {
-        for (int level = 0; level < 2; ++level) {
-            long done = 0;
-            const size_t m = jobs.size();
-            for (size_t k = m; k-- > 0 && done < round;) {
-                PairJob& j = jobs[k];
-                if (j.diag || j.ncol < 128) continue;
-                PairJob b = j;
-                const int first = j.ncol / 2;
-                j.ncol = first;
-                b.cols = j.cols + first;
-                b.ncol -= first;
-                jobs.push_back(b);
-                ++done;
-            }
-            by_cost();
-        }
-    }
-    for (size_t k = 0; k < jobs.size(); ++k) jobs[k].slot = (int)k;
+    const long round = resident_jobs(h, h->variant, h->K, h->n_chunks);
+    split_tail(jobs, (long)jobs.size() > 2 * round ? round : 0, cost);
 }
 
 std::vector<PairJob> forward_jobs(rsx_engine* h)
@@ -1341,7 +1355,10 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
     KernelChoice kc;
     incremental_kernel(h, kc.variant, kc.K, kc.n_chunks, kc.chunks);
     const int q1 = p_is_m ? mp : p1, q2 = p_is_m ? mp : p2;  // row counts of the segments against the rest
-    const int L = pick_range(h, (double)(q1 + q2) * (double)h->n, (q1 + q2) / 64, kc.variant, kc.K, kc.n_chunks);
+    // about two rounds of waves, the tail split (measured against six rounds unsplit:
+    // 0.834 -> 0.824 ms plain, 2.23 -> 2.17 ms hydrated per 500-atom move at 50k atoms)
+    const int L =
+        pick_range(h, (double)(q1 + q2) * (double)h->n, (q1 + q2) / 64, kc.variant, kc.K, kc.n_chunks, 2.0);
     rsx_engine::IncKey key;
     key.mp = mp;
     key.p1 = p1;
@@ -1363,7 +1380,8 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
         rect_jobs(jobs, P2, q2, h->d_cols.p, h->n, L, -2.0f);                   // old x rest
         rect_jobs(jobs, h->d_rows.p, mp, h->d_rows.p, mp, L, 1.0f);             // new x new
         rect_jobs(jobs, h->d_rows.p + mp, mp, h->d_rows.p + mp, mp, L, -1.0f);  // old x old
-        finalize_jobs(jobs, &h->inc_scratch);
+        split_tail(jobs, resident_jobs(h, kc.variant, kc.K, kc.n_chunks),
+                   [](const PairJob& j) { return (double)j.ncol * j.nrow; });
         h->d_jobs.ensure(jobs.size());
         // (the host table stays alive in the engine until the copy has been consumed)
         hipchk(hipMemcpyAsync(h->d_jobs.p, jobs.data(), jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice,
