@@ -158,8 +158,10 @@ int rsx_set_persistent(rsx_engine *h, int on);
 
 /* ---- differentiable fit (no reference counterpart; uses saxs_profile.hpp:137-166 scale/chi) ---- */
 /* chi2 = sum_k (I_exp - c I)^2 / sigma^2 at the least-squares c; grad = dchi2/dx [n][3].
- * The chi^2 weights amplify forward errors by I / sigma, so this and rsx_debye_vjp run the FP64
- * recurrence: an engine on an FP32 variant moves to RSX_VARIANT_WAVE on the first call. */
+ * The chi^2 weights amplify forward errors by I / sigma, so this and rsx_debye_vjp run the FP32
+ * per-lane-sum recurrence with runs of at most 51 q (its error grows with the run length): an
+ * engine on RSX_VARIANT_FP32R or on longer runs moves there on the first call (rsx_autotune,
+ * rsx_set_tuning or rsx_set_variant lift the cap until the next call); an FP64 variant is kept. */
 int rsx_chi2_grad(rsx_engine *h, const float *bodies, int n, const float *i_exp, const float *sigma,
                   double *out_chi2, double *out_scale, float *out_grad, double *out_iq);
 /* adjoint only: grad_i = sum_k w_k dI_k/dx_i for caller weights w [n_q] (backward of I(q)) */

@@ -299,6 +299,7 @@ struct rsx_engine {
 
     // tuning
     int k_req = 0, range_req = 0;
+    int k_max = 0;  // cap on K while the gradient forward is active (0: none)
     int variant = rsx::VARIANT_FP32R;  // recurrence kernel (rsx::VARIANT_*)
     int K = 0, n_chunks = 0, ldf = 0;
     double path_radius = 0.0;  // bounding radius the current path and series length were chosen for
@@ -449,6 +450,7 @@ void choose_chunks(rsx_engine* h)
     for (int i = 0; i < nk; ++i) {
         const int K = rsx::debye_k_value(path, i);
         if (h->k_req > 0 && K != h->k_req) continue;
+        if (h->k_max > 0 && K > h->k_max) continue;
         if (h->k_req <= 0 && !uni && K > 48) continue;  // the direct kernel spills at K = 64
         const int C = (h->nq + K - 1) / K;
         // cost per (pair, chunk) in FP64-op units: the FP64 recurrence 2K + ~44 for distance
@@ -2041,6 +2043,7 @@ void autotune_impl(rsx_engine* h, const char* cache_path)
 {
     if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "autotune needs a prior calc_curve");
     set_device(h);
+    h->k_max = 0;  // the tuner chooses among all K; a later gradient call caps it again
     const std::string key = tune_key(h);
     if (cache_path && *cache_path) {
         std::ifstream in(cache_path);
@@ -2131,14 +2134,21 @@ int guarded(rsx_engine* h, F&& f)
 }
 
 // The chi^2 weights w_k = -2 c (I_exp - c I) / sigma^2 amplify forward errors by
-// I / sigma (~100x at 1 % errors), so the gradient entry points run the FP64
-// recurrence: an engine on the FP32 variant moves to the one-wave FP64 kernel (and
-// stays there) on its first gradient call.
-void use_fp64_forward(rsx_engine* h)
+// I / sigma (~100x at 1 % errors).  The FP32 recurrence's error grows with its run
+// length K, so the gradient entry points run the per-lane-sum FP32 kernel with runs
+// of at most GRAD_K_MAX q (I(q) error ~5e-7, gradient error at 1 % noise no worse
+// than with the FP64 recurrence); an engine on the run-flush FP32 kernel or on longer
+// runs moves there (and stays there) on its first gradient call.  An engine set to
+// an FP64 variant keeps it.
+constexpr int GRAD_K_MAX = 51;
+
+void use_gradient_forward(rsx_engine* h)
 {
-    if ((h->variant != rsx::VARIANT_FP32 && h->variant != rsx::VARIANT_FP32R) || h->nq <= 0) return;
-    h->variant = rsx::VARIANT_WAVE;
-    h->k_req = 0;  // the FP64 kernels have their own K list
+    if (h->nq <= 0 || (h->variant != rsx::VARIANT_FP32 && h->variant != rsx::VARIANT_FP32R)) return;
+    if (h->variant == rsx::VARIANT_FP32 && h->k_max == GRAD_K_MAX && h->K <= GRAD_K_MAX) return;
+    h->variant = rsx::VARIANT_FP32;
+    h->k_req = 0;
+    h->k_max = GRAD_K_MAX;
     setup_q(h);
     h->params_dirty = true;
     batch_release_graph(h);
@@ -2358,6 +2368,7 @@ int rsx_set_tuning(rsx_engine* h, int k, int col_range)
         if (k < 0 || col_range < 0) fail(RSX_INVALID_ARG, "tuning values must be >= 0");
         if (col_range > 0 && col_range % 64 != 0) fail(RSX_INVALID_ARG, "col_range must be a multiple of 64");
         h->k_req = k;
+        h->k_max = 0;
         h->range_req = col_range;
         if (h->nq > 0) {
             set_device(h);
@@ -2540,7 +2551,7 @@ int rsx_chi2_grad(rsx_engine* h, const float* bodies, int n, const float* i_exp,
     return guarded(h, [&] {
         if (!i_exp || !sigma) fail(RSX_INVALID_ARG, "chi2_grad: null profile");
         if (!h->uniform) fail(RSX_NOT_SUPPORTED, "chi2_grad needs a uniform q grid");
-        use_fp64_forward(h);
+        use_gradient_forward(h);
         std::vector<double> I(h->nq);
         calc_curve_impl(h, bodies, n, nullptr, I.data());
         double num = 0.0, den = 0.0;
@@ -2570,7 +2581,7 @@ int rsx_debye_vjp(rsx_engine* h, const float* bodies, int n, const double* w, fl
     return guarded(h, [&] {
         if (!w || !out_grad) fail(RSX_INVALID_ARG, "vjp: null argument");
         if (!h->uniform) fail(RSX_NOT_SUPPORTED, "vjp needs a uniform q grid");
-        use_fp64_forward(h);
+        use_gradient_forward(h);
         calc_curve_impl(h, bodies, n, nullptr, nullptr);
         std::vector<double> wk(w, w + h->nq);
         adjoint_impl(h, wk, out_grad);
@@ -2595,6 +2606,7 @@ int rsx_set_variant(rsx_engine* h, int variant)
 {
     return guarded(h, [&] {
         if (variant < 0 || variant >= rsx::VARIANT_COUNT) fail(RSX_INVALID_ARG, "variant must be 0, 1, 2 or 3");
+        h->k_max = 0;
         if (variant == h->variant) return;
         h->variant = variant;
         if (h->nq > 0) {  // the K list may differ: lay out again on the next calc_curve
