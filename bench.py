@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the Debye-SAXS hot path on one MI355X.
 
-Workload (north-star configuration 2): 20,000 atoms at 0.045 A^-3 in a
+Workload (hydrated forward, sasa-hydrated-20000x256): 20,000 atoms at 0.045 A^-3 in a
 1:1.3:1.8 ellipsoid, SASA hydration (256 Fibonacci dots, probe 1.4 A,
 contrast 0.03 e/A^3), 256 q over 0-0.6 A^-1.  A step is one full device-
 resident forward pass: SASA dot counts -> hydration classes -> Debye pair
@@ -59,7 +59,7 @@ def read_traffic():
 
 def forward_bench(steps, warmup):
     from resaxs_amd import configs
-    w = configs.config2()
+    w = configs.sasa_ellipsoid_20000()
     n, nq = len(w.bodies), len(w.q)
     e = make_engine(w)
     log(f"bench: {w.name}: host calc_curve (layout + first result)")
@@ -112,10 +112,10 @@ def sasa_summary(w, ms):
 
 
 def forward_50k_bench():
-    """The north-star scale: plain forward of 50,000 atoms x 256 q (the incremental
+    """The 50,000-atom scale: plain forward of 50,000 atoms x 256 q (the incremental
     workload's structure), pair-kernel time from HIP events on the engine stream."""
     from resaxs_amd import configs
-    w = configs.config3()
+    w = configs.incremental_50000()
     n, nq = len(w.bodies), len(w.q)
     e = make_engine(w, hydrate=False)
     log(f"bench: {n} x {nq} plain forward (autotuned layout)")
@@ -137,7 +137,7 @@ def adjoint_bench(reps=3):
     Device times (HIP events): the default forward pair kernel, the fit path's FP64
     forward pair kernel and the adjoint pass; plus host wall clock of the C-ABI calls."""
     from resaxs_amd import configs
-    w = configs.config5()
+    w = configs.fit_10000()
     n, nq = len(w.bodies), len(w.q)
     e = make_engine(w, hydrate=False)
     I_ref = e.calc_curve(w.bodies)
@@ -183,7 +183,7 @@ def incremental_bench(hydrate, n_moves=2000):
     since the last re-baseline) is compared with a fresh full recompute by the FP64
     pair kernel."""
     from resaxs_amd import configs
-    w = configs.config3()
+    w = configs.incremental_50000()
     n, nq, m = len(w.pos), len(w.q), w.seg
     e = make_engine(w, hydrate=hydrate)
     e.set_rebaseline(200)
@@ -225,7 +225,7 @@ def batched_bench(reps=5):
     """The batched ensemble: 512 frames of 1,000-15,000 atoms x 128 q, one graph-captured
     launch per replay; per-frame I(q) and the uniform mean to device buffers."""
     from resaxs_amd import Engine, configs
-    w = configs.config4()
+    w = configs.ensemble_512()
     nq = len(w.q)
     e = Engine(0)
     e.set_q(w.q)
@@ -291,7 +291,7 @@ def histogram_bench(bin_width=0.01, reps=3):
     host wall clock of rsx_calc_curve_hist (bodies from the host, I(q) back) against the
     exact engine's forward on the same engine, and its error relative to that forward."""
     from resaxs_amd import configs
-    w = configs.config3()
+    w = configs.incremental_50000()
     n, nq = len(w.bodies), len(w.q)
     e = make_engine(w, hydrate=False)
     I_exact = e.calc_curve(w.bodies)

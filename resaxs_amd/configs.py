@@ -125,7 +125,7 @@ class Workload(dict):
     __getattr__ = dict.__getitem__
 
 
-def config1(seed=1):
+def plain_ball_1888(seed=1):
     """1,888 atoms uniform in a ball of radius 18 A; 101 q in [0, 0.5]; plain."""
     rng = np.random.default_rng(seed)
     n = 1888
@@ -144,7 +144,7 @@ def _protein_like(seed, n, q):
     return rng, pos, types, table
 
 
-def config2(seed=2, n=20000):
+def sasa_ellipsoid_20000(seed=2, n=20000):
     """20,000 atoms at 0.045 A^-3 in a 1:1.3:1.8 ellipsoid; 256 q in [0, 0.6];
     SASA hydration (256 Fibonacci dots, probe 1.4 A)."""
     q = np.linspace(0.0, 0.6, 256).astype(np.float32)
@@ -153,8 +153,8 @@ def config2(seed=2, n=20000):
                     w=water_weight(), radii=RADII.copy(), probe=PROBE, n_half=128, bodies=bodies_of(pos, types))
 
 
-def config3(seed=3, n=50000, n_moves=2000, seg=500):
-    """50,000 atoms (config-2 generator, Z-order sorted); moves of 500-atom
+def incremental_50000(seed=3, n=50000, n_moves=2000, seg=500):
+    """50,000 atoms (generator of sasa_ellipsoid_20000, Z-order sorted); moves of 500-atom
     segments: rotation <= 5 deg about the segment centroid, translation <= 1 A."""
     q = np.linspace(0.0, 0.6, 256).astype(np.float32)
     rng, pos, types, table = _protein_like(seed, n, q)
@@ -183,7 +183,7 @@ def random_move(rng, pos, seg):
     return start, newp.astype(np.float32)
 
 
-def config4(seed=4, n_frames=512, nmin=1000, nmax=15000):
+def ensemble_512(seed=4, n_frames=512, nmin=1000, nmax=15000):
     """512 frames, N_f = round(exp(U[ln 1000, ln 15000])), alternating ball /
     ellipsoid at 0.045 A^-3 with Gaussian-perturbed shape radii; 128 q in
     [0, 0.5]; plain."""
@@ -207,8 +207,8 @@ def config4(seed=4, n_frames=512, nmin=1000, nmax=15000):
                     bodies=np.concatenate(bodies), offsets=np.array(offsets, dtype=np.int32), sizes=sizes)
 
 
-def config5(seed=5, n=10000):
-    """10,000 atoms (config-2 generator), 200 q in [0.01, 0.5]; target from the
+def fit_10000(seed=5, n=10000):
+    """10,000 atoms (generator of sasa_ellipsoid_20000), 200 q in [0.01, 0.5]; target from the
     reference conformation with 1 % Gaussian noise; start displaced by N(0, 0.5 A)."""
     q = np.linspace(0.01, 0.5, 200).astype(np.float32)
     rng, pos, types, table = _protein_like(seed, n, q)

@@ -211,10 +211,10 @@ def test_host_generators_equal_oracle(oracle):
 
 def test_workload_generators():
     from resaxs_amd import configs
-    w = configs.config1()
+    w = configs.plain_ball_1888()
     assert list(np.bincount(w.types)) == [604, 170, 189, 19, 906]
     assert np.all(np.linalg.norm(w.pos, axis=1) <= 18.0 + 1e-5)
-    w2 = configs.config2(n=5000)
+    w2 = configs.sasa_ellipsoid_20000(n=5000)
     ax = configs.ellipsoid_axes(5000)
     u = w2.pos / ax
     assert np.all((u ** 2).sum(1) <= 1 + 1e-5)
@@ -222,5 +222,5 @@ def test_workload_generators():
     assert abs(configs.ellipsoid_axes(20000)[0] - 35.66) < 0.01
     assert w2.table.shape == (256, 6)
     assert 2.0 < w2.w < 3.5
-    w4 = configs.config4(n_frames=16)
+    w4 = configs.ensemble_512(n_frames=16)
     assert w4.offsets[-1] == len(w4.bodies) and np.all(np.diff(w4.offsets) >= 1000)

@@ -28,7 +28,7 @@ def _say(msg):
 def test_batch_frames_and_mean(oracle, variant, k, persistent):
     from resaxs_amd import configs
     _say(f"case: batch of 24 frames, 100..2500 atoms x 128 q, variant {variant} K={k} persistent={persistent}")
-    w = configs.config4(n_frames=24, nmin=100, nmax=2500)
+    w = configs.ensemble_512(n_frames=24, nmin=100, nmax=2500)
     e = engine_for(w.q, w.table, w.water_index, 0.0)
     e.set_variant(variant)
     if k:
@@ -65,7 +65,7 @@ def test_batch_full_size(oracle):
     a graph replay reproduces every bit; the persistent work-queue launch at K = 64
     agrees to rounding."""
     from resaxs_amd import configs
-    w = configs.config4()
+    w = configs.ensemble_512()
     nf = len(w.sizes)
     _say(f"case: batched full size, {nf} frames, {int(w.sizes.sum())} atoms x 128 q")
     e = engine_for(w.q, w.table, w.water_index, 0.0)

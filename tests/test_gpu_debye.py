@@ -1,6 +1,6 @@
 """GPU parity of the Debye pair sums (forward I(q)) against the FP64 oracle.
 
-Bar: I(q) within 1e-5 relative of the FP64 oracle at every q (north-star
+Bar: I(q) within 1e-5 relative of the FP64 oracle at every q (the stated
 tolerance); the reference's own golden fixture (test_resaxs/test1888.*) is
 matched to the fixture's stated accuracy (5e-5; it carries 1e-3 A position
 rounding).
@@ -62,8 +62,8 @@ def test_golden_vs_reference_host_debye():
 
 def test_config1_plain(oracle):
     from resaxs_amd import configs
-    w = configs.config1()
-    _say("case: config1 1888 x 101 plain")
+    w = configs.plain_ball_1888()
+    _say("case: plain_ball_1888 1888 x 101 plain")
     e = engine_for(w.q, w.table, w.water_index, 0.0, n=len(w.pos))
     I = e.calc_curve(w.bodies)
     ref = oracle_iq(w.q, w.pos, w.types, w.table)
@@ -167,7 +167,7 @@ def test_fp32_variants_hydrated(oracle, variant, k):
     range: every run flush of both FP32 kernels against the FP64 oracle."""
     from resaxs_amd import configs
     n = 4000
-    w = configs.config2(n=n)
+    w = configs.sasa_ellipsoid_20000(n=n)
     _say(f"case: hydrated 4000 x 256, variant {variant} K={k}")
     e = engine_for(w.q, w.table, w.water_index, w.w, n=n, radii=w.radii, probe=w.probe, n_half=w.n_half)
     e.set_variant(variant)
@@ -207,8 +207,8 @@ def test_invariances(oracle):
 def test_config2_plain_full_size(oracle):
     """20,000 atoms x 256 q without hydration against the full FP64 oracle."""
     from resaxs_amd import configs
-    w = configs.config2()
-    _say("case: config2 plain 20000 x 256")
+    w = configs.sasa_ellipsoid_20000()
+    _say("case: sasa_ellipsoid_20000 plain 20000 x 256")
     e = engine_for(w.q, w.table, w.water_index, 0.0, n=len(w.pos))
     I = e.calc_curve(w.bodies)
     ref = oracle_iq(w.q, w.pos, w.types, w.table)

@@ -35,7 +35,7 @@ def _target(w, e):
 @pytest.mark.parametrize("fp32", [True, False])
 def test_fit_gradient_vs_fd_full_size(oracle, fp32):
     from resaxs_amd import configs
-    w = configs.config5()
+    w = configs.fit_10000()
     _say(f"case: chi2 gradient at full size, 10000 atoms x 200 q, FD on 64 atoms x 3, "
          f"{'FP32' if fp32 else 'FP64'} adjoint")
     e = engine_for(w.q, w.table, n=len(w.pos))
@@ -97,7 +97,7 @@ def test_autograd_wrappers(oracle):
 def test_refinement_100_steps():
     from resaxs_amd import configs
     from resaxs_amd.torch_debye import refine
-    w = configs.config5()
+    w = configs.fit_10000()
     _say("case: 100 Adam steps on chi2, 10000 atoms x 200 q")
     e = engine_for(w.q, w.table, n=len(w.pos))
     i_exp, sigma = _target(w, e)

@@ -51,7 +51,7 @@ def test_hist_hydrated_accuracy(oracle):
     from resaxs_amd import configs
     _say("case: histogram Debye, 3000 atoms x 128 q, SASA hydration, bins 0.02 A")
     n = 3000
-    w = configs.config2(n=n)
+    w = configs.sasa_ellipsoid_20000(n=n)
     q = np.linspace(0.0, 0.5, 128).astype(np.float32)
     table = configs.factor_table(np.random.default_rng(62), q)
     e = engine_for(q, table, w.water_index, w.w, n=n, radii=w.radii, probe=w.probe, n_half=w.n_half)

@@ -121,7 +121,7 @@ def test_dirty_params_force_full_recompute(oracle):
     from resaxs_amd import configs
     _say("case: water weight change forces a full recompute")
     n = 2500
-    w = configs.config2(n=n)
+    w = configs.sasa_ellipsoid_20000(n=n)
     e = engine_for(w.q, w.table, w.water_index, 0.0, n=n, radii=w.radii, probe=w.probe, n_half=w.n_half)
     e.calc_curve(w.bodies)
     e.set_water_weight(w.w)
@@ -139,7 +139,7 @@ def test_hydrated_recalc(oracle):
     from resaxs_amd import configs
     _say("case: hydrated recalc, 10 moves of 150 atoms on 3000 atoms")
     n = 3000
-    w = configs.config3(n=n)
+    w = configs.incremental_50000(n=n)
     pos = w.pos.copy()
     e = engine_for(w.q, w.table, w.water_index, w.w, n=n, radii=w.radii, probe=w.probe, n_half=w.n_half)
     e.set_rebaseline(0)
@@ -169,7 +169,7 @@ def test_hydrated_recalc_kernels(oracle, variant, uniform_q):
     _say(f"case: hydrated recalc, variant {variant}, {'uniform' if uniform_q else 'non-uniform'} q, "
          f"6 moves of 150 atoms on 2500 atoms")
     n = 2500
-    w = configs.config3(n=n)
+    w = configs.incremental_50000(n=n)
     q = w.q
     if not uniform_q:
         rng_q = np.random.default_rng(3)
@@ -213,7 +213,7 @@ def test_incremental_full_size_plain():
     full recompute by the FP64 pair kernel within 1e-5; the 200th move re-baselines."""
     from resaxs_amd import configs
     _say("case: 50000 atoms x 256 q, 200 moves of 500 atoms, re-baseline every 200, plain")
-    w = configs.config3()
+    w = configs.incremental_50000()
     pos = w.pos.copy()
     e = engine_for(w.q, w.table, n=len(pos))
     e.set_rebaseline(200)
@@ -248,7 +248,7 @@ def test_incremental_full_size_hydrated(oracle):
     oracle's cell-list count) and the running I(q) follows a fresh FP64 recompute."""
     from resaxs_amd import configs
     _say("case: 50000 atoms x 256 q, 20 moves of 500 atoms, SASA hydration, no re-baseline")
-    w = configs.config3()
+    w = configs.incremental_50000()
     pos = w.pos.copy()
     e = engine_for(w.q, w.table, w.water_index, w.w, n=len(pos), radii=w.radii, probe=w.probe, n_half=w.n_half)
     e.set_rebaseline(0)
@@ -279,7 +279,7 @@ def test_hydrated_local_recount_with_revert(oracle):
     from resaxs_amd import configs
     _say("case: hydrated local recount with commit / revert, 8000 atoms, moves of 120 atoms")
     n = 8000
-    w = configs.config3(n=n)
+    w = configs.incremental_50000(n=n)
     pos = w.pos.copy()
     e = engine_for(w.q, w.table, w.water_index, w.w, n=n, radii=w.radii, probe=w.probe, n_half=w.n_half)
     e.set_rebaseline(0)

@@ -27,7 +27,7 @@ def _engine():
 
 def test_sasa_counts_config2_bit_exact(oracle):
     from resaxs_amd import configs
-    w = configs.config2()
+    w = configs.sasa_ellipsoid_20000()
     _say("case: SASA counts, 20000 atoms x 256 dots")
     dots = fib_dot_table(w.radii, w.probe, w.n_half)
     e = _engine()
@@ -86,7 +86,7 @@ def test_hydrated_debye_vs_oracle(oracle):
     """Hydration factors F = f_t + (count/D) w f_w from the GPU's own counts;
     I(q) against the FP64 oracle using the oracle's (identical) counts."""
     from resaxs_amd import configs
-    w = configs.config2(n=6000)
+    w = configs.sasa_ellipsoid_20000(n=6000)
     _say("case: hydrated Debye 6000 x 256")
     dots = fib_dot_table(w.radii, w.probe, w.n_half)
     e = engine_for(w.q, w.table, w.water_index, w.w, n=len(w.pos), radii=w.radii, probe=w.probe, n_half=w.n_half)
@@ -101,9 +101,9 @@ def test_hydrated_debye_vs_oracle(oracle):
 
 
 def test_hydrated_full_size(oracle):
-    """Configuration 2 at full size: 20,000 atoms x 256 q with SASA hydration."""
+    """The hydrated forward at full size: 20,000 atoms x 256 q with SASA hydration."""
     from resaxs_amd import configs
-    w = configs.config2()
+    w = configs.sasa_ellipsoid_20000()
     _say("case: hydrated Debye 20000 x 256 (full size)")
     dots = fib_dot_table(w.radii, w.probe, w.n_half)
     e = engine_for(w.q, w.table, w.water_index, w.w, n=len(w.pos), radii=w.radii, probe=w.probe, n_half=w.n_half)
