@@ -1825,8 +1825,9 @@ void adjoint_layout(rsx_engine* h)
         const int k = rsx::adjoint_k_value(fp32, i);
         if (h->adj_k_req > 0 && k != h->adj_k_req) continue;
         const int c = (h->nq + k - 1) / k;
-        // (the FP32 kernel keeps 3 waves per SIMD up to K = 67)
-        const double cost = fp32 ? c * (2.4 * k + 30.0) * (k > 67 ? 1.1 : 1.0) : c * (4.0 * k + 60.0);
+        // (the FP32 kernel keeps 3 waves per SIMD up to K = 67 and 2 at K = 100, where the
+        // per-pair seeds spread over more q make up for it: 1.08 vs 1.09-1.12 ms at 10k x 200)
+        const double cost = fp32 ? c * (2.4 * k + 30.0) : c * (4.0 * k + 60.0);
         if (cost < best) {
             best = cost;
             K = k;

@@ -1471,7 +1471,7 @@ constexpr int REC32_KS[] = {16, 32, 43, 48, 51, 64, 67, 80, 86, 96, 100};
 constexpr int REC32R_KS[] = {64, 86, 96, 100, 128};
 constexpr int DIRECT_KS[] = {8, 16, 32, 48, 64};
 constexpr int ADJOINT_KS[] = {8, 16, 32, 48, 64};
-constexpr int ADJOINT32_KS[] = {16, 32, 40, 50, 64, 67};
+constexpr int ADJOINT32_KS[] = {16, 32, 40, 50, 64, 67, 100};
 
 }  // namespace
 
@@ -2077,6 +2077,7 @@ hipError_t launch_adjoint(const AdjointArgs& a, int n_jobs, int K, hipStream_t s
         case 50: launch_adjoint32_k<50>(a, n_jobs, s); break;
         case 64: launch_adjoint32_k<64>(a, n_jobs, s); break;
         case 67: launch_adjoint32_k<67>(a, n_jobs, s); break;
+        case 100: launch_adjoint32_k<100>(a, n_jobs, s); break;
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
