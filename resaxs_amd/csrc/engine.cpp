@@ -406,6 +406,11 @@ struct rsx_engine {
         bool have_rsasa = false;
         int since_rebase = 0;
     } saved;
+    // bodies / counts / cls / rsasa entries written by incremental updates since the last
+    // commit or revert, which then copy only those; dirty_full: a path rewrote whole
+    // arrays or the slot layout since then
+    std::vector<int> dirty;
+    bool dirty_full = true;
     int since_rebase = 0;
 
     // batch
@@ -428,6 +433,23 @@ namespace {
 void set_device(rsx_engine* h)
 {
     hipchk(hipSetDevice(h->device), "hipSetDevice");
+}
+
+void mark_dirty_full(rsx_engine* h)
+{
+    h->dirty_full = true;
+    h->dirty.clear();
+}
+
+// record entry i before an incremental update writes it
+void mark_dirty(rsx_engine* h, int i)
+{
+    if (h->dirty_full) return;
+    if (4 * h->dirty.size() >= (size_t)h->n) {
+        mark_dirty_full(h);
+        return;
+    }
+    h->dirty.push_back(i);
 }
 
 // ---- q grid and chunking -------------------------------------------------
@@ -638,12 +660,14 @@ uint32_t class_of(const rsx_engine* h, int i)
 void assign_slots(rsx_engine* h)
 {
     const int n = h->n;
+    mark_dirty_full(h);
     h->cls.resize(n);
     for (int i = 0; i < n; ++i) h->cls[i] = class_of(h, i);
     h->cls_count_ok = false;
     std::vector<int> cnt(h->n_class + 1, 0);
     for (int i = 0; i < n; ++i) cnt[h->cls[i] + 1]++;
     for (int c = 0; c < h->n_class; ++c) cnt[c + 1] += cnt[c];
+    mark_dirty_full(h);
     h->caller_of.resize(n);
     h->slot_of.resize(n);
     for (int i = 0; i < n; ++i) {
@@ -753,6 +777,7 @@ void compute_sasa(rsx_engine* h)
            "upload sasa atoms");
     h->sasa_mirror = true;
     run_sasa_device(h, n);
+    mark_dirty_full(h);
     h->counts.resize(n);
     hipchk(hipMemcpyAsync(h->counts.data(), h->d_counts.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream),
            "download counts");
@@ -825,7 +850,10 @@ bool compute_sasa_local(rsx_engine* h, int start, int len, const std::vector<flo
     for (int t = 0; t < ns; ++t) {
         const int i = sub[t];
         const uint32_t c = h->pin_cnt.p[t];
-        if (h->counts[i] != c) changed.push_back(i);
+        if (h->counts[i] != c) {
+            changed.push_back(i);
+            mark_dirty(h, i);
+        }
         h->counts[i] = c;
         h->rsasa[i] = (float)c / (float)h->n_dots;
     }
@@ -1170,6 +1198,7 @@ void load_bodies(rsx_engine* h, const float* bodies, int start, int len)
         uint32_t t;
         std::memcpy(&t, &bodies[4 * i + 3], 4);
         if ((int)t >= h->n_types) fail(RSX_INVALID_ARG, "body type index outside the factor table");
+        mark_dirty(h, i);
         std::memcpy(&h->bodies[4 * i], &bodies[4 * i], 16);
         h->type[i] = t;
     }
@@ -1396,6 +1425,7 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
     // q = 0: (sum F_new)^2 - (sum F_old)^2 from the class populations
     const double s_old = sum_f0(h);
     for (int a = 0; a < m; ++a) {
+        mark_dirty(h, M[a]);
         h->cls[M[a]] = cls_new[a];
         if (h->cls_count_ok) {
             h->cls_count[cls_old[a]]--;
@@ -1410,6 +1440,25 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
     copy_out(h, out32, out64);
 }
 
+// the saved arrays have the current sizes (entry-wise copies are possible)
+bool saved_matches_sizes(const rsx_engine* h)
+{
+    return h->saved.bodies.size() == h->bodies.size() && h->saved.counts.size() == h->counts.size() &&
+           h->saved.cls.size() == h->cls.size() && h->saved.rsasa.size() == h->rsasa.size() &&
+           h->saved.slot_of.size() == h->slot_of.size() && h->saved.caller_of.size() == h->caller_of.size();
+}
+
+// entry i of bodies / counts / cls / rsasa from src to dst (same sizes)
+void copy_entry(std::vector<float>& db, std::vector<uint32_t>& dc, std::vector<uint32_t>& dk, std::vector<float>& dr,
+                const std::vector<float>& sb, const std::vector<uint32_t>& sc, const std::vector<uint32_t>& sk,
+                const std::vector<float>& sr, int i)
+{
+    std::memcpy(&db[4 * (size_t)i], &sb[4 * (size_t)i], 16);
+    if ((size_t)i < dc.size()) dc[i] = sc[i];
+    if ((size_t)i < dk.size()) dk[i] = sk[i];
+    if ((size_t)i < dr.size()) dr[i] = sr[i];
+}
+
 void commit_impl(rsx_engine* h)
 {
     if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "commit() called before an initial call to calc_curve()");
@@ -1421,12 +1470,19 @@ void commit_impl(rsx_engine* h)
     hipchk(hipMemcpyAsync(h->d_saved_iq.p, h->d_iq.p, h->nq * sizeof(double), hipMemcpyDeviceToDevice, h->stream),
            "save iq");
     hipchk(hipStreamSynchronize(h->stream), "sync commit");
-    h->saved.bodies = h->bodies;
-    h->saved.counts = h->counts;
-    h->saved.cls = h->cls;
-    h->saved.rsasa = h->rsasa;
-    h->saved.slot_of = h->slot_of;
-    h->saved.caller_of = h->caller_of;
+    if (saved_matches_sizes(h) && !h->dirty_full) {
+        for (int i : h->dirty) copy_entry(h->saved.bodies, h->saved.counts, h->saved.cls, h->saved.rsasa, h->bodies,
+                                          h->counts, h->cls, h->rsasa, i);
+    } else {
+        h->saved.bodies = h->bodies;
+        h->saved.counts = h->counts;
+        h->saved.cls = h->cls;
+        h->saved.rsasa = h->rsasa;
+        h->saved.slot_of = h->slot_of;
+        h->saved.caller_of = h->caller_of;
+    }
+    h->dirty.clear();
+    h->dirty_full = false;
     h->saved.have_rsasa = h->have_rsasa;
     h->saved.since_rebase = h->since_rebase;
     h->xaction = true;
@@ -1437,22 +1493,34 @@ void revert_impl(rsx_engine* h)
     if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "revert() called before an initial call to calc_curve()");
     if (!h->xaction) return;
     set_device(h);
-    const bool layout_changed = h->saved.caller_of != h->caller_of;
+    const bool partial = saved_matches_sizes(h) && !h->dirty_full;
+    const bool layout_changed = !partial && h->saved.caller_of != h->caller_of;
     h->sasa_mirror = false;  // the device copy may hold the reverted positions
-    h->bodies = h->saved.bodies;
-    h->counts = h->saved.counts;
-    h->cls = h->saved.cls;
+    auto restore_type = [h](int i) {
+        uint32_t t;
+        std::memcpy(&t, &h->bodies[4 * (size_t)i + 3], 4);
+        h->type[i] = t;
+    };
+    if (partial) {
+        for (int i : h->dirty) {
+            copy_entry(h->bodies, h->counts, h->cls, h->rsasa, h->saved.bodies, h->saved.counts, h->saved.cls,
+                       h->saved.rsasa, i);
+            restore_type(i);
+        }
+    } else {
+        h->bodies = h->saved.bodies;
+        h->counts = h->saved.counts;
+        h->cls = h->saved.cls;
+        h->rsasa = h->saved.rsasa;
+        h->slot_of = h->saved.slot_of;
+        h->caller_of = h->saved.caller_of;
+        for (int i = 0; i < h->n; ++i) restore_type(i);
+    }
+    h->dirty.clear();
+    h->dirty_full = false;
     h->cls_count_ok = false;
-    h->rsasa = h->saved.rsasa;
-    h->slot_of = h->saved.slot_of;
-    h->caller_of = h->saved.caller_of;
     h->have_rsasa = h->saved.have_rsasa;
     h->since_rebase = h->saved.since_rebase;
-    for (int i = 0; i < h->n; ++i) {
-        uint32_t t;
-        std::memcpy(&t, &h->bodies[4 * i + 3], 4);
-        h->type[i] = t;
-    }
     hipchk(hipMemcpyAsync(h->d_cols.p, h->d_saved_cols.p, (size_t)h->n * sizeof(float4), hipMemcpyDeviceToDevice,
                           h->stream),
            "restore cols");
@@ -2393,6 +2461,7 @@ int rsx_initialize(rsx_engine* h, int n_bodies)
         h->type.assign(n_bodies, 0);
         h->counts.assign(n_bodies, 0);
         h->rsasa.assign(n_bodies, 0.0f);
+        mark_dirty_full(h);
         h->have_rsasa = false;
         h->xaction = false;
         h->params_dirty = true;

@@ -296,6 +296,8 @@ def test_hydrated_local_recount_with_revert(oracle):
         I = e.recalc_curve(bt, start, 120)
         if step % 3 == 2:
             e.revert()                     # reject: back to pos
+            # the restored exposure fractions are the committed ones
+            assert np.array_equal(e.get_rsasa(), counts.astype(np.float32) / np.float32(256)), step
             continue
         e.commit()
         pos = trial
