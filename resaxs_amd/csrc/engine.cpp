@@ -382,6 +382,7 @@ struct rsx_engine {
     std::vector<PairJob> inc_jobs;  // host table of the incremental jobs (capacity kept)
     // pinned staging of the per-update transfers
     HBuf<float4> pin_rows, pin_vals, pin_moved;
+    HBuf<PairJob> pin_jobs;
     HBuf<int> pin_idx, pin_sub;
     HBuf<uint32_t> pin_cnt;
     HBuf<double> pin_iq;
@@ -1414,8 +1415,11 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
         split_tail(jobs, resident_jobs(h, kc.variant, kc.K, kc.n_chunks),
                    [](const PairJob& j) { return (double)j.ncol * j.nrow; });
         h->d_jobs.ensure(jobs.size());
-        // (the host table stays alive in the engine until the copy has been consumed)
-        hipchk(hipMemcpyAsync(h->d_jobs.p, jobs.data(), jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice,
+        // staged through pinned memory (the call waits for the move's result before
+        // returning, so the staging buffer is free again at the next move)
+        h->pin_jobs.ensure(jobs.size());
+        std::memcpy(h->pin_jobs.p, jobs.data(), jobs.size() * sizeof(PairJob));
+        hipchk(hipMemcpyAsync(h->d_jobs.p, h->pin_jobs.p, jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice,
                               h->stream),
                "upload jobs");
         h->inc_key = key;
@@ -2293,6 +2297,7 @@ int rsx_destroy(rsx_engine* h)
     h->inc_chunks_K = 0;
     h->d_subcnt.release();
     h->pin_rows.release();
+    h->pin_jobs.release();
     h->pin_vals.release();
     h->pin_moved.release();
     h->pin_idx.release();
