@@ -32,8 +32,8 @@
 // Dot-sphere occlusion (resaxslib/src/rsasa_dot_spheres.cl:94-163): a dot of
 // atom i is buried when (p + r_j)^2 - |dot - x_j|^2 > 1e-4 for any atom j.
 // One wave per atom, dots spread over lanes, candidates from a hashed cell
-// grid; the predicate is evaluated in FP32 without contraction so the integer
-// counts equal the CPU restatement bit for bit.
+// grid; the predicate is evaluated in FP32 without contraction (dot_occluded) so
+// the integer counts equal the CPU restatement bit for bit.
 
 #include "kernels.h"
 
@@ -973,6 +973,20 @@ __device__ __forceinline__ int cell_coord(float x, float inv_cell)
     return (int)floorf(x * inv_cell);
 }
 
+// The occlusion predicate of rsasa_dot_spheres.cl:122-132 / rsasa_cpu_alg.cpp:62-68:
+// (p + r_j)^2 - |dot - x_j|^2 > 1e-4, the squared length summed x, y, z in order with
+// each product and sum rounded on its own.  Contraction is off inside this function
+// (the build's default fuses a*b + c into one FMA, which rounds once and moves the
+// near-ties of the 1e-4 test), so the integer counts equal an unfused FP32 evaluation.
+__device__ __forceinline__ bool dot_occluded(float dx, float dy, float dz, float r2j)
+{
+#pragma clang fp contract(off)
+    float dist2 = dx * dx;
+    dist2 = dist2 + dy * dy;
+    dist2 = dist2 + dz * dz;
+    return (r2j - dist2) > 1e-4f;
+}
+
 __device__ __forceinline__ int cell_hash(int ix, int iy, int iz, int nb)
 {
     const uint32_t h = ((uint32_t)ix * 73856093u) ^ ((uint32_t)iy * 19349663u) ^ ((uint32_t)iz * 83492791u);
@@ -1145,11 +1159,7 @@ __global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
             const float jr2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r2j), l));
 #pragma unroll
             for (int t = 0; t < DPL; ++t) {
-                const float dx = px[t] - jx, dy = py[t] - jy, dz = pz[t] - jz;
-                float dist2 = dx * dx;
-                dist2 = dist2 + dy * dy;
-                dist2 = dist2 + dz * dz;
-                live[t] = live[t] && !((jr2 - dist2) > 1e-4f);
+                live[t] = live[t] && !dot_occluded(px[t] - jx, py[t] - jy, pz[t] - jz, jr2);
             }
         }
         bool any = false;
@@ -1163,7 +1173,6 @@ __global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
     cnt = wave_sum_i(cnt);
     if (lane == 0) a.counts[i] = (uint32_t)cnt;
 }
-#pragma clang fp contract(on)
 
 // ---------------------------------------------------------------------------
 // adjoint: grad_i = sum_k w_k dI_k/dx_i

@@ -138,6 +138,80 @@ def test_sasa_oracle_closed_forms(oracle):
     assert np.array_equal(a, b)
 
 
+SASA_TIES = os.path.join(ROOT, "tests", "golden", "sasa_ties.npz")
+
+
+def test_sasa_oracle_near_ties(oracle):
+    """Counts on the near-tie fixture (tests/golden/make_sasa_ties.py): pairs of atoms
+    where one dot lies within an ulp of the 1e-4 occlusion threshold.  The oracle's
+    unfused FP32 predicate (rsasa_dot_spheres.cl:122-132, rsasa_cpu_alg.cpp:62-68) gives
+    the fixture's counts, brute force and cells alike; every fused ordering of the squared
+    distance gives other counts on this fixture, so it tells an unfused evaluation from
+    any contracted one."""
+    g = np.load(SASA_TIES)
+    b = g["bodies"]
+    types = b[:, 3].view(np.uint32)
+    R2 = ((g["radii"] + g["probe"]) ** 2).astype(np.float32)
+    assert np.array_equal(R2, g["r2"])
+    for cells in (False, True):
+        c = oracle.sasa_counts(b[:, :3], types, R2, g["dots"], cells=cells)
+        assert np.array_equal(c, g["counts"]), cells
+    for v, name in enumerate(g["variants"]):
+        diff = int(np.sum(g["counts_fused"][v] != g["counts"]))
+        assert diff >= 10, (name, diff)
+        # each tie moves its atom's count by one dot at most
+        assert np.abs(g["counts_fused"][v].astype(int) - g["counts"].astype(int)).max() == 1
+    # the tie dots sit at the threshold: |(p + r_j)^2 - |d|^2 - 1e-4| within a few ulps
+    for a, d, j in zip(g["tie_atom"], g["tie_dot"], g["tie_partner"]):
+        p = b[a, :3].astype(np.float64) + g["dots"][types[a], d, :3]
+        gap = float(g["r2"][types[j]]) - np.sum((p - b[j, :3]) ** 2) - 1e-4
+        assert abs(gap) < 1e-5, gap
+
+
+def test_hydration_composition_closed_forms(oracle):
+    """The hydration factor F_i(q) = f_t(q) + s_i w f_water(q): the reference adds the body's
+    surface fraction times water_factor = t_factors[q][WATER_FF_INDEX] * water_weight to the
+    mapped type factor (gpu_saxs_pt_wf.cl:130-138).  Every column of the table differs, so a
+    swapped water index, s applied to f_t, or w applied twice fails these closed forms."""
+    rng = np.random.default_rng(11)
+    q = np.linspace(0.0, 0.5, 17).astype(np.float32)
+    nt, water = 6, 4
+    table = rng.uniform(0.5, 8.0, size=(len(q), nt)).astype(np.float32)
+    t64 = table.astype(np.float64)
+    w = np.float32(0.73)
+    origin = np.zeros((1, 3), dtype=np.float32)
+    # one atom of type t with surface fraction s: I(q) = (f_t + s w f_water)^2 at every q
+    for t in range(nt):
+        for s in (0.0, 0.25, 1.0):
+            F = oracle.body_factors(origin, [t], q, table, water, w, np.array([s], np.float32))
+            I = oracle.debye(q, origin, F)
+            expect = (t64[:, t] + s * np.float64(w) * t64[:, water]) ** 2
+            assert rel_err(I, expect).max() < 1e-14, (t, s)
+            if s > 0 and t != water:
+                wrong = [(t64[:, t] + s * np.float64(w) * t64[:, t]) ** 2,           # water column = own
+                         (t64[:, t] * (1 + s) + np.float64(w) * t64[:, water]) ** 2,  # s on f_t
+                         (t64[:, t] + s * np.float64(w) ** 2 * t64[:, water]) ** 2]   # w twice
+                for alt in wrong:
+                    assert rel_err(I, alt).max() > 1e-3
+    # two atoms of different types and fractions at distance r:
+    # I = F1^2 + F2^2 + 2 F1 F2 sin(q r) / (q r)
+    r = 7.25
+    pos = np.array([[0, 0, 0], [0, r, 0]], dtype=np.float32)
+    s = np.array([0.375, 0.8125], dtype=np.float32)
+    F = oracle.body_factors(pos, [1, 3], q, table, water, w, s)
+    I = oracle.debye(q, pos, F)
+    F1 = t64[:, 1] + np.float64(s[0]) * np.float64(w) * t64[:, water]
+    F2 = t64[:, 3] + np.float64(s[1]) * np.float64(w) * t64[:, water]
+    qr = q.astype(np.float64) * r
+    sinc = np.where(qr == 0, 1.0, np.sin(qr) / np.where(qr == 0, 1.0, qr))
+    assert rel_err(I, F1 ** 2 + F2 ** 2 + 2 * F1 * F2 * sinc).max() < 1e-13
+    # w = 0, s = 0 or no water index: the plain factors
+    plain = oracle.body_factors(pos, [1, 3], q, table)
+    assert np.array_equal(plain, np.stack([t64[:, 1], t64[:, 3]]))
+    for args in ((water, 0.0, s), (water, w, np.zeros(2, np.float32)), (-1, w, s)):
+        assert np.array_equal(oracle.body_factors(pos, [1, 3], q, table, *args), plain)
+
+
 # ------------------------------------------------------------- form factors
 def test_atomic_form_factors(oracle):
     q = np.array([0.0, 0.2, 0.5], dtype=np.float32)

@@ -4,6 +4,7 @@ The dot templates come from the library's own generator, which equals the
 oracle's restatement of rsasa_algorithm.cpp:96-116 bit for bit (checked in the
 CPU suite), so both sides test the same dots with the same FP32 predicate.
 """
+import os
 import sys
 
 import numpy as np
@@ -23,6 +24,59 @@ def _say(msg):
 def _engine():
     from resaxs_amd import Engine
     return Engine(0)
+
+
+SASA_TIES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sasa_ties.npz")
+
+
+def test_sasa_near_ties_bit_exact():
+    """Counts on the near-tie fixture (tests/golden/make_sasa_ties.py): 40 atom pairs with a
+    dot within an ulp of the 1e-4 occlusion threshold, where every fused ordering of the
+    squared distance flips 14-30 of the 80 counts.  The kernel's counts equal the unfused
+    FP32 predicate's bit for bit."""
+    g = np.load(SASA_TIES)
+    _say("case: SASA near-tie fixture, 80 atoms x 64 dots")
+    e = _engine()
+    c, frac = e.calc_rsasa(g["bodies"], g["radii"], g["probe"], g["dots"])
+    assert np.array_equal(c, g["counts"]), np.nonzero(c != g["counts"])[0]
+    assert np.array_equal(frac, g["counts"].astype(np.float32) / np.float32(64))
+
+
+def test_hydration_composition_closed_forms_gpu(oracle):
+    """The engine's hydrated I(q) against closed forms of F_i = f_t + (count_i / D) w f_water
+    (gpu_saxs_pt_wf.cl:130-138) with the engine's own exposed-dot counts: one isolated atom of
+    each type (every dot exposed) gives (f_t + w f_water)^2; two overlapping atoms of
+    different types give F1^2 + F2^2 + 2 F1 F2 sin(qr)/(qr).  Every table column differs, so
+    a swapped water index or a misapplied fraction fails."""
+    rng = np.random.default_rng(17)
+    q = np.linspace(0.0, 0.5, 64).astype(np.float32)
+    water = 4
+    table = rng.uniform(0.5, 8.0, size=(len(q), 5)).astype(np.float32)
+    t64 = table.astype(np.float64)
+    radii = np.array([1.7, 1.55, 1.52, 1.8], dtype=np.float32)
+    probe, n_half, w = 1.4, 32, 0.61
+    D = 2 * n_half
+    qr_sinc = lambda r: np.where(q == 0, 1.0, np.sin(q.astype(np.float64) * r) / np.where(q == 0, 1.0, q * r))
+    _say("case: hydration composition, closed forms")
+    for t in range(4):
+        e = engine_for(q, table, water, w, n=1, radii=radii, probe=probe, n_half=n_half)
+        I = e.calc_curve(oracle.bodies(np.zeros((1, 3)), [t]))
+        assert e.get_rsasa()[0] == 1.0
+        expect = (t64[:, t] + np.float64(np.float32(w)) * t64[:, water]) ** 2
+        assert rel_err(I, expect).max() < 1e-6, t
+        e.close()
+    r = 3.1
+    pos = np.array([[0, 0, 0], [r, 0, 0]], dtype=np.float32)
+    e = engine_for(q, table, water, w, n=2, radii=radii, probe=probe, n_half=n_half)
+    I = e.calc_curve(oracle.bodies(pos, [1, 3]))
+    counts = oracle.sasa_counts(pos, [1, 3], r2_table(radii, probe), fib_dot_table(radii, probe, n_half),
+                                cells=False)
+    assert 0 < counts.min() and counts.max() < D
+    s = (counts.astype(np.float32) / np.float32(D)).astype(np.float64)
+    assert np.array_equal(e.get_rsasa(), counts.astype(np.float32) / np.float32(D))
+    wf = np.float64(np.float32(w)) * t64[:, water]
+    F1, F2 = t64[:, 1] + s[0] * wf, t64[:, 3] + s[1] * wf
+    assert rel_err(I, F1 ** 2 + F2 ** 2 + 2 * F1 * F2 * qr_sinc(r)).max() < 1e-5
 
 
 def test_sasa_counts_config2_bit_exact(oracle):
