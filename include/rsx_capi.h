@@ -55,6 +55,17 @@ void *rsx_stream(rsx_engine *h);
 int rsx_set_q(rsx_engine *h, const float *q, int n_q);
 /* table [n_q][n_types]; water_index < 0: no hydration factor */
 int rsx_set_factors(rsx_engine *h, const float *table, int n_types, int water_index);
+/* the default atomic factor table (atomic_form_factors::generate, ff_coef.hpp:112-211, as the
+ * reference's engine sets it up, saxs_alg_cl_gpu_pt_wf.cpp:38-48): [n_q][16] at the engine's
+ * q, mean atomic radius avg_radius (atomic_form_factors::calc_avg_radius of the bodies) and
+ * the current expansion factor, water row OH2 (type 14) */
+int rsx_set_atomic_factors(rsx_engine *h, float avg_radius);
+/* excluded-volume expansion factor c > 0 (form_factor_params::set_expansion_factor,
+ * saxs_algorithm.hpp:101-107; saxs_alg_cl_gpu_pt_wf.cpp:53-62).  With the atomic table of
+ * rsx_set_atomic_factors the table is regenerated at c; a caller's table (rsx_set_factors) is
+ * kept and c only recorded, as the reference's generic form-factor parameters do.  Either way
+ * the next recalc is a full recompute (the parameters are dirty). */
+int rsx_set_expansion_factor(rsx_engine *h, float c);
 int rsx_set_water_weight(rsx_engine *h, float w);
 /* skip the SASA recomputation on recalc once one exists (implicit_water_params::no_rsasa_recalc_) */
 int rsx_set_no_rsasa_recalc(rsx_engine *h, int flag);
@@ -145,9 +156,16 @@ int rsx_pdb_read(const char *path, int all_models, int verbose, float *bodies, l
                  long long *n_atoms, int *offsets, int offsets_capacity, int *n_models);
 
 /* ---- batched ensemble (no reference counterpart; calc_saxs/calc_saxs.hpp:611-624 loops frames) ---- */
-/* frames: bodies concatenated, frame f is [offsets[f], offsets[f+1]); uploads and lays them out */
+/* frames: bodies concatenated, frame f is [offsets[f], offsets[f+1]) (offsets[0] = 0, never
+ * decreasing); uploads and lays them out.  The batch keeps a layout of its own (q chunks,
+ * classes, factor table, jobs), untouched by single-structure calls on the same engine.  With
+ * hydration on (water weight != 0, a water row and SASA parameters) each frame's exposed-dot
+ * counts are taken once, frame by frame, and its atoms get the classes type x count. */
 int rsx_batch_set_frames(rsx_engine *h, const float *bodies, const int *offsets, int n_frames);
-/* per-frame I(q) [n_frames][n_q] and the uniform-weight mean [n_q]; either may be NULL */
+/* per-frame I(q) [n_frames][n_q] and the uniform-weight mean [n_q]; either may be NULL.
+ * A changed factor table, water weight or expansion factor re-forms only the class factors
+ * (the captured launch is kept: a fit over (c, w) pays one factor upload per point); new q,
+ * tuning, SASA parameters or hydration switched on lay the kept frames out again. */
 int rsx_batch_calc(rsx_engine *h, double *out_frames, double *out_mean);
 /* same on resident frames, results left on the device (no host copy); graph-captured */
 int rsx_batch_calc_dev_async(rsx_engine *h, double *out_frames_dev, double *out_mean_dev);

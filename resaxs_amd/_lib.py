@@ -43,6 +43,8 @@ SIGNATURES = {
     "rsx_stream": (_vp, [_vp]),
     "rsx_set_q": (_i, [_vp, _fp, _i]),
     "rsx_set_factors": (_i, [_vp, _fp, _i, _i]),
+    "rsx_set_atomic_factors": (_i, [_vp, _f]),
+    "rsx_set_expansion_factor": (_i, [_vp, _f]),
     "rsx_set_water_weight": (_i, [_vp, _f]),
     "rsx_set_no_rsasa_recalc": (_i, [_vp, _i]),
     "rsx_set_sasa_params": (_i, [_vp, _fp, _i, _f, _i, _i]),

@@ -215,6 +215,8 @@ const float kCM[7][9] = {
 const float kExclVol[7] = {5.15f, 16.44f, 2.49f, 9.13f, 5.73f, 19.86f, 9.85f};
 const int kComplex[9][2] = {{1, 1}, {1, 2}, {1, 3}, {2, 1}, {2, 2}, {2, 3}, {3, 1}, {3, 2}, {5, 1}};
 constexpr float kRho0 = 0.334f;
+constexpr int ATOMIC_TYPES = 16;  // atom types of the default table (constants.hpp:28-51)
+constexpr int ATOMIC_OH2 = 14;    // its water row, the OH2 group
 
 // The exponentials and powers are evaluated in double from float arguments and
 // rounded to float once per expression, the evaluation order of ff_coef.hpp:86-102
@@ -283,6 +285,9 @@ struct rsx_engine {
     int n_types = 0;
     int water = -1;
     float w = 0.0f;
+    bool atomic_table = false;  // table = the default atomic factors (rsx_set_atomic_factors)
+    float avg_radius = 1.62f;   // their mean atomic radius
+    float expansion = 1.0f;     // excluded-volume expansion factor c
     bool params_dirty = true;  // factors / weight changed since the last full calc
     bool w_dirty = false;
     bool no_rsasa_recalc = false;
@@ -414,9 +419,36 @@ struct rsx_engine {
     bool dirty_full = true;
     int since_rebase = 0;
 
-    // batch
+    // inputs the batch layout depends on, stamped when they change
+    uint64_t factor_stamp = 1;  // factor table, water row or weight
+    uint64_t sasa_stamp = 1;    // SASA radii, probe or dot sphere
+
+    // batch: the frames as given, and a layout of its own (path, q chunks, classes, factor
+    // table, jobs), so single-structure calls on the same engine never touch it
     int b_frames = 0;
     std::vector<int> b_offsets;
+    std::vector<float> b_bodies;     // frames as given, {x, y, z, type bits}
+    bool b_layout_ok = false;        // the layout matches q, the tuning and the SASA set-up
+    uint64_t b_factor_stamp = 0;     // factor_stamp the class table was built from
+    uint64_t b_sasa_stamp = 0;
+    int b_nq = 0, b_n_types = 0;
+    bool b_hydrate = false;          // classes type x exposed-dot count
+    int b_d1 = 1, b_n_class = 0;
+    std::vector<uint32_t> b_counts;  // exposed-dot count per body (hydrated layouts)
+    std::vector<int> b_pop_off;      // frame f's class populations: b_pop[b_pop_off[f] .. b_pop_off[f+1])
+    std::vector<std::pair<int, int>> b_pop;  // (class, atoms)
+    bool b_uniform = false;
+    int b_variant = 0, b_K = 0, b_n_chunks = 0, b_ldf = 0, b_L = 0;
+    double b_radius = 0.0;
+    DBuf<ChunkConst> b_chunks;
+    DBuf<double> b_invq, b_qrev;
+    DBuf<float> b_qv, b_ftab;
+    HBuf<float> b_pin_ftab;
+    HBuf<double> b_pin_q0;
+    std::vector<double> b_f64;       // [class][n_q] FP64 class factors of the current table
+    // batch tuning (rsx_batch_autotune / its cache): variant, K, column range, launch form;
+    // -1 / 0 = follow the engine
+    int b_var_req = -1, b_k_req = 0, b_range_req = 0, b_pers_req = -1;
     DBuf<float4> b_cols;
     DBuf<PairJob> b_jobs;
     DBuf<double> b_partials, b_out, b_mean, b_q0, b_red;
@@ -454,27 +486,33 @@ void mark_dirty(rsx_engine* h, int i)
 }
 
 // ---- q grid and chunking -------------------------------------------------
-// K list of the active pair kernel
-int pair_path(const rsx_engine* h)
+// K list of a pair kernel: the path of a (uniform grid?, recurrence variant) choice
+int pair_path(bool uniform, int variant)
 {
-    return !h->uniform                          ? rsx::PATH_DIRECT
-           : h->variant == rsx::VARIANT_FP32  ? rsx::PATH_REC32
-           : h->variant == rsx::VARIANT_FP32R ? rsx::PATH_REC32R
-                                              : rsx::PATH_REC64;
+    return !uniform                          ? rsx::PATH_DIRECT
+           : variant == rsx::VARIANT_FP32  ? rsx::PATH_REC32
+           : variant == rsx::VARIANT_FP32R ? rsx::PATH_REC32R
+                                           : rsx::PATH_REC64;
 }
 
-void choose_chunks(rsx_engine* h)
+int pair_path(const rsx_engine* h)
 {
-    const bool uni = h->uniform;
-    const int path = pair_path(h);
+    return pair_path(h->uniform, h->variant);
+}
+
+// q per thread K (and the chunk count C) of a pair kernel over the engine's n_q: the
+// requested K when k_req > 0, else the cheapest of the kernel's K list at most k_max (> 0)
+void pick_chunks(const rsx_engine* h, bool uni, int variant, int k_req, int k_max, int& K_out, int& C_out)
+{
+    const int path = pair_path(uni, variant);
     const int nk = rsx::debye_k_count(path);
     int bestK = 0, bestC = 0;
     double best = 1e300;
     for (int i = 0; i < nk; ++i) {
         const int K = rsx::debye_k_value(path, i);
-        if (h->k_req > 0 && K != h->k_req) continue;
-        if (h->k_max > 0 && K > h->k_max) continue;
-        if (h->k_req <= 0 && !uni && K > 48) continue;  // the direct kernel spills at K = 64
+        if (k_req > 0 && K != k_req) continue;
+        if (k_max > 0 && K > k_max) continue;
+        if (k_req <= 0 && !uni && K > 48) continue;  // the direct kernel spills at K = 64
         const int C = (h->nq + K - 1) / K;
         // cost per (pair, chunk) in FP64-op units: the FP64 recurrence 2K + ~44 for distance
         // and seeds; the FP32 one 3K FP32 ops (~0.6 of an FP64 op each) + ~25
@@ -488,22 +526,25 @@ void choose_chunks(rsx_engine* h)
         }
     }
     if (bestK == 0) fail(RSX_INVALID_ARG, "unsupported K (q values per thread)");
-    h->K = bestK;
-    h->n_chunks = bestC;
-    h->ldf = bestK * bestC;
+    K_out = bestK;
+    C_out = bestC;
 }
 
-// Upload the per-chunk constants of the active path (uniform recurrence or direct).
-void apply_path(rsx_engine* h, bool uniform)
+void choose_chunks(rsx_engine* h)
 {
-    h->layout_stamp++;
-    const int nq = h->nq;
-    h->uniform = uniform;
-    choose_chunks(h);
-    const int ldf = h->ldf;
-    std::vector<ChunkConst> cc(h->n_chunks);
-    for (int c = 0; c < h->n_chunks; ++c) {
-        const int k0 = c * h->K;
+    pick_chunks(h, h->uniform, h->variant, h->k_req, h->k_max, h->K, h->n_chunks);
+    h->ldf = h->K * h->n_chunks;
+}
+
+// Per-chunk constants and the q tables ([ldf], zero padded) of a chunking of the
+// engine's q grid into n_chunks runs of K.
+void upload_q_consts(const rsx_engine* h, int K, int n_chunks, DBuf<ChunkConst>& chunks, DBuf<double>& d_invq,
+                     DBuf<double>& d_qrev, DBuf<float>& d_qv)
+{
+    const int nq = h->nq, ldf = K * n_chunks;
+    std::vector<ChunkConst> cc(n_chunks);
+    for (int c = 0; c < n_chunks; ++c) {
+        const int k0 = c * K;
         const double a = k0 < nq ? (double)h->q[k0] : 0.0;
         cc[c].a = (float)a;
         cc[c].a_rev = a / (2.0 * PI);
@@ -517,14 +558,23 @@ void apply_path(rsx_engine* h, bool uniform)
         qrev[k] = (double)h->q[k] / (2.0 * PI);
         qv[k] = h->q[k];
     }
-    h->d_chunks.ensure(cc.size());
-    h->d_invq.ensure(ldf);
-    h->d_qrev.ensure(ldf);
-    h->d_qv.ensure(ldf);
-    hipchk(hipMemcpy(h->d_chunks.p, cc.data(), cc.size() * sizeof(ChunkConst), hipMemcpyHostToDevice), "upload chunks");
-    hipchk(hipMemcpy(h->d_invq.p, invq.data(), ldf * sizeof(double), hipMemcpyHostToDevice), "upload invq");
-    hipchk(hipMemcpy(h->d_qrev.p, qrev.data(), ldf * sizeof(double), hipMemcpyHostToDevice), "upload qrev");
-    hipchk(hipMemcpy(h->d_qv.p, qv.data(), ldf * sizeof(float), hipMemcpyHostToDevice), "upload qv");
+    chunks.ensure(cc.size());
+    d_invq.ensure(ldf);
+    d_qrev.ensure(ldf);
+    d_qv.ensure(ldf);
+    hipchk(hipMemcpy(chunks.p, cc.data(), cc.size() * sizeof(ChunkConst), hipMemcpyHostToDevice), "upload chunks");
+    hipchk(hipMemcpy(d_invq.p, invq.data(), ldf * sizeof(double), hipMemcpyHostToDevice), "upload invq");
+    hipchk(hipMemcpy(d_qrev.p, qrev.data(), ldf * sizeof(double), hipMemcpyHostToDevice), "upload qrev");
+    hipchk(hipMemcpy(d_qv.p, qv.data(), ldf * sizeof(float), hipMemcpyHostToDevice), "upload qv");
+}
+
+// Upload the per-chunk constants of the active path (uniform recurrence or direct).
+void apply_path(rsx_engine* h, bool uniform)
+{
+    h->layout_stamp++;
+    h->uniform = uniform;
+    choose_chunks(h);
+    upload_q_consts(h, h->K, h->n_chunks, h->d_chunks, h->d_invq, h->d_qrev, h->d_qv);
     h->params_dirty = true;  // the class table is laid out with stride ldf
 }
 
@@ -604,6 +654,30 @@ bool hydration_on(const rsx_engine* h)
     return h->w != 0.0f && h->water >= 0 && h->sasa_set;
 }
 
+// Factor rows of the classes t * d1 + c (t < n_types, c < d1): FP64 [class][n_q] and
+// FP32 [class][ldf] (ft must hold n_types * d1 rows of ldf, zero padded by the caller).
+// With hydration (d1 = n_dots + 1) class count c has F = f_t + (c / n_dots) w f_water,
+// the exposed fraction formed as the reference forms it, float(count) / n_dots; a table
+// without a water row contributes nothing there.
+void class_factor_rows(const rsx_engine* h, int d1, int ldf, std::vector<double>& f64, float* ft)
+{
+    const int nq = h->nq;
+    const bool hyd = d1 > 1;
+    f64.assign((size_t)h->n_types * d1 * nq, 0.0);
+    for (int t = 0; t < h->n_types; ++t)
+        for (int c = 0; c < d1; ++c) {
+            const int id = t * d1 + c;
+            const float s = hyd ? (float)c / (float)h->n_dots : 0.0f;
+            for (int k = 0; k < nq; ++k) {
+                double f = h->table[(size_t)k * h->n_types + t];
+                if (hyd && h->water >= 0)
+                    f += (double)s * ((double)h->w * (double)h->table[(size_t)k * h->n_types + h->water]);
+                f64[(size_t)id * nq + k] = f;
+                ft[(size_t)id * ldf + k] = (float)f;
+            }
+        }
+}
+
 void build_class_table(rsx_engine* h)
 {
     h->layout_stamp++;
@@ -613,22 +687,10 @@ void build_class_table(rsx_engine* h)
     h->d1 = h->hydrate ? h->n_dots + 1 : 1;
     h->n_class = h->n_types * h->d1;
     const int nq = h->nq, ldf = h->ldf;
-    h->f64.assign((size_t)h->n_class * nq, 0.0);
     // + an all-zero row; with hydration + 2 D + 1 delta rows (delta_class)
     const int n_delta = h->hydrate ? 2 * h->n_dots + 1 : 0;
     std::vector<float> ft((size_t)(h->n_class + 1 + n_delta) * ldf, 0.0f);
-    for (int t = 0; t < h->n_types; ++t)
-        for (int c = 0; c < h->d1; ++c) {
-            const int id = t * h->d1 + c;
-            // exposed fraction as the reference forms it: float(count) / n_dots
-            const float s = h->hydrate ? (float)c / (float)h->n_dots : 0.0f;
-            for (int k = 0; k < nq; ++k) {
-                double f = h->table[(size_t)k * h->n_types + t];
-                if (h->hydrate) f += (double)s * ((double)h->w * (double)h->table[(size_t)k * h->n_types + h->water]);
-                h->f64[(size_t)id * nq + k] = f;
-                ft[(size_t)id * ldf + k] = (float)f;
-            }
-        }
+    class_factor_rows(h, h->d1, ldf, h->f64, ft.data());
     // delta rows: the factor change of an atom whose exposed-dot count changed by d,
     // (d / D) w f_water(q), for d in [-D, D]
     for (int d = -h->n_dots; h->hydrate && d <= h->n_dots; ++d) {
@@ -862,15 +924,16 @@ bool compute_sasa_local(rsx_engine* h, int start, int len, const std::vector<flo
 }
 
 // ---- pair jobs ---------------------------------------------------------------
-int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles, int variant, int K, int n_chunks,
-               double rounds = 6.0)
+// Column range L of the pair jobs of a launch on a layout of the given path (uniform
+// grid?, variant, K, chunks) and bounding radius.
+int pick_range_for(const rsx_engine* h, double column_atoms_total, bool uniform, int variant, int K, int n_chunks,
+                   double radius, double rounds)
 {
-    if (h->range_req > 0) return h->range_req;
     // resident waves of the pair kernel (the runtime's occupancy for the FP32 kernels,
     // two per SIMD otherwise)
     int per_cu = 0;
-    if (h->uniform && (variant == rsx::VARIANT_FP32 || variant == rsx::VARIANT_FP32R))
-        per_cu = rsx::rec32_blocks_per_cu(variant, K, rsx::sin_series_terms(h->dq * h->path_radius), false);
+    if (uniform && (variant == rsx::VARIANT_FP32 || variant == rsx::VARIANT_FP32R))
+        per_cu = rsx::rec32_blocks_per_cu(variant, K, rsx::sin_series_terms(h->dq * radius), false);
     if (per_cu <= 0) per_cu = 8;
     const double slots = (double)std::max(1, h->n_cu) * per_cu;
     // about `rounds` rounds of waves over the resident slots (default six): fewer leave a
@@ -879,14 +942,20 @@ int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles, in
     // set-up and epilogue more often.  64..1024 columns, rounded down.
     const double jobs = std::max(1.0, rounds * slots / std::max(1, n_chunks));
     double L = column_atoms_total / 64.0 / jobs;
-    (void)row_tiles;
     L = std::max(64.0, std::min(1024.0, L));
     return std::max(64, (int)L / 64 * 64);
 }
 
-int pick_range(const rsx_engine* h, double column_atoms_total, int row_tiles)
+// the same on the single-structure layout (the caller's column range when set)
+int pick_range(const rsx_engine* h, double column_atoms_total, int variant, int K, int n_chunks, double rounds = 6.0)
 {
-    return pick_range(h, column_atoms_total, row_tiles, h->variant, h->K, h->n_chunks);
+    if (h->range_req > 0) return h->range_req;
+    return pick_range_for(h, column_atoms_total, h->uniform, variant, K, n_chunks, h->path_radius, rounds);
+}
+
+int pick_range(const rsx_engine* h, double column_atoms_total)
+{
+    return pick_range(h, column_atoms_total, h->variant, h->K, h->n_chunks);
 }
 
 // Kernel of the incremental launches.  Their rows are the changed atoms sorted by class;
@@ -1156,7 +1225,7 @@ std::vector<PairJob> forward_jobs(rsx_engine* h)
 {
     const int n = h->n;
     std::vector<PairJob> jobs;
-    const int L = pick_range(h, (double)n * n / 2.0, (n + 63) / 64);
+    const int L = pick_range(h, (double)n * n / 2.0);
     h->last_L = L;
     triangle_jobs(jobs, h->d_cols.p, n, L, 2.0f);
     order_jobs(h, jobs);
@@ -1390,7 +1459,7 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
     // about two rounds of waves, the tail split (measured against six rounds unsplit:
     // 0.834 -> 0.824 ms plain, 2.23 -> 2.17 ms hydrated per 500-atom move at 50k atoms)
     const int L =
-        pick_range(h, (double)(q1 + q2) * (double)h->n, (q1 + q2) / 64, kc.variant, kc.K, kc.n_chunks, 2.0);
+        pick_range(h, (double)(q1 + q2) * (double)h->n, kc.variant, kc.K, kc.n_chunks, 2.0);
     rsx_engine::IncKey key;
     key.mp = mp;
     key.p1 = p1;
@@ -1742,64 +1811,120 @@ void batch_release_graph(rsx_engine* h)
     h->b_graph = nullptr;
 }
 
-void batch_set_frames_impl(rsx_engine* h, const float* bodies, const int* offsets, int nf)
+// the batch is laid out again (from the frames kept on the host) on its next launch
+void batch_invalidate(rsx_engine* h)
 {
-    if (h->nq <= 0 || h->n_types <= 0) fail(RSX_ARGS_NOT_SET, "batch: q and factors must be set first");
-    if (nf <= 0 || !offsets || !bodies) fail(RSX_INVALID_ARG, "batch: invalid frames");
-    set_device(h);
     batch_release_graph(h);
+    h->b_layout_ok = false;
+}
+
+// Exposed-dot counts of every frame's bodies (one SASA count per frame, each frame on
+// its own: atoms of other frames never occlude), into b_counts (body order).
+void batch_counts(rsx_engine* h)
+{
+    const int total = h->b_offsets[h->b_frames];
+    h->b_counts.assign(total, 0u);
+    for (int i = 0; i < total; ++i) {
+        uint32_t t;
+        std::memcpy(&t, &h->b_bodies[4 * (size_t)i + 3], 4);
+        if (t >= h->radii.size()) fail(RSX_INVALID_ARG, "batch: body type index outside the SASA radius table");
+    }
+    int nmax = 0;
+    for (int f = 0; f < h->b_frames; ++f) nmax = std::max(nmax, h->b_offsets[f + 1] - h->b_offsets[f]);
+    h->d_sasa_atoms.ensure(std::max(nmax, 1));
+    h->sasa_mirror = false;  // it holds the frames now, not the single structure
+    for (int f = 0; f < h->b_frames; ++f) {
+        const int b = h->b_offsets[f], m = h->b_offsets[f + 1] - b;
+        if (m <= 0) continue;
+        hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, &h->b_bodies[4 * (size_t)b], (size_t)m * 16, hipMemcpyHostToDevice,
+                              h->stream),
+               "upload frame atoms");
+        run_sasa_device(h, m);
+        hipchk(hipMemcpyAsync(&h->b_counts[b], h->d_counts.p, (size_t)m * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              h->stream),
+               "download frame counts");
+        hipchk(hipStreamSynchronize(h->stream), "sync frame sasa");
+    }
+}
+
+// Lay the frames out: the path and q chunks for their largest bounding radius, classes
+// (type, or type x exposed-dot count with hydration), columns class-sorted within each
+// frame, and one job table over every frame's triangle (longest first).  The class
+// factors come later (batch_factors): they change with the factor table, the layout
+// does not.
+void batch_layout(rsx_engine* h)
+{
+    batch_release_graph(h);
+    h->b_layout_ok = false;
+    const int nf = h->b_frames;
+    const std::vector<int>& off = h->b_offsets;
+    const float* bodies = h->b_bodies.data();
+    for (int i = 0; i < off[nf]; ++i) {
+        uint32_t t;
+        std::memcpy(&t, &bodies[4 * (size_t)i + 3], 4);
+        if ((int)t >= h->n_types) fail(RSX_INVALID_ARG, "batch: type index outside the factor table");
+    }
+    // path: recurrence where every frame fits it
     double rmax = 0.0;
     for (int f = 0; f < nf; ++f)
-        if (offsets[f + 1] > offsets[f])
-            rmax = std::max(rmax, bounding_radius(bodies + 4 * (size_t)offsets[f], offsets[f + 1] - offsets[f], nullptr));
-    select_path(h, rmax);
-    if (h->params_dirty || h->d_ftab.p == nullptr || h->hydrate) {
-        // frames use the plain (type) classes
-        const float keep_w = h->w;
-        h->w = 0.0f;
-        build_class_table(h);
-        h->w = keep_w;
-        h->params_dirty = true;  // the single-structure path rebuilds its table on next use
-    }
-    const int total = offsets[nf];
-    std::vector<float4> cols(std::max(total, 1));
-    std::vector<double> q0(nf, 0.0);
+        if (off[f + 1] > off[f]) rmax = std::max(rmax, bounding_radius(bodies + 4 * (size_t)off[f], off[f + 1] - off[f], nullptr));
+    h->b_radius = rmax;
+    h->b_uniform = fits_recurrence(h, rmax);
+    h->b_variant = h->b_var_req >= 0 ? h->b_var_req : h->variant;
+    pick_chunks(h, h->b_uniform, h->b_variant, h->b_k_req > 0 ? h->b_k_req : h->k_req, 0, h->b_K, h->b_n_chunks);
+    h->b_ldf = h->b_K * h->b_n_chunks;
+    upload_q_consts(h, h->b_K, h->b_n_chunks, h->b_chunks, h->b_invq, h->b_qrev, h->b_qv);
+    // classes
+    h->b_hydrate = hydration_on(h);
+    if (h->b_hydrate) batch_counts(h);
+    h->b_d1 = h->b_hydrate ? h->n_dots + 1 : 1;
+    h->b_n_types = h->n_types;
+    h->b_n_class = h->n_types * h->b_d1;
+    std::vector<float4> cols(std::max(off[nf], 1));
+    h->b_pop_off.assign(nf + 1, 0);
+    h->b_pop.clear();
+    std::vector<int> pop(h->b_n_class, 0);
     for (int f = 0; f < nf; ++f) {
-        const int b = offsets[f], e = offsets[f + 1];
-        if (e < b) fail(RSX_INVALID_ARG, "batch: offsets must not decrease");
-        std::vector<int> idx(e - b);
-        std::iota(idx.begin(), idx.end(), b);
-        std::vector<uint32_t> t(e - b);
+        const int b = off[f], e = off[f + 1];
+        std::vector<uint32_t> cl(e - b);
         for (int i = b; i < e; ++i) {
-            std::memcpy(&t[i - b], &bodies[4 * (size_t)i + 3], 4);
-            if ((int)t[i - b] >= h->n_types) fail(RSX_INVALID_ARG, "batch: type index outside the factor table");
+            uint32_t t;
+            std::memcpy(&t, &bodies[4 * (size_t)i + 3], 4);
+            cl[i - b] = t * (uint32_t)h->b_d1 + (h->b_hydrate ? h->b_counts[i] : 0u);
         }
-        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return t[x - b] < t[y - b]; });
-        double s = 0.0;
+        std::vector<int> idx(e - b);
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return cl[x] < cl[y]; });
         for (int a = 0; a < e - b; ++a) {
-            const int i = idx[a];
-            cols[b + a] = make_float4(bodies[4 * (size_t)i], bodies[4 * (size_t)i + 1], bodies[4 * (size_t)i + 2],
-                                      bitsf(t[i - b]));
-            if (h->q0_index >= 0) s += h->f64[(size_t)t[i - b] * h->nq + h->q0_index];
+            const size_t i = (size_t)(b + idx[a]);
+            cols[b + a] = make_float4(bodies[4 * i], bodies[4 * i + 1], bodies[4 * i + 2], bitsf(cl[idx[a]]));
+            pop[cl[idx[a]]]++;
         }
-        q0[f] = s * s;
+        for (int c = 0; c < h->b_n_class; ++c)
+            if (pop[c]) {
+                h->b_pop.emplace_back(c, pop[c]);
+                pop[c] = 0;
+            }
+        h->b_pop_off[f + 1] = (int)h->b_pop.size();
     }
     h->b_cols.ensure(cols.size());
     hipchk(hipMemcpy(h->b_cols.p, cols.data(), cols.size() * sizeof(float4), hipMemcpyHostToDevice), "upload frames");
     // jobs grouped by frame: slots of frame f are [sb[f], sb[f+1])
     double work = 0.0;
-    int row_tiles = 0;
     for (int f = 0; f < nf; ++f) {
-        const double m = offsets[f + 1] - offsets[f];
+        const double m = off[f + 1] - off[f];
         work += m * m / 2.0;
-        row_tiles += (offsets[f + 1] - offsets[f] + 63) / 64;
     }
-    const int L = pick_range(h, work, row_tiles);
+    const int L = h->b_range_req > 0   ? h->b_range_req
+                  : h->range_req > 0 ? h->range_req
+                                     : pick_range_for(h, work, h->b_uniform, h->b_variant, h->b_K, h->b_n_chunks,
+                                                      h->b_radius, 6.0);
+    h->b_L = L;
     std::vector<PairJob> jobs;
     std::vector<int> sb(nf + 1, 0);
     for (int f = 0; f < nf; ++f) {
         std::vector<PairJob> fj;
-        triangle_jobs(fj, h->b_cols.p + offsets[f], offsets[f + 1] - offsets[f], L, 2.0f);
+        triangle_jobs(fj, h->b_cols.p + off[f], off[f + 1] - off[f], L, 2.0f);
         for (auto& j : fj) {
             j.slot = (int)jobs.size();
             jobs.push_back(j);
@@ -1814,52 +1939,131 @@ void batch_set_frames_impl(rsx_engine* h, const float* bodies, const int* offset
     h->b_max_slots = 0;
     for (int f = 0; f < nf; ++f) h->b_max_slots = std::max(h->b_max_slots, sb[f + 1] - sb[f]);
     h->b_jobs.ensure(jobs.size());
-    h->b_partials.ensure((size_t)std::max(1, h->b_slots) * h->ldf);
+    h->b_partials.ensure((size_t)std::max(1, h->b_slots) * h->b_ldf);
     h->b_red.ensure(std::max<size_t>(1, rsx::reduce_scratch_doubles(h->nq, nf, h->b_max_slots)));
     h->b_slot_begin.ensure(nf + 1);
     h->b_out.ensure((size_t)nf * h->nq);
     h->b_mean.ensure(h->nq);
     h->b_q0.ensure(nf);
+    h->b_ftab.ensure((size_t)(h->b_n_class + 1) * h->b_ldf);
     hipchk(hipMemcpy(h->b_jobs.p, jobs.data(), jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice), "upload jobs");
     hipchk(hipMemcpy(h->b_slot_begin.p, sb.data(), sb.size() * sizeof(int), hipMemcpyHostToDevice), "upload sb");
-    hipchk(hipMemcpy(h->b_q0.p, q0.data(), nf * sizeof(double), hipMemcpyHostToDevice), "upload q0");
-    h->b_frames = nf;
+    h->b_nq = h->nq;
+    h->b_sasa_stamp = h->sasa_stamp;
+    h->b_factor_stamp = 0;  // class factors still to come
+    h->b_layout_ok = true;
+}
+
+// Class factors of the batch layout from the engine's current factor table and water
+// weight, and each frame's exact I(0) = (sum of F(0))^2 from its class populations.
+// Buffers keep their addresses, so a captured batch graph stays valid.
+void batch_factors(rsx_engine* h)
+{
+    const int nf = h->b_frames, ldf = h->b_ldf, nc = h->b_n_class;
+    hipchk(hipStreamSynchronize(h->stream), "sync before factor upload");  // pinned buffers free
+    h->b_pin_ftab.ensure((size_t)(nc + 1) * ldf);
+    h->b_pin_q0.ensure(nf);
+    std::memset(h->b_pin_ftab.p, 0, (size_t)(nc + 1) * ldf * sizeof(float));
+    class_factor_rows(h, h->b_d1, ldf, h->b_f64, h->b_pin_ftab.p);
+    for (int f = 0; f < nf; ++f) {
+        double s = 0.0;
+        if (h->q0_index >= 0)
+            for (int p = h->b_pop_off[f]; p < h->b_pop_off[f + 1]; ++p)
+                s += (double)h->b_pop[p].second * h->b_f64[(size_t)h->b_pop[p].first * h->nq + h->q0_index];
+        h->b_pin_q0.p[f] = s * s;
+    }
+    hipchk(hipMemcpyAsync(h->b_ftab.p, h->b_pin_ftab.p, (size_t)(nc + 1) * ldf * sizeof(float), hipMemcpyHostToDevice,
+                          h->stream),
+           "upload batch factors");
+    hipchk(hipMemcpyAsync(h->b_q0.p, h->b_pin_q0.p, nf * sizeof(double), hipMemcpyHostToDevice, h->stream),
+           "upload batch q0");
+    h->b_factor_stamp = h->factor_stamp;
+}
+
+void batch_set_frames_impl(rsx_engine* h, const float* bodies, const int* offsets, int nf)
+{
+    if (h->nq <= 0 || h->n_types <= 0) fail(RSX_ARGS_NOT_SET, "batch: q and factors must be set first");
+    if (nf <= 0 || !offsets || !bodies) fail(RSX_INVALID_ARG, "batch: invalid frames");
+    if (offsets[0] != 0) fail(RSX_INVALID_ARG, "batch: offsets[0] must be 0");
+    for (int f = 0; f < nf; ++f)
+        if (offsets[f + 1] < offsets[f]) fail(RSX_INVALID_ARG, "batch: offsets must not decrease");
+    set_device(h);
+    batch_invalidate(h);
+    h->b_bodies.assign(bodies, bodies + 4 * (size_t)offsets[nf]);
     h->b_offsets.assign(offsets, offsets + nf + 1);
+    h->b_frames = nf;
+    try {
+        batch_layout(h);
+    } catch (...) {
+        h->b_frames = 0;
+        throw;
+    }
 }
 
 // waves a persistent launch keeps resident: the CUs times the kernel's resident
 // one-wave blocks per CU (from the runtime's occupancy calculator); 0 (one wave per
 // item instead) where the work-queue loop would cost the kernel occupancy
-int persistent_waves(const rsx_engine* h)
+int persistent_waves(const rsx_engine* h, int variant, int K, double radius)
 {
-    const int st = rsx::sin_series_terms(h->dq * h->path_radius);
-    const int pb = rsx::rec32_blocks_per_cu(h->variant, h->K, st, true);
-    const int ib = rsx::rec32_blocks_per_cu(h->variant, h->K, st, false);
+    const int st = rsx::sin_series_terms(h->dq * radius);
+    const int pb = rsx::rec32_blocks_per_cu(variant, K, st, true);
+    const int ib = rsx::rec32_blocks_per_cu(variant, K, st, false);
     return pb >= ib ? h->n_cu * pb : 0;
+}
+
+rsx::DebyeArgs batch_debye_args(rsx_engine* h)
+{
+    rsx::DebyeArgs a;
+    a.jobs = h->b_jobs.p;
+    a.n_chunks = h->b_n_chunks;
+    a.ftab = h->b_ftab.p;
+    a.ldf = h->b_ldf;
+    a.zero_class = h->b_n_class;
+    a.chunks = h->b_chunks.p;
+    a.half_dq = 0.5 * h->dq;
+    a.dq = (float)h->dq;
+    a.invq = h->b_invq.p;
+    a.qrev = h->b_qrev.p;
+    a.qv = h->b_qv.p;
+    a.partials = h->b_partials.p;
+    a.variant = h->b_variant;
+    a.queue = nullptr;
+    a.persistent_waves = 0;
+    a.sin_terms = rsx::sin_series_terms(h->dq * h->b_radius);
+    return a;
 }
 
 void batch_enqueue(rsx_engine* h, double* out_frames, double* out_mean, int pers_waves)
 {
-    rsx::DebyeArgs a = debye_args(h, h->b_jobs.p, h->b_partials.p);
+    rsx::DebyeArgs a = batch_debye_args(h);
     if (pers_waves > 0) {
         a.queue = h->d_queue.p;
         a.persistent_waves = pers_waves;
     }
-    hipchk(rsx::launch_debye(a, h->b_njobs, h->K, h->uniform, h->stream), "batch debye");
-    hipchk(rsx::launch_reduce(h->b_partials.p, h->ldf, h->nq, h->b_slot_begin.p, h->b_frames, out_frames, h->nq, 0,
+    hipchk(rsx::launch_debye(a, h->b_njobs, h->b_K, h->b_uniform, h->stream), "batch debye");
+    hipchk(rsx::launch_reduce(h->b_partials.p, h->b_ldf, h->nq, h->b_slot_begin.p, h->b_frames, out_frames, h->nq, 0,
                               h->b_q0.p, h->q0_index, h->b_max_slots, h->b_red.p, h->stream),
            "batch reduce");
     hipchk(rsx::launch_frame_mean(out_frames, h->b_frames, h->nq, out_mean, h->stream), "frame mean");
 }
 
+// Launch the batch (graph-captured) into out_frames [n_frames][n_q] and out_mean [n_q] on
+// the device.  The layout is redone when q, the tuning, the factor table's type count,
+// the SASA set-up or a switch to hydration made it stale; the class factors are
+// refreshed when the factor table or the water weight changed (the graph stays).
 void batch_run_dev(rsx_engine* h, double* out_frames, double* out_mean)
 {
     if (h->b_frames <= 0) fail(RSX_ARGS_NOT_SET, "batch: no frames set");
     set_device(h);
+    const bool stale = !h->b_layout_ok || h->b_nq != h->nq || h->b_n_types != h->n_types ||
+                       (hydration_on(h) && !h->b_hydrate) || (h->b_hydrate && h->b_sasa_stamp != h->sasa_stamp);
+    if (stale) batch_layout(h);
+    if (h->b_factor_stamp != h->factor_stamp) batch_factors(h);
     if (!h->b_exec || h->b_graph_out != out_frames || h->b_graph_mean != out_mean) {
         batch_release_graph(h);
-        const bool pers = h->persistent && (h->variant == rsx::VARIANT_FP32 || h->variant == rsx::VARIANT_FP32R);
-        const int pw = pers ? persistent_waves(h) : 0;
+        const bool fp32 = h->b_variant == rsx::VARIANT_FP32 || h->b_variant == rsx::VARIANT_FP32R;
+        const bool pers = h->b_uniform && fp32 && (h->b_pers_req >= 0 ? h->b_pers_req != 0 : h->persistent);
+        const int pw = pers ? persistent_waves(h, h->b_variant, h->b_K, h->b_radius) : 0;
         h->b_pers_waves = pw;
         hipchk(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture");
         try {
@@ -2224,8 +2428,6 @@ void use_gradient_forward(rsx_engine* h)
     h->k_max = GRAD_K_MAX;
     setup_q(h);
     h->params_dirty = true;
-    batch_release_graph(h);
-    h->b_frames = 0;
 }
 
 }  // namespace
@@ -2282,17 +2484,21 @@ int rsx_destroy(rsx_engine* h)
                           &h->d_sorted_atoms};
     for (auto* b : f4) b->release();
     h->d_qrev.release();
-    DBuf<float>* ff[] = {&h->d_ftab, &h->d_qv, &h->d_r2, &h->d_rext, &h->d_grad,
-                         &h->a_ftab, &h->a_rowpart, &h->a_colpart, &h->a_invq32, &h->a_cb, &h->a_cg};
+    DBuf<float>* ff[] = {&h->d_ftab, &h->d_qv, &h->d_r2, &h->d_rext, &h->d_grad, &h->a_ftab, &h->a_rowpart,
+                         &h->a_colpart, &h->a_invq32, &h->a_cb, &h->a_cg, &h->b_qv, &h->b_ftab};
     for (auto* b : ff) b->release();
     DBuf<double>* fd[] = {&h->d_invq, &h->d_partials, &h->d_iq, &h->d_saved_iq, &h->d_out, &h->d_wq, &h->d_q0,
-                          &h->d_red, &h->b_partials, &h->b_out, &h->b_mean, &h->b_q0, &h->b_red};
+                          &h->d_red, &h->b_partials, &h->b_out, &h->b_mean, &h->b_q0, &h->b_red, &h->b_invq,
+                          &h->b_qrev};
     for (auto* b : fd) b->release();
     DBuf<int>* fi[] = {&h->d_slot_begin, &h->d_idx, &h->d_bucket_of, &h->d_bucket_count, &h->d_bucket_start,
                        &h->d_bucket_fill, &h->d_sorted, &h->d_s2c, &h->b_slot_begin, &h->d_subset,
                        &h->d_tile_sum};
     for (auto* b : fi) b->release();
     h->d_chunks.release();
+    h->b_chunks.release();
+    h->b_pin_ftab.release();
+    h->b_pin_q0.release();
     h->d_chunks_inc.release();
     h->inc_chunks_K = 0;
     h->d_subcnt.release();
@@ -2360,8 +2566,7 @@ int rsx_set_q(rsx_engine* h, const float* q, int n_q)
         setup_q(h);
         h->params_dirty = true;
         if (h->state > 3) h->state = 3;
-        batch_release_graph(h);
-        h->b_frames = 0;
+        batch_invalidate(h);
     });
 }
 
@@ -2373,7 +2578,37 @@ int rsx_set_factors(rsx_engine* h, const float* table, int n_types, int water_in
         h->table.assign(table, table + (size_t)h->nq * n_types);
         h->n_types = n_types;
         h->water = water_index;
+        h->atomic_table = false;
         h->params_dirty = true;
+        h->factor_stamp++;
+    });
+}
+
+int rsx_set_atomic_factors(rsx_engine* h, float avg_radius)
+{
+    return guarded(h, [&] {
+        if (h->nq <= 0) fail(RSX_ARGS_NOT_SET, "set q before the factor table");
+        if (!(avg_radius > 0.0f) || !std::isfinite(avg_radius)) fail(RSX_INVALID_ARG, "mean radius must be > 0");
+        h->table.assign((size_t)h->nq * ATOMIC_TYPES, 0.0f);
+        atomic_ff(h->q.data(), h->nq, avg_radius, h->expansion, h->table.data());
+        h->n_types = ATOMIC_TYPES;
+        h->water = ATOMIC_OH2;
+        h->atomic_table = true;
+        h->avg_radius = avg_radius;
+        h->params_dirty = true;
+        h->factor_stamp++;
+    });
+}
+
+int rsx_set_expansion_factor(rsx_engine* h, float c)
+{
+    return guarded(h, [&] {
+        if (!(c > 0.0f) || !std::isfinite(c)) fail(RSX_INVALID_ARG, "expansion factor must be > 0");
+        if (c == h->expansion) return;
+        h->expansion = c;
+        if (h->atomic_table) atomic_ff(h->q.data(), h->nq, h->avg_radius, c, h->table.data());
+        h->params_dirty = true;
+        h->factor_stamp++;
     });
 }
 
@@ -2385,6 +2620,7 @@ int rsx_set_water_weight(rsx_engine* h, float w)
             h->w = w;
             h->params_dirty = true;
             h->w_dirty = true;
+            h->factor_stamp++;
         }
     });
 }
@@ -2425,6 +2661,7 @@ int rsx_set_sasa_params(rsx_engine* h, const float* radii, int n_types, float pr
         h->sasa_set = true;
         h->have_rsasa = false;
         h->params_dirty = true;
+        h->sasa_stamp++;
     });
 }
 
@@ -2449,8 +2686,7 @@ int rsx_set_tuning(rsx_engine* h, int k, int col_range)
             setup_q(h);
             h->params_dirty = true;
             if (h->state > 3) h->state = 3;
-            batch_release_graph(h);
-            h->b_frames = 0;
+            batch_invalidate(h);
         }
     });
 }
@@ -2689,8 +2925,7 @@ int rsx_set_variant(rsx_engine* h, int variant)
             setup_q(h);
             h->params_dirty = true;
             if (h->state > 3) h->state = 3;
-            batch_release_graph(h);
-            h->b_frames = 0;
+            batch_invalidate(h);
         }
     });
 }

@@ -82,6 +82,13 @@ class Engine:
             raise ValueError("factor table must be [n_q, n_types]")
         self._check(self._L.rsx_set_factors(self.h, _ptr(table, ctypes.c_float), table.shape[1], int(water_index)))
 
+    def set_atomic_factors(self, avg_radius):
+        """The default atomic factor table [n_q][16] (water row OH2 = 14) at this mean radius."""
+        self._check(self._L.rsx_set_atomic_factors(self.h, float(avg_radius)))
+
+    def set_expansion_factor(self, c):
+        self._check(self._L.rsx_set_expansion_factor(self.h, float(c)))
+
     def set_water_weight(self, w):
         self._check(self._L.rsx_set_water_weight(self.h, float(w)))
 

@@ -101,7 +101,7 @@ int main(int argc, char** argv)
         fitted_params<float> best = calc.fit_ensemble(eval);
         CHECK(best.intensity_.size() == nq, "best intensity size");
         CHECK(eval.calc_count > 0 && eval.calc_count % (unsigned)n_models == 0, "whole ensembles evaluated");
-        CHECK(eval.n_engines() == (size_t)n_models, "one engine per model");
+        CHECK(eval.n_engines() == 1, "one batched engine holds the ensemble");
         std::printf("fit: c %.6f w %.5f scale %.6g chi %.6g after %u model evaluations\n", best.exp_factor_,
                     best.water_weight_, best.scale_, best.chi_, eval.calc_count);
         std::vector<double> res = {best.exp_factor_, best.water_weight_, best.scale_, best.chi_,
@@ -109,11 +109,25 @@ int main(int argc, char** argv)
         write_bin(dir + "/fit.bin", res);
         write_bin(dir + "/fit_iq.bin", best.intensity_);
 
-        // re-evaluating the best point reproduces it (the engines keep their state)
+        // re-evaluating the best point reproduces it (the engine keeps the laid-out ensemble)
         const unsigned before = eval.calc_count;
         fitted_params<float> again = calc.fit_ensemble(eval, best.exp_factor_, best.water_weight_);
         CHECK(eval.calc_count == before + (unsigned)n_models, "one evaluation per model");
         CHECK(again.chi_ == best.chi_ && again.scale_ == best.scale_, "best point reproduces");
+        // the tool's per-model call gives the models' own I(q); their mean is the ensemble mean
+        std::vector<double> acc(nq, 0.0);
+        std::vector<float> iq;
+        for (const auto& m : calc.v_models_) {
+            eval.calc(m, iq);
+            for (size_t k = 0; k < nq; ++k) acc[k] += iq[k];
+        }
+        double worst = 0.0;
+        for (size_t k = 0; k < nq; ++k) {
+            const double mean = acc[k] / n_models, ens = again.intensity_[k] / again.scale_;
+            worst = std::max(worst, std::fabs(mean - ens) / std::fabs(ens));
+        }
+        CHECK(worst < 1e-6, "per-model I(q) average to the ensemble mean");
+        CHECK(eval.n_engines() == 1, "per-model calls reuse the laid-out ensemble");
 
         if (argc >= 4) {
             const float c = (float)std::atof(argv[2]), w = (float)std::atof(argv[3]);

@@ -95,6 +95,120 @@ def test_batch_full_size(oracle):
     assert err < 5e-6
 
 
+def _frames(seed, n_frames, nmin, nmax):
+    from resaxs_amd import configs
+    w = configs.ensemble_512(n_frames=n_frames, nmin=nmin, nmax=nmax, seed=seed)
+    return w
+
+
+def test_batch_independent_of_single_structure(oracle):
+    """The batch keeps a layout of its own: single-structure calls on the same engine (a
+    hydrated calc_curve, which rebuilds the engine's class table, and an incremental
+    update) between batch launches leave the batch's results unchanged bit for bit, whether
+    they come before or after its first launch."""
+    from resaxs_amd import configs
+    w = _frames(5, 8, 200, 1500)
+    s = configs.sasa_ellipsoid_20000(n=3000)
+    _say("case: batch interleaved with a hydrated single structure")
+    e = engine_for(w.q, w.table, w.water_index, 0.0)
+    e.batch_set_frames(w.bodies, w.offsets)
+    ref_frames, ref_mean = e.batch_calc()
+    # hydrated single structure on the same engine (same q and table, SASA on)
+    e.set_sasa_params(s.radii, s.probe, n_half=s.n_half)
+    e.set_water_weight(s.w)
+    e.initialize(len(s.pos))
+    I1 = e.calc_curve(s.bodies)
+    frames, mean = e.batch_calc()   # hydration is now on: the frames get exposed-dot classes
+    hyd = [oracle.sasa_counts(w.bodies[w.offsets[f]:w.offsets[f + 1], :3],
+                              w.bodies[w.offsets[f]:w.offsets[f + 1], 3].view(np.uint32),
+                              (s.radii + np.float32(s.probe)) ** 2,
+                              np.stack([oracle.fibonacci_dots(s.n_half, r, s.probe) for r in s.radii]))
+           for f in range(len(w.offsets) - 1)]
+    for f in (0, 3, 7):
+        b = w.bodies[w.offsets[f]:w.offsets[f + 1]]
+        ref = oracle_iq(w.q, b[:, :3], b[:, 3].view(np.uint32), w.table, w.water_index, s.w, hyd[f], 2 * s.n_half)
+        assert rel_err(frames[f], ref).max() < 1e-5, f
+    # back to plain: the hydrated layout serves w = 0 with the plain factors
+    e.set_water_weight(0.0)
+    frames0, mean0 = e.batch_calc()
+    assert rel_err(frames0, ref_frames).max() < 1e-6
+    # a new engine: single-structure work before the first batch launch, and between launches
+    e2 = engine_for(w.q, w.table, w.water_index, 0.0)
+    e2.batch_set_frames(w.bodies, w.offsets)
+    e2.set_sasa_params(s.radii, s.probe, n_half=s.n_half)
+    e2.initialize(len(s.pos))
+    e2.set_water_weight(s.w)
+    I2 = e2.calc_curve(s.bodies)
+    e2.set_water_weight(0.0)
+    f2, m2 = e2.batch_calc()
+    assert np.array_equal(f2, ref_frames) and np.array_equal(m2, ref_mean)
+    e2.set_water_weight(s.w)
+    e2.calc_curve(s.bodies)
+    moved = s.bodies.copy()
+    moved[:50, :3] += np.float32(0.3)
+    e2.recalc_curve(moved, 0, 50)
+    e2.set_water_weight(0.0)
+    f3, m3 = e2.batch_calc()
+    assert np.array_equal(f3, ref_frames) and np.array_equal(m3, ref_mean)
+    assert np.array_equal(I1, I2)
+
+
+def test_batch_factor_swap_keeps_the_launch(oracle):
+    """A fit over (c, w) on resident frames: the default atomic factors regenerated at each
+    expansion factor, the water weight changed between launches.  Every launch matches the
+    FP64 oracle for its own (c, w), and the captured launch (jobs, shape) is kept."""
+    from resaxs_amd import configs
+    rng = np.random.default_rng(12)
+    q = np.linspace(0.01, 0.45, 70).astype(np.float32)
+    n, nf = 900, 3
+    base = configs.in_ball(rng, n, (n / 0.045 / (4.0 / 3.0 * np.pi)) ** (1.0 / 3.0))
+    types = rng.choice(np.array([1, 2, 3, 5, 8, 9, 13], dtype=np.uint32), size=n).astype(np.uint32)
+    models = [(base + rng.normal(scale=0.4, size=base.shape)).astype(np.float32) for _ in range(nf)]
+    bodies = np.concatenate([oracle.bodies(p, types) for p in models])
+    offsets = np.arange(nf + 1, dtype=np.int32) * n
+    radii = oracle.atomic_radii()
+    avg_r = np.float32(np.mean(radii[types].astype(np.float64)))
+    dots = np.stack([oracle.dodeca_icosa_dots(r, 1.8) for r in radii])
+    counts = [oracle.sasa_counts(p, types, ((radii + np.float32(1.8)) ** 2).astype(np.float32), dots) for p in models]
+    _say("case: batch factor swaps over (c, w), 3 frames x 900 atoms x 70 q")
+    from resaxs_amd import Engine, RSX_DOTS_DODECA_ICOSA
+    e = Engine(0)
+    e.set_q(q)
+    e.set_sasa_params(radii, 1.8, RSX_DOTS_DODECA_ICOSA, 0)
+    e.set_atomic_factors(avg_r)
+    e.set_water_weight(1.0)
+    e.batch_set_frames(bodies, offsets)
+    shape = None
+    for c, ww in ((1.0, 1.0), (0.97, 1.0), (1.03, -1.5), (1.03, 0.0), (0.95, 3.2)):
+        e.set_expansion_factor(c)
+        e.set_water_weight(ww)
+        frames, mean = e.batch_calc()
+        launch = e.batch_launch()
+        shape = shape or launch
+        assert launch == shape
+        table = oracle.atomic_ff(q, avg_r, np.float32(c))
+        for f in range(nf):
+            ref = oracle_iq(q, models[f], types, table, 14, ww, counts[f], 32)
+            err = rel_err(frames[f], ref).max()
+            assert err < 1e-5, (c, ww, f, err)
+    _say("  every (c, w) point within 1e-5 of the oracle, one captured launch")
+
+
+def test_batch_rejects_bad_offsets():
+    """Offsets are checked before any use: offsets[0] must be 0 and they must not decrease."""
+    from resaxs_amd import RsxError
+    from resaxs_amd._lib import RSX_INVALID_ARG
+    w = _frames(6, 3, 100, 300)
+    e = engine_for(w.q, w.table, w.water_index, 0.0)
+    for bad in (np.array([1, 100, 200, 300], np.int32), np.array([0, 200, 100, 300], np.int32),
+                np.array([-5, 100, 200, 300], np.int32)):
+        with pytest.raises(RsxError) as ei:
+            e.batch_set_frames(w.bodies, bad)
+        assert ei.value.code == RSX_INVALID_ARG
+    e.batch_set_frames(w.bodies, w.offsets)
+    assert e.batch_calc()[0].shape == (3, len(w.q))
+
+
 def _fit_case(seed=41, n=1200, nq=80):
     from resaxs_amd import configs
     rng = np.random.default_rng(seed)
