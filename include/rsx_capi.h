@@ -103,6 +103,24 @@ int rsx_synchronize(rsx_engine *h);
 int rsx_calc_rsasa(rsx_engine *h, const float *bodies, int n, const float *radii, int n_types,
                    float probe, const float *dots, int n_dots, unsigned *out_counts, float *out_frac);
 
+/* ---- SASA counts that allocate nothing (no counterpart in the reference) ----
+ * For callers that cannot allocate device memory: code under HIP graph capture, serving
+ * loops and harnesses that forbid hipMalloc.  The exposed-dot counts of rsx_calc_rsasa on
+ * device-resident data: bodies_dev n float4 {x, y, z, type bits}, dots_dev
+ * [n_types][n_dots] float4 templates, counts_dev n unsigned (output); radii [n_types] on the
+ * host (copied by value into the launch).  The cell list lives in a static device buffer of
+ * RSX_SASA_NOALLOC_SCRATCH_BYTES on each GPU it runs on (held until the process ends), so
+ * n <= RSX_SASA_NOALLOC_MAX_ATOMS, n_types <= 64 and 1 <= n_dots <= 512, else
+ * RSX_INVALID_ARG with nothing launched.  A body whose type is not below n_types has no dots
+ * (count 0) and occludes nothing.  Runs on the current HIP device, asynchronously on stream
+ * (a hipStream_t of that device; NULL = the null stream); no hipMalloc, hipFree or memset.
+ * Every call on one GPU shares its buffer: calls that overlap on one GPU (two streams or two
+ * host threads) race. */
+#define RSX_SASA_NOALLOC_MAX_ATOMS 65536
+#define RSX_SASA_NOALLOC_SCRATCH_BYTES 3147264
+int rsx_calc_rsasa_noalloc(const void *bodies_dev, int n, const float *radii, int n_types, float probe,
+                           const void *dots_dev, int n_dots, unsigned *counts_dev, void *stream);
+
 /* ---- host helpers (no device work) ---- */
 /* rsasa_algorithm.cpp:96-116: 2*n_half dots {x,y,z,radius} on a sphere of radius+probe */
 int rsx_fibonacci_dots(int n_half, float radius, float probe, float *out);

@@ -61,6 +61,7 @@ SIGNATURES = {
     "rsx_calc_curve_dev_async": (_i, [_vp, _vp, _i, _vp]),
     "rsx_synchronize": (_i, [_vp]),
     "rsx_calc_rsasa": (_i, [_vp, _fp, _i, _fp, _i, _f, _fp, _i, _up, _fp]),
+    "rsx_calc_rsasa_noalloc": (_i, [_vp, _i, _fp, _i, _f, _vp, _i, _vp, _vp]),
     "rsx_fibonacci_dots": (_i, [_i, _f, _f, _fp]),
     "rsx_dodeca_icosa_dots": (_i, [_f, _f, _fp]),
     "rsx_sphere_dots": (_i, [_f, _f, _f, _fp, _i, _ip]),

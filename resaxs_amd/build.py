@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libresaxs_amd.so")
 ARCH = os.environ.get("RSX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "engine.cpp", "pdb.cpp", "fit.cpp"]
+SOURCES = ["kernels.hip", "engine.cpp", "pdb.cpp", "fit.cpp", "noalloc.hip"]
 
 
 def hipcc():

@@ -809,6 +809,7 @@ void run_sasa_device(rsx_engine* h, int n, const int* subset = nullptr, int n_su
     rsx::SasaArgs a;
     a.atoms = h->d_sasa_atoms.p;
     a.n = n;
+    a.n_types = (int)h->rext.size();
     a.r2 = h->d_r2.p;
     a.rext = h->d_rext.p;
     a.dots = h->d_dots.p;

@@ -124,6 +124,8 @@ hipError_t launch_gather_cols(float4* cols, const float4* bodies, const int* src
 struct SasaArgs {
     const float4* atoms;     // x, y, z, type bits
     int n;
+    int n_types;             // an atom whose type is not below n_types has no dots (count 0)
+                             // and occludes nothing
     const float* r2;         // [n_types] (probe + r)^2
     const float* rext;       // [n_types] sqrt(r2) (prefilter)
     const float4* dots;      // [n_types][n_dots]

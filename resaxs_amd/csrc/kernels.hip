@@ -1098,6 +1098,10 @@ __global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
     const int lane = threadIdx.x;
     const float4 pi = a.atoms[i];
     const uint32_t ti = __float_as_uint(pi.w);
+    if (ti >= (uint32_t)a.n_types) {  // wave-uniform: no dots
+        if (lane == 0) a.counts[i] = 0u;
+        return;
+    }
     const float rext_i = a.rext[ti];
     float px[DPL], py[DPL], pz[DPL];
     bool live[DPL];
@@ -1144,10 +1148,12 @@ __global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
         if (p >= 0) {
             pj = a.sorted_atoms[p];
             const uint32_t tj = __float_as_uint(pj.w);
-            const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-            const float lim = rext_i + a.rext[tj] + 0.01f;
-            pass = ex * ex + ey * ey + ez * ez < lim * lim;
-            r2j = a.r2[tj];
+            if (tj < (uint32_t)a.n_types) {
+                const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+                const float lim = rext_i + a.rext[tj] + 0.01f;
+                pass = ex * ex + ey * ey + ez * ez < lim * lim;
+                r2j = a.r2[tj];
+            }
         }
         uint64_t m = __ballot(pass);
         while (m) {
