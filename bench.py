@@ -1,20 +1,24 @@
 #!/usr/bin/env python3
 """Benchmark of the Debye-SAXS hot path on one MI355X.
 
-Workload (hydrated forward, sasa-hydrated-20000x256): 20,000 atoms at 0.045 A^-3 in a
-1:1.3:1.8 ellipsoid, SASA hydration (256 Fibonacci dots, probe 1.4 A,
-contrast 0.03 e/A^3), 256 q over 0-0.6 A^-1.  A step is one full device-
-resident forward pass: SASA dot counts -> hydration classes -> Debye pair
-kernel -> FP64 reduction, over positions already in HBM.
+Headline workload (plain-50000x256): 50,000 atoms at 0.045 A^-3 in a 1:1.3:1.8
+ellipsoid (the incremental workload's structure, plain factors), 256 q over 0-0.6 A^-1.
+A step is one full device-resident forward pass on positions already in HBM: columns
+gathered from the bodies -> Debye pair kernel -> fixed-order FP64 reduction.  Every
+step runs on its own set of positions (each atom jittered by up to 0.2 A from the
+laid-out structure, all sets uploaded before the timed region), so the timed path
+reuses the layout's slot order and job table but not its positions.
 
-Prints ONE JSON line.  value = forward pair.q terms/s; roofline = the pair
-kernel's algorithmic FP32 FLOP rate (P (5 n_q + 9) + 2 N n_q per launch)
-over its HIP-event duration on the engine stream, against the FP32 peak.
+Prints ONE JSON line.  value = forward pair.q terms/s of the step; roofline = the pair
+kernel's algorithmic FP32 FLOP rate (P (5 n_q + 9) + 2 N n_q per launch) over its
+HIP-event duration on the engine stream, against the FP32 vector peak.  Further keys
+carry the other workloads (hydrated 20k step with SASA, forward + adjoint, batched
+ensemble, incremental moves, fit driver, histogram, non-uniform q) and the CPU
+baselines.
 """
 import argparse
 import json
 import os
-import platform
 import re
 import sys
 import time
@@ -25,7 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "debye_pair_kernel_sasa-hydrated-20000x256_counters.txt")
+PEAK_HBM_GBPS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
+# HBM bytes per launch from the committed rocprofv3 counter summaries
+TRAFFIC_SUMMARY = os.path.join(PROFILES, "debye_pair_kernel_plain-50000x256_counters.txt")
+SASA_SUMMARY = os.path.join(PROFILES, "sasa_kernels_sasa-hydrated-20000x256_counters.txt")
 # autotuner cache (one line per arch / path / n_q / size bucket / hydration key); a missing key is
 # tuned on the spot and appended
 TUNING_CACHE = os.path.join(ROOT, "resaxs_amd", "tuning_gfx950.txt")
@@ -47,89 +55,186 @@ def make_engine(w, hydrate=True):
     return e
 
 
-def read_traffic():
-    """HBM bytes per pair-kernel launch from the committed rocprofv3 counter summary."""
+def read_summary(path, key):
+    """A number `key: value` from a committed rocprofv3 summary (None if absent)."""
     try:
-        text = open(TRAFFIC_SUMMARY).read()
+        text = open(path).read()
     except OSError:
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    m = re.search(re.escape(key) + r":\s*([0-9.eE+]+)", text)
     return float(m.group(1)) if m else None
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo, as lscpu reports it)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def jittered(w, n_sets, seed=2024, shrink=0.99, amp=0.2):
+    """n_sets position sets: the structure pulled 1 % towards its centroid, every atom then
+    moved by up to amp A per axis (the sets stay inside the laid-out bounding radius)."""
+    rng = np.random.default_rng(seed)
+    c = w.pos.astype(np.float64).mean(axis=0)
+    from resaxs_amd import configs
+    out = []
+    for _ in range(n_sets):
+        p = c + shrink * (w.pos - c) + rng.uniform(-amp, amp, size=w.pos.shape)
+        out.append(configs.bodies_of(p.astype(np.float32), w.types))
+    return out
+
+
+def device_steps(e, sets, steps, warmup):
+    """Upload the position sets, run warmup steps, then time `steps` device-resident forward
+    passes, each on the next set (host wall clock between two stream synchronisations).
+    Returns (ms per step, the device buffers' last I(q), the bodies of the last step)."""
+    nq = e.nq
+    n = len(sets[0])
+    bufs = []
+    for b in sets:
+        p = e.dev_alloc(b.nbytes)
+        e.h2d(p, b)
+        bufs.append(p)
+    d_out = e.dev_alloc(nq * 8)
+    for k in range(warmup):
+        e.calc_curve_dev_async(bufs[k % len(bufs)], n, d_out)
+    e.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        e.calc_curve_dev_async(bufs[(warmup + k) % len(bufs)], n, d_out)
+    e.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    I = e.d2h(np.zeros(nq, dtype=np.float64), d_out)
+    last = sets[(warmup + steps - 1) % len(sets)]
+    for p in bufs:
+        e.dev_free(p)
+    e.dev_free(d_out)
+    return ms, I, last
+
+
 def forward_bench(steps, warmup):
+    """The headline: plain 50,000 x 256 forward, device-resident steps on fresh positions."""
+    from resaxs_amd import configs
+    w = configs.incremental_50000()
+    n, nq = len(w.bodies), len(w.q)
+    e = make_engine(w, hydrate=False)
+    log(f"bench: plain-{n}x{nq}: host calc_curve (layout + first result)")
+    e.calc_curve(w.bodies)
+    tuning = e.autotune(TUNING_CACHE)
+    log(f"bench: tuning {tuning}")
+    e.calc_curve(w.bodies)
+    sets = jittered(w, min(steps + warmup, 32))
+    log(f"bench: timing {steps} device-resident steps on fresh positions")
+    ms_per_step, I_dev, last = device_steps(e, sets, steps, warmup)
+    # pair-kernel duration from HIP events on the engine stream (the last step's columns)
+    _, pair_ms = e.time_forward(10)
+    t = e.get_tuning()
+    # accuracy of the last step: the same positions laid out afresh and summed by the FP64
+    # recurrence kernel (the oracle's 50,000-atom sum takes minutes on the host; the FP64
+    # kernel is held to it in the GPU tests)
+    f = make_engine(w, hydrate=False)
+    f.set_variant(0)
+    I_f64 = f.calc_curve(last)
+    f.close()
+    e.close()
+    err = np.abs(I_dev - I_f64) / np.abs(I_f64)
+    return w, ms_per_step, pair_ms, t, {"max": float(err.max()), "median": float(np.median(err)),
+                                         "reference": "GPU FP64 three-term recurrence kernel on the last step's "
+                                                      "positions, laid out afresh"}
+
+
+def hydrated_bench(steps, warmup):
+    """The 20,000-atom SASA-hydrated forward (256 Fibonacci dots, probe 1.4 A, contrast
+    0.03 e/A^3) x 256 q: device-resident steps on fresh positions (SASA counts, classes,
+    pair kernel, reduction on the device each step); the SASA count timed on its own."""
     from resaxs_amd import configs
     w = configs.sasa_ellipsoid_20000()
     n, nq = len(w.bodies), len(w.q)
     e = make_engine(w)
-    log(f"bench: {w.name}: host calc_curve (layout + first result)")
+    log(f"bench: {w.name}: host calc_curve")
     I_host = e.calc_curve(w.bodies)
     tuning = e.autotune(TUNING_CACHE)
-    log(f"bench: tuning {tuning}")
     I_host = e.calc_curve(w.bodies)
-    d_b = e.dev_alloc(n * 16)
-    d_out = e.dev_alloc(nq * 8)
-    e.h2d(d_b, w.bodies)
-    for _ in range(warmup):
-        e.calc_curve_dev_async(d_b, n, d_out)
-    e.synchronize()
-    log(f"bench: timing {steps} device-resident steps")
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        e.calc_curve_dev_async(d_b, n, d_out)
-    e.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / steps
-    I_dev = e.d2h(np.zeros(nq, dtype=np.float64), d_out)
-    # same kernels and layout; only the q = 0 value is summed in another order
-    if not np.all(np.abs(I_dev - I_host) <= 1e-12 * np.abs(I_host)):
-        raise RuntimeError("device-resident step differs from the host-path result")
-    # pair-kernel duration from HIP events on the engine stream
+    sets = jittered(w, min(steps + warmup, 32), seed=2025)
+    ms_per_step, _, _ = device_steps(e, sets, steps, warmup)
     _, pair_ms = e.time_forward(10)
+    # the laid-out positions once more: the device path equals the host path (same kernels and
+    # layout; only the q = 0 value is summed in another order)
+    _, same, _ = device_steps(e, [w.bodies], 1, 0)
+    if not np.all(np.abs(same - I_host) <= 1e-12 * np.abs(I_host)):
+        raise RuntimeError("device-resident step differs from the host-path result")
     sasa_ms = e.time_sasa(10)
-    e.dev_free(d_b)
-    e.dev_free(d_out)
+    t = e.get_tuning()
     e.close()
-    return w, I_host, ms_per_step, pair_ms, tuning, sasa_ms
+    flops = configs.forward_flops(n, nq)
+    kernel = ("debye_uniform_kernel", "debye_shared_kernel", "debye_rec32_kernel", "debye_rec32r_kernel")[t["variant"]]
+    return w, I_host, {
+        "workload": w.name, "ms_per_step": round(ms_per_step, 4),
+        "terms_per_s": configs.pair_terms(n, nq) / (ms_per_step * 1e-3),
+        "pair_kernel": f"{kernel}<{t['k']}> L={t['col_range']}", "pair_kernel_ms": round(pair_ms, 4),
+        "achieved_tflops": round(flops / (pair_ms * 1e-3) / 1e12, 3),
+        "frac": round(flops / (pair_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+        "positions": "every step on its own jittered position set (up to 0.2 A per axis); the layout's class "
+                     "order is that of the laid-out positions, so changed exposed-dot counts break class runs",
+        "timing": "host wall clock over device-resident steps; pair kernel from HIP events on the engine stream",
+    }, sasa_ms
 
 
 def sasa_summary(w, ms):
     """SASA at its algorithmic unit: one (dot, candidate) test over every atom j != i with
     |x_i - x_j| < r_i + r_j + 2p, 10 FP32 FLOP each (distance, subtract, compare); the
     count is a property of the structure, not of the kernel (which stops scanning an atom
-    once all its dots are buried, so it executes fewer)."""
+    once all its dots are buried, so it executes fewer).  HBM bytes per count from the
+    committed counter summary of the SASA kernels."""
     from scipy.spatial import cKDTree
     R = (w.radii.astype(np.float64) + w.probe)[w.types]
     pairs = cKDTree(w.pos.astype(np.float64)).query_pairs(2.0 * R.max(), output_type="ndarray")
     d = np.linalg.norm(w.pos[pairs[:, 0]].astype(np.float64) - w.pos[pairs[:, 1]], axis=1)
     close = int(np.count_nonzero(d < R[pairs[:, 0]] + R[pairs[:, 1]]))
     tests = 2.0 * close * 2 * w.n_half
-    return {"workload": f"sasa-{len(w.pos)}-atoms-{2 * w.n_half}-dots", "ms": round(ms, 4),
-            "kernels": "sasa_bin + sasa_scan_tiles/top/add + sasa_place + sasa_dots (HIP events, engine stream)",
-            "tests": tests, "tests_per_s": tests / (ms * 1e-3),
-            "achieved_tflops": round(10 * tests / (ms * 1e-3) / 1e12, 3),
-            "frac": round(10 * tests / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4)}
+    hbm = read_summary(SASA_SUMMARY, "hbm_bytes_per_count")
+    out = {"workload": f"sasa-{len(w.pos)}-atoms-{2 * w.n_half}-dots", "ms": round(ms, 4),
+           "kernels": "fill + sasa_bin + sasa_scan_tiles/top/add + sasa_place + sasa_dots (HIP events, engine stream)",
+           "tests": tests, "tests_per_s": tests / (ms * 1e-3),
+           "achieved_tflops": round(10 * tests / (ms * 1e-3) / 1e12, 3),
+           "frac": round(10 * tests / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+           "hbm_bytes": hbm}
+    if hbm:
+        out["achieved_GBps"] = round(hbm / (ms * 1e-3) / 1e9, 1)
+        out["hbm_frac"] = round(hbm / (ms * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4)
+    return out
 
 
-def forward_50k_bench():
-    """The 50,000-atom scale: plain forward of 50,000 atoms x 256 q (the incremental
-    workload's structure), pair-kernel time from HIP events on the engine stream."""
+def direct_bench(reps=5):
+    """A non-uniform q grid (the direct kernel: one v_sin per term, what a measured profile's
+    q values take): 20,000 plain atoms x 256 q spaced as q_k = 0.6 (k / 255)^1.5."""
     from resaxs_amd import configs
-    w = configs.incremental_50000()
-    n, nq = len(w.bodies), len(w.q)
-    e = make_engine(w, hydrate=False)
-    log(f"bench: {n} x {nq} plain forward (autotuned layout)")
+    w = configs.sasa_ellipsoid_20000()
+    q = (0.6 * (np.arange(256) / 255.0) ** 1.5).astype(np.float32)
+    n, nq = len(w.bodies), len(q)
+    from resaxs_amd import Engine
+    e = Engine(0)
+    e.set_q(q)
+    rng = np.random.default_rng(77)
+    e.set_factors(configs.factor_table(rng, q), configs.WATER)
+    e.initialize(n)
     e.calc_curve(w.bodies)
-    tuning = e.autotune(TUNING_CACHE)
-    e.calc_curve(w.bodies)
-    _, pair_ms = e.time_forward(5)
+    t = e.get_tuning()
+    _, pair_ms = e.time_forward(reps)
     e.close()
     flops = configs.forward_flops(n, nq)
-    return {"workload": f"plain-{n}x{nq}", "kernel_ms": round(pair_ms, 3),
+    return {"workload": f"plain-{n}x{nq}-nonuniform-q", "pair_kernel": f"debye_direct_kernel<{t['k']}>",
+            "recurrence": bool(t["recurrence"]), "pair_kernel_ms": round(pair_ms, 4),
             "terms_per_s": configs.pair_terms(n, nq) / (pair_ms * 1e-3),
-            "achieved_tflops": round(flops / (pair_ms * 1e-3) / 1e12, 2),
+            "achieved_tflops": round(flops / (pair_ms * 1e-3) / 1e12, 3),
             "frac": round(flops / (pair_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
-            "kernel": f"variant {tuning['variant']} K={tuning['k']} L={tuning['col_range']}"}
+            "timing": "HIP events on the engine stream"}
 
 
 def adjoint_bench(reps=3):
@@ -215,7 +320,9 @@ def incremental_bench(hydrate, n_moves=2000):
             "ms_per_move_median": round(float(np.median(ts)) * 1e3, 4),
             "terms_per_s": 2 * p_move * nq / (ms * 1e-3),
             "achieved_tflops": round(2 * p_move * (5 * nq + 9) / (ms * 1e-3) / 1e12, 2),
-            "rel_err_vs_fp64_full": err_last,
+            "rel_err_vs_gpu_fp64_full": err_last,
+            "rel_err_reference": "a fresh full recompute of the same positions by the GPU's FP64 three-term "
+                                 "recurrence kernel (variant 0)",
             "moves_since_rebaseline_at_check": (n_moves - 2) % 200 + 1,
             "timing": "host wall clock around rsx_recalc_curve_f64 + rsx_commit (host bodies in, I(q) out)"
                       + ("; SASA recounted for the atoms within reach of the moved segment" if hydrate else "")}
@@ -283,7 +390,8 @@ def fit_driver_bench(n_models=8, n=4000, nq=100):
     return {"workload": f"fit-{n_models}-models-{n}x{nq}", "seconds": round(sec, 3), "evaluations": ev,
             "ms_per_model_evaluation": round(sec * 1e3 / ev, 4), "found": [round(c, 5), round(w, 4), round(s, 5)],
             "target": [c0, w0, 2.0], "chi": chi,
-            "timing": "host wall clock around rsx_fit_ensemble (engines created, laid out and SASA counted inside)"}
+            "timing": "host wall clock around rsx_fit_ensemble (one engine created, the models laid out as batch "
+                      "frames and SASA counted inside)"}
 
 
 def histogram_bench(bin_width=0.01, reps=3):
@@ -308,13 +416,13 @@ def histogram_bench(bin_width=0.01, reps=3):
             "timing": "host wall clock around rsx_calc_curve_hist (histogram, transform, host combination)"}
 
 
-def cpu_baseline(w, I_gpu, rows=2560):
-    """The reference's CPU path timed on this host, 1 thread, on a bounded sample: the
-    pairs among the first `rows` atoms (the first rows of the lower triangle) at all
-    n_q.  kind "reference": host_debye<double, float, cl_float4>::calc_curve compiled
-    from the reference's sources (oracle/_ref); kind "port": its restatement in the
-    oracle where that library was not built.  Plus the FP64 oracle's full I(q) as the
-    accuracy check of the GPU result."""
+def cpu_baseline(w, rows=2900):
+    """The reference's CPU path timed on this host, 1 thread, on a bounded sample of the
+    headline workload: the pairs among its first `rows` atoms (a compact cluster: the
+    atoms are in space-filling-curve order) at all n_q.  kind "reference":
+    host_debye<double, float, cl_float4>::calc_curve compiled from the reference's sources
+    (oracle/_ref); kind "port": its restatement in the oracle where that library was not
+    built."""
     from oracle import pyoracle as po
     nq = len(w.q)
     terms = rows * (rows - 1) / 2.0 * nq
@@ -332,7 +440,50 @@ def cpu_baseline(w, I_gpu, rows=2560):
         po.debye_ref_rows(w.q, pos, types, w.table, 0, rows, nthreads=1)
         t = time.perf_counter() - t0
         kind, what = "port", "the oracle's restatement of host_debye.cpp:54-109"
-    log("bench: FP64 oracle of the full workload (accuracy check)")
+    return {"value": terms / t, "unit": "pair.q terms/s", "cores": 1, "kind": kind,
+            "sample": f"{what} on the first {rows} atoms of the {len(w.pos)}-atom workload x {nq} q = "
+                      f"{terms:.3e} pair.q terms in {t:.1f} s",
+            "cpu": cpu_model()}
+
+
+def cpu_config1():
+    """The plain 1,888-atom ball x 101 q timed in full on the CPU (the reference's own host_debye,
+    1 thread) and on the GPU (host-buffer calc_curve, wall clock), with both results against the
+    FP64 oracle."""
+    from oracle import pyoracle as po
+    from resaxs_amd import configs
+    w = configs.plain_ball_1888()
+    n, nq = len(w.pos), len(w.q)
+    terms = n * (n - 1) / 2.0 * nq
+    out = {"workload": w.name, "terms": terms}
+    F = po.body_factors(w.pos, w.types, w.q, w.table)
+    I_ref = po.debye(w.q, w.pos, F)
+    if po.ref_lib() is not None:
+        t0 = time.perf_counter()
+        I_cpu = po.ref_host_debye(w.q, w.pos, w.types, w.table)
+        t = time.perf_counter() - t0
+        out.update({"cpu_s": round(t, 3), "cpu_terms_per_s": terms / t, "cpu_kind": "reference (1 thread)",
+                    "cpu_rel_err_vs_fp64": float(np.max(np.abs(I_cpu - I_ref) / np.abs(I_ref)))})
+    e = make_engine(w, hydrate=False)
+    e.calc_curve(w.bodies)
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        I_gpu = e.calc_curve(w.bodies)
+    t = (time.perf_counter() - t0) / reps
+    e.close()
+    out.update({"gpu_ms": round(t * 1e3, 4), "gpu_terms_per_s": terms / t,
+                "gpu_rel_err_vs_fp64": float(np.max(np.abs(I_gpu - I_ref) / np.abs(I_ref))),
+                "timing": "CPU: one full host_debye call; GPU: host wall clock of rsx_calc_curve_f64 with host "
+                          "buffers (layout, upload, kernels, download)"})
+    return out
+
+
+def hydrated_accuracy(w, I_gpu):
+    """The hydrated 20,000-atom I(q) (host path) against the FP64 oracle with the oracle's own
+    exposed-dot counts."""
+    from oracle import pyoracle as po
+    log("bench: FP64 oracle of the hydrated 20,000-atom workload (accuracy check)")
     dots = np.stack([po.fibonacci_dots(w.n_half, r, w.probe) for r in w.radii])
     rr = (w.radii.astype(np.float32) + np.float32(w.probe)).astype(np.float32)
     counts = po.sasa_counts(w.pos, w.types, (rr * rr).astype(np.float32), dots)
@@ -340,15 +491,10 @@ def cpu_baseline(w, I_gpu, rows=2560):
     F = po.body_factors(w.pos, w.types, w.q, w.table, w.water_index, w.w, s)
     I_ref = po.debye(w.q, w.pos, F)
     err = np.abs(I_gpu - I_ref) / np.abs(I_ref)
-    base = {"value": terms / t, "unit": "pair.q terms/s", "cores": 1, "kind": kind,
-            "sample": f"{what} on the first {rows} atoms of the {len(w.pos)}-atom workload (plain factors) x "
-                      f"{nq} q = {terms:.3e} pair.q terms in {t:.1f} s",
-            "cpu": platform.processor() or platform.machine()}
-    acc = {"max": float(err.max()), "median": float(np.median(err)), "reference": "FP64 oracle, full I(q)"}
-    return base, acc
+    return {"max": float(err.max()), "median": float(np.median(err)), "reference": "FP64 oracle, full I(q)"}
 
 
-def cpu_baseline_all_cores(w, rows=6400):
+def cpu_baseline_all_cores(w, rows=11000):
     """The same loop on all of this host's cores (at most 16, the host share of one GPU):
     the oracle's OpenMP restatement of host_debye.cpp:54-109 over the pairs among the
     first `rows` atoms at all n_q (the reference's own loop is single-threaded)."""
@@ -362,8 +508,8 @@ def cpu_baseline_all_cores(w, rows=6400):
     t = time.perf_counter() - t0
     return {"value": terms / t, "unit": "pair.q terms/s", "cores": threads, "kind": "port",
             "sample": f"the oracle's restatement of host_debye.cpp:54-109 (OpenMP over rows) on the first {rows} "
-                      f"atoms of the {len(w.pos)}-atom workload (plain factors) x {nq} q = {terms:.3e} pair.q "
-                      f"terms in {t:.1f} s"}
+                      f"atoms of the {len(w.pos)}-atom workload x {nq} q = {terms:.3e} pair.q terms in {t:.1f} s",
+            "cpu": cpu_model()}
 
 
 def main():
@@ -374,10 +520,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-workloads", action="store_true",
-                    help="skip the batched-ensemble and incremental-move measurements")
+                    help="skip the batched-ensemble, incremental-move, fit-driver, histogram and non-uniform-q "
+                         "measurements")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of this many MiB (counter calibration)")
-    ap.add_argument("--section", choices=("fwd_adjoint", "incremental", "incremental_sasa", "batched"),
+    ap.add_argument("--section", choices=("forward_50k", "hydrated_20k", "sasa", "fwd_adjoint", "incremental",
+                                          "incremental_sasa", "batched", "direct"),
                     help="run only this workload section, shortened for profiling (incremental: 100 moves; "
                          "batched: 2 replays), and print its JSON")
     args = ap.parse_args()
@@ -385,13 +533,6 @@ def main():
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.section:
-        run = {"fwd_adjoint": lambda: adjoint_bench(),
-               "incremental": lambda: incremental_bench(False, n_moves=100),
-               "incremental_sasa": lambda: incremental_bench(True, n_moves=100),
-               "batched": lambda: batched_bench(reps=2)}[args.section]
-        print(json.dumps({"section": args.section, "result": run()}), flush=True)
-        return
     from resaxs_amd import configs
     if args.copy_probe:
         from resaxs_amd import Engine
@@ -400,7 +541,31 @@ def main():
         log(f"bench: copy probe {args.copy_probe} MiB: {ms:.4f} ms/launch, "
             f"{2 * (args.copy_probe << 20) / ms / 1e6:.1f} GB/s (read + write)")
         e.close()
-    w, I_gpu, ms_per_step, pair_ms, tuning, sasa_ms = forward_bench(args.steps, args.warmup)
+    if args.section:
+        def fwd():
+            w, ms, pair_ms, t, acc = forward_bench(args.steps, args.warmup)
+            return {"ms_per_step": ms, "pair_kernel_ms": pair_ms, "tuning": t, "rel_err": acc}
+
+        def hyd():
+            return hydrated_bench(args.steps, args.warmup)[2]
+
+        def sasa():
+            w = configs.sasa_ellipsoid_20000()
+            e = make_engine(w)
+            e.calc_curve(w.bodies)
+            ms = e.time_sasa(20)
+            e.close()
+            return {"ms": ms}
+
+        run = {"forward_50k": fwd, "hydrated_20k": hyd, "sasa": sasa,
+               "fwd_adjoint": lambda: adjoint_bench(),
+               "incremental": lambda: incremental_bench(False, n_moves=100),
+               "incremental_sasa": lambda: incremental_bench(True, n_moves=100),
+               "batched": lambda: batched_bench(reps=2),
+               "direct": lambda: direct_bench()}[args.section]
+        print(json.dumps({"section": args.section, "result": run()}), flush=True)
+        return
+    w, ms_per_step, pair_ms, tuning, acc = forward_bench(args.steps, args.warmup)
     kernel = ("debye_uniform_kernel", "debye_shared_kernel", "debye_rec32_kernel",
               "debye_rec32r_kernel")[tuning["variant"]]
     n, nq = len(w.pos), len(w.q)
@@ -408,7 +573,7 @@ def main():
     flops = configs.forward_flops(n, nq)
     achieved = flops / (pair_ms * 1e-3) / 1e12
     line = {
-        "metric": "Debye pair.q terms/s (forward, device-resident step incl. SASA hydration)",
+        "metric": "Debye pair.q terms/s (forward, device-resident step on fresh positions)",
         "value": terms / (ms_per_step * 1e-3),
         "unit": "pair.q terms/s",
         "n_gpus": 1,
@@ -419,18 +584,21 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
-        "data": "synthetic (seeded)",
-        "config": {"workload": w.name, "n_atoms": n, "n_q": nq, "q_max": float(w.q[-1]),
-                   "hydration": "SASA, 256 Fibonacci dots, probe 1.4 A, contrast 0.03 e/A^3"},
+        "data": "synthetic (seeded); each step on its own jittered position set, uploaded before timing",
+        "config": {"workload": f"plain-{n}x{nq}", "n_atoms": n, "n_q": nq, "q_max": float(w.q[-1]),
+                   "hydration": "none (plain factors); the SASA-hydrated 20,000-atom step is under hydrated_20k"},
         "roofline": {"bound": "mfma", "pipe": "VALU (FP32 + FP64 vector; no MFMA)",
                      "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": read_traffic(),
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": read_summary(TRAFFIC_SUMMARY, "traffic_bytes_per_launch"),
                      "kernel": f"{kernel}<{tuning['k']}>", "col_range": tuning["col_range"],
                      "kernel_ms": round(pair_ms, 4), "flop_per_launch": flops},
-        "sasa": sasa_summary(w, sasa_ms),
+        "rel_err": acc,
     }
+    hw, I_hyd, hyd, sasa_ms = hydrated_bench(args.steps, args.warmup)
+    hyd["sasa"] = sasa_summary(hw, sasa_ms)
+    line["hydrated_20k"] = hyd
     if not args.no_adjoint:
-        line["forward_50k"] = forward_50k_bench()
         log("bench: forward + adjoint on the fit configuration")
         line["fwd_adjoint"] = adjoint_bench()
     if not args.no_workloads:
@@ -441,11 +609,13 @@ def main():
         line["fit_driver"] = fit_driver_bench()
         log("bench: distance-histogram Debye")
         line["histogram"] = histogram_bench()
+        log("bench: non-uniform q grid (direct kernel)")
+        line["direct_q"] = direct_bench()
     if not args.no_cpu_baseline:
-        base, acc = cpu_baseline(w, I_gpu)
-        line["cpu_baseline"] = base
+        line["cpu_baseline"] = cpu_baseline(w)
         line["cpu_baseline_all_cores"] = cpu_baseline_all_cores(w)
-        line["rel_err_vs_fp64"] = acc
+        line["cpu_config1"] = cpu_config1()
+        line["hydrated_20k"]["rel_err_vs_fp64"] = hydrated_accuracy(hw, I_hyd)
     print(json.dumps(line), flush=True)
 
 

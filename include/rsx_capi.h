@@ -93,8 +93,15 @@ int rsx_get_rsasa(rsx_engine *h, float *out, int n);
 int rsx_state(const rsx_engine *h);
 
 /* ---- device-resident variants (inputs already in HBM; no host copies) ---- */
-/* bodies_dev: device pointer to n float4 bodies in the caller's order */
+/* bodies_dev: device pointer to n float4 bodies in the caller's order.  Reuses the layout of
+ * the last host calc_curve (slot order, jobs, path): RSX_NO_INITIAL_CALC before one,
+ * RSX_ARGS_NOT_SET when parameters changed since.  The bodies are checked on the device:
+ * a type outside the factor table is skipped, and (recurrence path) positions must stay
+ * within min(1.25 R, 1.5 / dq) of the layout's centre, R its bounding radius, for the
+ * recurrence's accuracy; a failed check is reported by the next rsx_synchronize. */
 int rsx_calc_curve_dev_async(rsx_engine *h, const void *bodies_dev, int n, double *out_dev);
+/* waits for the engine's stream; RSX_INVALID_ARG when a device-resident forward enqueued since
+ * the last synchronize failed its checks (the message says which) */
 int rsx_synchronize(rsx_engine *h);
 
 /* ---- SASA (i_rsasa::calc_rsasa, rsasa_algorithm.hpp:42-61; kernel rsasa_dot_spheres.cl:94-163) ---- */

@@ -364,6 +364,8 @@ struct rsx_engine {
     bool persistent = true;          // batched launch: persistent work-queue kernel (FP32 variant)
     int n_cu = 256;                  // compute units of the device
     DBuf<int> d_queue;               // work counter of the persistent launch
+    DBuf<int> d_guard;               // checks of device-resident bodies (rsx::GUARD_*)
+    bool dev_pending = false;        // a device-resident forward ran since the last synchronize
     uint64_t layout_stamp = 0;  // bumped whenever the class table or the slot layout changes
     size_t ftab_rows = 0;
 
@@ -2257,7 +2259,23 @@ void enqueue_device_forward(rsx_engine* h, const float4* bodies_dev, double* out
         run_sasa_device(h, n);
         counts = h->d_counts.p;
     }
-    hipchk(rsx::launch_gather_cols(h->d_cols.p, bodies_dev, h->d_s2c.p, counts, h->d1, n, h->stream), "gather");
+    // checks of the device-resident bodies (read back by rsx_synchronize): types inside the
+    // factor table, and positions inside the radius the layout's recurrence was chosen for
+    // (its series covers 1.25 times that radius; the step angle dq r stays below 3)
+    if (!h->dev_pending) {
+        h->d_guard.ensure(1);
+        hipchk(rsx::launch_fill_i32(h->d_guard.p, 0, 1, h->stream), "clear guard");
+    }
+    h->dev_pending = true;
+    rsx::GatherGuard g;
+    g.n_types = h->n_types;
+    g.cx = (float)h->centre[0];
+    g.cy = (float)h->centre[1];
+    g.cz = (float)h->centre[2];
+    const double r_ok = h->uniform ? std::min(1.25 * h->path_radius, 0.5 * MAX_STEP_ANGLE / h->dq) : 1e30;
+    g.r2_max = (float)std::min(r_ok * r_ok, 1e30);
+    g.flag = h->d_guard.p;
+    hipchk(rsx::launch_gather_cols(h->d_cols.p, bodies_dev, h->d_s2c.p, counts, h->d1, n, g, h->stream), "gather");
     hipchk(rsx::launch_q0(h->d_cols.p, n, h->d_f0col.p, h->d_q0.p, h->stream), "q0 kernel");
     const int nj = (int)h->last_jobs.size();
     const rsx::DebyeArgs a = debye_args(h, h->d_fjobs.p, h->d_partials.p);
@@ -2531,6 +2549,7 @@ int rsx_destroy(rsx_engine* h)
     h->b_jobs.release();
     h->d_counts.release();
     h->d_queue.release();
+    h->d_guard.release();
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->ev2) hipEventDestroy(h->ev2);
@@ -2757,6 +2776,8 @@ int rsx_calc_curve_dev_async(rsx_engine* h, const void* bodies_dev, int n, doubl
         check_args_set(h);
         if (n != h->n || !bodies_dev) fail(RSX_INVALID_ARG, "calc_curve_dev: body count mismatch");
         if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "calc_curve_dev needs one host calc_curve to lay out the atoms");
+        if (h->params_dirty)
+            fail(RSX_ARGS_NOT_SET, "calc_curve_dev: parameters changed since the last calc_curve (call it first)");
         set_device(h);
         enqueue_device_forward(h, (const float4*)bodies_dev, out_dev ? out_dev : h->d_iq.p);
     });
@@ -2764,7 +2785,18 @@ int rsx_calc_curve_dev_async(rsx_engine* h, const void* bodies_dev, int n, doubl
 
 int rsx_synchronize(rsx_engine* h)
 {
-    return guarded(h, [&] { hipchk(hipStreamSynchronize(h->stream), "synchronize"); });
+    return guarded(h, [&] {
+        hipchk(hipStreamSynchronize(h->stream), "synchronize");
+        if (!h->dev_pending) return;
+        h->dev_pending = false;
+        int flag = 0;
+        hipchk(hipMemcpy(&flag, h->d_guard.p, sizeof(int), hipMemcpyDeviceToHost), "read guard");
+        if (flag & rsx::GUARD_TYPE)
+            fail(RSX_INVALID_ARG, "calc_curve_dev: a body type index is outside the factor table (its pairs were skipped)");
+        if (flag & rsx::GUARD_RADIUS)
+            fail(RSX_INVALID_ARG, "calc_curve_dev: bodies moved beyond the radius the layout was chosen for "
+                                  "(the result may be inaccurate; lay out again with calc_curve)");
+    });
 }
 
 int rsx_calc_rsasa(rsx_engine* h, const float* bodies, int n, const float* radii, int n_types, float probe,

@@ -116,9 +116,20 @@ hipError_t launch_scatter(float4* cols, const int* slot, const float4* vals, int
 // dst[t] = src[idx[t]] for t < m
 hipError_t launch_gather_u32(uint32_t* dst, const uint32_t* src, const int* idx, int m, hipStream_t s);
 
-// cols[s] = {bodies[src[s]].xyz, class}, class = type(bodies[src[s]]) * d1 + (counts ? counts[src[s]] : 0)
+// cols[s] = {bodies[src[s]].xyz, class}, class = type(bodies[src[s]]) * d1 + (counts ? counts[src[s]] : 0).
+// Checks of device-resident bodies, reported through *flag (bits or-ed in, never cleared
+// here): a type not below n_types gets the skip class and sets GUARD_TYPE; a body farther
+// than sqrt(r2_max) from centre sets GUARD_RADIUS.
+constexpr int GUARD_TYPE = 1;
+constexpr int GUARD_RADIUS = 2;
+struct GatherGuard {
+    int n_types;
+    float cx, cy, cz;
+    float r2_max;
+    int* flag;
+};
 hipError_t launch_gather_cols(float4* cols, const float4* bodies, const int* src, const uint32_t* counts,
-                              int d1, int n, hipStream_t s);
+                              int d1, int n, const GatherGuard& g, hipStream_t s);
 
 // ---- SASA ----
 struct SasaArgs {

@@ -922,14 +922,22 @@ __global__ void gather_u32_kernel(uint32_t* dst, const uint32_t* __restrict__ sr
 }
 
 __global__ void gather_cols_kernel(float4* cols, const float4* __restrict__ bodies, const int* __restrict__ src,
-                                   const uint32_t* __restrict__ counts, int d1, int n)
+                                   const uint32_t* __restrict__ counts, int d1, int n, GatherGuard g)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int c = src[s];
     const float4 b = bodies[c];
     const uint32_t type = __float_as_uint(b.w);
-    const uint32_t cls = type * (uint32_t)d1 + (counts ? counts[c] : 0u);
+    int bad = 0;
+    uint32_t cls = CLS_SKIP;
+    if (type < (uint32_t)g.n_types)
+        cls = type * (uint32_t)d1 + (counts ? counts[c] : 0u);
+    else
+        bad |= GUARD_TYPE;
+    const float dx = b.x - g.cx, dy = b.y - g.cy, dz = b.z - g.cz;
+    if (!(dx * dx + dy * dy + dz * dz <= g.r2_max)) bad |= GUARD_RADIUS;
+    if (bad) atomicOr(g.flag, bad);
     cols[s] = make_float4(b.x, b.y, b.z, __uint_as_float(cls));
 }
 
@@ -1786,11 +1794,11 @@ hipError_t launch_gather_u32(uint32_t* dst, const uint32_t* src, const int* idx,
 }
 
 hipError_t launch_gather_cols(float4* cols, const float4* bodies, const int* src, const uint32_t* counts, int d1,
-                              int n, hipStream_t s)
+                              int n, const GatherGuard& g, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(gather_cols_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cols, bodies, src,
-                       counts, d1, n);
+                       counts, d1, n, g);
     return hipGetLastError();
 }
 

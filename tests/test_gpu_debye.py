@@ -284,3 +284,75 @@ def test_autotune_and_cache(oracle, tmp_path):
     assert np.array_equal(f.calc_curve(b), I)
     f.close()
     assert len(open(cache).read().split()) == 5
+
+
+def _dev_forward(e, bodies):
+    """I(q) from rsx_calc_curve_dev_async on a device copy of bodies (then synchronize)."""
+    b = np.ascontiguousarray(bodies, dtype=np.float32)
+    pb, po_ = e.dev_alloc(b.nbytes), e.dev_alloc(8 * e.nq)
+    try:
+        e.h2d(pb, b)
+        e.calc_curve_dev_async(pb, len(b), po_)
+        e.synchronize()
+        return e.d2h(np.zeros(e.nq, dtype=np.float64), po_)
+    finally:
+        e.dev_free(pb)
+        e.dev_free(po_)
+
+
+@pytest.mark.parametrize("hydrate", [False, True])
+def test_device_resident_forward_on_new_positions(oracle, hydrate):
+    """rsx_calc_curve_dev_async on positions other than the ones the host calc_curve laid
+    out (every atom jittered by up to 0.3 A, types unchanged): the slot order and jobs are
+    reused, the columns are gathered (and, hydrated, the SASA counts taken) on the device.
+    I(q) meets the FP64 bar for the new positions."""
+    from resaxs_amd import configs
+    w = configs.sasa_ellipsoid_20000(n=3000)
+    _say(f"case: device-resident forward on new positions, 3000 x 256, hydrated={hydrate}")
+    wt = w.w if hydrate else 0.0
+    e = engine_for(w.q, w.table, w.water_index, wt, n=len(w.pos), radii=w.radii, probe=w.probe, n_half=w.n_half)
+    I0 = e.calc_curve(w.bodies)
+    same = _dev_forward(e, w.bodies)
+    assert np.all(np.abs(same - I0) <= 1e-12 * np.abs(I0))
+    rng = np.random.default_rng(31)
+    centre = w.pos.astype(np.float64).mean(axis=0)
+    # shrink a little first so the jitter cannot push an atom past the laid-out radius
+    pos = (centre + 0.98 * (w.pos - centre) + rng.uniform(-0.3, 0.3, size=w.pos.shape)).astype(np.float32)
+    got = _dev_forward(e, oracle.bodies(pos, w.types))
+    counts = None
+    if hydrate:
+        counts = oracle.sasa_counts(pos, w.types, r2_table(w.radii, w.probe), fib_dot_table(w.radii, w.probe, w.n_half))
+    ref = oracle_iq(w.q, pos, w.types, w.table, w.water_index, wt, counts, 2 * w.n_half)
+    err = rel_err(got, ref).max()
+    _say(f"  max rel err {err:.3e}")
+    assert err < TOL
+    e.close()
+
+
+def test_device_resident_forward_checks():
+    """The device-resident forward checks its bodies on the device and reports through
+    rsx_synchronize: a type outside the factor table (its pairs skipped), positions beyond the
+    radius the layout was chosen for; parameters changed since the host calc_curve are refused
+    up front.  A clean call after a reported failure succeeds."""
+    from resaxs_amd import RsxError, configs
+    w = configs.sasa_ellipsoid_20000(n=2000)
+    _say("case: device-resident forward checks, 2000 x 256")
+    e = engine_for(w.q, w.table, w.water_index, 0.0, n=len(w.pos))
+    I0 = e.calc_curve(w.bodies)
+    bad = w.bodies.copy()
+    bad[5, 3] = np.array([w.table.shape[1] + 3], dtype=np.uint32).view(np.float32)[0]
+    with pytest.raises(RsxError) as ei:
+        _dev_forward(e, bad)
+    assert ei.value.code == -3 and "type" in str(ei.value)
+    far = w.bodies.copy()
+    far[:, :3] *= np.float32(1.6)
+    with pytest.raises(RsxError) as ei:
+        _dev_forward(e, far)
+    assert ei.value.code == -3 and "radius" in str(ei.value)
+    again = _dev_forward(e, w.bodies)
+    assert np.all(np.abs(again - I0) <= 1e-12 * np.abs(I0))
+    e.set_factors(w.table * np.float32(1.01), w.water_index)
+    with pytest.raises(RsxError) as ei:
+        _dev_forward(e, w.bodies)
+    assert ei.value.code == -5
+    e.close()
