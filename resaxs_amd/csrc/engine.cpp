@@ -366,6 +366,10 @@ struct rsx_engine {
     DBuf<int> d_queue;               // work counter of the persistent launch
     DBuf<int> d_guard;               // checks of device-resident bodies (rsx::GUARD_*)
     bool dev_pending = false;        // a device-resident forward ran since the last synchronize
+    DBuf<int> d_cs_table, d_cs_tot;  // scratch of the device class sort
+    // a device-resident forward overwrote the columns, slot map or running I(q): the host
+    // bodies' state has to be laid out again before an incremental update or a commit
+    bool host_stale = false;
     uint64_t layout_stamp = 0;  // bumped whenever the class table or the slot layout changes
     size_t ftab_rows = 0;
 
@@ -1292,6 +1296,7 @@ void calc_curve_impl(rsx_engine* h, const float* bodies, int n, float* out32, do
     full_forward(h);
     h->params_dirty = false;
     h->w_dirty = false;
+    h->host_stale = false;
     h->state = 4;
     copy_out(h, out32, out64);
 }
@@ -1327,14 +1332,16 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
         if (h->no_rsasa_recalc && h->have_rsasa) sasa_now = false;
     }
 
-    if (h->params_dirty || hyd_before != hydration_on(h)) {
-        // new parameters force a full recalculation
-        build_class_table(h);
+    if (h->params_dirty || hyd_before != hydration_on(h) || h->host_stale) {
+        // new parameters (or device state left by a device-resident forward) force a full
+        // recalculation
+        if (h->params_dirty || hyd_before != hydration_on(h)) build_class_table(h);
         if (h->hydrate && (sasa_now || !h->have_rsasa)) compute_sasa(h);
         assign_slots(h);
         full_forward(h);
         h->params_dirty = false;
         h->w_dirty = false;
+        h->host_stale = false;
         copy_out(h, out32, out64);
         return;
     }
@@ -1539,6 +1546,13 @@ void commit_impl(rsx_engine* h)
 {
     if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "commit() called before an initial call to calc_curve()");
     set_device(h);
+    if (h->host_stale) {
+        // the device holds a device-resident forward's bodies: commit the host bodies' state
+        // (their exposed-dot counts on the host are current)
+        assign_slots(h);
+        full_forward(h);
+        h->host_stale = false;
+    }
     h->d_saved_cols.ensure(h->n);
     hipchk(hipMemcpyAsync(h->d_saved_cols.p, h->d_cols.p, (size_t)h->n * sizeof(float4), hipMemcpyDeviceToDevice,
                           h->stream),
@@ -1602,10 +1616,11 @@ void revert_impl(rsx_engine* h)
            "restore cols");
     hipchk(hipMemcpyAsync(h->d_iq.p, h->d_saved_iq.p, h->nq * sizeof(double), hipMemcpyDeviceToDevice, h->stream),
            "restore iq");
-    if (layout_changed)
+    if (layout_changed || h->host_stale)
         hipchk(hipMemcpyAsync(h->d_s2c.p, h->caller_of.data(), h->n * sizeof(int), hipMemcpyHostToDevice, h->stream),
                "restore slot map");
     hipchk(hipStreamSynchronize(h->stream), "sync revert");
+    h->host_stale = false;
 }
 
 // ---- standalone SASA -----------------------------------------------------------
@@ -2275,7 +2290,29 @@ void enqueue_device_forward(rsx_engine* h, const float4* bodies_dev, double* out
     const double r_ok = h->uniform ? std::min(1.25 * h->path_radius, 0.5 * MAX_STEP_ANGLE / h->dq) : 1e30;
     g.r2_max = (float)std::min(r_ok * r_ok, 1e30);
     g.flag = h->d_guard.p;
-    hipchk(rsx::launch_gather_cols(h->d_cols.p, bodies_dev, h->d_s2c.p, counts, h->d1, n, g, h->stream), "gather");
+    h->host_stale = true;  // columns (and with hydration the slot map) now hold these bodies
+    if (counts && h->n_class + 1 <= rsx::CLASS_SORT_MAX_KEYS) {
+        // new positions change exposed-dot counts and so classes: the columns are sorted by
+        // class again (stable, the host layout's order), so the pair kernel keeps long runs
+        rsx::ClassSortArgs cs;
+        cs.bodies = bodies_dev;
+        cs.counts = counts;
+        cs.d1 = h->d1;
+        cs.n = n;
+        cs.n_types = h->n_types;
+        cs.n_class = h->n_class;
+        h->d_cs_table.ensure((size_t)((n + 1023) / 1024) * (h->n_class + 1));
+        h->d_cs_tot.ensure(h->n_class + 1);
+        cs.table = h->d_cs_table.p;
+        cs.tot = h->d_cs_tot.p;
+        cs.cols = h->d_cols.p;
+        cs.s2c = h->d_s2c.p;
+        cs.g = g;
+        hipchk(rsx::launch_class_sort(cs, h->stream), "class sort");
+    } else {
+        hipchk(rsx::launch_gather_cols(h->d_cols.p, bodies_dev, h->d_s2c.p, counts, h->d1, n, g, h->stream),
+               "gather");
+    }
     hipchk(rsx::launch_q0(h->d_cols.p, n, h->d_f0col.p, h->d_q0.p, h->stream), "q0 kernel");
     const int nj = (int)h->last_jobs.size();
     const rsx::DebyeArgs a = debye_args(h, h->d_fjobs.p, h->d_partials.p);
@@ -2550,6 +2587,8 @@ int rsx_destroy(rsx_engine* h)
     h->d_counts.release();
     h->d_queue.release();
     h->d_guard.release();
+    h->d_cs_table.release();
+    h->d_cs_tot.release();
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->ev2) hipEventDestroy(h->ev2);

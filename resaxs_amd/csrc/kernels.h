@@ -131,6 +131,25 @@ struct GatherGuard {
 hipError_t launch_gather_cols(float4* cols, const float4* bodies, const int* src, const uint32_t* counts,
                               int d1, int n, const GatherGuard& g, hipStream_t s);
 
+// Stable counting sort of device-resident bodies by class, class = type * d1 + counts[i]:
+// slot s receives the s-th body in (class, caller index) order, the order the host layout
+// gives them.  cols[s] = {x, y, z, class}, s2c[s] = caller index; a type not below n_types
+// sorts last with the skip class.  The GatherGuard checks apply.  Scratch: table
+// [ceil(n / 1024)][n_class + 1] ints, tot [n_class + 1] ints.  n_class + 1 <=
+// CLASS_SORT_MAX_KEYS (the per-block histogram lives in LDS).
+constexpr int CLASS_SORT_MAX_KEYS = 16384;
+struct ClassSortArgs {
+    const float4* bodies;
+    const uint32_t* counts;
+    int d1, n, n_types, n_class;
+    int* table;
+    int* tot;
+    float4* cols;
+    int* s2c;
+    GatherGuard g;
+};
+hipError_t launch_class_sort(const ClassSortArgs& a, hipStream_t s);
+
 // ---- SASA ----
 struct SasaArgs {
     const float4* atoms;     // x, y, z, type bits

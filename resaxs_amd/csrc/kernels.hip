@@ -941,6 +941,101 @@ __global__ void gather_cols_kernel(float4* cols, const float4* __restrict__ bodi
     cols[s] = make_float4(b.x, b.y, b.z, __uint_as_float(cls));
 }
 
+// ---- stable counting sort by class (device-resident hydrated steps) ----
+constexpr int CS_BLOCK = 1024;
+
+__device__ __forceinline__ int sort_key(const ClassSortArgs& a, uint32_t type, int i)
+{
+    return type < (uint32_t)a.n_types ? (int)(type * (uint32_t)a.d1 + a.counts[i]) : a.n_class;
+}
+
+// per block of CS_BLOCK bodies: how many of each key (LDS histogram)
+__global__ __launch_bounds__(CS_BLOCK) void class_hist_kernel(ClassSortArgs a)
+{
+    extern __shared__ int hist[];
+    const int nk = a.n_class + 1;
+    for (int c = threadIdx.x; c < nk; c += CS_BLOCK) hist[c] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * CS_BLOCK + threadIdx.x;
+    if (i < a.n) atomicAdd(&hist[sort_key(a, __float_as_uint(a.bodies[i].w), i)], 1);
+    __syncthreads();
+    for (int c = threadIdx.x; c < nk; c += CS_BLOCK) a.table[(size_t)blockIdx.x * nk + c] = hist[c];
+}
+
+// one workgroup: table[b][c] becomes the first slot of block b's bodies of key c, i.e. the
+// bodies of all smaller keys plus those of key c in blocks before b
+__global__ __launch_bounds__(1024) void class_scan_kernel(ClassSortArgs a, int n_blk)
+{
+    __shared__ int wave_tot[16];
+    __shared__ int carry_s;
+    const int nk = a.n_class + 1;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int c = tid; c < nk; c += 1024) {
+        int off = 0;
+        for (int b = 0; b < n_blk; ++b) {
+            int* p = a.table + (size_t)b * nk + c;
+            const int v = *p;
+            *p = off;
+            off += v;
+        }
+        a.tot[c] = off;
+    }
+    __syncthreads();
+    // exclusive scan of the key totals in passes of 1024 with a running carry
+    int carry = 0;
+    for (int c0 = 0; c0 < nk; c0 += 1024) {
+        const int c = c0 + tid;
+        const int v = c < nk ? a.tot[c] : 0;
+        int x = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wave_tot[w] = x;
+        __syncthreads();
+        int off = carry;
+        for (int u = 0; u < w; ++u) off += wave_tot[u];
+        if (c < nk) a.tot[c] = off + x - v;
+        if (tid == 1023) carry_s = off + x;
+        __syncthreads();
+        carry = carry_s;
+        __syncthreads();
+    }
+    for (int c = tid; c < nk; c += 1024) {
+        const int base = a.tot[c];
+        for (int b = 0; b < n_blk; ++b) a.table[(size_t)b * nk + c] += base;
+    }
+}
+
+// each body to its slot: its block's first slot of its key plus the bodies of the same key
+// before it in the block (caller order, so the sort is stable)
+__global__ __launch_bounds__(CS_BLOCK) void class_scatter_kernel(ClassSortArgs a)
+{
+    __shared__ int keys[CS_BLOCK];
+    const int i = blockIdx.x * CS_BLOCK + threadIdx.x;
+    int key = -1;
+    float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < a.n) {
+        b = a.bodies[i];
+        const uint32_t type = __float_as_uint(b.w);
+        key = sort_key(a, type, i);
+        int bad = type < (uint32_t)a.n_types ? 0 : GUARD_TYPE;
+        const float dx = b.x - a.g.cx, dy = b.y - a.g.cy, dz = b.z - a.g.cz;
+        if (!(dx * dx + dy * dy + dz * dz <= a.g.r2_max)) bad |= GUARD_RADIUS;
+        if (bad) atomicOr(a.g.flag, bad);
+    }
+    keys[threadIdx.x] = key;
+    __syncthreads();
+    if (i < a.n) {
+        int rank = 0;
+        for (int t = 0; t < (int)threadIdx.x; ++t) rank += keys[t] == key ? 1 : 0;
+        const int slot = a.table[(size_t)blockIdx.x * (a.n_class + 1) + key] + rank;
+        a.cols[slot] = make_float4(b.x, b.y, b.z, __uint_as_float(key == a.n_class ? CLS_SKIP : (uint32_t)key));
+        a.s2c[slot] = i;
+    }
+}
+
 __global__ __launch_bounds__(1024) void q0_kernel(const float4* __restrict__ cols, int n,
                                                   const double* __restrict__ f0col, double* out)
 {
@@ -1799,6 +1894,18 @@ hipError_t launch_gather_cols(float4* cols, const float4* bodies, const int* src
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(gather_cols_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cols, bodies, src,
                        counts, d1, n, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_class_sort(const ClassSortArgs& a, hipStream_t s)
+{
+    if (a.n <= 0) return hipSuccess;
+    if (a.n_class < 1 || a.n_class + 1 > CLASS_SORT_MAX_KEYS) return hipErrorInvalidValue;
+    const int n_blk = (a.n + CS_BLOCK - 1) / CS_BLOCK;
+    hipLaunchKernelGGL(class_hist_kernel, dim3((unsigned)n_blk), dim3(CS_BLOCK), (size_t)(a.n_class + 1) * sizeof(int),
+                       s, a);
+    hipLaunchKernelGGL(class_scan_kernel, dim3(1), dim3(1024), 0, s, a, n_blk);
+    hipLaunchKernelGGL(class_scatter_kernel, dim3((unsigned)n_blk), dim3(CS_BLOCK), 0, s, a);
     return hipGetLastError();
 }
 

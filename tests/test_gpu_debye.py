@@ -326,6 +326,24 @@ def test_device_resident_forward_on_new_positions(oracle, hydrate):
     err = rel_err(got, ref).max()
     _say(f"  max rel err {err:.3e}")
     assert err < TOL
+    # the engine's incremental state follows the host bodies again after device steps: an
+    # update, a commit, another device step and a revert
+    moved = w.bodies.copy()
+    moved[:40, :3] += np.float32(0.25)
+    I_inc = e.recalc_curve(moved, 0, 40)
+    f = engine_for(w.q, w.table, w.water_index, wt, n=len(w.pos), radii=w.radii, probe=w.probe, n_half=w.n_half)
+    I_full = f.calc_curve(moved)
+    f.close()
+    assert rel_err(I_inc, I_full).max() < 1e-6
+    e.commit()
+    _dev_forward(e, oracle.bodies(pos, w.types))
+    e.revert()
+    assert np.array_equal(e.recalc_curve(moved, 0, 0), I_inc)
+    moved[100:130, :3] -= np.float32(0.2)
+    I_inc2 = e.recalc_curve(moved, 100, 30)
+    g = engine_for(w.q, w.table, w.water_index, wt, n=len(w.pos), radii=w.radii, probe=w.probe, n_half=w.n_half)
+    assert rel_err(I_inc2, g.calc_curve(moved)).max() < 1e-6
+    g.close()
     e.close()
 
 
