@@ -237,6 +237,9 @@ def ref_lib():
             getattr(L, name).argtypes = [f, f, _fp, i]
         L.ref_density_dots.argtypes = [f, f, f, _fp, i]
         L.ref_rsasa_cpu.argtypes = [_fp, i, f, f, _dp]
+        L.ref_pdb_read.argtypes = [ctypes.c_char_p, i, _fp, ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong),
+                                   ctypes.POINTER(ctypes.c_int), i, ctypes.POINTER(ctypes.c_int)]
+        L.ref_pdb_atom_type.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
         _ref = L
     return _ref
 
@@ -298,3 +301,23 @@ def ref_rsasa_cpu(b, density, probe):
                                  _ptr(out, ctypes.c_double))
     assert rc == 0
     return out
+
+
+def ref_pdb_read(path, all_models=True):
+    """parse_pdb<float> of the reference: (bodies [N, 4], offsets [M + 1]), or the error code."""
+    L = ref_lib()
+    n, m = ctypes.c_longlong(), ctypes.c_int()
+    cap, ocap = 1 << 20, 4096
+    out = np.zeros((cap, 4), dtype=np.float32)
+    off = np.zeros(ocap, dtype=np.int32)
+    rc = L.ref_pdb_read(str(path).encode(), int(all_models), _ptr(out, ctypes.c_float), cap, ctypes.byref(n),
+                        off.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ocap, ctypes.byref(m))
+    if rc != 0:
+        return rc
+    assert n.value <= cap and m.value < ocap
+    return out[:n.value].copy(), off[:m.value + 1].copy()
+
+
+def ref_pdb_atom_type(label, res, element=None):
+    """map_pdb_atom_to_ff_type of the reference (element symbol, or None for the label overload)."""
+    return ref_lib().ref_pdb_atom_type(label.encode(), res.encode(), None if element is None else element.encode())

@@ -12,6 +12,7 @@
 #include "utils.hpp"
 #include "ff_coef.hpp"
 #include "host_debye.hpp"
+#include "parse_pdb.hpp"
 #include "rsasa_algorithm.hpp"
 
 #include <memory>
@@ -106,6 +107,51 @@ int ref_rsasa_cpu(const float* bodies, int n, float density, float probe, double
     alg->calc_rsasa(to_bodies(bodies, n), frac);
     for (int i = 0; i < n; ++i) out_frac[i] = frac[i];
     return 0;
+}
+
+// parse_pdb<float>::first_model_as_complex_atoms / all_models_as_complex_atoms
+// (parse_pdb.cpp:35-104, records parse_pdb.cpp:302-364), quiet.  Bodies of model m are
+// out[offsets[m] .. offsets[m+1]); returns 0, or the resaxs::error code.
+int ref_pdb_read(const char* path, int all_models, float* out, long long cap, long long* n_atoms, int* offsets,
+                 int off_cap, int* n_models)
+{
+    try {
+        resaxs::parse_pdb<float> p(path);
+        p.set_verbose_level(resaxs::parse_pdb<float>::QUIET);
+        std::vector<std::vector<cl_float4>> models;
+        if (all_models)
+            models = p.all_models_as_complex_atoms();
+        else
+            models.push_back(p.first_model_as_complex_atoms());
+        long long k = 0;
+        int m = 0;
+        for (const auto& mod : models) {
+            if (m < off_cap) offsets[m] = (int)k;
+            for (const auto& b : mod) {
+                if (k < cap)
+                    for (int c = 0; c < 4; ++c) out[4 * k + c] = b.s[c];
+                ++k;
+            }
+            ++m;
+        }
+        if (m < off_cap) offsets[m] = (int)k;
+        *n_atoms = k;
+        *n_models = m;
+        return 0;
+    } catch (const resaxs::error& e) {
+        return e.err();
+    }
+}
+
+// map_pdb_atom_to_ff_type (utils.cpp:283-314): element NULL -> the label overload (element
+// from the label's first letter); else the element symbol through pdb_to_ff_simple_atom_map,
+// an unknown symbol typed N as parse_pdb.cpp:329-337 does
+int ref_pdb_atom_type(const char* label, const char* res, const char* element)
+{
+    if (!element) return resaxs::map_pdb_atom_to_ff_type(std::string(label), std::string(res));
+    const auto it = resaxs::pdb_to_ff_simple_atom_map.find(element);
+    const resaxs::atoms::atom_type el = it != resaxs::pdb_to_ff_simple_atom_map.end() ? it->second : resaxs::atoms::N;
+    return resaxs::map_pdb_atom_to_ff_type(el, std::string(label), std::string(res));
 }
 
 }  // extern "C"

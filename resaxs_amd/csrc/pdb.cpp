@@ -29,6 +29,18 @@ std::string trim_spaces(const std::string& s)
     return s.substr(b, e - b + 1);
 }
 
+// The reference's trim of a record field (utils.cpp:319-325): spaces removed at both ends,
+// except that a field of spaces only keeps all but its last character.  Typing depends on
+// it: a blank element column reads as " ", an unknown element (typed N), while an element
+// column the line does not reach reads as "" and the label's first letter is used.
+std::string trim_field(const std::string& s)
+{
+    const size_t b = s.find_first_not_of(' ');
+    if (b == std::string::npos) return s.empty() ? s : s.substr(0, s.size() - 1);
+    const size_t e = s.find_last_not_of(' ');
+    return s.substr(b, e - b + 1);
+}
+
 // element symbol -> type; anything else is typed N by the caller (utils.cpp:201-210)
 bool element_type(const std::string& el, int& t)
 {
@@ -88,10 +100,10 @@ struct Atom {
 // sscanf format reads them); false when the record is skipped
 bool parse_atom(const std::string& line, int verbose, Atom& out)
 {
-    const std::string label = trim_spaces(field(line, 12, 4));
+    const std::string label = trim_field(field(line, 12, 4));
     const char alt = line.size() > 16 ? line[16] : ' ';
-    const std::string res = trim_spaces(field(line, 17, 3));
-    std::string el = trim_spaces(field(line, 76, 2));
+    const std::string res = trim_field(field(line, 17, 3));
+    std::string el = line.size() >= 78 ? trim_field(line.substr(76, 2)) : std::string();
     if (el.empty()) el = label.substr(0, 1);
     if (res == "HOH") return false;               // waters
     if (alt != ' ' && alt != 'A') return false;   // extra alternate locations

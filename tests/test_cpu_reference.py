@@ -11,7 +11,9 @@ What this pins, bit for bit:
   create_fibonacci_dots / create_dodeca_icosa_dots (rsasa_algorithm.cpp:96-208)
       == orc_*_dots and the library's generators;
   rsasa_cpu_alg::calc_rsasa (rsasa_cpu_alg.cpp:112-185, voxel-pruned) exposed
-      counts == the oracle's brute-force counts over the same dot spheres.
+      counts == the oracle's brute-force counts over the same dot spheres;
+  parse_pdb<float> (parse_pdb.cpp:35-104, 302-364) and map_pdb_atom_to_ff_type
+      (utils.cpp:201-314) == the library's rsx_pdb_read / rsx_pdb_atom_type.
 """
 import os
 import sys
@@ -113,3 +115,82 @@ def test_rsasa_cpu_counts_equal_oracle():
             assert np.allclose(ref_counts / nd[types], frac, rtol=0, atol=1e-12)
             got = po.sasa_counts(pos, types, R2, dots, cells=False).astype(np.int64)
             assert np.array_equal(ref_counts, got), (density, n)
+
+
+# residues and atom labels whose cross product covers the typing tables' default labels and
+# every residue exception, plus labels and residues the tables do not name
+PDB_RESIDUES = ["ALA", "ARG", "ASN", "ASP", "CYS", "GLN", "GLU", "GLY", "HIS", "ILE", "LEU", "LYS", "MET", "PHE",
+                "PRO", "SER", "THR", "TRP", "TYR", "VAL", "DA", "HEM", "UNK", "ZN"]
+PDB_LABELS = ["C", "CA", "CB", "CG", "CG1", "CG2", "CD", "CD1", "CD2", "CE", "CE1", "CE2", "CE3", "CZ", "CZ2", "CZ3",
+              "CH2", "CH3", "N", "ND1", "ND2", "NE", "NE1", "NE2", "NH1", "NH2", "NZ", "O", "OD1", "OD2", "OE1", "OE2",
+              "OG", "OG1", "OH", "OXT", "SD", "SG", "P", "ZN", "FE", "H", "HA", "SE"]
+
+
+def _pdb_line(rec, serial, name, res, x, y, z, element=None, alt=" ", chain="A", resseq=1):
+    nm = f" {name:<3}" if len(name) < 4 else name
+    line = (f"{rec:<6}{serial:>5} {nm:<4}{alt:1}{res:>3} {chain:1}{resseq:>4}    "
+            f"{x:8.3f}{y:8.3f}{z:8.3f}{1.0:6.2f}{0.0:6.2f}")
+    return line if element is None else line + f"          {element:>2}  "
+
+
+def test_pdb_atom_typing_equals_reference():
+    from resaxs_amd import pdb_atom_type
+    for res in PDB_RESIDUES:
+        for label in PDB_LABELS:
+            assert pdb_atom_type(label, res) == po.ref_pdb_atom_type(label, res), (label, res)
+            for el in ("H", "C", "N", "O", "P", "S", "Zn", "ZN", "FE", "Se", label[:1]):
+                assert pdb_atom_type(label, res, el) == po.ref_pdb_atom_type(label, res, el), (label, res, el)
+
+
+def test_pdb_reader_equals_reference(tmp_path):
+    """Synthetic PDB text: two MODELs, records before the first MODEL, every residue x label of
+    the typing cross product (with and without an element column, ATOM and HETATM), alternate
+    locations blank / A / B / C, waters, hydrogens, unknown and two-letter elements, an END
+    with records after it, a line over 99 characters and a last line without a newline.
+    Records are padded to the format's 80 columns: the reference takes a record's name from
+    6 bytes of its line buffer (parse_pdb.cpp:47, 88), which for a line shorter than that
+    includes bytes of the previous line, so an unpadded END after ENDMDL is no END there."""
+    from resaxs_amd import read_pdb
+    rng = np.random.default_rng(404)
+    lines, serial = [], 0
+
+    def rec(*args, **kw):
+        nonlocal serial
+        serial += 1
+        x, y, z = rng.uniform(-99, 99, 3)
+        lines.append(_pdb_line(args[0], serial, args[1], args[2], x, y, z, **kw))
+
+    rec("ATOM", "CA", "GLY", element="C")                     # before any MODEL
+    lines.append("MODEL        1")
+    for res in PDB_RESIDUES:
+        for k, label in enumerate(PDB_LABELS):
+            el = None if k % 3 == 0 else ("ZN" if label == "ZN" else label[:1])
+            rec("HETATM" if k % 7 == 0 else "ATOM", label, res, element=el,
+                alt=" ABC"[k % 4] if k % 5 == 0 else " ")
+    rec("HETATM", "O", "HOH", element="O")
+    rec("ATOM", "FE", "HEM", element="FE")
+    lines.append("ENDMDL")
+    lines.append("MODEL        2")
+    for label in PDB_LABELS[:12]:
+        rec("ATOM", label, "LYS", element=label[:1])
+    lines.append(_pdb_line("ATOM", 9999, "CB", "ALA", 1.0, 2.0, 3.0, element="C") + " " * 40)  # > 99 chars
+    rec("ATOM", "CG", "LEU", element="C")
+    lines.append("ENDMDL")
+    lines = [ln.ljust(80) for ln in lines]
+    tail = [ln.ljust(80) for ln in ("END", _pdb_line("ATOM", 1, "N", "GLY", 0.0, 0.0, 0.0, element="N"))]
+    path = tmp_path / "synthetic.pdb"
+    path.write_text("\n".join(lines + tail) + "\n")
+    for all_models in (True, False):
+        b, off = read_pdb(str(path), all_models=all_models)
+        rb, roff = po.ref_pdb_read(path, all_models)
+        assert np.array_equal(off, roff), all_models
+        assert np.array_equal(b.view(np.uint32), rb.view(np.uint32)), all_models
+        if all_models:
+            assert len(roff) == 4 and roff[2] - roff[1] > 700   # three models, the big one typed
+    # no END and no newline after the last record
+    path.write_text("\n".join(lines))
+    for all_models in (True, False):
+        b, off = read_pdb(str(path), all_models=all_models)
+        rb, roff = po.ref_pdb_read(path, all_models)
+        assert np.array_equal(off, roff) and np.array_equal(b.view(np.uint32), rb.view(np.uint32)), all_models
+    assert po.ref_pdb_read(tmp_path / "missing.pdb") == -1100
