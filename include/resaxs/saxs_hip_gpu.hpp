@@ -242,6 +242,45 @@ public:
         detail::check(rsx_revert(h_), h_, "revert");
     }
 
+    // ---- differentiable fit (no reference counterpart; the scale and chi of
+    // saxs_profile.hpp:137-166) ----
+    // chi^2 = sum_k (I_exp,k - c I_k)^2 / sigma_k^2 of the bodies' I(q) at the least-squares
+    // scale c, and grad = d chi^2 / dx [n][3] (x, y, z per body) with c held fixed (it is
+    // optimal).  The bodies become the engine's structure, as with calc_curve.  Returns chi^2.
+    double chi2_grad(const std::vector<real4>& bodies, const std::vector<FLT_T>& i_exp,
+                     const std::vector<FLT_T>& sigma, double& scale, std::vector<FLT_T>& grad)
+    {
+        verify(this->args_set(), error::SAXS_ARGS_NOT_SET);
+        verify(bodies.size() == params_.n_bodies_ && i_exp.size() == (size_t)nq_ && sigma.size() == (size_t)nq_,
+               error::SAXS_INVALID_ARG, "RESAXS::error -- chi2_grad: sizes do not match the engine");
+        push_factors();
+        push_water();
+        std::vector<float> ie(i_exp.begin(), i_exp.end()), sg(sigma.begin(), sigma.end()), g(bodies.size() * 3);
+        double chi2 = 0.0;
+        detail::check(rsx_chi2_grad(h_, body_ptr(bodies), (int)bodies.size(), ie.data(), sg.data(), &chi2, &scale,
+                                    g.data(), nullptr),
+                      h_, "chi2_grad");
+        params_.clear_all();
+        this->state = this->alg_computing;
+        grad.assign(g.begin(), g.end());
+        return chi2;
+    }
+
+    // vector-Jacobian product grad_i = sum_k w_k dI_k / dx_i, grad [n][3] (the backward of I(q))
+    void vjp(const std::vector<real4>& bodies, const std::vector<double>& w, std::vector<FLT_T>& grad)
+    {
+        verify(this->args_set(), error::SAXS_ARGS_NOT_SET);
+        verify(bodies.size() == params_.n_bodies_ && w.size() == (size_t)nq_, error::SAXS_INVALID_ARG,
+               "RESAXS::error -- vjp: sizes do not match the engine");
+        push_factors();
+        push_water();
+        std::vector<float> g(bodies.size() * 3);
+        detail::check(rsx_debye_vjp(h_, body_ptr(bodies), (int)bodies.size(), w.data(), g.data()), h_, "vjp");
+        params_.clear_all();
+        this->state = this->alg_computing;
+        grad.assign(g.begin(), g.end());
+    }
+
     // the engine handle, for callers that mix this object with the C ABI
     rsx_engine* engine() const { return h_; }
 };

@@ -209,6 +209,17 @@ int rsx_chi2_grad(rsx_engine *h, const float *bodies, int n, const float *i_exp,
                   double *out_chi2, double *out_scale, float *out_grad, double *out_iq);
 /* adjoint only: grad_i = sum_k w_k dI_k/dx_i for caller weights w [n_q] (backward of I(q)) */
 int rsx_debye_vjp(rsx_engine *h, const float *bodies, int n, const double *w, float *out_grad);
+/* device-resident forms of the two above (no host copies; asynchronous on the engine stream,
+ * rsx_synchronize reports the bodies' checks as for rsx_calc_curve_dev_async).  They reuse the
+ * layout of the last host call (RSX_NO_INITIAL_CALC before one, RSX_ARGS_NOT_SET when the
+ * parameters changed since) and need a uniform q grid.
+ * vjp: grad_dev [n][3] FP32 = sum_k w_dev[k] dI_k/dx_i, w_dev [n_q] FP64.
+ * chi2: i_exp_dev, sigma_dev [n_q] FP32; out_dev = {chi2, scale} FP64; grad_dev [n][3] FP32 (may be
+ * NULL); iq_dev [n_q] FP64 (may be NULL).  Needs the gradient configuration of the forward, which
+ * a host rsx_chi2_grad sets up (RSX_ARGS_NOT_SET otherwise). */
+int rsx_debye_vjp_dev_async(rsx_engine *h, const void *bodies_dev, int n, const double *w_dev, float *grad_dev);
+int rsx_chi2_grad_dev_async(rsx_engine *h, const void *bodies_dev, int n, const float *i_exp_dev,
+                            const float *sigma_dev, double *out_dev, float *grad_dev, double *iq_dev);
 /* adjoint kernel: fp32 1 = FP32 difference-form recurrence (default), 0 = FP64 recurrence;
  * k q values per thread (0 = automatic); col_range columns per job (0 = 256) */
 int rsx_set_adjoint_tuning(rsx_engine *h, int fp32, int k, int col_range);

@@ -78,6 +78,8 @@ SIGNATURES = {
     "rsx_set_persistent": (_i, [_vp, _i]),
     "rsx_chi2_grad": (_i, [_vp, _fp, _i, _fp, _fp, _dp, _dp, _fp, _dp]),
     "rsx_debye_vjp": (_i, [_vp, _fp, _i, _dp, _fp]),
+    "rsx_debye_vjp_dev_async": (_i, [_vp, _vp, _i, _vp, _vp]),
+    "rsx_chi2_grad_dev_async": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp]),
     "rsx_autotune": (_i, [_vp, _c.c_char_p]),
     "rsx_set_tuning": (_i, [_vp, _i, _i]),
     "rsx_set_variant": (_i, [_vp, _i]),

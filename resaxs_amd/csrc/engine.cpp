@@ -2200,8 +2200,9 @@ void adjoint_layout(rsx_engine* h)
     h->a_fp32 = fp32;
 }
 
-// adjoint kernel + reduction on the weights already in d_wq; grad [n][3] (caller order) -> d_grad
-void launch_adjoint_pass(rsx_engine* h)
+// adjoint kernel + reduction on the weights already in d_wq; grad [n][3] (caller order) -> grad
+// (d_grad when null)
+void launch_adjoint_pass(rsx_engine* h, float* grad = nullptr)
 {
     rsx::AdjointArgs a;
     a.jobs = h->a_jobs.p;
@@ -2235,7 +2236,7 @@ void launch_adjoint_pass(rsx_engine* h)
     r.n_chunks = h->a_nch;
     r.col_range = h->a_L;
     r.slot_to_caller = h->d_s2c.p;
-    r.grad = h->d_grad.p;
+    r.grad = grad ? grad : h->d_grad.p;
     hipchk(rsx::launch_adjoint_reduce(r, h->stream), "adjoint reduce");
 }
 
@@ -2260,9 +2261,10 @@ void adjoint_impl(rsx_engine* h, const std::vector<double>& wk, float* out_grad)
     hipchk(hipStreamSynchronize(h->stream), "sync adjoint");
 }
 
-// Device-resident forward step: (SASA) -> class records in slot order -> pair
-// kernel -> fixed-order reduction.  No host synchronisation.
-void enqueue_device_forward(rsx_engine* h, const float4* bodies_dev, double* out)
+// Device-resident bodies -> class records in slot order (d_cols, and d_s2c when sorted on
+// the device), with the exposed-dot counts taken on the device for hydration layouts and the
+// bodies' checks queued for rsx_synchronize.  No host synchronisation.
+void enqueue_device_columns(rsx_engine* h, const float4* bodies_dev)
 {
     const int n = h->n;
     const uint32_t* counts = nullptr;
@@ -2313,6 +2315,14 @@ void enqueue_device_forward(rsx_engine* h, const float4* bodies_dev, double* out
         hipchk(rsx::launch_gather_cols(h->d_cols.p, bodies_dev, h->d_s2c.p, counts, h->d1, n, g, h->stream),
                "gather");
     }
+}
+
+// Device-resident forward step: (SASA) -> class records in slot order -> pair kernel ->
+// fixed-order reduction into out [n_q].  No host synchronisation.
+void enqueue_device_forward(rsx_engine* h, const float4* bodies_dev, double* out)
+{
+    const int n = h->n;
+    enqueue_device_columns(h, bodies_dev);
     hipchk(rsx::launch_q0(h->d_cols.p, n, h->d_f0col.p, h->d_q0.p, h->stream), "q0 kernel");
     const int nj = (int)h->last_jobs.size();
     const rsx::DebyeArgs a = debye_args(h, h->d_fjobs.p, h->d_partials.p);
@@ -2968,6 +2978,59 @@ int rsx_debye_vjp(rsx_engine* h, const float* bodies, int n, const double* w, fl
         calc_curve_impl(h, bodies, n, nullptr, nullptr);
         std::vector<double> wk(w, w + h->nq);
         adjoint_impl(h, wk, out_grad);
+    });
+}
+
+// the engine's forward runs in the gradient configuration (that of rsx_chi2_grad: FP32 runs
+// of at most GRAD_K_MAX q, or an FP64 recurrence the caller chose)
+static bool gradient_forward_ready(const rsx_engine* h)
+{
+    if (h->variant == rsx::VARIANT_WAVE || h->variant == rsx::VARIANT_SHARED) return true;
+    return h->variant == rsx::VARIANT_FP32 && h->k_max == GRAD_K_MAX && h->K <= GRAD_K_MAX;
+}
+
+static void check_device_gradient(rsx_engine* h, const void* bodies_dev, int n)
+{
+    check_args_set(h);
+    if (n != h->n || !bodies_dev) fail(RSX_INVALID_ARG, "gradient_dev: body count mismatch");
+    if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "gradient_dev needs a host call to lay out the atoms");
+    if (h->params_dirty)
+        fail(RSX_ARGS_NOT_SET, "gradient_dev: parameters changed since the last host call (call it first)");
+    if (!h->uniform) fail(RSX_NOT_SUPPORTED, "gradient_dev needs a uniform q grid");
+}
+
+int rsx_debye_vjp_dev_async(rsx_engine* h, const void* bodies_dev, int n, const double* w_dev, float* grad_dev)
+{
+    return guarded(h, [&] {
+        if (!w_dev || !grad_dev) fail(RSX_INVALID_ARG, "vjp_dev: null argument");
+        check_device_gradient(h, bodies_dev, n);
+        set_device(h);
+        adjoint_layout(h);
+        enqueue_device_columns(h, (const float4*)bodies_dev);
+        hipchk(rsx::launch_adjoint_weights(w_dev, h->nq, h->q0_index, h->a_ldf, h->d_wq.p, h->stream),
+               "adjoint weights");
+        launch_adjoint_pass(h, grad_dev);
+        h->a_have_w = true;
+    });
+}
+
+int rsx_chi2_grad_dev_async(rsx_engine* h, const void* bodies_dev, int n, const float* i_exp_dev,
+                            const float* sigma_dev, double* out_dev, float* grad_dev, double* iq_dev)
+{
+    return guarded(h, [&] {
+        if (!i_exp_dev || !sigma_dev || !out_dev) fail(RSX_INVALID_ARG, "chi2_grad_dev: null argument");
+        check_device_gradient(h, bodies_dev, n);
+        if (!gradient_forward_ready(h))
+            fail(RSX_ARGS_NOT_SET, "chi2_grad_dev: lay the engine out for gradients with a host rsx_chi2_grad first");
+        set_device(h);
+        adjoint_layout(h);
+        double* iq = iq_dev ? iq_dev : h->d_out.p;
+        enqueue_device_forward(h, (const float4*)bodies_dev, iq);
+        hipchk(rsx::launch_chi2_weights(iq, i_exp_dev, sigma_dev, h->nq, h->q0_index, h->a_ldf, out_dev, h->d_wq.p,
+                                        h->stream),
+               "chi2 weights");
+        if (grad_dev) launch_adjoint_pass(h, grad_dev);
+        h->a_have_w = true;
     });
 }
 

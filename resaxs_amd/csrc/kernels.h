@@ -219,6 +219,15 @@ struct AdjointReduceArgs {
 // grad of every atom: its row and column partials over all chunks, fixed order
 hipError_t launch_adjoint_reduce(const AdjointReduceArgs& a, hipStream_t s);
 
+// Adjoint weights on the device.  wq[0 .. ldfa): wq[k] = w[k] for k < n_q except the q = 0
+// entry (q0_index >= 0), 0 there and in the padding.
+hipError_t launch_adjoint_weights(const double* w, int n_q, int q0_index, int ldfa, double* wq, hipStream_t s);
+// chi^2 at the least-squares scale, one workgroup, fixed-order FP64 sums over k < n_q:
+// c = sum I_exp I / sigma^2 / sum I^2 / sigma^2, chi2 = sum (I_exp - c I)^2 / sigma^2;
+// out = {chi2, c}; wq as launch_adjoint_weights with w_k = -2 c (I_exp,k - c I_k) / sigma_k^2.
+hipError_t launch_chi2_weights(const double* iq, const float* i_exp, const float* sigma, int n_q, int q0_index,
+                               int ldfa, double* out, double* wq, hipStream_t s);
+
 // out[0] = (sum_s f0col[class(cols[s])])^2, fixed-order FP64 sum (the q = 0 value)
 hipError_t launch_q0(const float4* cols, int n, const double* f0col, double* out, hipStream_t s);
 

@@ -941,6 +941,59 @@ __global__ void gather_cols_kernel(float4* cols, const float4* __restrict__ bodi
     cols[s] = make_float4(b.x, b.y, b.z, __uint_as_float(cls));
 }
 
+// ---- adjoint weights on the device ----
+__global__ void adjoint_weights_kernel(const double* __restrict__ w, int n_q, int q0_index, int ldfa, double* wq)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ldfa) wq[k] = (k < n_q && k != q0_index) ? w[k] : 0.0;
+}
+
+// one workgroup of 1024: each thread sums its q values in index order, then a fixed-shape
+// tree over the lanes and waves, so the result repeats bit for bit
+__device__ __forceinline__ double block_sum_1024(double v, double* part)
+{
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int u = 0; u < 16; ++u) t += part[u];
+    __syncthreads();
+    return t;
+}
+
+__global__ __launch_bounds__(1024) void chi2_weights_kernel(const double* __restrict__ iq,
+                                                           const float* __restrict__ i_exp,
+                                                           const float* __restrict__ sigma, int n_q, int q0_index,
+                                                           int ldfa, double* out, double* wq)
+{
+    __shared__ double part[16];
+    double num = 0.0, den = 0.0;
+    for (int k = threadIdx.x; k < n_q; k += 1024) {
+        const double p2 = 1.0 / ((double)sigma[k] * (double)sigma[k]);
+        num += (double)i_exp[k] * iq[k] * p2;
+        den += iq[k] * iq[k] * p2;
+    }
+    num = block_sum_1024(num, part);
+    den = block_sum_1024(den, part);
+    const double c = num / den;
+    double x2 = 0.0;
+    for (int k = threadIdx.x; k < ldfa; k += 1024) {
+        double wk = 0.0;
+        if (k < n_q) {
+            const double s2 = (double)sigma[k] * (double)sigma[k];
+            const double d = (double)i_exp[k] - c * iq[k];
+            x2 += d * d / s2;
+            if (k != q0_index) wk = -2.0 * c * d / s2;
+        }
+        wq[k] = wk;
+    }
+    x2 = block_sum_1024(x2, part);
+    if (threadIdx.x == 0) {
+        out[0] = x2;
+        out[1] = c;
+    }
+}
+
 // ---- stable counting sort by class (device-resident hydrated steps) ----
 constexpr int CS_BLOCK = 1024;
 
@@ -1894,6 +1947,21 @@ hipError_t launch_gather_cols(float4* cols, const float4* bodies, const int* src
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(gather_cols_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cols, bodies, src,
                        counts, d1, n, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_weights(const double* w, int n_q, int q0_index, int ldfa, double* wq, hipStream_t s)
+{
+    if (ldfa <= 0) return hipSuccess;
+    hipLaunchKernelGGL(adjoint_weights_kernel, dim3((unsigned)((ldfa + 255) / 256)), dim3(256), 0, s, w, n_q, q0_index,
+                       ldfa, wq);
+    return hipGetLastError();
+}
+
+hipError_t launch_chi2_weights(const double* iq, const float* i_exp, const float* sigma, int n_q, int q0_index,
+                               int ldfa, double* out, double* wq, hipStream_t s)
+{
+    hipLaunchKernelGGL(chi2_weights_kernel, dim3(1), dim3(1024), 0, s, iq, i_exp, sigma, n_q, q0_index, ldfa, out, wq);
     return hipGetLastError();
 }
 

@@ -226,6 +226,15 @@ class Engine:
                                           _ptr(grad, ctypes.c_float)))
         return grad
 
+    def vjp_dev_async(self, bodies_dev, n, w_dev, grad_dev):
+        """rsx_debye_vjp_dev_async on device pointers (ints or c_void_p)."""
+        self._check(self._L.rsx_debye_vjp_dev_async(self.h, bodies_dev, int(n), w_dev, grad_dev))
+
+    def chi2_grad_dev_async(self, bodies_dev, n, i_exp_dev, sigma_dev, out_dev, grad_dev=None, iq_dev=None):
+        """rsx_chi2_grad_dev_async on device pointers: out_dev receives {chi2, scale} (FP64)."""
+        self._check(self._L.rsx_chi2_grad_dev_async(self.h, bodies_dev, int(n), i_exp_dev, sigma_dev, out_dev,
+                                                     grad_dev, iq_dev))
+
     # -- measurement -----------------------------------------------------------
     def time_forward(self, reps):
         tot, pair = ctypes.c_float(), ctypes.c_float()

@@ -243,6 +243,57 @@ static int run_gpu(const std::string& dir)
         write_bin(dir + "/iq_double.bin", id);
     }
 
+    // ---- the differentiable-fit extension of the engine class: chi^2 and its gradient, the VJP;
+    // the gradient against central differences of chi^2 (h = 0.05 A) on the body with the
+    // largest gradient
+    {
+        saxs_hip_gpu<float> g;
+        g.access_params().initialize(q, bodies, devices);
+        g.initialize();
+        std::vector<float> i0;
+        g.calc_curve(bodies, i0);
+        std::vector<float> i_exp(i0.size()), sigma(i0.size());
+        for (size_t k = 0; k < i0.size(); ++k) {
+            i_exp[k] = i0[k] * (1.0f + 0.02f * std::sin(0.7f * (float)k));
+            sigma[k] = 0.01f * i0[k];
+        }
+        std::vector<cl_float4> moved = bodies;
+        moved[0].s[0] += 0.3f;
+        double scale = 0.0;
+        std::vector<float> grad;
+        const double chi2 = g.chi2_grad(moved, i_exp, sigma, scale, grad);
+        CHECK(grad.size() == 3 * bodies.size() && chi2 > 0.0 && scale > 0.0, "chi2_grad sizes and values");
+        size_t top = 0;
+        for (size_t i = 0; i < bodies.size(); ++i)
+            if (std::fabs(grad[3 * i]) + std::fabs(grad[3 * i + 1]) + std::fabs(grad[3 * i + 2]) >
+                std::fabs(grad[3 * top]) + std::fabs(grad[3 * top + 1]) + std::fabs(grad[3 * top + 2]))
+                top = i;
+        for (int c = 0; c < 3; ++c) {
+            const float h = 0.05f;
+            std::vector<cl_float4> p = moved, m = moved;
+            p[top].s[c] += h;
+            m[top].s[c] -= h;
+            double sp = 0.0, sm = 0.0;
+            std::vector<float> gp, gm;
+            const double cp = g.chi2_grad(p, i_exp, sigma, sp, gp), cm = g.chi2_grad(m, i_exp, sigma, sm, gm);
+            const double fd = (cp - cm) / (2.0 * h);
+            const double gmax = std::fabs(grad[3 * top]) + std::fabs(grad[3 * top + 1]) + std::fabs(grad[3 * top + 2]);
+            std::printf("  body %zu axis %d: gradient %.6g, central difference %.6g\n", top, c, grad[3 * top + c], fd);
+            CHECK(std::fabs(fd - grad[3 * top + c]) <= 1e-2 * gmax, "chi2 gradient vs central difference");
+        }
+        std::vector<double> w(q.size());
+        for (size_t k = 0; k < w.size(); ++k) w[k] = (double)k / (double)w.size() - 0.5;
+        std::vector<float> gv;
+        g.vjp(moved, w, gv);
+        std::vector<float> gc(3 * bodies.size());
+        std::vector<float> flat(4 * bodies.size());
+        for (size_t i = 0; i < bodies.size(); ++i)
+            for (int c = 0; c < 4; ++c) flat[4 * i + c] = moved[i].s[c];
+        detail::check(rsx_debye_vjp(g.engine(), flat.data(), (int)bodies.size(), w.data(), gc.data()), g.engine(),
+                      "vjp");
+        CHECK(gv == gc, "vjp equals the C ABI's");
+    }
+
     // ---- the saxs_hist_cl_gpu slot: the exact sum, as the reference's class of that name
     {
         std::unique_ptr<i_saxs<float, cl_base<float>>> sg(

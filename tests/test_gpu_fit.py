@@ -107,3 +107,77 @@ def test_refinement_100_steps():
     # the start (0.5 A random displacements) already fits to chi^2 / n_q ~ 2; 1 % noise puts
     # the floor near chi^2 = n_q: the refinement must get there
     assert hist[-1] < hist[0] and hist[-1] < 1.25 * len(w.q)
+
+
+def test_device_gradients_equal_host(oracle):
+    """rsx_debye_vjp_dev_async and rsx_chi2_grad_dev_async on device-resident bodies against
+    the host entry points: the same layout gives the same gradient, and on moved positions
+    the device path matches a host call on those positions."""
+    from resaxs_amd import configs
+    rng = np.random.default_rng(81)
+    n = 2000
+    q = np.linspace(0.01, 0.5, 64).astype(np.float32)
+    pos = configs.in_ellipsoid(rng, n, configs.ellipsoid_axes(n)).astype(np.float32)
+    types = configs.seeded_types(rng, n)
+    table = configs.factor_table(rng, q)
+    _say("case: device-resident VJP and chi2 gradient, 2000 atoms x 64 q")
+    e = engine_for(q, table, n=n)
+    b = configs.bodies_of(pos, types)
+    I_ref = oracle.debye(q, pos, oracle.body_factors(pos, types, q, table))
+    i_exp = (I_ref * (1 + 0.01 * rng.normal(size=len(q)))).astype(np.float32)
+    sigma = (0.01 * I_ref).astype(np.float32)
+    chi2_h, scale_h, grad_h, iq_h = e.chi2_grad(b, i_exp, sigma)   # host call: gradient layout
+    w = np.linspace(-1.0, 1.0, len(q))
+    g_vjp_h = e.vjp(b, w)
+    bufs = {k: e.dev_alloc(v) for k, v in (("b", b.nbytes), ("w", w.nbytes), ("g", n * 12), ("ie", i_exp.nbytes),
+                                           ("sg", sigma.nbytes), ("out", 16), ("iq", len(q) * 8))}
+    try:
+        e.h2d(bufs["b"], b)
+        e.h2d(bufs["w"], w)
+        e.h2d(bufs["ie"], i_exp)
+        e.h2d(bufs["sg"], sigma)
+        e.vjp_dev_async(bufs["b"], n, bufs["w"], bufs["g"])
+        e.synchronize()
+        g = e.d2h(np.zeros((n, 3), np.float32), bufs["g"])
+        assert np.array_equal(g, g_vjp_h)
+        e.chi2_grad_dev_async(bufs["b"], n, bufs["ie"], bufs["sg"], bufs["out"], bufs["g"], bufs["iq"])
+        e.synchronize()
+        out = e.d2h(np.zeros(2), bufs["out"])
+        g = e.d2h(np.zeros((n, 3), np.float32), bufs["g"])
+        iq = e.d2h(np.zeros(len(q)), bufs["iq"])
+        assert abs(out[0] / chi2_h - 1) < 1e-9 and abs(out[1] / scale_h - 1) < 1e-12
+        assert np.all(np.abs(iq - iq_h) <= 1e-12 * np.abs(iq_h))
+        assert np.linalg.norm(g - grad_h) / np.linalg.norm(grad_h) < 1e-6
+        # moved positions: the device path against host calls on them
+        moved = (pos + rng.uniform(-0.2, 0.2, pos.shape)).astype(np.float32)
+        bm = configs.bodies_of(0.98 * moved, types)
+        e.h2d(bufs["b"], bm)
+        e.chi2_grad_dev_async(bufs["b"], n, bufs["ie"], bufs["sg"], bufs["out"], bufs["g"], None)
+        e.synchronize()
+        out = e.d2h(np.zeros(2), bufs["out"])
+        g = e.d2h(np.zeros((n, 3), np.float32), bufs["g"])
+        chi2_m, scale_m, grad_m, _ = e.chi2_grad(bm, i_exp, sigma)
+        assert abs(out[0] / chi2_m - 1) < 1e-6
+        assert np.linalg.norm(g - grad_m) / np.linalg.norm(grad_m) < 1e-5
+    finally:
+        for p in bufs.values():
+            e.dev_free(p)
+    e.close()
+
+
+def test_autograd_wrappers_on_device():
+    """The torch wrappers with positions on the GPU (device-resident entry points, no host
+    copies after the first layout) against the same wrappers on the CPU, and a short
+    refinement on the device.  Runs tests/torch_device_check.py in a child process that
+    imports torch before the library, as a torch program does: the library then shares the
+    HIP runtime torch ships (in this process the library was loaded first)."""
+    import os
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    _say("case: autograd wrappers on the device (child process), 1500 atoms x 64 q")
+    r = subprocess.run([sys.executable, os.path.join(here, "torch_device_check.py")], capture_output=True, text=True,
+                       timeout=180)
+    _say(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "torch device check: ok" in r.stdout
