@@ -341,6 +341,7 @@ def batched_bench(reps=5):
     t0 = time.perf_counter()
     e.batch_set_frames(w.bodies, w.offsets)
     layout_s = time.perf_counter() - t0
+    tuning = e.batch_autotune(TUNING_CACHE)
     fr = e.dev_alloc(len(w.sizes) * nq * 8)
     mn = e.dev_alloc(nq * 8)
     e.batch_calc_dev_async(fr, mn)
@@ -350,7 +351,6 @@ def batched_bench(reps=5):
         e.batch_calc_dev_async(fr, mn)
     e.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / reps
-    tuning = e.get_tuning()
     launch = e.batch_launch()
     e.dev_free(fr)
     e.dev_free(mn)
@@ -360,7 +360,7 @@ def batched_bench(reps=5):
     return {"workload": f"ensemble-{len(w.sizes)}-frames-1000..15000x{nq}", "ms_per_batch": round(ms, 3),
             "terms_per_s": terms / (ms * 1e-3), "achieved_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
             "frac": round(flops / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4), "layout_s": round(layout_s, 3),
-            "kernel": f"variant {tuning['variant']} K={tuning['k']} " + (
+            "kernel": f"variant {tuning['variant']} K={tuning['k']} L={tuning['col_range']} " + (
                 f"(work queue, {launch['persistent_waves']} resident waves)" if launch["persistent_waves"] > 0
                 else "(one wave per (job, q chunk) item)") + f", {launch['n_jobs']} jobs",
             "timing": "host wall clock over graph replays on device-resident frames, outputs to device"}

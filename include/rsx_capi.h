@@ -227,6 +227,14 @@ int rsx_set_adjoint_tuning(rsx_engine *h, int fp32, int k, int col_range);
 /* ---- tuning and measurement ---- */
 /* choose variant, q-per-thread and column-range length for the current n_q / size; cache in a text file */
 int rsx_autotune(rsx_engine *h, const char *cache_path);
+/* the same for the batch (no reference counterpart): the FP32 kernels over q per thread, column
+ * range and launch form (work queue or one wave per item), timed as graph replays on the
+ * frames set; the key is architecture, n_q, frame count and atom total (powers of two), class
+ * kind; a cached key is applied without timing */
+int rsx_batch_autotune(rsx_engine *h, const char *cache_path);
+/* the batch's launch configuration (laid out on demand): variant, q per thread, column range,
+ * resident waves of the work-queue form (0: one wave per item); pointers may be NULL */
+int rsx_batch_get_tuning(rsx_engine *h, int *variant, int *k, int *col_range, int *persistent_waves);
 /* tuning knobs: k (q per thread, 0 = automatic), col_range (atoms per job, 0 = automatic) */
 int rsx_set_tuning(rsx_engine *h, int k, int col_range);
 /* recurrence kernel for uniform q grids (the next calc_curve lays out again):

@@ -81,6 +81,8 @@ SIGNATURES = {
     "rsx_debye_vjp_dev_async": (_i, [_vp, _vp, _i, _vp, _vp]),
     "rsx_chi2_grad_dev_async": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp]),
     "rsx_autotune": (_i, [_vp, _c.c_char_p]),
+    "rsx_batch_autotune": (_i, [_vp, _c.c_char_p]),
+    "rsx_batch_get_tuning": (_i, [_vp, _ip, _ip, _ip, _ip]),
     "rsx_set_tuning": (_i, [_vp, _i, _i]),
     "rsx_set_variant": (_i, [_vp, _i]),
     "rsx_get_tuning": (_i, [_vp, _ip, _ip, _ip, _ip]),

@@ -2459,6 +2459,109 @@ void autotune_impl(rsx_engine* h, const char* cache_path)
     h->tune_report = key + " K=" + std::to_string(bestK) + " L=" + std::to_string(bestL) + " ms=" + std::to_string(best);
 }
 
+// ---- batch autotuner --------------------------------------------------------------
+// Key of a batch tuning entry: architecture, n_q, frame count and atom total (powers of
+// two), class kind.
+std::string batch_tune_key(rsx_engine* h)
+{
+    hipDeviceProp_t prop;
+    std::string arch = "unknown";
+    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) arch = prop.gcnArchName;
+    for (char& c : arch)
+        if (c == ' ' || c == ':') c = '_';
+    auto log2i = [](long long v) {
+        int b = 0;
+        while ((1LL << (b + 1)) <= v) ++b;
+        return b;
+    };
+    std::ostringstream k;
+    k << arch << ":batch:nq" << h->nq << ":f2^" << log2i(h->b_frames) << ":a2^" << log2i(h->b_offsets[h->b_frames])
+      << ":" << (hydration_on(h) ? "hyd" : "plain");
+    return k.str();
+}
+
+// average ms of `reps` replays of the batch graph (after one warm-up), HIP events
+float time_batch(rsx_engine* h, int reps)
+{
+    batch_run_dev(h, h->b_out.p, h->b_mean.p);
+    hipchk(hipEventRecord(h->ev0, h->stream), "event");
+    for (int r = 0; r < reps; ++r) batch_run_dev(h, h->b_out.p, h->b_mean.p);
+    hipchk(hipEventRecord(h->ev1, h->stream), "event");
+    hipchk(hipEventSynchronize(h->ev1), "event sync");
+    float ms = 0.0f;
+    hipchk(hipEventElapsedTime(&ms, h->ev0, h->ev1), "elapsed");
+    return ms / reps;
+}
+
+void set_batch_tuning(rsx_engine* h, int V, int K, int L, int P)
+{
+    h->b_var_req = V;
+    h->b_k_req = K;
+    h->b_range_req = L;
+    h->b_pers_req = P;
+    batch_invalidate(h);
+}
+
+// Look the batch up in cache_path ("key K L variant persistent ms" lines); if absent, time the
+// FP32 kernels over K, column range and launch form on the laid-out frames, keep the
+// fastest, append it.  Needs frames (rsx_batch_set_frames) on a uniform q grid.
+void batch_autotune_impl(rsx_engine* h, const char* cache_path)
+{
+    if (h->b_frames <= 0) fail(RSX_ARGS_NOT_SET, "batch autotune needs frames");
+    set_device(h);
+    const std::string key = batch_tune_key(h);
+    if (cache_path && *cache_path) {
+        std::ifstream in(cache_path);
+        std::string line;
+        while (std::getline(in, line)) {
+            std::istringstream ls(line);
+            std::string k;
+            int K = 0, L = 0, V = 0, P = 0;
+            if ((ls >> k >> K >> L >> V >> P) && k == key && V >= 0 && V < rsx::VARIANT_COUNT) {
+                set_batch_tuning(h, V, K, L, P);
+                return;
+            }
+        }
+    }
+    if (h->b_layout_ok && !h->b_uniform) fail(RSX_NOT_SUPPORTED, "batch autotune needs a uniform q grid");
+    h->b_out.ensure((size_t)h->b_frames * h->nq);
+    h->b_mean.ensure(h->nq);
+    struct Cand {
+        float ms;
+        int V, K, L, P;
+    };
+    std::vector<Cand> cands;
+    const int r32r[] = {96, 100, 128}, r32[] = {64, 86};
+    const int Ls[] = {256, 512, 1024};
+    for (int K : r32r)
+        for (int L : Ls)
+            for (int P = 0; P < 2; ++P) cands.push_back({0.0f, rsx::VARIANT_FP32R, K, L, P});
+    for (int K : r32)
+        for (int P = 0; P < 2; ++P) cands.push_back({0.0f, rsx::VARIANT_FP32, K, 512, P});
+    for (auto& c : cands) {
+        set_batch_tuning(h, c.V, c.K, c.L, c.P);
+        c.ms = time_batch(h, 2);
+    }
+    // the per-lane-sum kernel keeps its run totals in FP32 per lane (about twice the error of
+    // the run-flush kernel's FP64 row sums at deep minima of I(q)): it has to be 3 % faster
+    const auto eff = [](const Cand& c) { return c.V == rsx::VARIANT_FP32 ? c.ms * 1.03f : c.ms; };
+    std::sort(cands.begin(), cands.end(), [&](const Cand& a, const Cand& b) { return eff(a) < eff(b); });
+    if (cands.size() > 3) cands.resize(3);
+    for (auto& c : cands) c.ms = 1e30f;
+    for (int round = 0; round < 2; ++round)
+        for (auto& c : cands) {
+            set_batch_tuning(h, c.V, c.K, c.L, c.P);
+            c.ms = std::min(c.ms, time_batch(h, 3));
+        }
+    const Cand top = *std::min_element(cands.begin(), cands.end(),
+                                       [&](const Cand& a, const Cand& b) { return eff(a) < eff(b); });
+    set_batch_tuning(h, top.V, top.K, top.L, top.P);
+    if (cache_path && *cache_path) {
+        std::ofstream out(cache_path, std::ios::app);
+        out << key << " " << top.K << " " << top.L << " " << top.V << " " << top.P << " " << top.ms << "\n";
+    }
+}
+
 template <typename F>
 int guarded(rsx_engine* h, F&& f)
 {
@@ -3046,6 +3149,30 @@ int rsx_batch_launch(rsx_engine* h, int* persistent_waves_out, int* n_jobs)
 int rsx_autotune(rsx_engine* h, const char* cache_path)
 {
     return guarded(h, [&] { autotune_impl(h, cache_path); });
+}
+
+int rsx_batch_autotune(rsx_engine* h, const char* cache_path)
+{
+    return guarded(h, [&] { batch_autotune_impl(h, cache_path); });
+}
+
+int rsx_batch_get_tuning(rsx_engine* h, int* variant, int* k, int* col_range, int* waves)
+{
+    return guarded(h, [&] {
+        if (h->b_frames <= 0) fail(RSX_ARGS_NOT_SET, "batch: no frames set");
+        if (!h->b_layout_ok) {
+            set_device(h);
+            batch_layout(h);
+        }
+        if (variant) *variant = h->b_variant;
+        if (k) *k = h->b_K;
+        if (col_range) *col_range = h->b_L;
+        if (waves) {
+            const bool fp32 = h->b_variant == rsx::VARIANT_FP32 || h->b_variant == rsx::VARIANT_FP32R;
+            const bool pers = h->b_uniform && fp32 && (h->b_pers_req >= 0 ? h->b_pers_req != 0 : h->persistent);
+            *waves = pers ? persistent_waves(h, h->b_variant, h->b_K, h->b_radius) : 0;
+        }
+    });
 }
 
 int rsx_set_variant(rsx_engine* h, int variant)

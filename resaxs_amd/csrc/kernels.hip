@@ -715,7 +715,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void de
 
 // (the work-queue loop needs a few more registers: K = 128 runs at two waves per SIMD)
 template <int K, int M>
-__global__ __launch_bounds__(64, 2) void debye_rec32r_persistent(const DebyeArgs a, int n_items)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void debye_rec32r_persistent(const DebyeArgs a, int n_items)
 {
     __shared__ float T[RSLICE][65];
     for (;;) {

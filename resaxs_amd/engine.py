@@ -119,6 +119,17 @@ class Engine:
                                            ctypes.byref(sg)))
         return {"k": k.value, "col_range": L.value, "recurrence": bool(rec.value), "variant": sg.value}
 
+    def batch_autotune(self, cache_path=None):
+        """Tune the batch launch (cached per key in cache_path); returns batch_tuning()."""
+        self._check(self._L.rsx_batch_autotune(self.h, cache_path.encode() if cache_path else None))
+        return self.batch_tuning()
+
+    def batch_tuning(self):
+        v, k, L, pw = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.rsx_batch_get_tuning(self.h, ctypes.byref(v), ctypes.byref(k), ctypes.byref(L),
+                                                 ctypes.byref(pw)))
+        return {"variant": v.value, "k": k.value, "col_range": L.value, "persistent_waves": pw.value}
+
     def batch_launch(self):
         """Launch shape of the current batch graph: resident waves of the work-queue kernel
         (0: one wave per (job, q chunk) item) and the pair-job count."""

@@ -194,6 +194,36 @@ def test_batch_factor_swap_keeps_the_launch(oracle):
     _say("  every (c, w) point within 1e-5 of the oracle, one captured launch")
 
 
+def test_batch_autotune_and_cache(oracle, tmp_path):
+    """rsx_batch_autotune times the FP32 kernels over K, column range and launch form on the
+    frames, keeps the fastest and appends it to the cache; a second engine takes the cached
+    entry and reproduces the first one's results bit for bit; frames still meet the bar."""
+    w = _frames(7, 24, 100, 2500)
+    _say("case: batch autotune + cache, 24 frames")
+    cache = str(tmp_path / "tune.txt")
+    e = engine_for(w.q, w.table, w.water_index, 0.0)
+    e.batch_set_frames(w.bodies, w.offsets)
+    t = e.batch_autotune(cache)
+    _say(f"  tuned: {t}")
+    tok = open(cache).read().split()
+    assert len(tok) == 6 and ":batch:nq128:" in tok[0]
+    assert (int(tok[1]), int(tok[2]), int(tok[3])) == (t["k"], t["col_range"], t["variant"])
+    if int(tok[4]) == 0:
+        assert t["persistent_waves"] == 0
+    frames, mean = e.batch_calc()
+    for f in (0, 11, 23):
+        b = w.bodies[w.offsets[f]:w.offsets[f + 1]]
+        assert rel_err(frames[f], oracle_iq(w.q, b[:, :3], b[:, 3].view(np.uint32), w.table)).max() < 1e-5, f
+    g = engine_for(w.q, w.table, w.water_index, 0.0)
+    g.batch_set_frames(w.bodies, w.offsets)
+    assert g.batch_autotune(cache) == t
+    f2, m2 = g.batch_calc()
+    assert np.array_equal(f2, frames) and np.array_equal(m2, mean)
+    assert len(open(cache).read().split()) == 6
+    e.close()
+    g.close()
+
+
 def test_batch_rejects_bad_offsets():
     """Offsets are checked before any use: offsets[0] must be 0 and they must not decrease."""
     from resaxs_amd import RsxError
