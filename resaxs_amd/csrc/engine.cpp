@@ -340,7 +340,7 @@ struct rsx_engine {
     DBuf<ChunkConst> d_chunks;
     DBuf<double> d_invq, d_partials, d_iq, d_saved_iq, d_out, d_wq, d_q0;
     DBuf<double> d_red;              // chunk sums of the two-stage reduction (single structure)
-    DBuf<PairJob> d_jobs, d_fjobs;  // incremental jobs; forward (triangle) jobs
+    DBuf<PairJob> d_fjobs;          // forward (triangle) jobs
     DBuf<double> d_f0col;           // FP64 F_class(q = 0) per class
     DBuf<int> d_slot_begin_f;  // [0, n forward jobs]
     DBuf<int> d_slot_begin, d_idx, d_bucket_of, d_bucket_count, d_bucket_start, d_bucket_fill, d_sorted, d_s2c,
@@ -375,7 +375,8 @@ struct rsx_engine {
 
     std::string tune_report;
     int last_L = 0;  // column range of the last forward job layout
-    // job table of the last incremental update, reused while its shape is unchanged
+    // job tables of incremental updates, reused while their shape recurs (the changed set's
+    // size varies from move to move with hydration, over a few row-tile counts)
     struct IncKey {
         int mp = -1, p1 = -1, p2 = -1, n = -1, L = -1, K = -1, n_chunks = -1, variant = -1, uniform = -1;
         const float4* rows = nullptr;
@@ -385,11 +386,16 @@ struct rsx_engine {
             return mp == o.mp && p1 == o.p1 && p2 == o.p2 && n == o.n && L == o.L && K == o.K && n_chunks == o.n_chunks &&
                    variant == o.variant && uniform == o.uniform && rows == o.rows && cols == o.cols;
         }
-    } inc_key;
-    int inc_njobs = 0;
-    DBuf<ChunkConst> d_chunks_inc;  // q chunks of the incremental kernel
-    int inc_chunks_K = 0, inc_chunks_n = 0;
-    uint64_t inc_chunks_stamp = ~0ull;
+    };
+    struct IncTable {
+        IncKey key;
+        DBuf<PairJob> jobs;
+        int njobs = 0;
+        uint64_t used = 0;
+    };
+    static constexpr int INC_TABLES = 8;  // least recently used one replaced
+    IncTable inc_tab[INC_TABLES];
+    uint64_t inc_clock = 0;
     std::vector<PairJob> inc_jobs;  // host table of the incremental jobs (capacity kept)
     // pinned staging of the per-update transfers
     HBuf<float4> pin_rows, pin_vals, pin_moved;
@@ -423,6 +429,12 @@ struct rsx_engine {
     // arrays or the slot layout since then
     std::vector<int> dirty;
     bool dirty_full = true;
+    bool slots_dirty = false;  // incremental updates moved atoms between slots since the last transaction
+    std::vector<char> slot_mark;                  // slots touched by a relocation (all zero between calls)
+    DBuf<int> d_rslot, d_rcaller;                 // relocated slots and their atoms
+    DBuf<float4> d_rvals;
+    HBuf<int> pin_rslot, pin_rcaller;
+    HBuf<float4> pin_rvals;
     int since_rebase = 0;
 
     // inputs the batch layout depends on, stamped when they change
@@ -478,6 +490,7 @@ void mark_dirty_full(rsx_engine* h)
 {
     h->dirty_full = true;
     h->dirty.clear();
+    h->slots_dirty = false;  // the full copy of a transaction takes the slot maps too
 }
 
 // record entry i before an incremental update writes it
@@ -688,7 +701,7 @@ void build_class_table(rsx_engine* h)
 {
     h->layout_stamp++;
     h->cls_count_ok = false;
-    h->inc_njobs = 0;
+    for (auto& t : h->inc_tab) t.njobs = 0;
     h->hydrate = hydration_on(h);
     h->d1 = h->hydrate ? h->n_dots + 1 : 1;
     h->n_class = h->n_types * h->d1;
@@ -965,49 +978,18 @@ int pick_range(const rsx_engine* h, double column_atoms_total)
     return pick_range(h, column_atoms_total, h->variant, h->K, h->n_chunks);
 }
 
-// Kernel of the incremental launches.  Their rows are the changed atoms sorted by class;
-// with hydration classes (type x exposed-dot count) a 64-row tile holds tens of classes,
-// so the run-flush kernel's per-flush loads of the 64 rows' factors miss the caches.  The
-// per-lane-sum kernel applies each lane's row factor once at the end, and the column
-// factor of a flush is one scalar load: hydrated moves take it (3.63 -> 2.70 ms per
-// 500-atom move at 50k atoms), plain ones keep the forward kernel (0.89 vs 1.01 ms).  It
-// runs on the forward's factor table (stride ldf) with its own q chunks of K_inc: the
-// largest K of that kernel at most 64 whose chunks fit in ldf.
+// Kernel of the incremental launches: the forward's.  With hydration classes a 64-row tile
+// of changed atoms holds tens of classes, so each flush loads the rows' factors from L2;
+// that costs little while the columns keep long class runs (relocate_classes keeps them
+// sorted): at 50k atoms a hydrated 500-atom move takes 1.51 ms (median) with it, the same as
+// with the per-lane-sum kernel at K = 64, and its FP64 row sums are the more accurate at deep
+// minima of I(q).
 void incremental_kernel(rsx_engine* h, int& variant, int& K, int& n_chunks, const ChunkConst*& chunks)
 {
     variant = h->variant;
     K = h->K;
     n_chunks = h->n_chunks;
     chunks = h->d_chunks.p;
-    if (!h->uniform || h->variant != rsx::VARIANT_FP32R || !h->hydrate) return;
-    int best = 0;
-    for (int i = 0; i < rsx::debye_k_count(rsx::PATH_REC32); ++i) {
-        const int k = rsx::debye_k_value(rsx::PATH_REC32, i);
-        if (k <= 64 && k > best && (long)((h->nq + k - 1) / k) * k <= h->ldf) best = k;
-    }
-    if (best <= 0) return;
-    if (h->inc_chunks_K != best || h->inc_chunks_stamp != h->layout_stamp) {
-        const int nc = (h->nq + best - 1) / best;
-        std::vector<ChunkConst> cc(nc);
-        for (int c = 0; c < nc; ++c) {
-            const int k0 = c * best;
-            const double a = k0 < h->nq ? (double)h->q[k0] : 0.0;
-            cc[c].a = (float)a;
-            cc[c].a_rev = a / (2.0 * PI);
-            cc[c].mid_rev = (a + 0.5 * h->dq) / (2.0 * PI);
-            cc[c].pad = 0.0f;
-        }
-        h->d_chunks_inc.ensure(nc);
-        hipchk(hipMemcpy(h->d_chunks_inc.p, cc.data(), nc * sizeof(ChunkConst), hipMemcpyHostToDevice),
-               "upload incremental chunks");
-        h->inc_chunks_K = best;
-        h->inc_chunks_n = nc;
-        h->inc_chunks_stamp = h->layout_stamp;
-    }
-    variant = rsx::VARIANT_FP32;
-    K = best;
-    n_chunks = h->inc_chunks_n;
-    chunks = h->d_chunks_inc.p;
 }
 
 // triangle jobs over records [0, m) of cols (rows == cols); appends to jobs
@@ -1301,6 +1283,54 @@ void calc_curve_impl(rsx_engine* h, const float* bodies, int n, float* out32, do
     copy_out(h, out32, out64);
 }
 
+// Keep the columns sorted by class through incremental updates: atom M[a] changed from
+// class cls_old[a] to cls_new[a] while keeping its slot, which sits in the run of its old
+// class; it is moved into the run of its new class by swaps across the runs in between,
+// each swap exchanging the atom with the boundary slot of a run and shifting that boundary
+// by one (so a count change of d dots costs d swaps).  run_start holds the old classes'
+// run starts and is updated in place.  Appends to touched every slot whose atom changed and
+// the final slots of all of M (whose records changed).
+void relocate_classes(rsx_engine* h, const std::vector<int>& M, const std::vector<uint32_t>& cls_old,
+                      const std::vector<uint32_t>& cls_new, std::vector<int>& run_start, std::vector<int>& touched)
+{
+    std::vector<char>& mark = h->slot_mark;
+    mark.resize(h->n, 0);
+    auto touch = [&](int sl) {
+        if (!mark[sl]) {
+            mark[sl] = 1;
+            touched.push_back(sl);
+        }
+    };
+    auto swap_slots = [&](int s1, int s2) {
+        if (s1 == s2) return;
+        const int i1 = h->caller_of[s1], i2 = h->caller_of[s2];
+        h->caller_of[s1] = i2;
+        h->caller_of[s2] = i1;
+        h->slot_of[i1] = s2;
+        h->slot_of[i2] = s1;
+        touch(s1);
+        touch(s2);
+        h->slots_dirty = true;
+    };
+    for (size_t a = 0; a < M.size(); ++a) {
+        const int i = M[a];
+        int c = (int)cls_old[a];
+        const int b = (int)cls_new[a];
+        while (c < b) {  // to the last slot of run c, then across the boundary into run c + 1
+            swap_slots(h->slot_of[i], run_start[c + 1] - 1);
+            run_start[c + 1]--;
+            ++c;
+        }
+        while (c > b) {  // to the first slot of run c, then across the boundary into run c - 1
+            swap_slots(h->slot_of[i], run_start[c]);
+            run_start[c]++;
+            --c;
+        }
+    }
+    for (int i : M) touch(h->slot_of[i]);
+    for (int sl : touched) mark[sl] = 0;
+}
+
 void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int len, float* out32, double* out64)
 {
     if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "recalc_curve() called before an initial call to calc_curve()");
@@ -1482,7 +1512,13 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
     key.uniform = h->uniform ? 1 : 0;
     key.rows = h->d_rows.p;
     key.cols = h->d_cols.p;
-    if (!(h->inc_njobs > 0 && key == h->inc_key)) {
+    rsx_engine::IncTable* tab = nullptr;
+    for (auto& t : h->inc_tab)
+        if (t.njobs > 0 && t.key == key) tab = &t;
+    if (!tab) {
+        tab = &h->inc_tab[0];
+        for (auto& t : h->inc_tab)
+            if (t.used < tab->used) tab = &t;
         std::vector<PairJob>& jobs = h->inc_jobs;
         jobs.clear();
         const float4* P1 = p_is_m ? h->d_rows.p : h->d_rows.p + 2 * (size_t)mp;
@@ -1493,20 +1529,24 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
         rect_jobs(jobs, h->d_rows.p + mp, mp, h->d_rows.p + mp, mp, L, -1.0f);  // old x old
         split_tail(jobs, resident_jobs(h, kc.variant, kc.K, kc.n_chunks),
                    [](const PairJob& j) { return (double)j.ncol * j.nrow; });
-        h->d_jobs.ensure(jobs.size());
+        tab->jobs.ensure(jobs.size());
         // staged through pinned memory (the call waits for the move's result before
         // returning, so the staging buffer is free again at the next move)
         h->pin_jobs.ensure(jobs.size());
         std::memcpy(h->pin_jobs.p, jobs.data(), jobs.size() * sizeof(PairJob));
-        hipchk(hipMemcpyAsync(h->d_jobs.p, h->pin_jobs.p, jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice,
+        hipchk(hipMemcpyAsync(tab->jobs.p, h->pin_jobs.p, jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice,
                               h->stream),
                "upload jobs");
-        h->inc_key = key;
-        h->inc_njobs = (int)jobs.size();
+        tab->key = key;
+        tab->njobs = (int)jobs.size();
     }
+    tab->used = ++h->inc_clock;
 
     // q = 0: (sum F_new)^2 - (sum F_old)^2 from the class populations
     const double s_old = sum_f0(h);
+    // run starts of the classes before the update (the populations are current: sum_f0)
+    std::vector<int> run_start(h->n_class + 1, 0);
+    for (int c = 0; c < h->n_class; ++c) run_start[c + 1] = run_start[c] + h->cls_count[c];
     for (int a = 0; a < m; ++a) {
         mark_dirty(h, M[a]);
         h->cls[M[a]] = cls_new[a];
@@ -1517,8 +1557,35 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
     }
     const double s_new = sum_f0(h);
     const double q0d = s_new * s_new - s_old * s_old;
-    launch_jobs(h, h->d_jobs.p, h->inc_njobs, h->d_iq.p, 1, q0d, false, &kc);
-    hipchk(rsx::launch_scatter(h->d_cols.p, h->d_idx.p, h->d_vals.p + m, m, h->stream), "scatter new");
+    launch_jobs(h, tab->jobs.p, tab->njobs, h->d_iq.p, 1, q0d, false, &kc);
+    // the new records, each in the run of its class (the slots of atoms whose class changed
+    // are exchanged across runs), written after the kernel on the same stream
+    std::vector<int> touched;
+    touched.reserve(2 * (size_t)m);
+    relocate_classes(h, M, cls_old, cls_new, run_start, touched);
+    const int nt = (int)touched.size();
+    h->d_rslot.ensure(nt);
+    h->d_rcaller.ensure(nt);
+    h->d_rvals.ensure(nt);
+    h->pin_rslot.ensure(nt);
+    h->pin_rcaller.ensure(nt);
+    h->pin_rvals.ensure(nt);
+    for (int t = 0; t < nt; ++t) {
+        const int sl = touched[t], i = h->caller_of[sl];
+        const float* bb = &h->bodies[4 * (size_t)i];
+        h->pin_rslot.p[t] = sl;
+        h->pin_rcaller.p[t] = i;
+        h->pin_rvals.p[t] = make_float4(bb[0], bb[1], bb[2], bitsf(h->cls[i]));
+    }
+    hipchk(hipMemcpyAsync(h->d_rslot.p, h->pin_rslot.p, nt * sizeof(int), hipMemcpyHostToDevice, h->stream),
+           "upload relocated slots");
+    hipchk(hipMemcpyAsync(h->d_rcaller.p, h->pin_rcaller.p, nt * sizeof(int), hipMemcpyHostToDevice, h->stream),
+           "upload relocated atoms");
+    hipchk(hipMemcpyAsync(h->d_rvals.p, h->pin_rvals.p, nt * sizeof(float4), hipMemcpyHostToDevice, h->stream),
+           "upload relocated records");
+    hipchk(rsx::launch_scatter_slots(h->d_cols.p, h->d_s2c.p, h->d_rslot.p, h->d_rvals.p, h->d_rcaller.p, nt,
+                                     h->stream),
+           "scatter new");
     h->since_rebase++;
     copy_out(h, out32, out64);
 }
@@ -1563,6 +1630,10 @@ void commit_impl(rsx_engine* h)
     if (saved_matches_sizes(h) && !h->dirty_full) {
         for (int i : h->dirty) copy_entry(h->saved.bodies, h->saved.counts, h->saved.cls, h->saved.rsasa, h->bodies,
                                           h->counts, h->cls, h->rsasa, i);
+        if (h->slots_dirty) {
+            h->saved.slot_of = h->slot_of;
+            h->saved.caller_of = h->caller_of;
+        }
     } else {
         h->saved.bodies = h->bodies;
         h->saved.counts = h->counts;
@@ -1573,6 +1644,7 @@ void commit_impl(rsx_engine* h)
     }
     h->dirty.clear();
     h->dirty_full = false;
+    h->slots_dirty = false;
     h->saved.have_rsasa = h->have_rsasa;
     h->saved.since_rebase = h->since_rebase;
     h->xaction = true;
@@ -1591,11 +1663,16 @@ void revert_impl(rsx_engine* h)
         std::memcpy(&t, &h->bodies[4 * (size_t)i + 3], 4);
         h->type[i] = t;
     };
+    const bool slots_moved = partial && h->slots_dirty;
     if (partial) {
         for (int i : h->dirty) {
             copy_entry(h->bodies, h->counts, h->cls, h->rsasa, h->saved.bodies, h->saved.counts, h->saved.cls,
                        h->saved.rsasa, i);
             restore_type(i);
+        }
+        if (slots_moved) {
+            h->slot_of = h->saved.slot_of;
+            h->caller_of = h->saved.caller_of;
         }
     } else {
         h->bodies = h->saved.bodies;
@@ -1608,6 +1685,7 @@ void revert_impl(rsx_engine* h)
     }
     h->dirty.clear();
     h->dirty_full = false;
+    h->slots_dirty = false;
     h->cls_count_ok = false;
     h->have_rsasa = h->saved.have_rsasa;
     h->since_rebase = h->saved.since_rebase;
@@ -1616,7 +1694,7 @@ void revert_impl(rsx_engine* h)
            "restore cols");
     hipchk(hipMemcpyAsync(h->d_iq.p, h->d_saved_iq.p, h->nq * sizeof(double), hipMemcpyDeviceToDevice, h->stream),
            "restore iq");
-    if (layout_changed || h->host_stale)
+    if (layout_changed || slots_moved || h->host_stale)
         hipchk(hipMemcpyAsync(h->d_s2c.p, h->caller_of.data(), h->n * sizeof(int), hipMemcpyHostToDevice, h->stream),
                "restore slot map");
     hipchk(hipStreamSynchronize(h->stream), "sync revert");
@@ -2668,8 +2746,6 @@ int rsx_destroy(rsx_engine* h)
     h->b_chunks.release();
     h->b_pin_ftab.release();
     h->b_pin_q0.release();
-    h->d_chunks_inc.release();
-    h->inc_chunks_K = 0;
     h->d_subcnt.release();
     h->pin_rows.release();
     h->pin_jobs.release();
@@ -2691,9 +2767,9 @@ int rsx_destroy(rsx_engine* h)
     h->h_q.release();
     h->h_pidx.release();
     h->h_tiles.release();
-    h->d_jobs.release();
+    for (auto& t : h->inc_tab) t.jobs.release();
     h->d_fjobs.release();
-    h->inc_njobs = 0;
+    for (auto& t : h->inc_tab) t.njobs = 0;
     h->d_f0col.release();
     h->d_slot_begin_f.release();
     h->b_jobs.release();
@@ -2701,6 +2777,12 @@ int rsx_destroy(rsx_engine* h)
     h->d_queue.release();
     h->d_guard.release();
     h->d_cs_table.release();
+    h->d_rslot.release();
+    h->d_rcaller.release();
+    h->d_rvals.release();
+    h->pin_rslot.release();
+    h->pin_rcaller.release();
+    h->pin_rvals.release();
     h->d_cs_tot.release();
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);

@@ -113,6 +113,9 @@ hipError_t launch_frame_mean(const double* in, int n_frames, int n_q, double* ou
 
 // cols[slot[t]] = vals[t]
 hipError_t launch_scatter(float4* cols, const int* slot, const float4* vals, int m, hipStream_t s);
+// cols[slot[t]] = vals[t] and s2c[slot[t]] = callers[t] (slots whose atom changed)
+hipError_t launch_scatter_slots(float4* cols, int* s2c, const int* slot, const float4* vals, const int* callers, int m,
+                                hipStream_t s);
 // dst[t] = src[idx[t]] for t < m
 hipError_t launch_gather_u32(uint32_t* dst, const uint32_t* src, const int* idx, int m, hipStream_t s);
 

@@ -915,6 +915,16 @@ __global__ void scatter_kernel(float4* cols, const int* __restrict__ slot, const
     if (t < m) cols[slot[t]] = vals[t];
 }
 
+__global__ void scatter_slots_kernel(float4* cols, int* s2c, const int* __restrict__ slot,
+                                     const float4* __restrict__ vals, const int* __restrict__ callers, int m)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) {
+        cols[slot[t]] = vals[t];
+        s2c[slot[t]] = callers[t];
+    }
+}
+
 __global__ void gather_u32_kernel(uint32_t* dst, const uint32_t* __restrict__ src, const int* __restrict__ idx, int m)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1924,6 +1934,15 @@ hipError_t launch_frame_mean(const double* in, int n_frames, int n_q, double* ou
     if (n_q <= 0) return hipSuccess;
     hipLaunchKernelGGL(frame_mean_kernel, dim3((unsigned)((n_q + 255) / 256)), dim3(256), 0, s, in, n_frames, n_q,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_slots(float4* cols, int* s2c, const int* slot, const float4* vals, const int* callers, int m,
+                                hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(scatter_slots_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, cols, s2c, slot, vals,
+                       callers, m);
     return hipGetLastError();
 }
 
