@@ -266,6 +266,10 @@ void atomic_ff(const float* q, int nq, float avg_radius, float c, float* out)
 struct rsx_engine {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream of incremental updates (default priority, like the first): the
+    // new-records launch runs there beside the old-records launch
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_skip = nullptr, ev_new = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     std::string err;
     int state = 0;  // 0 blank, 1 initialized, 3 args set, 4 computing
@@ -327,9 +331,12 @@ struct rsx_engine {
     bool hydrate = false;
 
     // device buffers
-    DBuf<float4> d_cols, d_saved_cols, d_rows, d_vals, d_sasa_atoms, d_sorted_atoms;
-    bool sasa_mirror = false;        // d_sasa_atoms holds exactly the host bodies
-    DBuf<int> d_subset;              // atom ids of a local SASA recount
+    DBuf<float4> d_cols, d_saved_cols, d_sasa_atoms, d_sorted_atoms;
+    bool sasa_mirror = false;        // d_sasa_atoms and d_counts hold exactly the host bodies and counts
+    // local SASA recount: the listed atoms, and the block read back
+    // ([listed, changed, (atom, count) x LOCAL_CHANGED_CAP])
+    DBuf<int> d_local_list, d_local;
+    hipEvent_t ev_sasa = nullptr;    // recorded after the local recount's read-back
     DBuf<unsigned int> h_u32;        // distance histograms (rsx_calc_curve_hist)
     DBuf<unsigned long long> h_u64;
     DBuf<double> h_dbl, h_out, h_q, h_part;
@@ -343,7 +350,7 @@ struct rsx_engine {
     DBuf<PairJob> d_fjobs;          // forward (triangle) jobs
     DBuf<double> d_f0col;           // FP64 F_class(q = 0) per class
     DBuf<int> d_slot_begin_f;  // [0, n forward jobs]
-    DBuf<int> d_slot_begin, d_idx, d_bucket_of, d_bucket_count, d_bucket_start, d_bucket_fill, d_sorted, d_s2c,
+    DBuf<int> d_slot_begin, d_bucket_of, d_bucket_count, d_bucket_start, d_bucket_fill, d_sorted, d_s2c,
         d_tile_sum;
     DBuf<uint32_t> d_counts;
     DBuf<float4> d_dots;
@@ -378,12 +385,12 @@ struct rsx_engine {
     // job tables of incremental updates, reused while their shape recurs (the changed set's
     // size varies from move to move with hydration, over a few row-tile counts)
     struct IncKey {
-        int mp = -1, p1 = -1, p2 = -1, n = -1, L = -1, K = -1, n_chunks = -1, variant = -1, uniform = -1;
+        int part = -1, nrows = -1, n = -1, L = -1, K = -1, n_chunks = -1, variant = -1, uniform = -1;
         const float4* rows = nullptr;
         const float4* cols = nullptr;
         bool operator==(const IncKey& o) const
         {
-            return mp == o.mp && p1 == o.p1 && p2 == o.p2 && n == o.n && L == o.L && K == o.K && n_chunks == o.n_chunks &&
+            return part == o.part && nrows == o.nrows && n == o.n && L == o.L && K == o.K && n_chunks == o.n_chunks &&
                    variant == o.variant && uniform == o.uniform && rows == o.rows && cols == o.cols;
         }
     };
@@ -393,17 +400,24 @@ struct rsx_engine {
         int njobs = 0;
         uint64_t used = 0;
     };
-    static constexpr int INC_TABLES = 8;  // least recently used one replaced
+    static constexpr int INC_TABLES = 12;  // least recently used one replaced
     IncTable inc_tab[INC_TABLES];
     uint64_t inc_clock = 0;
     std::vector<PairJob> inc_jobs;  // host table of the incremental jobs (capacity kept)
-    // pinned staging of the per-update transfers
-    HBuf<float4> pin_rows, pin_vals, pin_moved;
-    HBuf<PairJob> pin_jobs;
-    HBuf<int> pin_idx, pin_sub;
-    HBuf<uint32_t> pin_cnt;
+    // Incremental updates: the arrays of each launch packed into one pinned block and copied
+    // with one transfer into a device block that only grows, so arrays at fixed offsets keep
+    // their addresses from update to update (the cached job tables point into them).
+    struct Staging {
+        HBuf<unsigned char> host;
+        DBuf<unsigned char> dev;
+    };
+    Staging st_old, st_new, st_reloc;
+    DBuf<double> d_partials_new;  // partials of an update's second launch
+    // pinned staging of the per-update transfers (job tables: one per launch of an update)
+    HBuf<float4> pin_moved;
+    HBuf<PairJob> pin_jobs[2];
+    HBuf<int> pin_local;
     HBuf<double> pin_iq;
-    DBuf<uint32_t> d_subcnt;                      // counts of a local recount's subset, gathered
     std::vector<char> in_m;                       // membership of the changed set (all zero between calls)
     // host sources of small uploads that may still be in flight when launch_jobs returns
     int up_sb[2] = {0, 0};
@@ -431,10 +445,6 @@ struct rsx_engine {
     bool dirty_full = true;
     bool slots_dirty = false;  // incremental updates moved atoms between slots since the last transaction
     std::vector<char> slot_mark;                  // slots touched by a relocation (all zero between calls)
-    DBuf<int> d_rslot, d_rcaller;                 // relocated slots and their atoms
-    DBuf<float4> d_rvals;
-    HBuf<int> pin_rslot, pin_rcaller;
-    HBuf<float4> pin_rvals;
     int since_rebase = 0;
 
     // inputs the batch layout depends on, stamped when they change
@@ -790,6 +800,9 @@ double sum_f0(rsx_engine* h)
 }
 
 // ---- SASA ----------------------------------------------------------------
+// changed counts a local recount reads back in its one transfer (more: all counts compared)
+constexpr int LOCAL_CHANGED_CAP = 4096;
+
 void upload_sasa_tables(rsx_engine* h)
 {
     const int nt = (int)h->radii.size();
@@ -809,9 +822,10 @@ void upload_sasa_tables(rsx_engine* h)
            "upload dots");
 }
 
-// counts for n atoms already on the device in d_sasa_atoms; with a subset (device list
-// of n_subset atom ids) only those atoms' counts are written
-void run_sasa_device(rsx_engine* h, int n, const int* subset = nullptr, int n_subset = 0)
+// Counts for n atoms already on the device in d_sasa_atoms.  With a box (lo[3], hi[3]) only
+// the atoms inside it are recounted, and the changed counts are listed in d_local.
+void run_sasa_device(rsx_engine* h, int n, const float* box_lo = nullptr, const float* box_hi = nullptr,
+                     hipStream_t stream = nullptr)
 {
     float rmax = 0.0f;
     for (float v : h->rext) rmax = std::max(rmax, v);
@@ -844,9 +858,21 @@ void run_sasa_device(rsx_engine* h, int n, const int* subset = nullptr, int n_su
     a.sorted = h->d_sorted.p;
     a.sorted_atoms = h->d_sorted_atoms.p;
     a.counts = h->d_counts.p;
-    a.subset = subset;
-    a.n_subset = n_subset;
-    hipchk(rsx::launch_sasa(a, h->stream), "sasa kernels");
+    a.subset = nullptr;
+    a.n_subset = 0;
+    if (box_lo) {
+        h->d_local_list.ensure(n);
+        h->d_local.ensure(2 + 2 * (size_t)LOCAL_CHANGED_CAP);
+        a.local_hdr = h->d_local.p;
+        a.local_list = h->d_local_list.p;
+        a.local_changed = h->d_local.p + 2;
+        a.changed_cap = LOCAL_CHANGED_CAP;
+        for (int d = 0; d < 3; ++d) {
+            a.box_lo[d] = box_lo[d];
+            a.box_hi[d] = box_hi[d];
+        }
+    }
+    hipchk(rsx::launch_sasa(a, stream ? stream : h->stream), "sasa kernels");
 }
 
 // recompute counts / rsasa for the host mirror bodies (caller order)
@@ -870,40 +896,33 @@ void compute_sasa(rsx_engine* h)
     h->have_rsasa = true;
 }
 
-// Recount after bodies [start, start+len) moved (old positions in old_bodies): only an
-// atom within reach (2 max(r + p) + 0.02) of a moved atom's old or new position can
-// change its count, so the atoms inside the moved set's bounding box widened by that
-// reach are recounted against the full, updated cell list; every other count is
-// unchanged by construction.  Returns false (nothing done) where the box holds more
-// than half the atoms: the full recount is then cheaper.
-bool compute_sasa_local(rsx_engine* h, int start, int len, const std::vector<float>& old_bodies,
-                        std::vector<int>& changed)
+// Local recount after bodies [start, start+len) moved (old positions in old_bodies),
+// enqueued: only an atom within reach (2 max(r + p) + 0.02) of a moved atom's old or new
+// position can change its count, so the atoms inside the moved set's bounding box widened by
+// that reach are recounted on the device against the full, rebuilt cell list, and the
+// counts that changed come back in one transfer (sasa_local_finish waits for it).  Every
+// other count is unchanged by construction.  Returns false (nothing enqueued) without
+// current counts to compare with.
+bool sasa_local_enqueue(rsx_engine* h, int start, int len, const std::vector<float>& old_bodies)
 {
-    changed.clear();
     const int n = h->n;
     if (len <= 0 || (int)h->counts.size() != n || !h->have_rsasa) return false;
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int a = 0; a < len; ++a)
+    for (int a = 0; a < len; ++a) {
         for (int d = 0; d < 3; ++d) {
             const float vo = old_bodies[4 * (size_t)a + d], vn = h->bodies[4 * (size_t)(start + a) + d];
             lo[d] = std::min(lo[d], std::min(vo, vn));
             hi[d] = std::max(hi[d], std::max(vo, vn));
         }
+    }
     float rmax = 0.0f;
     for (float v : h->rext) rmax = std::max(rmax, v);
     const float reach = 2.0f * rmax + 0.02f;
-    std::vector<int> sub;
-    sub.reserve(4096);
-    for (int i = 0; i < n; ++i) {
-        const float* b = &h->bodies[4 * (size_t)i];
-        if ((i >= start && i < start + len) ||
-            (b[0] >= lo[0] - reach && b[0] <= hi[0] + reach && b[1] >= lo[1] - reach && b[1] <= hi[1] + reach &&
-             b[2] >= lo[2] - reach && b[2] <= hi[2] + reach))
-            sub.push_back(i);
+    float box_lo[3], box_hi[3];
+    for (int d = 0; d < 3; ++d) {
+        box_lo[d] = lo[d] - reach;
+        box_hi[d] = hi[d] + reach;
     }
-    if (2 * sub.size() > (size_t)n) return false;
-    for (int i : sub)
-        if (h->type[i] >= h->radii.size()) fail(RSX_INVALID_ARG, "body type index outside the SASA radius table");
     h->d_sasa_atoms.ensure(n);
     if (h->sasa_mirror) {
         h->pin_moved.ensure(len);
@@ -912,35 +931,51 @@ bool compute_sasa_local(rsx_engine* h, int start, int len, const std::vector<flo
                               h->stream),
                "upload moved sasa atoms");
     } else {
+        // the device copies hold other data: the host bodies and counts go up whole
+        h->d_counts.ensure(n);
         hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, h->bodies.data(), (size_t)n * 16, hipMemcpyHostToDevice, h->stream),
                "upload sasa atoms");
+        hipchk(hipMemcpyAsync(h->d_counts.p, h->counts.data(), (size_t)n * 4, hipMemcpyHostToDevice, h->stream),
+               "upload counts");
+        hipchk(hipStreamSynchronize(h->stream), "sync sasa upload");  // pageable sources
     }
     h->sasa_mirror = true;
-    const int ns = (int)sub.size();
-    h->d_subset.ensure(ns);
-    h->pin_sub.ensure(ns);
-    std::memcpy(h->pin_sub.p, sub.data(), ns * sizeof(int));
-    hipchk(hipMemcpyAsync(h->d_subset.p, h->pin_sub.p, ns * sizeof(int), hipMemcpyHostToDevice, h->stream),
-           "upload subset");
-    run_sasa_device(h, n, h->d_subset.p, ns);
-    // only the subset's counts come back (gathered on the device)
-    h->d_subcnt.ensure(ns);
-    h->pin_cnt.ensure(ns);
-    hipchk(rsx::launch_gather_u32(h->d_subcnt.p, h->d_counts.p, h->d_subset.p, ns, h->stream), "gather counts");
-    hipchk(hipMemcpyAsync(h->pin_cnt.p, h->d_subcnt.p, ns * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream),
-           "download counts");
-    hipchk(hipStreamSynchronize(h->stream), "sync sasa");
-    for (int t = 0; t < ns; ++t) {
-        const int i = sub[t];
-        const uint32_t c = h->pin_cnt.p[t];
-        if (h->counts[i] != c) {
-            changed.push_back(i);
-            mark_dirty(h, i);
-        }
+    run_sasa_device(h, n, box_lo, box_hi, h->stream);
+    h->pin_local.ensure(2 + 2 * (size_t)LOCAL_CHANGED_CAP);
+    hipchk(hipMemcpyAsync(h->pin_local.p, h->d_local.p, (2 + 2 * (size_t)LOCAL_CHANGED_CAP) * sizeof(int),
+                          hipMemcpyDeviceToHost, h->stream),
+           "download changed counts");
+    hipchk(hipEventRecord(h->ev_sasa, h->stream), "record sasa");
+    return true;
+}
+
+// Waits for the local recount and applies it: changed receives the atoms whose count
+// changed, ascending.
+void sasa_local_finish(rsx_engine* h, std::vector<int>& changed)
+{
+    hipchk(hipEventSynchronize(h->ev_sasa), "sync sasa");
+    changed.clear();
+    const int n_changed = h->pin_local.p[1];
+    auto apply = [&](int i, uint32_t c) {
+        changed.push_back(i);
+        mark_dirty(h, i);
         h->counts[i] = c;
         h->rsasa[i] = (float)c / (float)h->n_dots;
+    };
+    if (n_changed <= LOCAL_CHANGED_CAP) {
+        std::vector<std::pair<int, uint32_t>> e((size_t)n_changed);
+        for (int t = 0; t < n_changed; ++t) e[t] = {h->pin_local.p[2 + 2 * t], (uint32_t)h->pin_local.p[3 + 2 * t]};
+        std::sort(e.begin(), e.end());
+        for (const auto& x : e) apply(x.first, x.second);
+        return;
     }
-    return true;
+    // more than the block holds: every count compared
+    std::vector<uint32_t> c((size_t)h->n);
+    hipchk(hipMemcpyAsync(c.data(), h->d_counts.p, (size_t)h->n * 4, hipMemcpyDeviceToHost, h->stream),
+           "download counts");
+    hipchk(hipStreamSynchronize(h->stream), "sync counts");
+    for (int i = 0; i < h->n; ++i)
+        if (c[i] != h->counts[i]) apply(i, c[i]);
 }
 
 // ---- pair jobs ---------------------------------------------------------------
@@ -1331,6 +1366,151 @@ void relocate_classes(rsx_engine* h, const std::vector<int>& M, const std::vecto
     for (int sl : touched) mark[sl] = 0;
 }
 
+// byte offsets of arrays packed into one staging block, each 16-byte aligned
+struct Pack {
+    size_t size = 0;
+    size_t add(size_t bytes)
+    {
+        const size_t off = size;
+        size += (bytes + 15) / 16 * 16;
+        return off;
+    }
+};
+
+// The staging block made to hold `bytes`; returns its host side to fill.  The device side
+// grows to powers of two from 64 KiB, so its address rarely changes.
+unsigned char* stage_begin(rsx_engine::Staging& st, size_t bytes)
+{
+    st.host.ensure(bytes);
+    if (bytes > st.dev.cap) {
+        size_t c = 65536;
+        while (c < bytes) c <<= 1;
+        st.dev.ensure(c);
+    }
+    return st.host.p;
+}
+
+void stage_upload(rsx_engine::Staging& st, size_t bytes, hipStream_t stream, const char* what)
+{
+    hipchk(hipMemcpyAsync(st.dev.p, st.host.p, bytes, hipMemcpyHostToDevice, stream), what);
+}
+
+// Job table of one launch of an update (part 0: old records, 1: new records): the nrows
+// rows (on the device) against the columns with weight 2 w and against themselves with
+// weight w.  Tables depend only on the shape and the buffers and are kept for the shapes
+// that recur (the least recently used one is replaced).
+const rsx_engine::IncTable* update_jobs(rsx_engine* h, int part, const float4* rows, int nrows, float w,
+                                        const KernelChoice& kc, hipStream_t stream)
+{
+    // about two rounds of waves, the tail split (measured against six rounds unsplit:
+    // 0.834 -> 0.824 ms plain, 2.23 -> 2.17 ms hydrated per 500-atom move at 50k atoms)
+    const int L = pick_range(h, (double)nrows * (double)h->n, kc.variant, kc.K, kc.n_chunks, 2.0);
+    rsx_engine::IncKey key;
+    key.part = part;
+    key.nrows = nrows;
+    key.n = h->n;
+    key.L = L;
+    key.K = kc.K;
+    key.n_chunks = kc.n_chunks;
+    key.variant = kc.variant;
+    key.uniform = h->uniform ? 1 : 0;
+    key.rows = rows;
+    key.cols = h->d_cols.p;
+    rsx_engine::IncTable* tab = nullptr;
+    for (auto& t : h->inc_tab)
+        if (t.njobs > 0 && t.key == key) tab = &t;
+    if (!tab) {
+        tab = &h->inc_tab[0];
+        for (auto& t : h->inc_tab)
+            if (t.used < tab->used) tab = &t;
+        std::vector<PairJob>& jobs = h->inc_jobs;
+        jobs.clear();
+        rect_jobs(jobs, rows, nrows, h->d_cols.p, h->n, L, 2.0f * w);
+        rect_jobs(jobs, rows, nrows, rows, nrows, L, w);
+        split_tail(jobs, resident_jobs(h, kc.variant, kc.K, kc.n_chunks),
+                   [](const PairJob& j) { return (double)j.ncol * j.nrow; });
+        tab->jobs.ensure(jobs.size());
+        // staged through pinned memory, one buffer per part (the call waits for the
+        // update's result before returning, so the buffer is free again at the next one)
+        HBuf<PairJob>& pin = h->pin_jobs[part];
+        pin.ensure(jobs.size());
+        std::memcpy(pin.p, jobs.data(), jobs.size() * sizeof(PairJob));
+        hipchk(hipMemcpyAsync(tab->jobs.p, pin.p, jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice, stream),
+               "upload jobs");
+        tab->key = key;
+        tab->njobs = (int)jobs.size();
+    }
+    tab->used = ++h->inc_clock;
+    return tab;
+}
+
+// the pair kernel over nj jobs, partial sums in rows 0..nj-1 of partials
+void launch_update_pairs(rsx_engine* h, const PairJob* d_jobs, int nj, double* partials, const KernelChoice& kc,
+                         hipStream_t stream)
+{
+    rsx::DebyeArgs a = debye_args(h, d_jobs, partials);
+    a.variant = kc.variant;
+    a.n_chunks = kc.n_chunks;
+    a.chunks = kc.chunks;
+    hipchk(rsx::launch_debye(a, nj, kc.K, h->uniform, stream), "debye kernel");
+}
+
+// the sum of those partials added to d_iq (slot range and q = 0 value read from the
+// device), on the engine stream (the two launches of an update add in a fixed order)
+void reduce_update_part(rsx_engine* h, int nj, const double* partials, const int* d_sb, const double* d_q0)
+{
+    // the largest reduction scratch of one output: both launches of an update share it
+    h->d_red.ensure(std::max<size_t>(1, rsx::reduce_scratch_doubles(h->nq, 1, 1 << 24)));
+    hipchk(rsx::launch_reduce(partials, h->ldf, h->nq, d_sb, 1, h->d_iq.p, h->nq, 1, d_q0, h->q0_index, nj,
+                              h->d_red.p, h->stream),
+           "reduce kernel");
+}
+
+// The old-records launch of an update: S = [start, start + len) removed from the columns
+// (skip records), its old records (old_bodies, old classes) sorted by class as rows, with
+// weight -1 (nothing is read from the host afterwards, so the work stays queued).
+void enqueue_old_part(rsx_engine* h, int start, int len, const std::vector<float>& old_bodies, const KernelChoice& kc)
+{
+    if (len <= 0) return;
+    const int mp = (len + 63) / 64 * 64;
+    std::vector<std::pair<uint32_t, int>> order((size_t)len);
+    for (int a = 0; a < len; ++a) order[a] = {h->cls[start + a], a};
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<uint32_t, int>& x, const std::pair<uint32_t, int>& y) { return x.first < y.first; });
+    Pack pk;
+    const size_t off_rows = pk.add((size_t)mp * sizeof(float4));
+    const size_t off_idx = pk.add((size_t)len * sizeof(int));
+    const size_t off_sb = pk.add(2 * sizeof(int));
+    const size_t off_q0 = pk.add(sizeof(double));
+    unsigned char* hb = stage_begin(h->st_old, pk.size);
+    float4* rows = reinterpret_cast<float4*>(hb + off_rows);
+    for (int t = 0; t < len; ++t) {
+        const float* ob = &old_bodies[4 * (size_t)order[t].second];
+        rows[t] = make_float4(ob[0], ob[1], ob[2], bitsf(order[t].first));
+    }
+    for (int t = len; t < mp; ++t) rows[t] = make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP));
+    int* idx = reinterpret_cast<int*>(hb + off_idx);
+    for (int a = 0; a < len; ++a) idx[a] = h->slot_of[start + a];
+    unsigned char* db = h->st_old.dev.p;
+    const rsx_engine::IncTable* tab =
+        update_jobs(h, 0, reinterpret_cast<const float4*>(db + off_rows), mp, -1.0f, kc, h->stream);
+    reinterpret_cast<int*>(hb + off_sb)[0] = 0;
+    reinterpret_cast<int*>(hb + off_sb)[1] = tab->njobs;
+    *reinterpret_cast<double*>(hb + off_q0) = 0.0;
+    stage_upload(h->st_old, pk.size, h->stream, "upload old rows");
+    hipchk(rsx::launch_scatter_value(h->d_cols.p, reinterpret_cast<const int*>(db + off_idx),
+                                     make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP)), len, h->stream),
+           "scatter skip");
+    hipchk(hipEventRecord(h->ev_skip, h->stream), "record skip");
+    h->d_partials.ensure((size_t)tab->njobs * h->ldf);
+    launch_update_pairs(h, tab->jobs.p, tab->njobs, h->d_partials.p, kc, h->stream);
+    reduce_update_part(h, tab->njobs, h->d_partials.p, reinterpret_cast<const int*>(db + off_sb),
+                       reinterpret_cast<const double*>(db + off_q0));
+}
+
+void incremental_update(rsx_engine* h, int start, int len, const std::vector<float>& old_bodies, bool sasa_now,
+                        float* out32, double* out64);
+
 void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int len, float* out32, double* out64)
 {
     if (h->state != 4) fail(RSX_NO_INITIAL_CALC, "recalc_curve() called before an initial call to calc_curve()");
@@ -1376,17 +1556,61 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
         return;
     }
 
-    // atoms whose exposed-dot count changed: from the local recount's subset, or by
-    // comparing all counts after a full recount
+    // moved atoms' types against the SASA radius table before any device work
+    if (sasa_now)
+        for (int i = start; i < start + len; ++i)
+            if (h->type[i] >= h->radii.size()) fail(RSX_INVALID_ARG, "body type index outside the SASA radius table");
+    try {
+        incremental_update(h, start, len, old_bodies, sasa_now, out32, out64);
+    } catch (...) {
+        // a failure part way leaves the columns and the running I(q) part updated: the
+        // next call lays the host bodies out again
+        h->host_stale = true;
+        h->sasa_mirror = false;
+        (void)hipStreamSynchronize(h->stream2);
+        (void)hipStreamSynchronize(h->stream);
+        throw;
+    }
+}
+
+// The incremental part of recalc_curve_impl (bodies loaded, full-recalculation cases
+// handled).
+void incremental_update(rsx_engine* h, int start, int len, const std::vector<float>& old_bodies, bool sasa_now,
+                        float* out32, double* out64)
+{
+    // An update over the moved atoms S = [start, start + len) and the atoms C whose
+    // exposed-dot count (so class) changed without moving runs as two launches against the
+    // columns as they were, S's slots skipped and C's records still the old ones:
+    //   dI = 2 P1 x cols + P1 x P1  -  (2 P2 x cols + P2 x P2)
+    // P2: the old records of S.  P1: the new records of S and, per atom of C, one record of
+    // the delta class (factor F' - F) at its position.  Pairs within the rest cancel; a pair
+    // (c, j), j unchanged, gets 2 (F'_c - F_c) F_j; a pair (c, d) of C gets
+    // (F'_c - F_c) F_d + (F'_d - F_d) F_c from P1 x cols and (F'_c - F_c)(F'_d - F_d) from
+    // P1 x P1, in all F'_c F'_d - F_c F_d; a pair (s, c) gets 2 (F'_s F_c - F_s F_c) from the
+    // rectangles and 2 F'_s (F'_c - F_c) from P1 x P1.  The P2 launch needs no new count:
+    // it is queued on the engine stream right behind the SASA recount (which it would
+    // otherwise hold up) and runs while the host reads the counts and builds P1; the P1
+    // launch joins it on the second stream, and both sums are added to I(q) on the engine
+    // stream, P2's first.
+    const bool rebase = h->rebase_every > 0 && h->since_rebase + 1 >= h->rebase_every;
+    const bool try_local = sasa_now && len > 0 && (int)h->counts.size() == h->n && h->have_rsasa;
+    KernelChoice kc;
+    incremental_kernel(h, kc.variant, kc.K, kc.n_chunks, kc.chunks);
+    const bool local = try_local && sasa_local_enqueue(h, start, len, old_bodies);
+    const bool old_early = len > 0 && !rebase && (local || !sasa_now);
+    if (old_early) enqueue_old_part(h, start, len, old_bodies, kc);
+
+    // atoms whose exposed-dot count changed
     std::vector<int> changed;
-    if (sasa_now && !compute_sasa_local(h, start, len, old_bodies, changed)) {
+    if (local) {
+        sasa_local_finish(h, changed);
+    } else if (sasa_now) {
         const std::vector<uint32_t> old_counts = h->counts;
         compute_sasa(h);
-        changed.clear();
         for (int i = 0; i < h->n; ++i)
             if (old_counts[i] != h->counts[i]) changed.push_back(i);
     }
-    // changed set: moved atoms plus atoms whose class changed
+    // changed set M: S, then C ascending
     std::vector<int> M;
     M.reserve((size_t)len + changed.size());
     std::vector<char>& in_m = h->in_m;
@@ -1405,142 +1629,46 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
         copy_out(h, out32, out64);
         return;
     }
-    if (h->rebase_every > 0 && h->since_rebase + 1 >= h->rebase_every) {
+    if (rebase) {
         assign_slots(h);
         full_forward(h);
         copy_out(h, out32, out64);
         return;
     }
+    if (!old_early) enqueue_old_part(h, start, len, old_bodies, kc);
 
     const int m = (int)M.size();
-    const int mp = (m + 63) / 64 * 64;
     std::vector<uint32_t> cls_new(m), cls_old(m);
-    std::vector<float4> rec_new(m), rec_old(m);
     for (int a = 0; a < m; ++a) {
-        const int i = M[a];
-        cls_old[a] = h->cls[i];
-        cls_new[a] = class_of(h, i);
-        const float* nb = &h->bodies[4 * (size_t)i];
-        const float* ob = (i >= start && i < start + len) ? &old_bodies[4 * (size_t)(i - start)] : nb;
-        rec_new[a] = make_float4(nb[0], nb[1], nb[2], bitsf(cls_new[a]));
-        rec_old[a] = make_float4(ob[0], ob[1], ob[2], bitsf(cls_old[a]));
+        cls_old[a] = h->cls[M[a]];
+        cls_new[a] = class_of(h, M[a]);
     }
-    // Rows, four segments, each sorted by class (row tiles of few classes, and class runs
-    // where rows serve as columns) and padded with skip records to a multiple of 64:
-    //   [M new (mp) | M old (mp)]  the M x M blocks (+1 / -1);
-    //   [P1 (p1) | P2 (p2)]        against the unchanged atoms: P1 = moved atoms' new
-    //                              records and, for atoms whose exposed-dot count changed
-    //                              without moving (u'_ij = u_ij there), one delta-class
-    //                              record carrying F' - F (+2); P2 = the moved atoms' old
-    //                              records (-2).
-    std::vector<int> p1_of, p2_of;  // indices into M
-    std::vector<float4> rec_p1;
+    // P1, sorted by class (stable), padded with skip records to a multiple of 64
+    std::vector<std::pair<uint32_t, int>> p1_order((size_t)m);
     for (int a = 0; a < m; ++a) {
-        const int i = M[a];
-        if (i >= start && i < start + len) {
-            p1_of.push_back(a);
-            p2_of.push_back(a);
-            rec_p1.push_back(rec_new[a]);
-        } else {
-            const int d = (int)cls_new[a] - (int)cls_old[a];  // same type: the count change
-            p1_of.push_back(a);
-            rec_p1.push_back(make_float4(rec_new[a].x, rec_new[a].y, rec_new[a].z, bitsf(delta_class(h, d))));
-        }
+        const bool moved = M[a] >= start && M[a] < start + len;
+        const uint32_t c = moved ? cls_new[a] : delta_class(h, (int)cls_new[a] - (int)cls_old[a]);
+        p1_order[a] = {c, a};
     }
-    const int n1 = (int)p1_of.size(), n2 = (int)p2_of.size();
-    // every changed atom moved (always so without hydration): P1, P2 are the M segments
-    const bool p_is_m = n2 == m;
-    const int p1 = p_is_m ? 0 : (n1 + 63) / 64 * 64, p2 = p_is_m ? 0 : (n2 + 63) / 64 * 64;
-    std::vector<float4> rows(2 * (size_t)mp + p1 + p2, make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP)));
-    auto place = [&](size_t at, int cnt, auto&& rec, auto&& cls) {
-        std::vector<int> order(cnt);
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cls(x) < cls(y); });
-        for (int t = 0; t < cnt; ++t) rows[at + t] = rec(order[t]);
-    };
-    place(0, m, [&](int a) { return rec_new[a]; }, [&](int a) { return cls_new[a]; });
-    place(mp, m, [&](int a) { return rec_old[a]; }, [&](int a) { return cls_old[a]; });
-    if (!p_is_m) {
-        place(2 * (size_t)mp, n1, [&](int t) { return rec_p1[t]; }, [&](int t) { return fbits(rec_p1[t].w); });
-        place(2 * (size_t)mp + p1, n2, [&](int t) { return rec_old[p2_of[t]]; },
-              [&](int t) { return cls_old[p2_of[t]]; });
+    std::stable_sort(p1_order.begin(), p1_order.end(),
+                     [](const std::pair<uint32_t, int>& x, const std::pair<uint32_t, int>& y) { return x.first < y.first; });
+    const int p1 = (m + 63) / 64 * 64;
+    Pack pk;
+    const size_t off_rows = pk.add((size_t)p1 * sizeof(float4));
+    const size_t off_sb = pk.add(2 * sizeof(int));
+    const size_t off_q0 = pk.add(sizeof(double));
+    unsigned char* hb = stage_begin(h->st_new, pk.size);
+    float4* rows = reinterpret_cast<float4*>(hb + off_rows);
+    for (int t = 0; t < m; ++t) {
+        const float* nb = &h->bodies[4 * (size_t)M[p1_order[t].second]];
+        rows[t] = make_float4(nb[0], nb[1], nb[2], bitsf(p1_order[t].first));
     }
-    // column edits: first remove M from the columns, later write the new records
-    std::vector<int> slots(m);
-    std::vector<float4> vals(2 * (size_t)m);
-    for (int a = 0; a < m; ++a) {
-        slots[a] = h->slot_of[M[a]];
-        vals[a] = make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP));
-        vals[m + a] = rec_new[a];
-    }
-    h->d_rows.ensure(rows.size());
-    h->d_idx.ensure(m);
-    h->d_vals.ensure(2 * (size_t)m);
-    h->pin_rows.ensure(rows.size());
-    h->pin_idx.ensure(m);
-    h->pin_vals.ensure(vals.size());
-    std::memcpy(h->pin_rows.p, rows.data(), rows.size() * sizeof(float4));
-    std::memcpy(h->pin_idx.p, slots.data(), m * sizeof(int));
-    std::memcpy(h->pin_vals.p, vals.data(), vals.size() * sizeof(float4));
-    hipchk(hipMemcpyAsync(h->d_rows.p, h->pin_rows.p, rows.size() * sizeof(float4), hipMemcpyHostToDevice, h->stream),
-           "upload rows");
-    hipchk(hipMemcpyAsync(h->d_idx.p, h->pin_idx.p, m * sizeof(int), hipMemcpyHostToDevice, h->stream),
-           "upload idx");
-    hipchk(hipMemcpyAsync(h->d_vals.p, h->pin_vals.p, vals.size() * sizeof(float4), hipMemcpyHostToDevice,
-                          h->stream),
-           "upload vals");
-    hipchk(rsx::launch_scatter(h->d_cols.p, h->d_idx.p, h->d_vals.p, m, h->stream), "scatter skip");
-
-    // the job table depends only on the shape (moved-set size, N, column range, kernel) and
-    // the buffers: reused as it is on the device while those are unchanged
-    KernelChoice kc;
-    incremental_kernel(h, kc.variant, kc.K, kc.n_chunks, kc.chunks);
-    const int q1 = p_is_m ? mp : p1, q2 = p_is_m ? mp : p2;  // row counts of the segments against the rest
-    // about two rounds of waves, the tail split (measured against six rounds unsplit:
-    // 0.834 -> 0.824 ms plain, 2.23 -> 2.17 ms hydrated per 500-atom move at 50k atoms)
-    const int L =
-        pick_range(h, (double)(q1 + q2) * (double)h->n, kc.variant, kc.K, kc.n_chunks, 2.0);
-    rsx_engine::IncKey key;
-    key.mp = mp;
-    key.p1 = p1;
-    key.p2 = p2;
-    key.n = h->n;
-    key.L = L;
-    key.K = kc.K;
-    key.n_chunks = kc.n_chunks;
-    key.variant = kc.variant;
-    key.uniform = h->uniform ? 1 : 0;
-    key.rows = h->d_rows.p;
-    key.cols = h->d_cols.p;
-    rsx_engine::IncTable* tab = nullptr;
-    for (auto& t : h->inc_tab)
-        if (t.njobs > 0 && t.key == key) tab = &t;
-    if (!tab) {
-        tab = &h->inc_tab[0];
-        for (auto& t : h->inc_tab)
-            if (t.used < tab->used) tab = &t;
-        std::vector<PairJob>& jobs = h->inc_jobs;
-        jobs.clear();
-        const float4* P1 = p_is_m ? h->d_rows.p : h->d_rows.p + 2 * (size_t)mp;
-        const float4* P2 = p_is_m ? h->d_rows.p + mp : P1 + p1;
-        rect_jobs(jobs, P1, q1, h->d_cols.p, h->n, L, 2.0f);                    // new (+ delta) x rest
-        rect_jobs(jobs, P2, q2, h->d_cols.p, h->n, L, -2.0f);                   // old x rest
-        rect_jobs(jobs, h->d_rows.p, mp, h->d_rows.p, mp, L, 1.0f);             // new x new
-        rect_jobs(jobs, h->d_rows.p + mp, mp, h->d_rows.p + mp, mp, L, -1.0f);  // old x old
-        split_tail(jobs, resident_jobs(h, kc.variant, kc.K, kc.n_chunks),
-                   [](const PairJob& j) { return (double)j.ncol * j.nrow; });
-        tab->jobs.ensure(jobs.size());
-        // staged through pinned memory (the call waits for the move's result before
-        // returning, so the staging buffer is free again at the next move)
-        h->pin_jobs.ensure(jobs.size());
-        std::memcpy(h->pin_jobs.p, jobs.data(), jobs.size() * sizeof(PairJob));
-        hipchk(hipMemcpyAsync(tab->jobs.p, h->pin_jobs.p, jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice,
-                              h->stream),
-               "upload jobs");
-        tab->key = key;
-        tab->njobs = (int)jobs.size();
-    }
-    tab->used = ++h->inc_clock;
+    for (int t = m; t < p1; ++t) rows[t] = make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP));
+    unsigned char* db = h->st_new.dev.p;
+    // the new-records launch reads the columns once S is skipped there
+    if (len > 0) hipchk(hipStreamWaitEvent(h->stream2, h->ev_skip, 0), "wait skip");
+    const rsx_engine::IncTable* tab =
+        update_jobs(h, 1, reinterpret_cast<const float4*>(db + off_rows), p1, 1.0f, kc, h->stream2);
 
     // q = 0: (sum F_new)^2 - (sum F_old)^2 from the class populations
     const double s_old = sum_f0(h);
@@ -1556,35 +1684,40 @@ void recalc_curve_impl(rsx_engine* h, const float* bodies, int n, int start, int
         }
     }
     const double s_new = sum_f0(h);
-    const double q0d = s_new * s_new - s_old * s_old;
-    launch_jobs(h, tab->jobs.p, tab->njobs, h->d_iq.p, 1, q0d, false, &kc);
+    reinterpret_cast<int*>(hb + off_sb)[0] = 0;
+    reinterpret_cast<int*>(hb + off_sb)[1] = tab->njobs;
+    *reinterpret_cast<double*>(hb + off_q0) = s_new * s_new - s_old * s_old;
+    stage_upload(h->st_new, pk.size, h->stream2, "upload new rows");
+    h->d_partials_new.ensure((size_t)tab->njobs * h->ldf);
+    launch_update_pairs(h, tab->jobs.p, tab->njobs, h->d_partials_new.p, kc, h->stream2);
+    hipchk(hipEventRecord(h->ev_new, h->stream2), "record new");
+    hipchk(hipStreamWaitEvent(h->stream, h->ev_new, 0), "wait new");
+    reduce_update_part(h, tab->njobs, h->d_partials_new.p, reinterpret_cast<const int*>(db + off_sb),
+                       reinterpret_cast<const double*>(db + off_q0));
+
     // the new records, each in the run of its class (the slots of atoms whose class changed
-    // are exchanged across runs), written after the kernel on the same stream
+    // are exchanged across runs), written after the launches on the same stream
     std::vector<int> touched;
     touched.reserve(2 * (size_t)m);
     relocate_classes(h, M, cls_old, cls_new, run_start, touched);
     const int nt = (int)touched.size();
-    h->d_rslot.ensure(nt);
-    h->d_rcaller.ensure(nt);
-    h->d_rvals.ensure(nt);
-    h->pin_rslot.ensure(nt);
-    h->pin_rcaller.ensure(nt);
-    h->pin_rvals.ensure(nt);
+    Pack pr;
+    const size_t off_slot = pr.add((size_t)nt * sizeof(int));
+    const size_t off_caller = pr.add((size_t)nt * sizeof(int));
+    const size_t off_vals = pr.add((size_t)nt * sizeof(float4));
+    unsigned char* hr = stage_begin(h->st_reloc, pr.size);
     for (int t = 0; t < nt; ++t) {
         const int sl = touched[t], i = h->caller_of[sl];
         const float* bb = &h->bodies[4 * (size_t)i];
-        h->pin_rslot.p[t] = sl;
-        h->pin_rcaller.p[t] = i;
-        h->pin_rvals.p[t] = make_float4(bb[0], bb[1], bb[2], bitsf(h->cls[i]));
+        reinterpret_cast<int*>(hr + off_slot)[t] = sl;
+        reinterpret_cast<int*>(hr + off_caller)[t] = i;
+        reinterpret_cast<float4*>(hr + off_vals)[t] = make_float4(bb[0], bb[1], bb[2], bitsf(h->cls[i]));
     }
-    hipchk(hipMemcpyAsync(h->d_rslot.p, h->pin_rslot.p, nt * sizeof(int), hipMemcpyHostToDevice, h->stream),
-           "upload relocated slots");
-    hipchk(hipMemcpyAsync(h->d_rcaller.p, h->pin_rcaller.p, nt * sizeof(int), hipMemcpyHostToDevice, h->stream),
-           "upload relocated atoms");
-    hipchk(hipMemcpyAsync(h->d_rvals.p, h->pin_rvals.p, nt * sizeof(float4), hipMemcpyHostToDevice, h->stream),
-           "upload relocated records");
-    hipchk(rsx::launch_scatter_slots(h->d_cols.p, h->d_s2c.p, h->d_rslot.p, h->d_rvals.p, h->d_rcaller.p, nt,
-                                     h->stream),
+    stage_upload(h->st_reloc, pr.size, h->stream, "upload relocated records");
+    unsigned char* dr = h->st_reloc.dev.p;
+    hipchk(rsx::launch_scatter_slots(h->d_cols.p, h->d_s2c.p, reinterpret_cast<const int*>(dr + off_slot),
+                                     reinterpret_cast<const float4*>(dr + off_vals),
+                                     reinterpret_cast<const int*>(dr + off_caller), nt, h->stream),
            "scatter new");
     h->since_rebase++;
     copy_out(h, out32, out64);
@@ -2703,7 +2836,11 @@ int rsx_create(int device, rsx_engine** out)
     h->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
-        hipEventCreate(&h->ev2) != hipSuccess || hipEventCreate(&h->ev3) != hipSuccess) {
+        hipEventCreate(&h->ev2) != hipSuccess || hipEventCreate(&h->ev3) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_sasa, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_skip, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_new, hipEventDisableTiming) != hipSuccess) {
         delete h;
         return RSX_HIP_ERROR;
     }
@@ -2727,8 +2864,8 @@ int rsx_destroy(rsx_engine* h)
     hipSetDevice(h->device);
     batch_release_graph(h);
     if (h->stream) hipStreamSynchronize(h->stream);
-    DBuf<float4>* f4[] = {&h->d_cols, &h->d_saved_cols, &h->d_rows, &h->d_vals, &h->d_sasa_atoms, &h->d_dots, &h->b_cols,
-                          &h->d_sorted_atoms};
+    if (h->stream2) hipStreamSynchronize(h->stream2);
+    DBuf<float4>* f4[] = {&h->d_cols, &h->d_saved_cols, &h->d_sasa_atoms, &h->d_dots, &h->b_cols, &h->d_sorted_atoms};
     for (auto* b : f4) b->release();
     h->d_qrev.release();
     DBuf<float>* ff[] = {&h->d_ftab, &h->d_qv, &h->d_r2, &h->d_rext, &h->d_grad, &h->a_ftab, &h->a_rowpart,
@@ -2736,24 +2873,22 @@ int rsx_destroy(rsx_engine* h)
     for (auto* b : ff) b->release();
     DBuf<double>* fd[] = {&h->d_invq, &h->d_partials, &h->d_iq, &h->d_saved_iq, &h->d_out, &h->d_wq, &h->d_q0,
                           &h->d_red, &h->b_partials, &h->b_out, &h->b_mean, &h->b_q0, &h->b_red, &h->b_invq,
-                          &h->b_qrev};
+                          &h->b_qrev, &h->d_partials_new};
     for (auto* b : fd) b->release();
-    DBuf<int>* fi[] = {&h->d_slot_begin, &h->d_idx, &h->d_bucket_of, &h->d_bucket_count, &h->d_bucket_start,
-                       &h->d_bucket_fill, &h->d_sorted, &h->d_s2c, &h->b_slot_begin, &h->d_subset,
-                       &h->d_tile_sum};
+    DBuf<int>* fi[] = {&h->d_slot_begin, &h->d_bucket_of, &h->d_bucket_count, &h->d_bucket_start, &h->d_bucket_fill,
+                       &h->d_sorted, &h->d_s2c, &h->b_slot_begin, &h->d_local_list, &h->d_local, &h->d_tile_sum};
     for (auto* b : fi) b->release();
     h->d_chunks.release();
     h->b_chunks.release();
     h->b_pin_ftab.release();
     h->b_pin_q0.release();
-    h->d_subcnt.release();
-    h->pin_rows.release();
-    h->pin_jobs.release();
-    h->pin_vals.release();
+    for (rsx_engine::Staging* st : {&h->st_old, &h->st_new, &h->st_reloc}) {
+        st->host.release();
+        st->dev.release();
+    }
+    for (auto& b : h->pin_jobs) b.release();
     h->pin_moved.release();
-    h->pin_idx.release();
-    h->pin_sub.release();
-    h->pin_cnt.release();
+    h->pin_local.release();
     h->pin_iq.release();
     h->a_chunks.release();
     h->a_invq.release();
@@ -2777,17 +2912,15 @@ int rsx_destroy(rsx_engine* h)
     h->d_queue.release();
     h->d_guard.release();
     h->d_cs_table.release();
-    h->d_rslot.release();
-    h->d_rcaller.release();
-    h->d_rvals.release();
-    h->pin_rslot.release();
-    h->pin_rcaller.release();
-    h->pin_rvals.release();
     h->d_cs_tot.release();
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->ev2) hipEventDestroy(h->ev2);
     if (h->ev3) hipEventDestroy(h->ev3);
+    if (h->ev_sasa) hipEventDestroy(h->ev_sasa);
+    if (h->ev_skip) hipEventDestroy(h->ev_skip);
+    if (h->ev_new) hipEventDestroy(h->ev_new);
+    if (h->stream2) hipStreamDestroy(h->stream2);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
     return RSX_OK;

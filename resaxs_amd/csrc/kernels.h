@@ -111,13 +111,11 @@ size_t reduce_scratch_doubles(int n_q, int n_out, int max_slots);
 // out[i] = mean over frames (uniform weights) of in[f*n_q + i]
 hipError_t launch_frame_mean(const double* in, int n_frames, int n_q, double* out, hipStream_t s);
 
-// cols[slot[t]] = vals[t]
-hipError_t launch_scatter(float4* cols, const int* slot, const float4* vals, int m, hipStream_t s);
+// cols[slot[t]] = v for t < m
+hipError_t launch_scatter_value(float4* cols, const int* slot, float4 v, int m, hipStream_t s);
 // cols[slot[t]] = vals[t] and s2c[slot[t]] = callers[t] (slots whose atom changed)
 hipError_t launch_scatter_slots(float4* cols, int* s2c, const int* slot, const float4* vals, const int* callers, int m,
                                 hipStream_t s);
-// dst[t] = src[idx[t]] for t < m
-hipError_t launch_gather_u32(uint32_t* dst, const uint32_t* src, const int* idx, int m, hipStream_t s);
 
 // cols[s] = {bodies[src[s]].xyz, class}, class = type(bodies[src[s]]) * d1 + (counts ? counts[src[s]] : 0).
 // Checks of device-resident bodies, reported through *flag (bits or-ed in, never cleared
@@ -176,6 +174,18 @@ struct SasaArgs {
     uint32_t* counts;        // [n] exposed dots (only the listed atoms when subset is set)
     const int* subset;       // null: count every atom; else count atoms subset[0 .. n_subset)
     int n_subset;
+    // Local recount (local_hdr set; subset unused): the place kernel lists the atoms whose
+    // position lies in the box [box_lo, box_hi] in local_list (in no fixed order); the dots
+    // kernel counts only those, over a fixed grid, and for each count that differs from the
+    // one in counts appends (atom, new count) to local_changed before overwriting it.
+    // local_hdr[0] = atoms listed, local_hdr[1] = counts changed (entries from changed_cap on
+    // are counted but not written); the scan clears both.
+    int* local_hdr = nullptr;
+    int* local_list = nullptr;   // [n]
+    int* local_changed = nullptr;  // [changed_cap][2]
+    int changed_cap = 0;
+    float box_lo[3] = {0.f, 0.f, 0.f};
+    float box_hi[3] = {0.f, 0.f, 0.f};
 };
 hipError_t launch_sasa(const SasaArgs& a, hipStream_t s);
 

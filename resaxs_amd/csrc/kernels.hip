@@ -909,10 +909,10 @@ __global__ void frame_mean_kernel(const double* __restrict__ in, int n_frames, i
     out[k] = s / (double)n_frames;
 }
 
-__global__ void scatter_kernel(float4* cols, const int* __restrict__ slot, const float4* __restrict__ vals, int m)
+__global__ void scatter_value_kernel(float4* cols, const int* __restrict__ slot, float4 v, int m)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < m) cols[slot[t]] = vals[t];
+    if (t < m) cols[slot[t]] = v;
 }
 
 __global__ void scatter_slots_kernel(float4* cols, int* s2c, const int* __restrict__ slot,
@@ -923,12 +923,6 @@ __global__ void scatter_slots_kernel(float4* cols, int* s2c, const int* __restri
         cols[slot[t]] = vals[t];
         s2c[slot[t]] = callers[t];
     }
-}
-
-__global__ void gather_u32_kernel(uint32_t* dst, const uint32_t* __restrict__ src, const int* __restrict__ idx, int m)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < m) dst[t] = src[idx[t]];
 }
 
 __global__ void gather_cols_kernel(float4* cols, const float4* __restrict__ bodies, const int* __restrict__ src,
@@ -1235,11 +1229,30 @@ __global__ __launch_bounds__(256) void sasa_scan_add_kernel(SasaArgs a)
     const int v = a.bucket_start[b] + a.tile_sum[b / SCAN_TILE];
     a.bucket_start[b] = v;
     a.bucket_fill[b] = v;
+    if (a.local_hdr && b < 2) a.local_hdr[b] = 0;
 }
 
+// Places atom i in its bucket; for a local recount also lists it when inside the box (one
+// atomic per wave for the wave's listed atoms).
 __global__ void sasa_place_kernel(SasaArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.local_hdr) {  // uniform
+        bool in = false;
+        if (i < a.n) {
+            const float4 p = a.atoms[i];
+            in = p.x >= a.box_lo[0] && p.x <= a.box_hi[0] && p.y >= a.box_lo[1] && p.y <= a.box_hi[1] &&
+                 p.z >= a.box_lo[2] && p.z <= a.box_hi[2];
+        }
+        const uint64_t m = __ballot(in);
+        if (m) {
+            const int lane = threadIdx.x & 63, first = __builtin_ctzll(m);
+            int base = 0;
+            if (lane == first) base = atomicAdd(&a.local_hdr[0], __builtin_popcountll(m));
+            base = __shfl(base, first, 64);
+            if (in) a.local_list[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = i;
+        }
+    }
     if (i >= a.n) return;
     const int pos = atomicAdd(&a.bucket_fill[a.bucket_of[i]], 1);
     a.sorted[pos] = i;
@@ -1258,16 +1271,11 @@ constexpr int SASA_MAX_DPL = 8;  // dots per lane (n_dots <= 512)
 // expression in the same order whatever the visiting order, so the counts do not
 // depend on it.
 template <int DPL>
-__global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
+__device__ __forceinline__ int sasa_count_atom(const SasaArgs& a, int i, int lane)
 {
-    const int i = a.subset ? a.subset[blockIdx.x] : (int)blockIdx.x;
-    const int lane = threadIdx.x;
     const float4 pi = a.atoms[i];
     const uint32_t ti = __float_as_uint(pi.w);
-    if (ti >= (uint32_t)a.n_types) {  // wave-uniform: no dots
-        if (lane == 0) a.counts[i] = 0u;
-        return;
-    }
+    if (ti >= (uint32_t)a.n_types) return 0;  // wave-uniform: no dots
     const float rext_i = a.rext[ti];
     float px[DPL], py[DPL], pz[DPL];
     bool live[DPL];
@@ -1342,7 +1350,33 @@ __global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
     int cnt = 0;
 #pragma unroll
     for (int t = 0; t < DPL; ++t) cnt += live[t] ? 1 : 0;
-    cnt = wave_sum_i(cnt);
+    return wave_sum_i(cnt);
+}
+
+template <int DPL>
+__global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
+{
+    const int lane = threadIdx.x;
+    if (a.local_hdr) {
+        const int listed = a.local_hdr[0];
+        for (int k = blockIdx.x; k < listed; k += gridDim.x) {
+            const int i = a.local_list[k];
+            const uint32_t c = (uint32_t)sasa_count_atom<DPL>(a, i, lane);
+            if (lane == 0) {
+                if (a.counts[i] != c) {
+                    const int e = atomicAdd(&a.local_hdr[1], 1);
+                    if (e < a.changed_cap) {
+                        a.local_changed[2 * e] = i;
+                        a.local_changed[2 * e + 1] = (int)c;
+                    }
+                }
+                a.counts[i] = c;
+            }
+        }
+        return;
+    }
+    const int i = a.subset ? a.subset[blockIdx.x] : (int)blockIdx.x;
+    const int cnt = sasa_count_atom<DPL>(a, i, lane);
     if (lane == 0) a.counts[i] = (uint32_t)cnt;
 }
 
@@ -1946,17 +1980,10 @@ hipError_t launch_scatter_slots(float4* cols, int* s2c, const int* slot, const f
     return hipGetLastError();
 }
 
-hipError_t launch_scatter(float4* cols, const int* slot, const float4* vals, int m, hipStream_t s)
+hipError_t launch_scatter_value(float4* cols, const int* slot, float4 v, int m, hipStream_t s)
 {
     if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, cols, slot, vals, m);
-    return hipGetLastError();
-}
-
-hipError_t launch_gather_u32(uint32_t* dst, const uint32_t* src, const int* idx, int m, hipStream_t s)
-{
-    if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL(gather_u32_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, dst, src, idx, m);
+    hipLaunchKernelGGL(scatter_value_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, cols, slot, v, m);
     return hipGetLastError();
 }
 
@@ -2234,7 +2261,8 @@ hipError_t launch_sasa(const SasaArgs& a, hipStream_t s)
     hipLaunchKernelGGL(sasa_scan_top_kernel, dim3(1), dim3(1024), 0, s, a);
     hipLaunchKernelGGL(sasa_scan_add_kernel, dim3((unsigned)(a.n_buckets / 256)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(sasa_place_kernel, g, b, 0, s, a);
-    const int n_waves = a.subset ? a.n_subset : a.n;
+    // a local recount loops over its list with a fixed grid (the list's length is on the device)
+    const int n_waves = a.local_hdr ? (a.n < 8192 ? a.n : 8192) : a.subset ? a.n_subset : a.n;
     if (n_waves <= 0) return hipGetLastError();
     const dim3 gd((unsigned)n_waves), bd(64);
     const int dpl = (a.n_dots + 63) / 64;
