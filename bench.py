@@ -186,20 +186,22 @@ def hydrated_bench(steps, warmup):
     }, sasa_ms
 
 
-def sasa_summary(w, ms):
+def sasa_summary(w, ms, probe=None, n_dots=None, hbm=True):
     """SASA at its algorithmic unit: one (dot, candidate) test over every atom j != i with
     |x_i - x_j| < r_i + r_j + 2p, 10 FP32 FLOP each (distance, subtract, compare); the
     count is a property of the structure, not of the kernel (which stops scanning an atom
     once all its dots are buried, so it executes fewer).  HBM bytes per count from the
-    committed counter summary of the SASA kernels."""
+    committed counter summary of the SASA kernels (taken on the 256-dot sphere)."""
     from scipy.spatial import cKDTree
-    R = (w.radii.astype(np.float64) + w.probe)[w.types]
+    probe = w.probe if probe is None else probe
+    n_dots = 2 * w.n_half if n_dots is None else n_dots
+    R = (w.radii.astype(np.float64) + probe)[w.types]
     pairs = cKDTree(w.pos.astype(np.float64)).query_pairs(2.0 * R.max(), output_type="ndarray")
     d = np.linalg.norm(w.pos[pairs[:, 0]].astype(np.float64) - w.pos[pairs[:, 1]], axis=1)
     close = int(np.count_nonzero(d < R[pairs[:, 0]] + R[pairs[:, 1]]))
-    tests = 2.0 * close * 2 * w.n_half
-    hbm = read_summary(SASA_SUMMARY, "hbm_bytes_per_count")
-    out = {"workload": f"sasa-{len(w.pos)}-atoms-{2 * w.n_half}-dots", "ms": round(ms, 4),
+    tests = 2.0 * close * n_dots
+    hbm = read_summary(SASA_SUMMARY, "hbm_bytes_per_count") if hbm else None
+    out = {"workload": f"sasa-{len(w.pos)}-atoms-{n_dots}-dots-probe-{probe}", "ms": round(ms, 4),
            "kernels": "fill + sasa_bin + sasa_scan_tiles/top/add + sasa_place + sasa_dots (HIP events, engine stream)",
            "tests": tests, "tests_per_s": tests / (ms * 1e-3),
            "achieved_tflops": round(10 * tests / (ms * 1e-3) / 1e12, 3),
@@ -209,6 +211,20 @@ def sasa_summary(w, ms):
         out["achieved_GBps"] = round(hbm / (ms * 1e-3) / 1e9, 1)
         out["hbm_frac"] = round(hbm / (ms * 1e-3) / 1e9 / PEAK_HBM_GBPS, 4)
     return out
+
+
+def sasa_reference_sphere_bench(reps=20):
+    """The SASA count on the 20,000-atom ellipsoid with the reference GPU hydration's dot
+    sphere: 32 dodecahedron + icosahedron dots, probe 1.8 A (saxs_alg_cl_gpu_pt_wf.cpp:91,
+    rsasa_cl_gpu_alg.cpp:73)."""
+    from resaxs_amd import RSX_DOTS_DODECA_ICOSA, configs
+    w = configs.sasa_ellipsoid_20000()
+    e = make_engine(w)
+    e.set_sasa_params(w.radii, 1.8, RSX_DOTS_DODECA_ICOSA, 0)
+    e.calc_curve(w.bodies)
+    ms = e.time_sasa(reps)
+    e.close()
+    return sasa_summary(w, ms, probe=1.8, n_dots=32, hbm=False)
 
 
 def direct_bench(reps=5):
@@ -555,7 +571,7 @@ def main():
             e.calc_curve(w.bodies)
             ms = e.time_sasa(20)
             e.close()
-            return {"ms": ms}
+            return {"fibonacci_256": sasa_summary(w, ms), "dodeca_icosa_32": sasa_reference_sphere_bench()}
 
         run = {"forward_50k": fwd, "hydrated_20k": hyd, "sasa": sasa,
                "fwd_adjoint": lambda: adjoint_bench(),
@@ -597,6 +613,7 @@ def main():
     }
     hw, I_hyd, hyd, sasa_ms = hydrated_bench(args.steps, args.warmup)
     hyd["sasa"] = sasa_summary(hw, sasa_ms)
+    hyd["sasa_reference_sphere"] = sasa_reference_sphere_bench()
     line["hydrated_20k"] = hyd
     if not args.no_adjoint:
         log("bench: forward + adjoint on the fit configuration")
