@@ -114,6 +114,24 @@ def test_nonuniform_grid_direct_path(oracle):
     assert rel_err(I, ref).max() < TOL
 
 
+def test_nonuniform_grid_direct_path_full_size(oracle):
+    """20,000 atoms x 256 q on the non-uniform grid q_k = 0.6 (k / 255)^1.5 (the bench's
+    direct workload: one FP32 sin per term) against the full FP64 oracle."""
+    from resaxs_amd import configs
+    w = configs.sasa_ellipsoid_20000()
+    q = (0.6 * (np.arange(256) / 255.0) ** 1.5).astype(np.float32)
+    table = configs.factor_table(np.random.default_rng(77), q)
+    _say("case: non-uniform q, sasa_ellipsoid_20000 plain 20000 x 256")
+    e = engine_for(q, table, n=len(w.pos))
+    I = e.calc_curve(w.bodies)
+    assert not e.get_tuning()["recurrence"]
+    ref = oracle_iq(q, w.pos, w.types, table)
+    err = rel_err(I, ref)
+    k = int(np.argmax(err))
+    _say(f"  max rel err {err.max():.3e} at q={q[k]:.4f}, median {np.median(err):.3e}")
+    assert err.max() < TOL
+
+
 def test_coincident_atoms_and_tiny_n(oracle):
     """r = 0 pairs give sinc = 1 (gpu_saxs_pt_wf.cl:55); N = 1, 2, 65 edge sizes."""
     _say("case: coincident atoms, N in {1, 2, 3, 65}")
