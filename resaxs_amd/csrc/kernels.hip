@@ -1356,28 +1356,35 @@ __device__ __forceinline__ int sasa_count_atom(const SasaArgs& a, int i, float4*
     }
     const int excl = incl - ccount;
     const int total = __builtin_amdgcn_readlane(incl, 26);
-    for (int base = 0; base < total; base += 64) {
-        // this lane's candidate: position q of the list -> its cell (the first c with
-        // incl[c] > q, by a binary search over lanes 0..31: incl does not decrease and
-        // lanes 27..31 hold the total) and its bucket slot
-        const int q = base + lane;
+    // the candidate at list position q: its cell (the first c with incl[c] > q, by a binary
+    // search over lanes 0..31: incl does not decrease and lanes 27..31 hold the total) and
+    // its record, loaded unconditionally (slot 0 past the list, which `valid` marks) so
+    // that nothing waits for the load before its use
+    auto load = [&](int q, bool& valid) {
         int c = 0;
 #pragma unroll
         for (int step = 16; step >= 1; step >>= 1)
             if (__shfl(incl, c + step - 1, 64) <= q) c += step;
         const int e0 = __shfl(excl, c, 64), s_c = __shfl(cstart, c, 64);
+        valid = q < total;
+        return a.sorted_atoms[valid ? s_c + (q - e0) : 0];
+    };
+    bool next_valid;
+    float4 next = load(lane, next_valid);
+    for (int base = 0; base < total; base += 64) {
+        // this chunk's candidate per lane; the next chunk's lookup and load go out after the
+        // prefilter's table loads and before this chunk's tests, which hide their latency
+        const float4 pj = next;
         bool pass = false;
         float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (q < total) {
-            const float4 pj = a.sorted_atoms[s_c + (q - e0)];
-            const uint32_t tj = __float_as_uint(pj.w);
-            if (tj < (uint32_t)a.n_types) {
-                const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-                const float lim = rext_i + a.rext[tj] + 0.01f;
-                pass = ex * ex + ey * ey + ez * ez < lim * lim;
-                rec = make_float4(pj.x, pj.y, pj.z, a.r2[tj]);
-            }
+        const uint32_t tj = next_valid ? __float_as_uint(pj.w) : 0xffffffffu;
+        if (tj < (uint32_t)a.n_types) {
+            const float ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+            const float lim = rext_i + a.rext[tj] + 0.01f;
+            pass = ex * ex + ey * ey + ez * ez < lim * lim;
+            rec = make_float4(pj.x, pj.y, pj.z, a.r2[tj]);
         }
+        next = load(base + 64 + lane, next_valid);  // unconditional: one path for the wait counts
         const uint64_t m = __ballot(pass);
         const int np = __builtin_popcountll(m);
         if (pass) cand[__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = rec;
