@@ -72,7 +72,8 @@ int rsx_set_no_rsasa_recalc(rsx_engine *h, int flag);
 /* SASA set-up used for hydration: per-type radii, probe radius, dot sphere.
  * kind RSX_DOTS_FIBONACCI uses n_dots = 2*n_half; RSX_DOTS_DODECA_ICOSA ignores n_half. */
 int rsx_set_sasa_params(rsx_engine *h, const float *radii, int n_types, float probe, int kind, int n_half);
-/* full-recompute period of the running incremental I(q) (0 = never) */
+/* full-recompute period of the running incremental I(q) (default 10000 updates; 0 = never).
+ * No counterpart in the reference, which re-reduces its stored block sums on every update. */
 int rsx_set_rebaseline(rsx_engine *h, int every_n_recalcs);
 /* size the engine for n_bodies (saxs_cl_gpu::initialize + set_args, saxs_alg_cl_gpu_pt_wf.cpp:176-263) */
 int rsx_initialize(rsx_engine *h, int n_bodies);

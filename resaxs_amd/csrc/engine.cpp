@@ -312,7 +312,12 @@ struct rsx_engine {
     int variant = rsx::VARIANT_FP32R;  // recurrence kernel (rsx::VARIANT_*)
     int K = 0, n_chunks = 0, ldf = 0;
     double path_radius = 0.0;  // bounding radius the current path and series length were chosen for
-    int rebase_every = 200;
+    // full recomputes of the running incremental I(q): its FP64 sum of FP32 increments was
+    // measured not to drift (50k atoms, 2000 committed 500-atom moves with and without
+    // hydration: 2.6e-6 / 3.0e-6 after 199 moves, 1.9e-6 / 2.3e-6 after 1999, against a fresh
+    // FP64 full recompute; the error is the first forward's), so the period only bounds
+    // drift over very long runs
+    int rebase_every = 10000;
 
     // bodies (host mirror, caller order)
     int n = 0;
@@ -337,6 +342,15 @@ struct rsx_engine {
     // ([listed, changed, (atom, count) x LOCAL_CHANGED_CAP])
     DBuf<int> d_local_list, d_local;
     hipEvent_t ev_sasa = nullptr;    // recorded after the local recount's read-back
+    // the local recount's stage (rsx::LocalStage + moved atoms) and its captured graph,
+    // kept while its arguments and buffers are unchanged
+    DBuf<unsigned char> d_lstage;
+    HBuf<unsigned char> pin_lstage;
+    hipGraph_t l_graph = nullptr;
+    hipGraphExec_t l_exec = nullptr;
+    rsx::SasaArgs l_args;
+    const void* l_pin_stage = nullptr;
+    const void* l_pin_local = nullptr;
     DBuf<unsigned int> h_u32;        // distance histograms (rsx_calc_curve_hist)
     DBuf<unsigned long long> h_u64;
     DBuf<double> h_dbl, h_out, h_q, h_part;
@@ -802,6 +816,8 @@ double sum_f0(rsx_engine* h)
 // ---- SASA ----------------------------------------------------------------
 // changed counts a local recount reads back in its one transfer (more: all counts compared)
 constexpr int LOCAL_CHANGED_CAP = 4096;
+// moved atoms a local recount's captured graph carries in its stage (more: eager launches)
+constexpr int LOCAL_PATCH_CAP = 4096;
 
 void upload_sasa_tables(rsx_engine* h)
 {
@@ -822,10 +838,21 @@ void upload_sasa_tables(rsx_engine* h)
            "upload dots");
 }
 
-// Counts for n atoms already on the device in d_sasa_atoms.  With a box (lo[3], hi[3]) only
-// the atoms inside it are recounted, and the changed counts are listed in d_local.
-void run_sasa_device(rsx_engine* h, int n, const float* box_lo = nullptr, const float* box_hi = nullptr,
-                     hipStream_t stream = nullptr)
+static_assert(sizeof(rsx::LocalStage) <= rsx::LOCAL_STAGE_HEADER, "stage header");
+
+void release_local_graph(rsx_engine* h)
+{
+    if (h->l_exec) (void)hipGraphExecDestroy(h->l_exec);
+    if (h->l_graph) (void)hipGraphDestroy(h->l_graph);
+    h->l_exec = nullptr;
+    h->l_graph = nullptr;
+}
+
+// SASA arguments over the engine's buffers for n atoms in d_sasa_atoms (zeroed first, so
+// that two argument sets compare byte for byte).  local_cap >= 0: a local recount (the
+// changed counts listed in d_local, the box and moved range in d_lstage; with
+// local_cap > 0 the first kernel also writes up to local_cap moved atoms from the stage).
+rsx::SasaArgs sasa_args(rsx_engine* h, int n, int local_cap = -1)
 {
     float rmax = 0.0f;
     for (float v : h->rext) rmax = std::max(rmax, v);
@@ -840,6 +867,7 @@ void run_sasa_device(rsx_engine* h, int n, const float* box_lo = nullptr, const 
     h->d_sorted_atoms.ensure(n);
     h->d_counts.ensure(n);
     rsx::SasaArgs a;
+    std::memset(static_cast<void*>(&a), 0, sizeof a);
     a.atoms = h->d_sasa_atoms.p;
     a.n = n;
     a.n_types = (int)h->rext.size();
@@ -858,21 +886,25 @@ void run_sasa_device(rsx_engine* h, int n, const float* box_lo = nullptr, const 
     a.sorted = h->d_sorted.p;
     a.sorted_atoms = h->d_sorted_atoms.p;
     a.counts = h->d_counts.p;
-    a.subset = nullptr;
-    a.n_subset = 0;
-    if (box_lo) {
+    if (local_cap >= 0) {
         h->d_local_list.ensure(n);
         h->d_local.ensure(2 + 2 * (size_t)LOCAL_CHANGED_CAP);
+        h->d_lstage.ensure(rsx::LOCAL_STAGE_HEADER + 16 * (size_t)LOCAL_PATCH_CAP);
         a.local_hdr = h->d_local.p;
         a.local_list = h->d_local_list.p;
         a.local_changed = h->d_local.p + 2;
         a.changed_cap = LOCAL_CHANGED_CAP;
-        for (int d = 0; d < 3; ++d) {
-            a.box_lo[d] = box_lo[d];
-            a.box_hi[d] = box_hi[d];
-        }
+        a.local_stage = reinterpret_cast<const rsx::LocalStage*>(h->d_lstage.p);
+        a.atoms_w = h->d_sasa_atoms.p;
+        a.patch_cap = local_cap;
     }
-    hipchk(rsx::launch_sasa(a, stream ? stream : h->stream), "sasa kernels");
+    return a;
+}
+
+// counts for n atoms already on the device in d_sasa_atoms
+void run_sasa_device(rsx_engine* h, int n)
+{
+    hipchk(rsx::launch_sasa(sasa_args(h, n), h->stream), "sasa kernels");
 }
 
 // recompute counts / rsasa for the host mirror bodies (caller order)
@@ -918,33 +950,73 @@ bool sasa_local_enqueue(rsx_engine* h, int start, int len, const std::vector<flo
     float rmax = 0.0f;
     for (float v : h->rext) rmax = std::max(rmax, v);
     const float reach = 2.0f * rmax + 0.02f;
-    float box_lo[3], box_hi[3];
+    const size_t stage_bytes = rsx::LOCAL_STAGE_HEADER + 16 * (size_t)LOCAL_PATCH_CAP;
+    h->pin_lstage.ensure(stage_bytes);
+    rsx::LocalStage* st = reinterpret_cast<rsx::LocalStage*>(h->pin_lstage.p);
     for (int d = 0; d < 3; ++d) {
-        box_lo[d] = lo[d] - reach;
-        box_hi[d] = hi[d] + reach;
+        st->lo[d] = lo[d] - reach;
+        st->hi[d] = hi[d] + reach;
     }
+    st->start = start;
+    st->len = len;
     h->d_sasa_atoms.ensure(n);
-    if (h->sasa_mirror) {
-        h->pin_moved.ensure(len);
-        std::memcpy(h->pin_moved.p, &h->bodies[4 * (size_t)start], (size_t)len * 16);
-        hipchk(hipMemcpyAsync(h->d_sasa_atoms.p + start, h->pin_moved.p, (size_t)len * 16, hipMemcpyHostToDevice,
-                              h->stream),
-               "upload moved sasa atoms");
+    const size_t local_bytes = (2 + 2 * (size_t)LOCAL_CHANGED_CAP) * sizeof(int);
+    h->pin_local.ensure(2 + 2 * (size_t)LOCAL_CHANGED_CAP);
+    if (h->sasa_mirror && len <= LOCAL_PATCH_CAP) {
+        // one replay of a captured graph: stage upload, the moved atoms patched in with the
+        // bucket counts cleared, the cell list, the listed counts, the read-back
+        std::memcpy(h->pin_lstage.p + rsx::LOCAL_STAGE_HEADER, &h->bodies[4 * (size_t)start], (size_t)len * 16);
+        const rsx::SasaArgs a = sasa_args(h, n, LOCAL_PATCH_CAP);
+        const bool same = h->l_exec && std::memcmp(&a, &h->l_args, sizeof a) == 0 &&
+                          h->l_pin_stage == h->pin_lstage.p && h->l_pin_local == h->pin_local.p;
+        if (!same) {
+            release_local_graph(h);
+            hipchk(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+            try {
+                hipchk(hipMemcpyAsync(h->d_lstage.p, h->pin_lstage.p, stage_bytes, hipMemcpyHostToDevice, h->stream),
+                       "upload stage");
+                hipchk(rsx::launch_sasa(a, h->stream), "sasa kernels");
+                hipchk(hipMemcpyAsync(h->pin_local.p, h->d_local.p, local_bytes, hipMemcpyDeviceToHost, h->stream),
+                       "download changed counts");
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(h->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hipchk(hipStreamEndCapture(h->stream, &h->l_graph), "end capture");
+            hipchk(hipGraphInstantiate(&h->l_exec, h->l_graph, nullptr, nullptr, 0), "instantiate graph");
+            h->l_args = a;
+            h->l_pin_stage = h->pin_lstage.p;
+            h->l_pin_local = h->pin_local.p;
+        }
+        hipchk(hipGraphLaunch(h->l_exec, h->stream), "graph launch");
     } else {
-        // the device copies hold other data: the host bodies and counts go up whole
-        h->d_counts.ensure(n);
-        hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, h->bodies.data(), (size_t)n * 16, hipMemcpyHostToDevice, h->stream),
-               "upload sasa atoms");
-        hipchk(hipMemcpyAsync(h->d_counts.p, h->counts.data(), (size_t)n * 4, hipMemcpyHostToDevice, h->stream),
-               "upload counts");
-        hipchk(hipStreamSynchronize(h->stream), "sync sasa upload");  // pageable sources
+        if (h->sasa_mirror) {
+            h->pin_moved.ensure(len);
+            std::memcpy(h->pin_moved.p, &h->bodies[4 * (size_t)start], (size_t)len * 16);
+            hipchk(hipMemcpyAsync(h->d_sasa_atoms.p + start, h->pin_moved.p, (size_t)len * 16, hipMemcpyHostToDevice,
+                                  h->stream),
+                   "upload moved sasa atoms");
+        } else {
+            // the device copies hold other data: the host bodies and counts go up whole
+            h->d_counts.ensure(n);
+            hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, h->bodies.data(), (size_t)n * 16, hipMemcpyHostToDevice,
+                                  h->stream),
+                   "upload sasa atoms");
+            hipchk(hipMemcpyAsync(h->d_counts.p, h->counts.data(), (size_t)n * 4, hipMemcpyHostToDevice, h->stream),
+                   "upload counts");
+            hipchk(hipStreamSynchronize(h->stream), "sync sasa upload");  // pageable sources
+        }
+        const rsx::SasaArgs a = sasa_args(h, n, 0);
+        hipchk(hipMemcpyAsync(h->d_lstage.p, h->pin_lstage.p, rsx::LOCAL_STAGE_HEADER, hipMemcpyHostToDevice,
+                              h->stream),
+               "upload stage");
+        hipchk(rsx::launch_sasa(a, h->stream), "sasa kernels");
+        hipchk(hipMemcpyAsync(h->pin_local.p, h->d_local.p, local_bytes, hipMemcpyDeviceToHost, h->stream),
+               "download changed counts");
     }
     h->sasa_mirror = true;
-    run_sasa_device(h, n, box_lo, box_hi, h->stream);
-    h->pin_local.ensure(2 + 2 * (size_t)LOCAL_CHANGED_CAP);
-    hipchk(hipMemcpyAsync(h->pin_local.p, h->d_local.p, (2 + 2 * (size_t)LOCAL_CHANGED_CAP) * sizeof(int),
-                          hipMemcpyDeviceToHost, h->stream),
-           "download changed counts");
     hipchk(hipEventRecord(h->ev_sasa, h->stream), "record sasa");
     return true;
 }
@@ -1591,7 +1663,8 @@ void incremental_update(rsx_engine* h, int start, int len, const std::vector<flo
     // it is queued on the engine stream right behind the SASA recount (which it would
     // otherwise hold up) and runs while the host reads the counts and builds P1; the P1
     // launch joins it on the second stream, and both sums are added to I(q) on the engine
-    // stream, P2's first.
+    // stream, P2's first.  (Uploading P2 and skipping S on the second stream, beside the
+    // recount, measured slower: the cross-stream wait costs more than it overlaps.)
     const bool rebase = h->rebase_every > 0 && h->since_rebase + 1 >= h->rebase_every;
     const bool try_local = sasa_now && len > 0 && (int)h->counts.size() == h->n && h->have_rsasa;
     KernelChoice kc;
@@ -2863,6 +2936,7 @@ int rsx_destroy(rsx_engine* h)
     if (!h) return RSX_INVALID_ARG;
     hipSetDevice(h->device);
     batch_release_graph(h);
+    release_local_graph(h);
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->stream2) hipStreamSynchronize(h->stream2);
     DBuf<float4>* f4[] = {&h->d_cols, &h->d_saved_cols, &h->d_sasa_atoms, &h->d_dots, &h->b_cols, &h->d_sorted_atoms};
@@ -2887,6 +2961,8 @@ int rsx_destroy(rsx_engine* h)
         st->dev.release();
     }
     for (auto& b : h->pin_jobs) b.release();
+    h->d_lstage.release();
+    h->pin_lstage.release();
     h->pin_moved.release();
     h->pin_local.release();
     h->pin_iq.release();

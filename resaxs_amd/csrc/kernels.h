@@ -152,6 +152,14 @@ struct ClassSortArgs {
 hipError_t launch_class_sort(const ClassSortArgs& a, hipStream_t s);
 
 // ---- SASA ----
+// Inputs of a local recount on the device: the box [lo, hi], the moved range and, from
+// LOCAL_STAGE_HEADER bytes on, the moved atoms.
+struct LocalStage {
+    float lo[3], hi[3];
+    int start, len;
+};
+constexpr int LOCAL_STAGE_HEADER = 32;  // bytes before the moved atoms (float4-aligned)
+
 struct SasaArgs {
     const float4* atoms;     // x, y, z, type bits
     int n;
@@ -175,7 +183,7 @@ struct SasaArgs {
     const int* subset;       // null: count every atom; else count atoms subset[0 .. n_subset)
     int n_subset;
     // Local recount (local_hdr set; subset unused): the place kernel lists the atoms whose
-    // position lies in the box [box_lo, box_hi] in local_list (in no fixed order); the dots
+    // position lies in the box of local_stage in local_list (in no fixed order); the dots
     // kernel counts only those, over a fixed grid, and for each count that differs from the
     // one in counts appends (atom, new count) to local_changed before overwriting it.
     // local_hdr[0] = atoms listed, local_hdr[1] = counts changed (entries from changed_cap on
@@ -184,8 +192,12 @@ struct SasaArgs {
     int* local_list = nullptr;   // [n]
     int* local_changed = nullptr;  // [changed_cap][2]
     int changed_cap = 0;
-    float box_lo[3] = {0.f, 0.f, 0.f};
-    float box_hi[3] = {0.f, 0.f, 0.f};
+    // The local recount's inputs on the device (LocalStage).  With patch_cap > 0 the first
+    // kernel also writes the stage's moved atoms into atoms[start, start + len) (len <=
+    // patch_cap; atoms_w is the writable view of atoms).
+    const LocalStage* local_stage = nullptr;
+    float4* atoms_w = nullptr;
+    int patch_cap = 0;
 };
 hipError_t launch_sasa(const SasaArgs& a, hipStream_t s);
 

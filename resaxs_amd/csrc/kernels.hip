@@ -1181,6 +1181,20 @@ __device__ __forceinline__ int cell_hash(int ix, int iy, int iz, int nb)
     return (int)(h & (uint32_t)(nb - 1));
 }
 
+// First kernel of a local recount with a patch: the bucket counts cleared (as the fill kernel
+// does otherwise) and the stage's moved atoms written into the atom array.
+__global__ void sasa_patch_kernel(SasaArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t <= a.n_buckets) a.bucket_count[t] = 0;
+    const LocalStage& st = *a.local_stage;
+    if (t < st.len && t < a.patch_cap) {
+        const float4* moved = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(a.local_stage) +
+                                                              LOCAL_STAGE_HEADER);
+        a.atoms_w[st.start + t] = moved[t];
+    }
+}
+
 __global__ void sasa_bin_kernel(SasaArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1266,11 +1280,12 @@ __global__ void sasa_place_kernel(SasaArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (a.local_hdr) {  // uniform
+        const LocalStage& st = *a.local_stage;
         bool in = false;
         if (i < a.n) {
             const float4 p = a.atoms[i];
-            in = p.x >= a.box_lo[0] && p.x <= a.box_hi[0] && p.y >= a.box_lo[1] && p.y <= a.box_hi[1] &&
-                 p.z >= a.box_lo[2] && p.z <= a.box_hi[2];
+            in = p.x >= st.lo[0] && p.x <= st.hi[0] && p.y >= st.lo[1] && p.y <= st.hi[1] && p.z >= st.lo[2] &&
+                 p.z <= st.hi[2];
         }
         const uint64_t m = __ballot(in);
         if (m) {
@@ -2306,8 +2321,14 @@ hipError_t launch_sasa(const SasaArgs& a, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
     if (a.n_dots <= 0 || a.n_dots > 64 * SASA_MAX_DPL) return hipErrorInvalidValue;
-    hipError_t e = launch_fill_i32(a.bucket_count, 0, (long long)a.n_buckets + 1, s);
-    if (e != hipSuccess) return e;
+    if (a.local_hdr && !a.local_stage) return hipErrorInvalidValue;
+    if (a.patch_cap > 0) {
+        const long long nt = (long long)a.n_buckets + 1 > a.patch_cap ? (long long)a.n_buckets + 1 : a.patch_cap;
+        hipLaunchKernelGGL(sasa_patch_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a);
+    } else {
+        hipError_t e = launch_fill_i32(a.bucket_count, 0, (long long)a.n_buckets + 1, s);
+        if (e != hipSuccess) return e;
+    }
     const dim3 g((unsigned)((a.n + 255) / 256)), b(256);
     if (a.n_buckets < SCAN_TILE || (a.n_buckets & (a.n_buckets - 1))) return hipErrorInvalidValue;
     hipLaunchKernelGGL(sasa_bin_kernel, g, b, 0, s, a);
