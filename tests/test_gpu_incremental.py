@@ -307,3 +307,57 @@ def test_hydrated_local_recount_with_revert(oracle):
         err = rel_err(I, ref)
         _say(f"  step {step}: max rel err {err.max():.3e}")
         assert err.max() < 1e-5
+
+
+def test_hydrated_recount_fallbacks_and_size_change(oracle):
+    """The local recount's other paths.  A jittered move of 9000 atoms is more than the
+    captured recount's stage carries (the recount runs as eager launches) and changes 6130
+    exposed-dot counts, more than the read-back block holds (every count is compared);
+    small moves before and after it take the captured graph.  Then the engine takes a
+    structure of another size, whose moves capture the graph again.  Counts stay bit-exact
+    and I(q) follows the FP64 oracle throughout."""
+    from resaxs_amd import configs
+
+    def check(e, w, pos, what):
+        dots = fib_dot_table(w.radii, w.probe, w.n_half)
+        counts = oracle.sasa_counts(pos, w.types, r2_table(w.radii, w.probe), dots)
+        assert np.array_equal(e.get_rsasa(), counts.astype(np.float32) / np.float32(256)), what
+        return counts
+
+    _say("case: hydrated recount fallbacks, 12000 atoms: 200-atom move, jitter of 9000 atoms, 150-atom moves")
+    rng = np.random.default_rng(41)
+    w = configs.incremental_50000(n=12000)
+    e = engine_for(w.q, w.table, w.water_index, w.w, n=len(w.pos), radii=w.radii, probe=w.probe, n_half=w.n_half)
+    e.set_rebaseline(0)
+    pos = w.pos.copy()
+    e.calc_curve(configs.bodies_of(pos, w.types))
+    moves = [("move 200", *configs.random_move(rng, pos, 200))]
+    jitter = pos[1000:10000] + np.random.default_rng(5).normal(scale=0.4, size=(9000, 3)).astype(np.float32)
+    moves.append(("jitter 9000", 1000, jitter.astype(np.float32)))
+    for what, start, newp in moves + [("move 150", None, None), ("move 150", None, None)]:
+        if start is None:
+            start, newp = configs.random_move(rng, pos, 150)
+        pos[start:start + len(newp)] = newp
+        I = e.recalc_curve(configs.bodies_of(pos, w.types), start, len(newp))
+        e.commit()
+        counts = check(e, w, pos, what)
+        err = rel_err(I, oracle_iq(w.q, pos, w.types, w.table, w.water_index, w.w, counts, 256)).max()
+        _say(f"  {what}: max rel err {err:.3e}")
+        assert err < 1e-5, what
+
+    _say("case: the same engine on a structure of 7000 atoms, two 150-atom moves")
+    w2 = configs.incremental_50000(seed=11, n=7000)
+    pos2 = w2.pos.copy()
+    e.initialize(len(pos2))
+    e.calc_curve(configs.bodies_of(pos2, w2.types))
+    for k in range(2):
+        start, newp = configs.random_move(rng, pos2, 150)
+        pos2[start:start + 150] = newp
+        I = e.recalc_curve(configs.bodies_of(pos2, w2.types), start, 150)
+        e.commit()
+        counts = check(e, w2, pos2, f"size change move {k}")
+        # the engine keeps the first workload's q and factor table
+        err = rel_err(I, oracle_iq(w.q, pos2, w2.types, w.table, w.water_index, w.w, counts, 256)).max()
+        _say(f"  move {k}: max rel err {err:.3e}")
+        assert err < 1e-5
+    e.close()
