@@ -1088,9 +1088,9 @@ int pick_range(const rsx_engine* h, double column_atoms_total)
 // Kernel of the incremental launches: the forward's.  With hydration classes a 64-row tile
 // of changed atoms holds tens of classes, so each flush loads the rows' factors from L2;
 // that costs little while the columns keep long class runs (relocate_classes keeps them
-// sorted): at 50k atoms a hydrated 500-atom move takes 1.51 ms (median) with it, the same as
-// with the per-lane-sum kernel at K = 64, and its FP64 row sums are the more accurate at deep
-// minima of I(q).
+// sorted): at 50k atoms a hydrated 500-atom move measured the same with it as with the
+// per-lane-sum kernel at K = 64, and its FP64 row sums are the more accurate at deep minima
+// of I(q).
 void incremental_kernel(rsx_engine* h, int& variant, int& K, int& n_chunks, const ChunkConst*& chunks)
 {
     variant = h->variant;
