@@ -60,7 +60,7 @@ struct DBuf {
     void ensure(size_t n)
     {
         if (n <= cap && p) return;
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
         if (n == 0) n = 1;
@@ -70,7 +70,7 @@ struct DBuf {
     }
     void release()
     {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
     }
@@ -102,12 +102,6 @@ struct HBuf {
     }
 };
 
-inline uint32_t fbits(float f)
-{
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    return u;
-}
 
 inline float bitsf(uint32_t u)
 {
@@ -2107,8 +2101,8 @@ void calc_rsasa_impl(rsx_engine* h, const float* bodies, int n, const float* rad
 // ---- batch --------------------------------------------------------------------
 void batch_release_graph(rsx_engine* h)
 {
-    if (h->b_exec) hipGraphExecDestroy(h->b_exec);
-    if (h->b_graph) hipGraphDestroy(h->b_graph);
+    if (h->b_exec) (void)hipGraphExecDestroy(h->b_exec);
+    if (h->b_graph) (void)hipGraphDestroy(h->b_graph);
     h->b_exec = nullptr;
     h->b_graph = nullptr;
 }
@@ -2372,8 +2366,8 @@ void batch_run_dev(rsx_engine* h, double* out_frames, double* out_mean)
             batch_enqueue(h, out_frames, out_mean, pw);
         } catch (...) {
             hipGraph_t g;
-            hipStreamEndCapture(h->stream, &g);
-            if (g) hipGraphDestroy(g);
+            (void)hipStreamEndCapture(h->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
             throw;
         }
         hipchk(hipStreamEndCapture(h->stream, &h->b_graph), "end capture");
@@ -2934,11 +2928,11 @@ int rsx_create(int device, rsx_engine** out)
 int rsx_destroy(rsx_engine* h)
 {
     if (!h) return RSX_INVALID_ARG;
-    hipSetDevice(h->device);
+    (void)hipSetDevice(h->device);
     batch_release_graph(h);
     release_local_graph(h);
-    if (h->stream) hipStreamSynchronize(h->stream);
-    if (h->stream2) hipStreamSynchronize(h->stream2);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->stream2) (void)hipStreamSynchronize(h->stream2);
     DBuf<float4>* f4[] = {&h->d_cols, &h->d_saved_cols, &h->d_sasa_atoms, &h->d_dots, &h->b_cols, &h->d_sorted_atoms};
     for (auto* b : f4) b->release();
     h->d_qrev.release();
@@ -2989,15 +2983,15 @@ int rsx_destroy(rsx_engine* h)
     h->d_guard.release();
     h->d_cs_table.release();
     h->d_cs_tot.release();
-    if (h->ev0) hipEventDestroy(h->ev0);
-    if (h->ev1) hipEventDestroy(h->ev1);
-    if (h->ev2) hipEventDestroy(h->ev2);
-    if (h->ev3) hipEventDestroy(h->ev3);
-    if (h->ev_sasa) hipEventDestroy(h->ev_sasa);
-    if (h->ev_skip) hipEventDestroy(h->ev_skip);
-    if (h->ev_new) hipEventDestroy(h->ev_new);
-    if (h->stream2) hipStreamDestroy(h->stream2);
-    if (h->stream) hipStreamDestroy(h->stream);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->ev2) (void)hipEventDestroy(h->ev2);
+    if (h->ev3) (void)hipEventDestroy(h->ev3);
+    if (h->ev_sasa) (void)hipEventDestroy(h->ev_sasa);
+    if (h->ev_skip) (void)hipEventDestroy(h->ev_skip);
+    if (h->ev_new) (void)hipEventDestroy(h->ev_new);
+    if (h->stream2) (void)hipStreamDestroy(h->stream2);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return RSX_OK;
 }

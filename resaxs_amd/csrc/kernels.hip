@@ -81,20 +81,6 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-__device__ __forceinline__ float wave_sum_f(float v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
-}
-
-__device__ __forceinline__ int wave_sum_i(int v)
-{
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
-}
-
 // ---------------------------------------------------------------------------
 // uniform-grid policy: FP64 accumulators, three-term recurrence
 // ---------------------------------------------------------------------------
@@ -102,28 +88,13 @@ struct Seeds {
     double u0, u1, c2;
 };
 
-// sin(x) for 0 <= x <= 1.5 in FP64 (odd Taylor series to x^17; truncation
-// below 2e-14 relative).  The recurrence's frequency 2 cos(dq r) = 2 - 4 sin^2(dq r / 2)
-// must be far more accurate than v_sin's ~4e-7 absolute error: a relative
-// error e in sin(dq r / 2) shifts every later term's phase by ~ e q r.
-__device__ __forceinline__ double sin_poly(double x)
-{
-    const double x2 = x * x;
-    double p = 2.8114572543455206e-15;
-    p = fma(p, x2, -7.647163731819816e-13);
-    p = fma(p, x2, 1.6059043836821613e-10);
-    p = fma(p, x2, -2.505210838544172e-08);
-    p = fma(p, x2, 2.7557319223985893e-06);
-    p = fma(p, x2, -1.984126984126984e-04);
-    p = fma(p, x2, 8.333333333333333e-03);
-    p = fma(p, x2, -1.6666666666666666e-01);
-    p = fma(p, x2, 1.0);
-    return x * p;
-}
-
-// The same series cut after x^(2M+1) (M a template argument, so the Horner chain
-// is unrolled and branch-free); the host picks M so that the first omitted term
-// stays below 1e-16 relative for the layout's largest dq r / 2.
+// sin(x) for small x >= 0 in FP64 from the odd Taylor series, cut after x^(2M+1)
+// (M a template argument, so the Horner chain is unrolled and branch-free); the
+// host picks M so that the first omitted term stays below 1e-16 relative for the
+// layout's largest dq r / 2.  The recurrence's frequency 2 cos(dq r) =
+// 2 - 4 sin^2(dq r / 2) must be far more accurate than v_sin's ~4e-7 absolute
+// error: a relative error e in sin(dq r / 2) shifts every later term's phase by
+// ~ e q r.
 template <int M>
 __device__ __forceinline__ double sin_series(double x)
 {
