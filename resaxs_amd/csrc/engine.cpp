@@ -264,6 +264,7 @@ struct rsx_engine {
     // new-records launch runs there beside the old-records launch
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_skip = nullptr, ev_new = nullptr;
+    hipEvent_t ev_out = nullptr;  // recorded after the download of I(q)
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     std::string err;
     int state = 0;  // 0 blank, 1 initialized, 3 args set, 4 computing
@@ -419,7 +420,7 @@ struct rsx_engine {
         HBuf<unsigned char> host;
         DBuf<unsigned char> dev;
     };
-    Staging st_old, st_new, st_reloc;
+    Staging st_old, st_new;
     DBuf<double> d_partials_new;  // partials of an update's second launch
     // pinned staging of the per-update transfers (job tables: one per launch of an update)
     HBuf<float4> pin_moved;
@@ -1335,16 +1336,30 @@ void full_forward(rsx_engine* h)
     h->since_rebase = 0;
 }
 
-void copy_out(rsx_engine* h, float* out32, double* out64)
+// I(q) to the host in two steps: the download is queued on the engine stream (work queued
+// after it may still run when the host has the result), then the host waits for it alone
+void enqueue_iq_download(rsx_engine* h)
 {
     h->pin_iq.ensure(h->nq);
-    double* tmp = h->pin_iq.p;
-    hipchk(hipMemcpyAsync(tmp, h->d_iq.p, h->nq * sizeof(double), hipMemcpyDeviceToHost, h->stream), "download iq");
-    hipchk(hipStreamSynchronize(h->stream), "sync download iq");
+    hipchk(hipMemcpyAsync(h->pin_iq.p, h->d_iq.p, h->nq * sizeof(double), hipMemcpyDeviceToHost, h->stream),
+           "download iq");
+    hipchk(hipEventRecord(h->ev_out, h->stream), "record download iq");
+}
+
+void finish_iq_download(rsx_engine* h, float* out32, double* out64)
+{
+    hipchk(hipEventSynchronize(h->ev_out), "sync download iq");
+    const double* tmp = h->pin_iq.p;
     for (int k = 0; k < h->nq; ++k) {
         if (out32) out32[k] = (float)tmp[k];
         if (out64) out64[k] = tmp[k];
     }
+}
+
+void copy_out(rsx_engine* h, float* out32, double* out64)
+{
+    enqueue_iq_download(h);
+    finish_iq_download(h, out32, out64);
 }
 
 void check_args_set(rsx_engine* h)
@@ -1444,11 +1459,15 @@ struct Pack {
 };
 
 // The staging block made to hold `bytes`; returns its host side to fill.  The device side
-// grows to powers of two from 64 KiB, so its address rarely changes.
-unsigned char* stage_begin(rsx_engine::Staging& st, size_t bytes)
+// grows to powers of two from 64 KiB, so its address rarely changes; before it is
+// replaced, both streams are drained (work queued after an update's result, such as the
+// relocation scatter, may still read the old block).
+unsigned char* stage_begin(rsx_engine* h, rsx_engine::Staging& st, size_t bytes)
 {
     st.host.ensure(bytes);
     if (bytes > st.dev.cap) {
+        hipchk(hipStreamSynchronize(h->stream), "sync before staging growth");
+        hipchk(hipStreamSynchronize(h->stream2), "sync before staging growth");
         size_t c = 65536;
         while (c < bytes) c <<= 1;
         st.dev.ensure(c);
@@ -1534,10 +1553,19 @@ void reduce_update_part(rsx_engine* h, int nj, const double* partials, const int
 
 // The old-records launch of an update: S = [start, start + len) removed from the columns
 // (skip records), its old records (old_bodies, old classes) sorted by class as rows, with
-// weight -1 (nothing is read from the host afterwards, so the work stays queued).
-void enqueue_old_part(rsx_engine* h, int start, int len, const std::vector<float>& old_bodies, const KernelChoice& kc)
+// weight -1 (nothing is read from the host afterwards, so the work stays queued).  Staged
+// in two steps: the rows and job table uploaded and S skipped (ev_skip recorded), then the
+// launch and its reduction.
+struct OldPart {
+    const rsx_engine::IncTable* tab = nullptr;
+    const int* d_sb = nullptr;
+    const double* d_q0 = nullptr;
+};
+
+OldPart stage_old_part(rsx_engine* h, int start, int len, const std::vector<float>& old_bodies, const KernelChoice& kc)
 {
-    if (len <= 0) return;
+    OldPart op;
+    if (len <= 0) return op;
     const int mp = (len + 63) / 64 * 64;
     std::vector<std::pair<uint32_t, int>> order((size_t)len);
     for (int a = 0; a < len; ++a) order[a] = {h->cls[start + a], a};
@@ -1548,7 +1576,7 @@ void enqueue_old_part(rsx_engine* h, int start, int len, const std::vector<float
     const size_t off_idx = pk.add((size_t)len * sizeof(int));
     const size_t off_sb = pk.add(2 * sizeof(int));
     const size_t off_q0 = pk.add(sizeof(double));
-    unsigned char* hb = stage_begin(h->st_old, pk.size);
+    unsigned char* hb = stage_begin(h, h->st_old, pk.size);
     float4* rows = reinterpret_cast<float4*>(hb + off_rows);
     for (int t = 0; t < len; ++t) {
         const float* ob = &old_bodies[4 * (size_t)order[t].second];
@@ -1568,10 +1596,18 @@ void enqueue_old_part(rsx_engine* h, int start, int len, const std::vector<float
                                      make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP)), len, h->stream),
            "scatter skip");
     hipchk(hipEventRecord(h->ev_skip, h->stream), "record skip");
-    h->d_partials.ensure((size_t)tab->njobs * h->ldf);
-    launch_update_pairs(h, tab->jobs.p, tab->njobs, h->d_partials.p, kc, h->stream);
-    reduce_update_part(h, tab->njobs, h->d_partials.p, reinterpret_cast<const int*>(db + off_sb),
-                       reinterpret_cast<const double*>(db + off_q0));
+    op.tab = tab;
+    op.d_sb = reinterpret_cast<const int*>(db + off_sb);
+    op.d_q0 = reinterpret_cast<const double*>(db + off_q0);
+    return op;
+}
+
+void launch_old_part(rsx_engine* h, const OldPart& op, const KernelChoice& kc)
+{
+    if (!op.tab) return;
+    h->d_partials.ensure((size_t)op.tab->njobs * h->ldf);
+    launch_update_pairs(h, op.tab->jobs.p, op.tab->njobs, h->d_partials.p, kc, h->stream);
+    reduce_update_part(h, op.tab->njobs, h->d_partials.p, op.d_sb, op.d_q0);
 }
 
 void incremental_update(rsx_engine* h, int start, int len, const std::vector<float>& old_bodies, bool sasa_now,
@@ -1654,18 +1690,24 @@ void incremental_update(rsx_engine* h, int start, int len, const std::vector<flo
     // (F'_c - F_c) F_d + (F'_d - F_d) F_c from P1 x cols and (F'_c - F_c)(F'_d - F_d) from
     // P1 x P1, in all F'_c F'_d - F_c F_d; a pair (s, c) gets 2 (F'_s F_c - F_s F_c) from the
     // rectangles and 2 F'_s (F'_c - F_c) from P1 x P1.  The P2 launch needs no new count:
-    // it is queued on the engine stream right behind the SASA recount (which it would
-    // otherwise hold up) and runs while the host reads the counts and builds P1; the P1
-    // launch joins it on the second stream, and both sums are added to I(q) on the engine
-    // stream, P2's first.  (Uploading P2 and skipping S on the second stream, beside the
-    // recount, measured slower: the cross-stream wait costs more than it overlaps.)
+    // its rows are uploaded and S skipped ahead of the SASA recount, and it is queued on the
+    // engine stream right behind the recount (which it would otherwise hold up), running
+    // while the host reads the counts and builds P1; the P1 launch joins it on the second
+    // stream, and both sums are added to I(q) on the engine stream, P2's first.  The records
+    // that move between class runs travel with P1's upload and are written after the sums,
+    // once I(q) is on its way to the host.  (Uploading P2 and skipping S on the second
+    // stream, beside the recount, measured slower: the cross-stream wait costs more than it
+    // overlaps.)
     const bool rebase = h->rebase_every > 0 && h->since_rebase + 1 >= h->rebase_every;
-    const bool try_local = sasa_now && len > 0 && (int)h->counts.size() == h->n && h->have_rsasa;
+    // sasa_local_enqueue recounts exactly when these hold
+    const bool local = sasa_now && len > 0 && (int)h->counts.size() == h->n && h->have_rsasa;
     KernelChoice kc;
     incremental_kernel(h, kc.variant, kc.K, kc.n_chunks, kc.chunks);
-    const bool local = try_local && sasa_local_enqueue(h, start, len, old_bodies);
     const bool old_early = len > 0 && !rebase && (local || !sasa_now);
-    if (old_early) enqueue_old_part(h, start, len, old_bodies, kc);
+    OldPart op;
+    if (old_early) op = stage_old_part(h, start, len, old_bodies, kc);
+    if (local && !sasa_local_enqueue(h, start, len, old_bodies)) fail(RSX_HIP_ERROR, "local recount not queued");
+    launch_old_part(h, op, kc);
 
     // atoms whose exposed-dot count changed
     std::vector<int> changed;
@@ -1702,7 +1744,7 @@ void incremental_update(rsx_engine* h, int start, int len, const std::vector<flo
         copy_out(h, out32, out64);
         return;
     }
-    if (!old_early) enqueue_old_part(h, start, len, old_bodies, kc);
+    if (!old_early) launch_old_part(h, stage_old_part(h, start, len, old_bodies, kc), kc);
 
     const int m = (int)M.size();
     std::vector<uint32_t> cls_new(m), cls_old(m);
@@ -1720,22 +1762,6 @@ void incremental_update(rsx_engine* h, int start, int len, const std::vector<flo
     std::stable_sort(p1_order.begin(), p1_order.end(),
                      [](const std::pair<uint32_t, int>& x, const std::pair<uint32_t, int>& y) { return x.first < y.first; });
     const int p1 = (m + 63) / 64 * 64;
-    Pack pk;
-    const size_t off_rows = pk.add((size_t)p1 * sizeof(float4));
-    const size_t off_sb = pk.add(2 * sizeof(int));
-    const size_t off_q0 = pk.add(sizeof(double));
-    unsigned char* hb = stage_begin(h->st_new, pk.size);
-    float4* rows = reinterpret_cast<float4*>(hb + off_rows);
-    for (int t = 0; t < m; ++t) {
-        const float* nb = &h->bodies[4 * (size_t)M[p1_order[t].second]];
-        rows[t] = make_float4(nb[0], nb[1], nb[2], bitsf(p1_order[t].first));
-    }
-    for (int t = m; t < p1; ++t) rows[t] = make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP));
-    unsigned char* db = h->st_new.dev.p;
-    // the new-records launch reads the columns once S is skipped there
-    if (len > 0) hipchk(hipStreamWaitEvent(h->stream2, h->ev_skip, 0), "wait skip");
-    const rsx_engine::IncTable* tab =
-        update_jobs(h, 1, reinterpret_cast<const float4*>(db + off_rows), p1, 1.0f, kc, h->stream2);
 
     // q = 0: (sum F_new)^2 - (sum F_old)^2 from the class populations
     const double s_old = sum_f0(h);
@@ -1751,6 +1777,41 @@ void incremental_update(rsx_engine* h, int start, int len, const std::vector<flo
         }
     }
     const double s_new = sum_f0(h);
+    // the new records, each in the run of its class (the slots of atoms whose class changed
+    // are exchanged across runs)
+    std::vector<int> touched;
+    touched.reserve(2 * (size_t)m);
+    relocate_classes(h, M, cls_old, cls_new, run_start, touched);
+    const int nt = (int)touched.size();
+
+    Pack pk;
+    const size_t off_rows = pk.add((size_t)p1 * sizeof(float4));
+    const size_t off_sb = pk.add(2 * sizeof(int));
+    const size_t off_q0 = pk.add(sizeof(double));
+    const size_t off_slot = pk.add((size_t)nt * sizeof(int));
+    const size_t off_caller = pk.add((size_t)nt * sizeof(int));
+    const size_t off_vals = pk.add((size_t)nt * sizeof(float4));
+    unsigned char* hb = stage_begin(h, h->st_new, pk.size);
+    float4* rows = reinterpret_cast<float4*>(hb + off_rows);
+    for (int t = 0; t < m; ++t) {
+        const float* nb = &h->bodies[4 * (size_t)M[p1_order[t].second]];
+        rows[t] = make_float4(nb[0], nb[1], nb[2], bitsf(p1_order[t].first));
+    }
+    for (int t = m; t < p1; ++t) rows[t] = make_float4(0.f, 0.f, 0.f, bitsf(CLS_SKIP));
+    for (int t = 0; t < nt; ++t) {
+        const int sl = touched[t], i = h->caller_of[sl];
+        const float* bb = &h->bodies[4 * (size_t)i];
+        reinterpret_cast<int*>(hb + off_slot)[t] = sl;
+        reinterpret_cast<int*>(hb + off_caller)[t] = i;
+        reinterpret_cast<float4*>(hb + off_vals)[t] = make_float4(bb[0], bb[1], bb[2], bitsf(h->cls[i]));
+    }
+    unsigned char* db = h->st_new.dev.p;
+    // the new-records launch reads the columns once S is skipped there and the engine
+    // stream's earlier writes (the last update's relocation scatter) are done
+    if (len == 0) hipchk(hipEventRecord(h->ev_skip, h->stream), "record skip");
+    hipchk(hipStreamWaitEvent(h->stream2, h->ev_skip, 0), "wait skip");
+    const rsx_engine::IncTable* tab =
+        update_jobs(h, 1, reinterpret_cast<const float4*>(db + off_rows), p1, 1.0f, kc, h->stream2);
     reinterpret_cast<int*>(hb + off_sb)[0] = 0;
     reinterpret_cast<int*>(hb + off_sb)[1] = tab->njobs;
     *reinterpret_cast<double*>(hb + off_q0) = s_new * s_new - s_old * s_old;
@@ -1761,33 +1822,15 @@ void incremental_update(rsx_engine* h, int start, int len, const std::vector<flo
     hipchk(hipStreamWaitEvent(h->stream, h->ev_new, 0), "wait new");
     reduce_update_part(h, tab->njobs, h->d_partials_new.p, reinterpret_cast<const int*>(db + off_sb),
                        reinterpret_cast<const double*>(db + off_q0));
-
-    // the new records, each in the run of its class (the slots of atoms whose class changed
-    // are exchanged across runs), written after the launches on the same stream
-    std::vector<int> touched;
-    touched.reserve(2 * (size_t)m);
-    relocate_classes(h, M, cls_old, cls_new, run_start, touched);
-    const int nt = (int)touched.size();
-    Pack pr;
-    const size_t off_slot = pr.add((size_t)nt * sizeof(int));
-    const size_t off_caller = pr.add((size_t)nt * sizeof(int));
-    const size_t off_vals = pr.add((size_t)nt * sizeof(float4));
-    unsigned char* hr = stage_begin(h->st_reloc, pr.size);
-    for (int t = 0; t < nt; ++t) {
-        const int sl = touched[t], i = h->caller_of[sl];
-        const float* bb = &h->bodies[4 * (size_t)i];
-        reinterpret_cast<int*>(hr + off_slot)[t] = sl;
-        reinterpret_cast<int*>(hr + off_caller)[t] = i;
-        reinterpret_cast<float4*>(hr + off_vals)[t] = make_float4(bb[0], bb[1], bb[2], bitsf(h->cls[i]));
-    }
-    stage_upload(h->st_reloc, pr.size, h->stream, "upload relocated records");
-    unsigned char* dr = h->st_reloc.dev.p;
-    hipchk(rsx::launch_scatter_slots(h->d_cols.p, h->d_s2c.p, reinterpret_cast<const int*>(dr + off_slot),
-                                     reinterpret_cast<const float4*>(dr + off_vals),
-                                     reinterpret_cast<const int*>(dr + off_caller), nt, h->stream),
+    enqueue_iq_download(h);
+    // written after both launches (same stream, behind the wait for P1); the next call's
+    // device work is queued behind it on this stream
+    hipchk(rsx::launch_scatter_slots(h->d_cols.p, h->d_s2c.p, reinterpret_cast<const int*>(db + off_slot),
+                                     reinterpret_cast<const float4*>(db + off_vals),
+                                     reinterpret_cast<const int*>(db + off_caller), nt, h->stream),
            "scatter new");
     h->since_rebase++;
-    copy_out(h, out32, out64);
+    finish_iq_download(h, out32, out64);
 }
 
 // the saved arrays have the current sizes (entry-wise copies are possible)
@@ -1826,7 +1869,8 @@ void commit_impl(rsx_engine* h)
            "save cols");
     hipchk(hipMemcpyAsync(h->d_saved_iq.p, h->d_iq.p, h->nq * sizeof(double), hipMemcpyDeviceToDevice, h->stream),
            "save iq");
-    hipchk(hipStreamSynchronize(h->stream), "sync commit");
+    // no wait: the copies are ordered before every later use of these buffers (all on the
+    // engine stream; the second stream reads the columns only behind an event recorded there)
     if (saved_matches_sizes(h) && !h->dirty_full) {
         for (int i : h->dirty) copy_entry(h->saved.bodies, h->saved.counts, h->saved.cls, h->saved.rsasa, h->bodies,
                                           h->counts, h->cls, h->rsasa, i);
@@ -2907,7 +2951,8 @@ int rsx_create(int device, rsx_engine** out)
         hipEventCreateWithFlags(&h->ev_sasa, hipEventDisableTiming) != hipSuccess ||
         hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_skip, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&h->ev_new, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&h->ev_new, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_out, hipEventDisableTiming) != hipSuccess) {
         delete h;
         return RSX_HIP_ERROR;
     }
@@ -2950,7 +2995,7 @@ int rsx_destroy(rsx_engine* h)
     h->b_chunks.release();
     h->b_pin_ftab.release();
     h->b_pin_q0.release();
-    for (rsx_engine::Staging* st : {&h->st_old, &h->st_new, &h->st_reloc}) {
+    for (rsx_engine::Staging* st : {&h->st_old, &h->st_new}) {
         st->host.release();
         st->dev.release();
     }
@@ -2990,6 +3035,7 @@ int rsx_destroy(rsx_engine* h)
     if (h->ev_sasa) (void)hipEventDestroy(h->ev_sasa);
     if (h->ev_skip) (void)hipEventDestroy(h->ev_skip);
     if (h->ev_new) (void)hipEventDestroy(h->ev_new);
+    if (h->ev_out) (void)hipEventDestroy(h->ev_out);
     if (h->stream2) (void)hipStreamDestroy(h->stream2);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
