@@ -351,7 +351,7 @@ struct rsx_engine {
     DBuf<double> h_dbl, h_out, h_q, h_part;
     DBuf<int> h_pidx;
     DBuf<rsx::HistTile> h_tiles;
-    DBuf<float> d_ftab, d_qv, d_r2, d_rext;
+    DBuf<float> d_ftab, d_qv, d_qsplit, d_r2, d_rext;
     DBuf<double> d_qrev;
     DBuf<ChunkConst> d_chunks;
     DBuf<double> d_invq, d_partials, d_iq, d_saved_iq, d_out, d_wq, d_q0;
@@ -479,7 +479,7 @@ struct rsx_engine {
     double b_radius = 0.0;
     DBuf<ChunkConst> b_chunks;
     DBuf<double> b_invq, b_qrev;
-    DBuf<float> b_qv, b_ftab;
+    DBuf<float> b_qv, b_qsplit, b_ftab;
     HBuf<float> b_pin_ftab;
     HBuf<double> b_pin_q0;
     std::vector<double> b_f64;       // [class][n_q] FP64 class factors of the current table
@@ -577,7 +577,7 @@ void choose_chunks(rsx_engine* h)
 // Per-chunk constants and the q tables ([ldf], zero padded) of a chunking of the
 // engine's q grid into n_chunks runs of K.
 void upload_q_consts(const rsx_engine* h, int K, int n_chunks, DBuf<ChunkConst>& chunks, DBuf<double>& d_invq,
-                     DBuf<double>& d_qrev, DBuf<float>& d_qv)
+                     DBuf<double>& d_qrev, DBuf<float>& d_qsplit, DBuf<float>& d_qv)
 {
     const int nq = h->nq, ldf = K * n_chunks;
     std::vector<ChunkConst> cc(n_chunks);
@@ -590,17 +590,22 @@ void upload_q_consts(const rsx_engine* h, int K, int n_chunks, DBuf<ChunkConst>&
         cc[c].pad = 0.0f;
     }
     std::vector<double> invq(ldf, 0.0), qrev(ldf, 0.0);
-    std::vector<float> qv(ldf, 0.0f);
+    std::vector<float> qv(ldf, 0.0f), qsplit(2 * (size_t)ldf, 0.0f);
     for (int k = 0; k < nq; ++k) {
         invq[k] = h->q[k] != 0.0f ? 1.0 / (double)h->q[k] : 0.0;
         qrev[k] = (double)h->q[k] / (2.0 * PI);
         qv[k] = h->q[k];
+        qsplit[2 * k] = (float)qrev[k];
+        qsplit[2 * k + 1] = (float)(qrev[k] - (double)qsplit[2 * k]);
     }
     chunks.ensure(cc.size());
     d_invq.ensure(ldf);
     d_qrev.ensure(ldf);
     d_qv.ensure(ldf);
+    d_qsplit.ensure(2 * (size_t)ldf);
     hipchk(hipMemcpy(chunks.p, cc.data(), cc.size() * sizeof(ChunkConst), hipMemcpyHostToDevice), "upload chunks");
+    hipchk(hipMemcpy(d_qsplit.p, qsplit.data(), 2 * (size_t)ldf * sizeof(float), hipMemcpyHostToDevice),
+           "upload qsplit");
     hipchk(hipMemcpy(d_invq.p, invq.data(), ldf * sizeof(double), hipMemcpyHostToDevice), "upload invq");
     hipchk(hipMemcpy(d_qrev.p, qrev.data(), ldf * sizeof(double), hipMemcpyHostToDevice), "upload qrev");
     hipchk(hipMemcpy(d_qv.p, qv.data(), ldf * sizeof(float), hipMemcpyHostToDevice), "upload qv");
@@ -612,7 +617,7 @@ void apply_path(rsx_engine* h, bool uniform)
     h->layout_stamp++;
     h->uniform = uniform;
     choose_chunks(h);
-    upload_q_consts(h, h->K, h->n_chunks, h->d_chunks, h->d_invq, h->d_qrev, h->d_qv);
+    upload_q_consts(h, h->K, h->n_chunks, h->d_chunks, h->d_invq, h->d_qrev, h->d_qsplit, h->d_qv);
     h->params_dirty = true;  // the class table is laid out with stride ldf
 }
 
@@ -654,6 +659,15 @@ constexpr double MAX_STEP_ANGLE = 3.0;
 bool fits_recurrence(const rsx_engine* h, double radius)
 {
     return h->uniform_grid && h->dq * 2.0 * radius <= MAX_STEP_ANGLE;
+}
+
+// The direct kernel may form its phases in FP32 when every q r / (2 pi) with r <= 2 R
+// stays within rsx::DIRECT_PHASE32_MAX_REV revolutions.
+bool phase32_ok(const rsx_engine* h, double radius)
+{
+    double qmax = 0.0;
+    for (float v : h->q) qmax = std::max(qmax, (double)std::fabs(v));
+    return qmax * 2.0 * radius / (2.0 * PI) <= rsx::DIRECT_PHASE32_MAX_REV;
 }
 
 double bounding_radius(const float* bodies, int n, double* centre)
@@ -1174,6 +1188,8 @@ rsx::DebyeArgs debye_args(rsx_engine* h, const PairJob* d_jobs, double* partials
     a.dq = (float)h->dq;
     a.invq = h->d_invq.p;
     a.qrev = h->d_qrev.p;
+    a.qsplit = h->d_qsplit.p;
+    a.phase32 = phase32_ok(h, h->path_radius) ? 1 : 0;
     a.qv = h->d_qv.p;
     a.partials = partials;
     a.variant = h->variant;
@@ -2213,7 +2229,7 @@ void batch_layout(rsx_engine* h)
     h->b_variant = h->b_var_req >= 0 ? h->b_var_req : h->variant;
     pick_chunks(h, h->b_uniform, h->b_variant, h->b_k_req > 0 ? h->b_k_req : h->k_req, 0, h->b_K, h->b_n_chunks);
     h->b_ldf = h->b_K * h->b_n_chunks;
-    upload_q_consts(h, h->b_K, h->b_n_chunks, h->b_chunks, h->b_invq, h->b_qrev, h->b_qv);
+    upload_q_consts(h, h->b_K, h->b_n_chunks, h->b_chunks, h->b_invq, h->b_qrev, h->b_qsplit, h->b_qv);
     // classes
     h->b_hydrate = hydration_on(h);
     if (h->b_hydrate) batch_counts(h);
@@ -2364,6 +2380,8 @@ rsx::DebyeArgs batch_debye_args(rsx_engine* h)
     a.dq = (float)h->dq;
     a.invq = h->b_invq.p;
     a.qrev = h->b_qrev.p;
+    a.qsplit = h->b_qsplit.p;
+    a.phase32 = phase32_ok(h, h->b_radius) ? 1 : 0;
     a.qv = h->b_qv.p;
     a.partials = h->b_partials.p;
     a.variant = h->b_variant;
@@ -2982,7 +3000,7 @@ int rsx_destroy(rsx_engine* h)
     for (auto* b : f4) b->release();
     h->d_qrev.release();
     DBuf<float>* ff[] = {&h->d_ftab, &h->d_qv, &h->d_r2, &h->d_rext, &h->d_grad, &h->a_ftab, &h->a_rowpart,
-                         &h->a_colpart, &h->a_invq32, &h->a_cb, &h->a_cg, &h->b_qv, &h->b_ftab};
+                         &h->a_colpart, &h->a_invq32, &h->a_cb, &h->a_cg, &h->b_qv, &h->b_ftab, &h->d_qsplit, &h->b_qsplit};
     for (auto* b : ff) b->release();
     DBuf<double>* fd[] = {&h->d_invq, &h->d_partials, &h->d_iq, &h->d_saved_iq, &h->d_out, &h->d_wq, &h->d_q0,
                           &h->d_red, &h->b_partials, &h->b_out, &h->b_mean, &h->b_q0, &h->b_red, &h->b_invq,

@@ -78,6 +78,9 @@ struct DebyeArgs {
     float dq;
     const double* invq;   // [ldf] 1/q, 0 for q == 0 and padding
     const double* qrev;   // [ldf] q / (2 pi)  (direct path)
+    const float* qsplit;  // [2 ldf] q / (2 pi) as FP32 pairs (hi, lo), hi + lo = qrev to ~1e-14
+                          // relative (direct path, FP32 phase)
+    int phase32;          // direct path: phases in FP32 (every q r / (2 pi) <= DIRECT_PHASE32_MAX_REV)
     const float* qv;      // [ldf] q           (direct path)
     double* partials;     // [n_slots][ldf]
     int variant;          // recurrence kernel: VARIANT_WAVE, VARIANT_SHARED, VARIANT_FP32 or VARIANT_FP32R
@@ -86,6 +89,12 @@ struct DebyeArgs {
     int sin_terms;        // terms the sin(dq r / 2) series needs (sin_series_terms); the launcher
                           // picks the shortest instantiated length >= it
 };
+
+// Largest phase, in revolutions, of the direct kernel's FP32 phase form: v_sin_f32 takes
+// revolutions and is exact in its reduction over its documented domain |x| <= 256 (measured
+// on gfx950: 1.2e-7 absolute error, unbiased, to 32 revolutions); the bound leaves the
+// rounding of the phase itself (half an ulp, 4e-6 revolutions at 64) random per pair.
+constexpr double DIRECT_PHASE32_MAX_REV = 64.0;
 
 // Debye pair sums.  uniform: q_k = a + k dq within each chunk (FP64
 // three-term recurrence); otherwise direct FP32 sin per term.

@@ -719,9 +719,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void de
 // ---------------------------------------------------------------------------
 // direct policy (any q grid): FP32 accumulators, one v_sin per term
 // ---------------------------------------------------------------------------
-template <int K, bool WEIGHTED>
+// One column against the wave's rows at K q.  The phase q_k r / (2 pi) goes to v_sin_f32
+// in revolutions.  PH32: formed in FP32 as fma(hi_k, r, lo_k r) from the split
+// hi_k + lo_k = q_k / (2 pi), a single rounding of a phase that is otherwise exact for the
+// FP32 r (v_sin_f32 reduces its argument exactly, so no fraction is taken); the caller
+// guarantees the phase stays within DIRECT_PHASE32_MAX_REV.  Otherwise formed in FP64 and
+// reduced to [0, 1) before the conversion.
+template <int K, bool WEIGHTED, bool PH32>
 __device__ __forceinline__ void direct_step(float (&acc)[K], float& zsum, float xi, float yi, float zi,
-                                            const float4 c, const double* __restrict__ qrev, float w)
+                                            const float4 c, const double* __restrict__ qrev,
+                                            const float* __restrict__ qsplit, float w)
 {
     const float dx = xi - c.x, dy = yi - c.y, dz = zi - c.z;
     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
@@ -731,16 +738,24 @@ __device__ __forceinline__ void direct_step(float (&acc)[K], float& zsum, float 
     const float wr = WEIGHTED ? w * rinv : rinv;
     // r == 0 pairs contribute w * q_k; their weights are summed and applied at the flush
     zsum += nz ? 0.0f : (WEIGHTED ? w : 1.0f);
-    const double rd = (double)r;
+    if (PH32) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = fmaf(sin_rev(qrev[k] * rd), wr, acc[k]);
+        for (int k = 0; k < K; ++k) {
+            const float ph = fmaf(qsplit[2 * k], r, qsplit[2 * k + 1] * r);
+            acc[k] = fmaf(__builtin_amdgcn_sinf(ph), wr, acc[k]);
+        }
+    } else {
+        const double rd = (double)r;
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] = fmaf(sin_rev(qrev[k] * rd), wr, acc[k]);
+    }
 }
 
 // columns summed in FP32 before their sums go to the FP64 accumulators (a class run of
 // plain factors can span a job's whole column range, up to 1024 columns)
 constexpr int DIRECT_FLUSH = 32;
 
-template <int K>
+template <int K, bool PH32>
 __global__ __launch_bounds__(64, 2) void debye_direct_kernel(const DebyeArgs a)
 {
     const int job_id = blockIdx.x / a.n_chunks;
@@ -754,6 +769,7 @@ __global__ __launch_bounds__(64, 2) void debye_direct_kernel(const DebyeArgs a)
     const float* __restrict__ ftab = a.ftab;
     const int ldf = a.ldf;
     const double* __restrict__ qrev = a.qrev + k0;
+    const float* __restrict__ qsplit = a.qsplit + 2 * k0;
     const float* __restrict__ qv = a.qv + k0;
 
     float acc[K];
@@ -793,7 +809,7 @@ __global__ __launch_bounds__(64, 2) void debye_direct_kernel(const DebyeArgs a)
             }
             if (cl != CLS_SKIP) {
                 const float w = j > lane ? 1.0f : (j == lane ? 0.5f : 0.0f);
-                direct_step<K, true>(acc, zsum, ri.x, ri.y, ri.z, c, qrev, w);
+                direct_step<K, true, PH32>(acc, zsum, ri.x, ri.y, ri.z, c, qrev, qsplit, w);
                 ++since;
             }
         }
@@ -807,7 +823,7 @@ __global__ __launch_bounds__(64, 2) void debye_direct_kernel(const DebyeArgs a)
             since = 0;
         }
         if (cl != CLS_SKIP) {
-            direct_step<K, false>(acc, zsum, ri.x, ri.y, ri.z, c, qrev, 1.0f);
+            direct_step<K, false, PH32>(acc, zsum, ri.x, ri.y, ri.z, c, qrev, qsplit, 1.0f);
             ++since;
         }
     }
@@ -1809,6 +1825,16 @@ void launch_recurrence_k(const DebyeArgs& a, int n_jobs, hipStream_t s)
 }
 
 template <int K>
+void launch_direct_k(const DebyeArgs& a, int n_jobs, hipStream_t s)
+{
+    const dim3 grid((unsigned)(n_jobs * a.n_chunks)), block(64);
+    if (a.phase32 && a.qsplit)
+        hipLaunchKernelGGL((debye_direct_kernel<K, true>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((debye_direct_kernel<K, false>), grid, block, 0, s, a);
+}
+
+template <int K>
 void launch_rec32_k(const DebyeArgs& a, int n_jobs, hipStream_t s)
 {
     const int n_items = n_jobs * a.n_chunks;
@@ -1949,13 +1975,12 @@ hipError_t launch_debye(const DebyeArgs& a, int n_jobs, int K, bool uniform, hip
         default: return hipErrorInvalidValue;
         }
     } else {
-        const dim3 grid((unsigned)(n_jobs * a.n_chunks)), block(64);
         switch (K) {
-        case 8: hipLaunchKernelGGL(debye_direct_kernel<8>, grid, block, 0, s, a); break;
-        case 16: hipLaunchKernelGGL(debye_direct_kernel<16>, grid, block, 0, s, a); break;
-        case 32: hipLaunchKernelGGL(debye_direct_kernel<32>, grid, block, 0, s, a); break;
-        case 48: hipLaunchKernelGGL(debye_direct_kernel<48>, grid, block, 0, s, a); break;
-        case 64: hipLaunchKernelGGL(debye_direct_kernel<64>, grid, block, 0, s, a); break;
+        case 8: launch_direct_k<8>(a, n_jobs, s); break;
+        case 16: launch_direct_k<16>(a, n_jobs, s); break;
+        case 32: launch_direct_k<32>(a, n_jobs, s); break;
+        case 48: launch_direct_k<48>(a, n_jobs, s); break;
+        case 64: launch_direct_k<64>(a, n_jobs, s); break;
         default: return hipErrorInvalidValue;
         }
     }

@@ -132,6 +132,28 @@ def test_nonuniform_grid_direct_path_full_size(oracle):
     assert err.max() < TOL
 
 
+@pytest.mark.parametrize("radius", [60.0, 400.0])
+def test_nonuniform_grid_phase_forms(oracle, radius):
+    """The direct kernel forms phases q r / (2 pi) in FP32 while every one stays within 64
+    revolutions (a 60 A ball at q <= 0.6: 11.5) and in FP64 beyond (a sparse 400 A ball:
+    76 revolutions at the largest pairs); both against the FP64 oracle."""
+    from resaxs_amd import configs
+    rng = np.random.default_rng(31)
+    n = 2500
+    pos = configs.in_ball(rng, n, radius).astype(np.float32)
+    types = configs.seeded_types(rng, n)
+    q = np.sort(rng.uniform(0.0, 0.6, 96)).astype(np.float32)
+    q[-1] = 0.6
+    table = configs.factor_table(rng, q)
+    _say(f"case: non-uniform q, {n} atoms in a {radius:.0f} A ball x {len(q)} q")
+    e = engine_for(q, table, n=n)
+    I = e.calc_curve(configs.bodies_of(pos, types))
+    assert not e.get_tuning()["recurrence"]
+    err = rel_err(I, oracle_iq(q, pos, types, table))
+    _say(f"  max rel err {err.max():.3e}, median {np.median(err):.3e}")
+    assert err.max() < TOL
+
+
 def test_coincident_atoms_and_tiny_n(oracle):
     """r = 0 pairs give sinc = 1 (gpu_saxs_pt_wf.cl:55); N = 1, 2, 65 edge sizes."""
     _say("case: coincident atoms, N in {1, 2, 3, 65}")
