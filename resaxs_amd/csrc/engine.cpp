@@ -333,9 +333,21 @@ struct rsx_engine {
     // device buffers
     DBuf<float4> d_cols, d_saved_cols, d_sasa_atoms, d_sorted_atoms;
     bool sasa_mirror = false;        // d_sasa_atoms and d_counts hold exactly the host bodies and counts
-    // local SASA recount: the listed atoms, and the block read back
-    // ([listed, changed, (atom, count) x LOCAL_CHANGED_CAP])
+    // local SASA recount: the listed atoms, and the block read back (two counter sets of
+    // rsx::LOCAL_HDR_INTS, then (atom, count) x LOCAL_CHANGED_CAP)
     DBuf<int> d_local_list, d_local;
+    int l_parity = 0;                // counter set of the next local recount
+    int l_used_parity = 0;           // counter set of the last one enqueued
+    // slab cell list of the local recounts (rsx::SasaArgs::slab): records, per-bucket counts,
+    // each atom's record.  slab_age counts the recounts since it was built from the device
+    // atoms (-1: not built for them); slab_off: the atoms overflowed a bucket right after a
+    // build, so local recounts rebuild the hashed cell list instead
+    DBuf<float4> d_slab;
+    DBuf<int> d_slab_count, d_slab_pos;
+    int slab_age = -1;
+    bool slab_off = false;
+    const void* slab_built[3] = {nullptr, nullptr, nullptr};  // slab, counts, positions built
+    int slab_built_nb = 0, slab_built_n = 0;
     hipEvent_t ev_sasa = nullptr;    // recorded after the local recount's read-back
     // the local recount's stage (rsx::LocalStage + moved atoms) and its captured graph,
     // kept while its arguments and buffers are unchanged
@@ -825,6 +837,8 @@ double sum_f0(rsx_engine* h)
 // ---- SASA ----------------------------------------------------------------
 // changed counts a local recount reads back in its one transfer (more: all counts compared)
 constexpr int LOCAL_CHANGED_CAP = 4096;
+// local recounts between rebuilds of the slab (clears the tombstones moves leave in it)
+constexpr int SLAB_REBUILD = 256;
 // moved atoms a local recount's captured graph carries in its stage (more: eager launches)
 constexpr int LOCAL_PATCH_CAP = 4096;
 
@@ -860,8 +874,9 @@ void release_local_graph(rsx_engine* h)
 // SASA arguments over the engine's buffers for n atoms in d_sasa_atoms (zeroed first, so
 // that two argument sets compare byte for byte).  local_cap >= 0: a local recount (the
 // changed counts listed in d_local, the box and moved range in d_lstage; with
-// local_cap > 0 the first kernel also writes up to local_cap moved atoms from the stage).
-rsx::SasaArgs sasa_args(rsx_engine* h, int n, int local_cap = -1)
+// local_cap > 0 the first kernel also writes up to local_cap moved atoms from the stage),
+// on the slab cell list when slab is set.
+rsx::SasaArgs sasa_args(rsx_engine* h, int n, int local_cap = -1, bool slab = false)
 {
     float rmax = 0.0f;
     for (float v : h->rext) rmax = std::max(rmax, v);
@@ -897,15 +912,31 @@ rsx::SasaArgs sasa_args(rsx_engine* h, int n, int local_cap = -1)
     a.counts = h->d_counts.p;
     if (local_cap >= 0) {
         h->d_local_list.ensure(n);
-        h->d_local.ensure(2 + 2 * (size_t)LOCAL_CHANGED_CAP);
+        const size_t local_ints = rsx::LOCAL_HDR_TOTAL + 2 * (size_t)LOCAL_CHANGED_CAP;
+        if (h->d_local.cap < local_ints) {
+            h->d_local.ensure(local_ints);
+            // counter sets start cleared (each recount then clears the next one's)
+            hipchk(hipMemsetAsync(h->d_local.p, 0, rsx::LOCAL_HDR_TOTAL * sizeof(int), h->stream), "clear counters");
+        }
         h->d_lstage.ensure(rsx::LOCAL_STAGE_HEADER + 16 * (size_t)LOCAL_PATCH_CAP);
         a.local_hdr = h->d_local.p;
         a.local_list = h->d_local_list.p;
-        a.local_changed = h->d_local.p + 2;
+        a.local_changed = h->d_local.p + rsx::LOCAL_HDR_TOTAL;
         a.changed_cap = LOCAL_CHANGED_CAP;
         a.local_stage = reinterpret_cast<const rsx::LocalStage*>(h->d_lstage.p);
         a.atoms_w = h->d_sasa_atoms.p;
         a.patch_cap = local_cap;
+        if (slab) {
+            int nbs = 1024;
+            while (nbs < n) nbs <<= 1;
+            h->d_slab.ensure((size_t)nbs * rsx::SLAB_CAP);
+            h->d_slab_count.ensure(nbs);
+            h->d_slab_pos.ensure(n);
+            a.slab = h->d_slab.p;
+            a.slab_count = h->d_slab_count.p;
+            a.slab_pos = h->d_slab_pos.p;
+            a.slab_buckets = nbs;
+        }
     }
     return a;
 }
@@ -926,6 +957,7 @@ void compute_sasa(rsx_engine* h)
     hipchk(hipMemcpyAsync(h->d_sasa_atoms.p, h->bodies.data(), (size_t)n * 16, hipMemcpyHostToDevice, h->stream),
            "upload sasa atoms");
     h->sasa_mirror = true;
+    h->slab_age = -1;
     run_sasa_device(h, n);
     mark_dirty_full(h);
     h->counts.resize(n);
@@ -940,10 +972,12 @@ void compute_sasa(rsx_engine* h)
 // Local recount after bodies [start, start+len) moved (old positions in old_bodies),
 // enqueued: only an atom within reach (2 max(r + p) + 0.02) of a moved atom's old or new
 // position can change its count, so the atoms inside the moved set's bounding box widened by
-// that reach are recounted on the device against the full, rebuilt cell list, and the
-// counts that changed come back in one transfer (sasa_local_finish waits for it).  Every
-// other count is unchanged by construction.  Returns false (nothing enqueued) without
-// current counts to compare with.
+// that reach are recounted on the device against a current cell list, and the counts that
+// changed come back in one transfer (sasa_local_finish waits for it).  Every other count is
+// unchanged by construction.  The cell list is the slab (moved atoms moved in it by one
+// kernel; rebuilt from the device atoms every SLAB_REBUILD recounts and whenever those were
+// written otherwise) or, without it, the hashed cell list rebuilt over all atoms.  Returns
+// false (nothing enqueued) without current counts to compare with.
 bool sasa_local_enqueue(rsx_engine* h, int start, int len, const std::vector<float>& old_bodies)
 {
     const int n = h->n;
@@ -968,14 +1002,31 @@ bool sasa_local_enqueue(rsx_engine* h, int start, int len, const std::vector<flo
     }
     st->start = start;
     st->len = len;
+    st->parity = h->l_parity;
+    h->l_used_parity = h->l_parity;
+    h->l_parity ^= 1;
     h->d_sasa_atoms.ensure(n);
-    const size_t local_bytes = (2 + 2 * (size_t)LOCAL_CHANGED_CAP) * sizeof(int);
-    h->pin_local.ensure(2 + 2 * (size_t)LOCAL_CHANGED_CAP);
+    const size_t local_ints = rsx::LOCAL_HDR_TOTAL + 2 * (size_t)LOCAL_CHANGED_CAP;
+    const size_t local_bytes = local_ints * sizeof(int);
+    h->pin_local.ensure(local_ints);
     if (h->sasa_mirror && len <= LOCAL_PATCH_CAP) {
-        // one replay of a captured graph: stage upload, the moved atoms patched in with the
-        // bucket counts cleared, the cell list, the listed counts, the read-back
+        // one replay of a captured graph: stage upload, the cell-list work (on the slab: one
+        // kernel), the listed counts, the read-back
         std::memcpy(h->pin_lstage.p + rsx::LOCAL_STAGE_HEADER, &h->bodies[4 * (size_t)start], (size_t)len * 16);
-        const rsx::SasaArgs a = sasa_args(h, n, LOCAL_PATCH_CAP);
+        const bool slab = !h->slab_off;
+        const rsx::SasaArgs a = sasa_args(h, n, LOCAL_PATCH_CAP, slab);
+        if (slab && (h->slab_age < 0 || h->slab_age >= SLAB_REBUILD || h->slab_built[0] != a.slab ||
+                     h->slab_built[1] != a.slab_count || h->slab_built[2] != a.slab_pos ||
+                     h->slab_built_nb != a.slab_buckets || h->slab_built_n != n)) {
+            // from the device atoms as they are before this move (the graph patches them)
+            hipchk(rsx::launch_slab_build(a, h->stream), "slab build");
+            h->slab_age = 0;
+            h->slab_built[0] = a.slab;
+            h->slab_built[1] = a.slab_count;
+            h->slab_built[2] = a.slab_pos;
+            h->slab_built_nb = a.slab_buckets;
+            h->slab_built_n = n;
+        }
         const bool same = h->l_exec && std::memcmp(&a, &h->l_args, sizeof a) == 0 &&
                           h->l_pin_stage == h->pin_lstage.p && h->l_pin_local == h->pin_local.p;
         if (!same) {
@@ -1000,7 +1051,9 @@ bool sasa_local_enqueue(rsx_engine* h, int start, int len, const std::vector<flo
             h->l_pin_local = h->pin_local.p;
         }
         hipchk(hipGraphLaunch(h->l_exec, h->stream), "graph launch");
+        if (slab) h->slab_age++;
     } else {
+        h->slab_age = -1;  // the device atoms are written here without the slab
         if (h->sasa_mirror) {
             h->pin_moved.ensure(len);
             std::memcpy(h->pin_moved.p, &h->bodies[4 * (size_t)start], (size_t)len * 16);
@@ -1031,12 +1084,21 @@ bool sasa_local_enqueue(rsx_engine* h, int start, int len, const std::vector<flo
 }
 
 // Waits for the local recount and applies it: changed receives the atoms whose count
-// changed, ascending.
-void sasa_local_finish(rsx_engine* h, std::vector<int>& changed)
+// changed, ascending.  Returns false, with nothing applied, when a slab bucket overflowed
+// (that recount missed atoms: the caller counts every atom instead).
+bool sasa_local_finish(rsx_engine* h, std::vector<int>& changed)
 {
     hipchk(hipEventSynchronize(h->ev_sasa), "sync sasa");
     changed.clear();
-    const int n_changed = h->pin_local.p[1];
+    const int* hdr = h->pin_local.p + rsx::LOCAL_HDR_INTS * h->l_used_parity;
+    const int* pairs = h->pin_local.p + rsx::LOCAL_HDR_TOTAL;
+    if (hdr[2] != 0) {
+        // overflow in the first recount after a build: these atoms do not fit the slab
+        h->slab_off = h->slab_age <= 1;
+        h->slab_age = -1;
+        return false;
+    }
+    const int n_changed = hdr[1];
     auto apply = [&](int i, uint32_t c) {
         changed.push_back(i);
         mark_dirty(h, i);
@@ -1045,10 +1107,10 @@ void sasa_local_finish(rsx_engine* h, std::vector<int>& changed)
     };
     if (n_changed <= LOCAL_CHANGED_CAP) {
         std::vector<std::pair<int, uint32_t>> e((size_t)n_changed);
-        for (int t = 0; t < n_changed; ++t) e[t] = {h->pin_local.p[2 + 2 * t], (uint32_t)h->pin_local.p[3 + 2 * t]};
+        for (int t = 0; t < n_changed; ++t) e[t] = {pairs[2 * t], (uint32_t)pairs[2 * t + 1]};
         std::sort(e.begin(), e.end());
         for (const auto& x : e) apply(x.first, x.second);
-        return;
+        return true;
     }
     // more than the block holds: every count compared
     std::vector<uint32_t> c((size_t)h->n);
@@ -1057,6 +1119,7 @@ void sasa_local_finish(rsx_engine* h, std::vector<int>& changed)
     hipchk(hipStreamSynchronize(h->stream), "sync counts");
     for (int i = 0; i < h->n; ++i)
         if (c[i] != h->counts[i]) apply(i, c[i]);
+    return true;
 }
 
 // ---- pair jobs ---------------------------------------------------------------
@@ -1727,9 +1790,8 @@ void incremental_update(rsx_engine* h, int start, int len, const std::vector<flo
 
     // atoms whose exposed-dot count changed
     std::vector<int> changed;
-    if (local) {
-        sasa_local_finish(h, changed);
-    } else if (sasa_now) {
+    const bool recounted = local && sasa_local_finish(h, changed);
+    if (!recounted && sasa_now) {  // (a local recount that overflowed the slab lands here too)
         const std::vector<uint32_t> old_counts = h->counts;
         compute_sasa(h);
         for (int i = 0; i < h->n; ++i)
@@ -2996,7 +3058,8 @@ int rsx_destroy(rsx_engine* h)
     release_local_graph(h);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->stream2) (void)hipStreamSynchronize(h->stream2);
-    DBuf<float4>* f4[] = {&h->d_cols, &h->d_saved_cols, &h->d_sasa_atoms, &h->d_dots, &h->b_cols, &h->d_sorted_atoms};
+    DBuf<float4>* f4[] = {&h->d_cols, &h->d_saved_cols, &h->d_sasa_atoms, &h->d_dots, &h->b_cols, &h->d_sorted_atoms,
+                          &h->d_slab};
     for (auto* b : f4) b->release();
     h->d_qrev.release();
     DBuf<float>* ff[] = {&h->d_ftab, &h->d_qv, &h->d_r2, &h->d_rext, &h->d_grad, &h->a_ftab, &h->a_rowpart,
@@ -3007,7 +3070,8 @@ int rsx_destroy(rsx_engine* h)
                           &h->b_qrev, &h->d_partials_new};
     for (auto* b : fd) b->release();
     DBuf<int>* fi[] = {&h->d_slot_begin, &h->d_bucket_of, &h->d_bucket_count, &h->d_bucket_start, &h->d_bucket_fill,
-                       &h->d_sorted, &h->d_s2c, &h->b_slot_begin, &h->d_local_list, &h->d_local, &h->d_tile_sum};
+                       &h->d_sorted, &h->d_s2c, &h->b_slot_begin, &h->d_local_list, &h->d_local, &h->d_tile_sum,
+                       &h->d_slab_count, &h->d_slab_pos};
     for (auto* b : fi) b->release();
     h->d_chunks.release();
     h->b_chunks.release();

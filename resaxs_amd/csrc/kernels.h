@@ -161,13 +161,21 @@ struct ClassSortArgs {
 hipError_t launch_class_sort(const ClassSortArgs& a, hipStream_t s);
 
 // ---- SASA ----
-// Inputs of a local recount on the device: the box [lo, hi], the moved range and, from
-// LOCAL_STAGE_HEADER bytes on, the moved atoms.
+// Inputs of a local recount on the device: the box [lo, hi], the moved range, the counter
+// set this recount uses (parity) and, from LOCAL_STAGE_HEADER bytes on, the moved atoms.
 struct LocalStage {
     float lo[3], hi[3];
     int start, len;
+    int parity;
 };
-constexpr int LOCAL_STAGE_HEADER = 32;  // bytes before the moved atoms (float4-aligned)
+constexpr int LOCAL_STAGE_HEADER = 48;  // bytes before the moved atoms (float4-aligned)
+// Counters of a local recount: two sets of LOCAL_HDR_INTS ints ([listed, changed, slab
+// overflow, unused]); recount set p is cleared by the recount before it, which works on set
+// 1 - p (or by the slab build).  The changed (atom, count) pairs follow the two sets.
+constexpr int LOCAL_HDR_INTS = 4;
+constexpr int LOCAL_HDR_TOTAL = 2 * LOCAL_HDR_INTS;
+// Records per bucket of the slab cell list.
+constexpr int SLAB_CAP = 64;
 
 struct SasaArgs {
     const float4* atoms;     // x, y, z, type bits
@@ -191,12 +199,13 @@ struct SasaArgs {
     uint32_t* counts;        // [n] exposed dots (only the listed atoms when subset is set)
     const int* subset;       // null: count every atom; else count atoms subset[0 .. n_subset)
     int n_subset;
-    // Local recount (local_hdr set; subset unused): the place kernel lists the atoms whose
-    // position lies in the box of local_stage in local_list (in no fixed order); the dots
-    // kernel counts only those, over a fixed grid, and for each count that differs from the
-    // one in counts appends (atom, new count) to local_changed before overwriting it.
-    // local_hdr[0] = atoms listed, local_hdr[1] = counts changed (entries from changed_cap on
-    // are counted but not written); the scan clears both.
+    // Local recount (local_hdr set; subset unused): the place kernel (or, with a slab, the
+    // slab update) lists the atoms whose position lies in the box of local_stage in local_list
+    // (in no fixed order); the dots kernel counts only those, over a fixed grid, and for each
+    // count that differs from the one in counts appends (atom, new count) to local_changed
+    // before overwriting it.  Counters in set local_stage->parity of local_hdr
+    // (LOCAL_HDR_TOTAL ints): [0] atoms listed, [1] counts changed (entries from changed_cap
+    // on are counted but not written), [2] slab overflow.
     int* local_hdr = nullptr;
     int* local_list = nullptr;   // [n]
     int* local_changed = nullptr;  // [changed_cap][2]
@@ -207,8 +216,23 @@ struct SasaArgs {
     const LocalStage* local_stage = nullptr;
     float4* atoms_w = nullptr;
     int patch_cap = 0;
+    // Slab cell list of a local recount (slab set): bucket b (of slab_buckets, a power of two)
+    // holds its records at slab[b * SLAB_CAP, b * SLAB_CAP + min(slab_count[b], SLAB_CAP)), in
+    // no fixed order; slab_pos[i] is the index of atom i's record (-1: none).  A local recount
+    // keeps it current for the moved atoms instead of rebuilding the hashed cell list: a moved
+    // atom's record is rewritten in place when its bucket is unchanged, otherwise replaced by a
+    // tombstone (type bits all ones: no dots, occludes nothing) and appended to its new bucket.
+    // A bucket that would pass SLAB_CAP raises the set's overflow counter; the counts of that
+    // recount are then not to be used.
+    float4* slab = nullptr;
+    int* slab_count = nullptr;
+    int* slab_pos = nullptr;
+    int slab_buckets = 0;
 };
 hipError_t launch_sasa(const SasaArgs& a, hipStream_t s);
+// The slab of a local recount built from atoms[0, n): slab_count and both counter sets
+// cleared, every atom appended to its bucket.
+hipError_t launch_slab_build(const SasaArgs& a, hipStream_t s);
 
 // ---- adjoint ----
 // Adjoint over the forward's triangle pair jobs (slot-ordered atoms).

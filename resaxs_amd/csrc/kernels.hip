@@ -1162,9 +1162,21 @@ __device__ __forceinline__ bool dot_occluded(float dx, float dy, float dz, float
     return (r2j - dist2) > 1e-4f;
 }
 
+// Bucket of cell (ix, iy, iz) among nb (a power of two): the low 10 bits of each coordinate
+// packed into one word (distinct for cells within 1024 of each other per axis, so no two
+// cells of a structure share the key), mixed by the 32-bit finalizer of MurmurHash3
+// (A. Appleby, public domain) so that the low bits the mask keeps depend on every key bit.
+// (A XOR of the coordinates times three large primes, taken modulo a power of two, gave
+// 3117 distinct values for the 4805 cells of a 50,000-atom ellipsoid: 1.5 cells per bucket.)
+// Atoms of other cells in a bucket only cost prefilter work, never a count.
 __device__ __forceinline__ int cell_hash(int ix, int iy, int iz, int nb)
 {
-    const uint32_t h = ((uint32_t)ix * 73856093u) ^ ((uint32_t)iy * 19349663u) ^ ((uint32_t)iz * 83492791u);
+    uint32_t h = ((uint32_t)ix & 1023u) | (((uint32_t)iy & 1023u) << 10) | (((uint32_t)iz & 1023u) << 20);
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
     return (int)(h & (uint32_t)(nb - 1));
 }
 
@@ -1258,35 +1270,118 @@ __global__ __launch_bounds__(256) void sasa_scan_add_kernel(SasaArgs a)
     const int v = a.bucket_start[b] + a.tile_sum[b / SCAN_TILE];
     a.bucket_start[b] = v;
     a.bucket_fill[b] = v;
-    if (a.local_hdr && b < 2) a.local_hdr[b] = 0;
+    // both counter sets: this recount's, and the next one's (which the next recount, on the
+    // slab, takes as cleared)
+    if (a.local_hdr && b < LOCAL_HDR_TOTAL) a.local_hdr[b] = 0;
 }
 
-// Places atom i in its bucket; for a local recount also lists it when inside the box (one
-// atomic per wave for the wave's listed atoms).
+// Appends atom i to the local recount's list (set hdr) when its position p lies inside the
+// box: one atomic per wave for the wave's listed atoms.
+__device__ __forceinline__ void list_if_in_box(const SasaArgs& a, int* hdr, const LocalStage& st, int i, bool valid,
+                                               const float4 p)
+{
+    const bool in = valid && p.x >= st.lo[0] && p.x <= st.hi[0] && p.y >= st.lo[1] && p.y <= st.hi[1] &&
+                    p.z >= st.lo[2] && p.z <= st.hi[2];
+    const uint64_t m = __ballot(in);
+    if (m) {
+        const int lane = threadIdx.x & 63, first = __builtin_ctzll(m);
+        int base = 0;
+        if (lane == first) base = atomicAdd(&hdr[0], __builtin_popcountll(m));
+        base = __shfl(base, first, 64);
+        const int pos = base + __builtin_popcountll(m & ((1ull << lane) - 1ull));
+        if (in && pos < a.n) a.local_list[pos] = i;  // (each atom is listed once: pos < n)
+    }
+}
+
+// Places atom i in its bucket; for a local recount also lists it when inside the box.
 __global__ void sasa_place_kernel(SasaArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (a.local_hdr) {  // uniform
         const LocalStage& st = *a.local_stage;
-        bool in = false;
-        if (i < a.n) {
-            const float4 p = a.atoms[i];
-            in = p.x >= st.lo[0] && p.x <= st.hi[0] && p.y >= st.lo[1] && p.y <= st.hi[1] && p.z >= st.lo[2] &&
-                 p.z <= st.hi[2];
-        }
-        const uint64_t m = __ballot(in);
-        if (m) {
-            const int lane = threadIdx.x & 63, first = __builtin_ctzll(m);
-            int base = 0;
-            if (lane == first) base = atomicAdd(&a.local_hdr[0], __builtin_popcountll(m));
-            base = __shfl(base, first, 64);
-            if (in) a.local_list[base + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = i;
-        }
+        list_if_in_box(a, a.local_hdr + LOCAL_HDR_INTS * st.parity, st, i, i < a.n,
+                       i < a.n ? a.atoms[i] : make_float4(0.f, 0.f, 0.f, 0.f));
     }
     if (i >= a.n) return;
     const int pos = atomicAdd(&a.bucket_fill[a.bucket_of[i]], 1);
     a.sorted[pos] = i;
     a.sorted_atoms[pos] = a.atoms[i];
+}
+
+// ---- slab cell list (local recounts) ----
+__device__ __forceinline__ int slab_bucket(const SasaArgs& a, const float4 p)
+{
+    return cell_hash(cell_coord(p.x, a.inv_cell), cell_coord(p.y, a.inv_cell), cell_coord(p.z, a.inv_cell),
+                     a.slab_buckets);
+}
+
+// Appends record p of atom i to bucket b; returns false (nothing written, slab_pos[i] = -1)
+// when the bucket is full.
+__device__ __forceinline__ bool slab_append(const SasaArgs& a, int i, int b, const float4 p)
+{
+    const int s = atomicAdd(&a.slab_count[b], 1);
+    if (s >= SLAB_CAP) {
+        a.slab_pos[i] = -1;
+        return false;
+    }
+    a.slab[(size_t)b * SLAB_CAP + s] = p;
+    a.slab_pos[i] = b * SLAB_CAP + s;
+    return true;
+}
+
+__global__ void slab_clear_kernel(SasaArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < a.slab_buckets) a.slab_count[t] = 0;
+    if (t < LOCAL_HDR_TOTAL) a.local_hdr[t] = 0;
+}
+
+__global__ void slab_build_kernel(SasaArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const float4 p = a.atoms[i];
+    if (!slab_append(a, i, slab_bucket(a, p), p)) {
+        // flagged in both counter sets: whichever the next recount uses reports it
+        atomicOr(&a.local_hdr[2], 1);
+        atomicOr(&a.local_hdr[LOCAL_HDR_INTS + 2], 1);
+    }
+}
+
+// The whole cell-list work of a local recount on the slab, one thread per atom: the stage's
+// moved atoms written into the atom array and moved in the slab (in place when the bucket is
+// unchanged, else a tombstone at the old record and an append to the new bucket), every atom
+// inside the box listed, and the other counter set cleared for the next recount.  The moved
+// atoms' old records are replaced only by their own threads, and appends take distinct
+// records by the atomic, so no two threads write one record.
+__global__ void slab_update_kernel(SasaArgs a)
+{
+    const LocalStage& st = *a.local_stage;
+    int* hdr = a.local_hdr + LOCAL_HDR_INTS * st.parity;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < LOCAL_HDR_INTS) a.local_hdr[LOCAL_HDR_INTS * (1 - st.parity) + i] = 0;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < a.n) {
+        const int m = i - st.start;
+        if (m >= 0 && m < st.len && m < a.patch_cap) {
+            const float4* moved = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(a.local_stage) +
+                                                                  LOCAL_STAGE_HEADER);
+            p = moved[m];
+            a.atoms_w[i] = p;
+            const int b = slab_bucket(a, p);
+            int s_old = a.slab_pos[i];
+            if (s_old >= a.slab_buckets * SLAB_CAP) s_old = -1;  // (never: slab_pos indexes the slab)
+            if (s_old >= 0 && s_old / SLAB_CAP == b) {
+                a.slab[s_old] = p;
+            } else {
+                if (s_old >= 0) a.slab[s_old] = make_float4(0.f, 0.f, 0.f, __uint_as_float(0xffffffffu));
+                if (!slab_append(a, i, b, p)) atomicOr(&hdr[2], 1);
+            }
+        } else {
+            p = a.atoms[i];
+        }
+    }
+    list_if_in_box(a, hdr, st, i, i < a.n, p);
 }
 
 constexpr int SASA_MAX_DPL = 8;  // dots per lane (n_dots <= 512)
@@ -1344,11 +1439,18 @@ __device__ __forceinline__ int sasa_count_atom(const SasaArgs& a, int i, float4*
     // lane c < 27: neighbour cell c's bucket range
     const int cx = cell_coord(pi.x, a.inv_cell), cy = cell_coord(pi.y, a.inv_cell), cz = cell_coord(pi.z, a.inv_cell);
     int cstart = 0, ccount = 0;
+    const float4* __restrict__ src = a.slab ? a.slab : a.sorted_atoms;
     if (lane < 27) {
         const int oz = lane / 9 - 1, oy = (lane / 3) % 3 - 1, ox = lane % 3 - 1;
-        const int b = cell_hash(cx + ox, cy + oy, cz + oz, a.n_buckets);
-        cstart = a.bucket_start[b];
-        ccount = a.bucket_start[b + 1] - cstart;
+        if (a.slab) {  // uniform
+            const int b = cell_hash(cx + ox, cy + oy, cz + oz, a.slab_buckets);
+            cstart = b * SLAB_CAP;
+            ccount = min(a.slab_count[b], SLAB_CAP);
+        } else {
+            const int b = cell_hash(cx + ox, cy + oy, cz + oz, a.n_buckets);
+            cstart = a.bucket_start[b];
+            ccount = a.bucket_start[b + 1] - cstart;
+        }
     }
     int incl = ccount;  // inclusive prefix of the counts (lanes >= 27 add 0)
 #pragma unroll
@@ -1369,7 +1471,7 @@ __device__ __forceinline__ int sasa_count_atom(const SasaArgs& a, int i, float4*
             if (__shfl(incl, c + step - 1, 64) <= q) c += step;
         const int e0 = __shfl(excl, c, 64), s_c = __shfl(cstart, c, 64);
         valid = q < total;
-        return a.sorted_atoms[valid ? s_c + (q - e0) : 0];
+        return src[valid ? s_c + (q - e0) : 0];
     };
     bool next_valid;
     float4 next = load(lane, next_valid);
@@ -1414,13 +1516,14 @@ __global__ __launch_bounds__(64) void sasa_dots_kernel(SasaArgs a)
 {
     __shared__ float4 cand[64];
     if (a.local_hdr) {
-        const int listed = a.local_hdr[0];
+        int* hdr = a.local_hdr + LOCAL_HDR_INTS * a.local_stage->parity;
+        const int listed = min(hdr[0], a.n);
         for (int k = blockIdx.x; k < listed; k += gridDim.x) {
             const int i = a.local_list[k];
             const uint32_t c = (uint32_t)sasa_count_atom<DPL>(a, i, cand);
             if (threadIdx.x == 0) {
                 if (a.counts[i] != c) {
-                    const int e = atomicAdd(&a.local_hdr[1], 1);
+                    const int e = atomicAdd(&hdr[1], 1);
                     if (e < a.changed_cap) {
                         a.local_changed[2 * e] = i;
                         a.local_changed[2 * e + 1] = (int)c;
@@ -2313,25 +2416,43 @@ hipError_t launch_fill_i32(int* p, int value, long long n, hipStream_t s)
     return hipGetLastError();
 }
 
+hipError_t launch_slab_build(const SasaArgs& a, hipStream_t s)
+{
+    if (!a.slab || !a.local_hdr || a.slab_buckets <= 0 || (a.slab_buckets & (a.slab_buckets - 1)))
+        return hipErrorInvalidValue;
+    const int nc = a.slab_buckets > LOCAL_HDR_TOTAL ? a.slab_buckets : LOCAL_HDR_TOTAL;
+    hipLaunchKernelGGL(slab_clear_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, a);
+    if (a.n > 0) hipLaunchKernelGGL(slab_build_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_sasa(const SasaArgs& a, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
     if (a.n_dots <= 0 || a.n_dots > 64 * SASA_MAX_DPL) return hipErrorInvalidValue;
     if (a.local_hdr && !a.local_stage) return hipErrorInvalidValue;
-    if (a.patch_cap > 0) {
+    if (a.slab && (!a.local_hdr || a.slab_buckets <= 0 || (a.slab_buckets & (a.slab_buckets - 1))))
+        return hipErrorInvalidValue;
+    if (a.slab) {
+        // local recount on the slab: one update kernel (patch, slab, list), then the dots
+        const int nt = a.n > LOCAL_HDR_INTS ? a.n : LOCAL_HDR_INTS;
+        hipLaunchKernelGGL(slab_update_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a);
+    } else if (a.patch_cap > 0) {
         const long long nt = (long long)a.n_buckets + 1 > a.patch_cap ? (long long)a.n_buckets + 1 : a.patch_cap;
         hipLaunchKernelGGL(sasa_patch_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a);
     } else {
         hipError_t e = launch_fill_i32(a.bucket_count, 0, (long long)a.n_buckets + 1, s);
         if (e != hipSuccess) return e;
     }
-    const dim3 g((unsigned)((a.n + 255) / 256)), b(256);
-    if (a.n_buckets < SCAN_TILE || (a.n_buckets & (a.n_buckets - 1))) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(sasa_bin_kernel, g, b, 0, s, a);
-    hipLaunchKernelGGL(sasa_scan_tiles_kernel, dim3((unsigned)(a.n_buckets / SCAN_TILE)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(sasa_scan_top_kernel, dim3(1), dim3(1024), 0, s, a);
-    hipLaunchKernelGGL(sasa_scan_add_kernel, dim3((unsigned)(a.n_buckets / 256)), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(sasa_place_kernel, g, b, 0, s, a);
+    if (!a.slab) {
+        const dim3 g((unsigned)((a.n + 255) / 256)), b(256);
+        if (a.n_buckets < SCAN_TILE || (a.n_buckets & (a.n_buckets - 1))) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(sasa_bin_kernel, g, b, 0, s, a);
+        hipLaunchKernelGGL(sasa_scan_tiles_kernel, dim3((unsigned)(a.n_buckets / SCAN_TILE)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(sasa_scan_top_kernel, dim3(1), dim3(1024), 0, s, a);
+        hipLaunchKernelGGL(sasa_scan_add_kernel, dim3((unsigned)(a.n_buckets / 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(sasa_place_kernel, g, b, 0, s, a);
+    }
     // a local recount loops over its list with a fixed grid (the list's length is on the device)
     const int n_waves = a.local_hdr ? (a.n < 8192 ? a.n : 8192) : a.subset ? a.n_subset : a.n;
     if (n_waves <= 0) return hipGetLastError();

@@ -361,3 +361,51 @@ def test_hydrated_recount_fallbacks_and_size_change(oracle):
         _say(f"  move {k}: max rel err {err:.3e}")
         assert err < 1e-5
     e.close()
+
+
+def test_hydrated_recount_slab_rebuild_and_overflow(oracle):
+    """The slab cell list of the local recounts.  300 committed 40-atom moves pass its
+    periodic rebuild (every 256 recounts, which clears the tombstones moves leave).  Then a
+    move gathers 80 atoms into a 0.6 A ball, more than a bucket holds: that recount reports
+    the overflow and the counts come from a full count instead; the next moves rebuild the
+    slab, overflow again right after the build, and go on without it.  Counts stay bit-exact
+    and I(q) follows the FP64 oracle."""
+    from resaxs_amd import configs
+
+    def check(e, w, pos, what, iq):
+        dots = fib_dot_table(w.radii, w.probe, w.n_half)
+        counts = oracle.sasa_counts(pos, w.types, r2_table(w.radii, w.probe), dots)
+        assert np.array_equal(e.get_rsasa(), counts.astype(np.float32) / np.float32(256)), what
+        err = rel_err(iq, oracle_iq(w.q, pos, w.types, w.table, w.water_index, w.w, counts, 256)).max()
+        _say(f"  {what}: max rel err {err:.3e}")
+        assert err < 1e-5, what
+
+    _say("case: slab rebuild and overflow, 6000 atoms: 300 moves of 40 atoms, then a gathering move")
+    rng = np.random.default_rng(43)
+    w = configs.incremental_50000(n=6000)
+    e = engine_for(w.q, w.table, w.water_index, w.w, n=len(w.pos), radii=w.radii, probe=w.probe, n_half=w.n_half)
+    pos = w.pos.copy()
+    e.calc_curve(configs.bodies_of(pos, w.types))
+    for k in range(300):
+        start, newp = configs.random_move(rng, pos, 40)
+        pos[start:start + 40] = newp
+        I = e.recalc_curve(configs.bodies_of(pos, w.types), start, 40)
+        e.commit()
+    check(e, w, pos, "after 300 moves", I)
+    # 80 atoms into a 0.7 A cube at the centre of the cell of the segment's first atom (cell
+    # edge 2 max(r + p) + 0.02, as the engine's cell lists use), so all land in one bucket
+    start = 2000
+    cell = 2.0 * float(np.float32(max(w.radii)) + np.float32(w.probe)) + 0.02
+    centre = (np.floor(pos[start].astype(np.float64) / cell) + 0.5) * cell
+    gather = centre + np.random.default_rng(9).uniform(-0.35, 0.35, size=(80, 3))
+    pos[start:start + 80] = gather.astype(np.float32)
+    I = e.recalc_curve(configs.bodies_of(pos, w.types), start, 80)
+    e.commit()
+    check(e, w, pos, "gathering move", I)
+    for k in range(3):
+        start, newp = configs.random_move(rng, pos, 40)
+        pos[start:start + 40] = newp
+        I = e.recalc_curve(configs.bodies_of(pos, w.types), start, 40)
+        e.commit()
+        check(e, w, pos, f"move {k} after the gathering", I)
+    e.close()
