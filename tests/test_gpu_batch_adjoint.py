@@ -63,7 +63,7 @@ def test_batch_full_size(oracle):
     graph-captured launch.  Sampled frames (smallest, largest and four more) against the
     FP64 oracle; every frame's q = 0 value is (sum F)^2; the mean is the frames' average;
     a graph replay reproduces every bit; the persistent work-queue launch at K = 64
-    agrees to rounding."""
+    agrees to 1e-5."""
     from resaxs_amd import configs
     w = configs.ensemble_512()
     nf = len(w.sizes)
@@ -92,7 +92,9 @@ def test_batch_full_size(oracle):
     assert e.batch_launch()["persistent_waves"] > 0
     err = rel_err(frames3, frames).max()
     _say(f"  K=64 work-queue launch vs default: max rel diff {err:.3e}")
-    assert err < 5e-6
+    # two FP32 kernels, each within a few 1e-6 of FP64 at these frames' deepest minima: they
+    # agree to the accuracy contract, not to each other's rounding
+    assert err < 1e-5
 
 
 def _frames(seed, n_frames, nmin, nmax):
