@@ -348,6 +348,7 @@ struct rsx_engine {
     bool slab_off = false;
     const void* slab_built[3] = {nullptr, nullptr, nullptr};  // slab, counts, positions built
     int slab_built_nb = 0, slab_built_n = 0;
+    float slab_built_cell = 0.0f;
     hipEvent_t ev_sasa = nullptr;    // recorded after the local recount's read-back
     // the local recount's stage (rsx::LocalStage + moved atoms) and its captured graph,
     // kept while its arguments and buffers are unchanged
@@ -1017,7 +1018,7 @@ bool sasa_local_enqueue(rsx_engine* h, int start, int len, const std::vector<flo
         const rsx::SasaArgs a = sasa_args(h, n, LOCAL_PATCH_CAP, slab);
         if (slab && (h->slab_age < 0 || h->slab_age >= SLAB_REBUILD || h->slab_built[0] != a.slab ||
                      h->slab_built[1] != a.slab_count || h->slab_built[2] != a.slab_pos ||
-                     h->slab_built_nb != a.slab_buckets || h->slab_built_n != n)) {
+                     h->slab_built_nb != a.slab_buckets || h->slab_built_n != n || h->slab_built_cell != a.cell)) {
             // from the device atoms as they are before this move (the graph patches them)
             hipchk(rsx::launch_slab_build(a, h->stream), "slab build");
             h->slab_age = 0;
@@ -1026,6 +1027,7 @@ bool sasa_local_enqueue(rsx_engine* h, int start, int len, const std::vector<flo
             h->slab_built[2] = a.slab_pos;
             h->slab_built_nb = a.slab_buckets;
             h->slab_built_n = n;
+            h->slab_built_cell = a.cell;
         }
         const bool same = h->l_exec && std::memcmp(&a, &h->l_args, sizeof a) == 0 &&
                           h->l_pin_stage == h->pin_lstage.p && h->l_pin_local == h->pin_local.p;
