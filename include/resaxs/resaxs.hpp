@@ -17,6 +17,9 @@
 #include "../rsx_capi.h"
 
 #if __has_include(<CL/cl_platform.h>)
+#ifndef CL_TARGET_OPENCL_VERSION
+#define CL_TARGET_OPENCL_VERSION 120  // the OpenCL 1.2 types the reference's records use
+#endif
 #include <CL/cl_platform.h>
 #else
 typedef float cl_float;
