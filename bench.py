@@ -34,9 +34,11 @@ PROFILES = os.path.join(ROOT, "profiles")
 # HBM bytes per launch from the committed rocprofv3 counter summaries
 TRAFFIC_SUMMARY = os.path.join(PROFILES, "debye_pair_kernel_plain-50000x256_counters.txt")
 SASA_SUMMARY = os.path.join(PROFILES, "sasa_kernels_sasa-hydrated-20000x256_counters.txt")
-# full-recompute period of the incremental workload (the engine's default; the running FP64
-# I(q) does not drift measurably over the 2000 moves, so none falls inside the run)
-REBASELINE = 10000
+# full-recompute period of the incremental workload: every 200 moves, as the workload defines
+# it; the same moves are also timed with the engine's default period (10000: the running FP64
+# I(q) does not drift measurably over the 2000 moves, so no full recompute falls inside them)
+REBASELINE = 200
+REBASELINE_DEFAULT = 10000
 # autotuner cache (one line per arch / path / n_q / size bucket / hydration key); a missing key is
 # tuned on the spot and appended
 TUNING_CACHE = os.path.join(ROOT, "resaxs_amd", "tuning_gfx950.txt")
@@ -298,19 +300,19 @@ def adjoint_bench(reps=3):
             "timing": "device: HIP events on the engine stream; wall: host clock around rsx_chi2_grad"}
 
 
-def incremental_bench(hydrate, n_moves=2000):
+def incremental_bench(hydrate, n_moves=2000, rebaseline=REBASELINE):
     """The incremental workload: 50,000 atoms x 256 q, n_moves committed rigid moves of
-    500-atom segments, running FP64 I(q) re-baselined every REBASELINE moves.  Host wall
+    500-atom segments, running FP64 I(q) re-baselined (a full recompute in place of the
+    update) every `rebaseline` moves, those moves timed with the rest.  Host wall
     clock around recalc_curve + commit (the synchronous calls a Monte Carlo caller makes),
     work per move 2 P_move (5 n_q + 9) with P_move = m (N - m) + m (m - 1) / 2.  The
-    running I(q) after the next-to-last move (with 2000 moves: move 1999, all of them since
-    the first full forward) is compared with a fresh full recompute by the FP64 pair
-    kernel."""
+    running I(q) after the next-to-last move is compared with a fresh full recompute by the
+    FP64 pair kernel."""
     from resaxs_amd import configs
     w = configs.incremental_50000()
     n, nq, m = len(w.pos), len(w.q), w.seg
     e = make_engine(w, hydrate=hydrate)
-    e.set_rebaseline(REBASELINE)
+    e.set_rebaseline(rebaseline)
     pos = w.pos.copy()
     b = configs.bodies_of(pos, w.types)
     e.calc_curve(b)
@@ -335,14 +337,17 @@ def incremental_bench(hydrate, n_moves=2000):
     p_move = m * (n - m) + m * (m - 1) / 2.0
     ms = float(np.mean(ts)) * 1e3
     return {"workload": f"{w.name}{'-sasa' if hydrate else ''}", "moves": n_moves, "segment": m,
-            "rebaseline_every": REBASELINE, "ms_per_move_mean": round(ms, 4),
+            "rebaseline_every": rebaseline, "ms_per_move_mean": round(ms, 4),
             "ms_per_move_median": round(float(np.median(ts)) * 1e3, 4),
+            "rebaselines": n_moves // rebaseline,
+            "ms_rebaseline_move_mean": (round(float(np.mean(ts[rebaseline - 1::rebaseline])) * 1e3, 3)
+                                        if n_moves >= rebaseline else None),
             "terms_per_s": 2 * p_move * nq / (ms * 1e-3),
             "achieved_tflops": round(2 * p_move * (5 * nq + 9) / (ms * 1e-3) / 1e12, 2),
             "rel_err_vs_gpu_fp64_full": err_last,
             "rel_err_reference": "a fresh full recompute of the same positions by the GPU's FP64 three-term "
                                  "recurrence kernel (variant 0)",
-            "moves_since_rebaseline_at_check": (n_moves - 2) % REBASELINE + 1,
+            "moves_since_rebaseline_at_check": (n_moves - 2) % rebaseline + 1,
             "timing": "host wall clock around rsx_recalc_curve_f64 + rsx_commit (host bodies in, I(q) out)"
                       + ("; SASA recounted for the atoms within reach of the moved segment" if hydrate else "")}
 
@@ -625,6 +630,8 @@ def main():
         log("bench: batched ensemble")
         line["batched"] = batched_bench()
         line["incremental"] = [incremental_bench(False), incremental_bench(True)]
+        line["incremental_rebaseline_10000"] = [incremental_bench(False, rebaseline=REBASELINE_DEFAULT),
+                                                incremental_bench(True, rebaseline=REBASELINE_DEFAULT)]
         log("bench: ensemble fit driver")
         line["fit_driver"] = fit_driver_bench()
         log("bench: distance-histogram Debye")

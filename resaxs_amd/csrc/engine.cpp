@@ -447,6 +447,17 @@ struct rsx_engine {
 
     // last forward launch (for timing)
     std::vector<PairJob> last_jobs;
+    // what the forward job table on the device (d_fjobs, last_jobs) was built for: atoms,
+    // column range, column buffer, kernel (variant, K, chunks) and path
+    struct FwdKey {
+        int n = -1, L = 0, variant = -1, K = 0, n_chunks = 0, uniform = -1;
+        const void* cols = nullptr;
+        bool operator==(const FwdKey& o) const
+        {
+            return n == o.n && L == o.L && variant == o.variant && K == o.K && n_chunks == o.n_chunks &&
+                   uniform == o.uniform && cols == o.cols;
+        }
+    } fjobs_key;
     int last_slots = 0;
     double last_q0 = 0.0;
 
@@ -1393,6 +1404,19 @@ void order_jobs(rsx_engine* h, std::vector<PairJob>& jobs)
     split_tail(jobs, (long)jobs.size() > 2 * round ? round : 0, cost);
 }
 
+rsx_engine::FwdKey forward_key(const rsx_engine* h, int L)
+{
+    rsx_engine::FwdKey k;
+    k.n = h->n;
+    k.L = L;
+    k.variant = h->variant;
+    k.K = h->K;
+    k.n_chunks = h->n_chunks;
+    k.uniform = h->uniform ? 1 : 0;
+    k.cols = h->d_cols.p;
+    return k;
+}
+
 std::vector<PairJob> forward_jobs(rsx_engine* h)
 {
     const int n = h->n;
@@ -1401,19 +1425,29 @@ std::vector<PairJob> forward_jobs(rsx_engine* h)
     h->last_L = L;
     triangle_jobs(jobs, h->d_cols.p, n, L, 2.0f);
     order_jobs(h, jobs);
+    h->fjobs_key = forward_key(h, L);
     return jobs;
 }
 
+// The full forward into d_iq.  The job table is built (host triangle tiling, cost order,
+// upload) only when the layout's key changed: a re-baseline of the incremental path repeats
+// the forward on the same columns, whose table is still on the device.
 void full_forward(rsx_engine* h)
 {
-    const std::vector<PairJob> jobs = forward_jobs(h);
     const double sf = sum_f0(h);
-    run_jobs(h, h->d_fjobs, jobs, h->d_iq.p, 0, sf * sf);
-    h->last_jobs = jobs;
+    const int L = pick_range(h, (double)h->n * h->n / 2.0);
+    if (h->last_jobs.empty() || !(h->fjobs_key == forward_key(h, L))) {
+        const std::vector<PairJob> jobs = forward_jobs(h);
+        run_jobs(h, h->d_fjobs, jobs, h->d_iq.p, 0, sf * sf);
+        h->last_jobs = jobs;
+        int sbf[2] = {0, (int)jobs.size()};
+        h->d_slot_begin_f.ensure(2);
+        hipchk(hipMemcpy(h->d_slot_begin_f.p, sbf, sizeof(sbf), hipMemcpyHostToDevice), "upload forward slots");
+    } else {
+        h->last_L = L;
+        launch_jobs(h, h->d_fjobs.p, (int)h->last_jobs.size(), h->d_iq.p, 0, sf * sf);
+    }
     h->last_q0 = sf * sf;
-    int sbf[2] = {0, (int)jobs.size()};
-    h->d_slot_begin_f.ensure(2);
-    hipchk(hipMemcpy(h->d_slot_begin_f.p, sbf, sizeof(sbf), hipMemcpyHostToDevice), "upload forward slots");
     h->since_rebase = 0;
 }
 
