@@ -851,6 +851,10 @@ double sum_f0(rsx_engine* h)
 constexpr int LOCAL_CHANGED_CAP = 4096;
 // local recounts between rebuilds of the slab (clears the tombstones moves leave in it)
 constexpr int SLAB_REBUILD = 256;
+// largest structure whose local recounts use the slab: it holds 64 float4 records per bucket
+// and a power of two >= n buckets, 1 KB per atom (4 GiB at this size); larger structures
+// rebuild the hashed cell list instead
+constexpr int SLAB_MAX_ATOMS = 1 << 22;
 // moved atoms a local recount's captured graph carries in its stage (more: eager launches)
 constexpr int LOCAL_PATCH_CAP = 4096;
 
@@ -1025,7 +1029,7 @@ bool sasa_local_enqueue(rsx_engine* h, int start, int len, const std::vector<flo
         // one replay of a captured graph: stage upload, the cell-list work (on the slab: one
         // kernel), the listed counts, the read-back
         std::memcpy(h->pin_lstage.p + rsx::LOCAL_STAGE_HEADER, &h->bodies[4 * (size_t)start], (size_t)len * 16);
-        const bool slab = !h->slab_off;
+        const bool slab = !h->slab_off && n <= SLAB_MAX_ATOMS;
         const rsx::SasaArgs a = sasa_args(h, n, LOCAL_PATCH_CAP, slab);
         if (slab && (h->slab_age < 0 || h->slab_age >= SLAB_REBUILD || h->slab_built[0] != a.slab ||
                      h->slab_built[1] != a.slab_count || h->slab_built[2] != a.slab_pos ||
