@@ -1706,10 +1706,39 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 67 ? 3 
     const size_t part = (size_t)jb.slot * a.n_chunks + chunk;
     float* __restrict__ colpart = a.col_partial + part * (size_t)a.col_range * 3;
     uint32_t cur = CLS_SKIP;
+    // per-column seeds of the difference recurrence and the pair geometry
+    struct Col {
+        float u, d, delta, R, scale, dx, dy, dz;
+    };
+    auto seed = [&](const float4 c, float w) {
+        Col o;
+        o.dx = ri.x - c.x;
+        o.dy = ri.y - c.y;
+        o.dz = ri.z - c.z;
+        const float r2 = fmaf(o.dz, o.dz, fmaf(o.dy, o.dy, o.dx * o.dx));
+        const float nzf = r2 > 1e-30f ? 1.0f : 0.0f;
+        const PairR pr = pair_r(r2);
+        const float rinv = pr.rinv, r = pr.r;
+        const double rd = (double)r;
+        const float sh = sin_small_f32<M>(r * h_dq);                    // sin(dq r / 2)
+        const float ch = __builtin_sqrtf(fmaxf(1.0f - sh * sh, 0.0f));  // cos(dq r / 2)
+        const float fa = (float)__builtin_amdgcn_fract(cc.a_rev * rd);  // a r in revolutions
+        const float sa = __builtin_amdgcn_sinf(fa);                     // sin(a r)
+        const float clo = fmaf(__builtin_amdgcn_cosf(fa), ch, sa * sh);  // cos((a - dq/2) r)
+        o.delta = 4.0f * sh * sh;
+        // R = r / (2 sin(dq r)) = r / (4 sin(dq r/2) cos(dq r/2)); 0 for r == 0
+        o.R = nzf * r * __builtin_amdgcn_rcpf(fmaxf(4.0f * sh * ch, 1e-30f));
+        o.u = sa * rinv;
+        o.d = (clo + clo) * sh * rinv;
+        o.scale = rinv * rinv * w;
+        return o;
+    };
     for (int t0 = 0; t0 < ncol; t0 += ADJ_TC) {
         const int tn = ncol - t0 < ADJ_TC ? ncol - t0 : ADJ_TC;
+        // columns in pairs of one class (two independent recurrence chains per step); a
+        // column without a partner runs with a weight-0 copy of itself
 #pragma unroll 1
-        for (int t = 0; t < tn; ++t) {
+        for (int t = 0; t < tn;) {
             const int j = t0 + t;
             const float4 c = cols[j];
             const uint32_t cl = __float_as_uint(c.w);
@@ -1734,41 +1763,43 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 67 ? 3 
                 }
                 beta[K] = prev;
             }
-            const float w = j < jd ? (j > lane ? row_w : 0.0f) : row_w;
-            const float dx = ri.x - c.x, dy = ri.y - c.y, dz = ri.z - c.z;
-            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            const float nzf = r2 > 1e-30f ? 1.0f : 0.0f;
-            const PairR pr = pair_r(r2);
-            const float rinv = pr.rinv, r = pr.r;
-            const double rd = (double)r;
-            const float sh = sin_small_f32<M>(r * h_dq);             // sin(dq r / 2)
-            const float ch = __builtin_sqrtf(fmaxf(1.0f - sh * sh, 0.0f));  // cos(dq r / 2)
-            const float fa = (float)__builtin_amdgcn_fract(cc.a_rev * rd);  // a r in revolutions
-            const float sa = __builtin_amdgcn_sinf(fa);              // sin(a r)
-            const float clo = fmaf(__builtin_amdgcn_cosf(fa), ch, sa * sh);  // cos((a - dq/2) r)
-            const float delta = 4.0f * sh * sh;
-            // R = r / (2 sin(dq r)) = r / (4 sin(dq r/2) cos(dq r/2)); 0 for r == 0
-            const float R = nzf * r * __builtin_amdgcn_rcpf(fmaxf(4.0f * sh * ch, 1e-30f));
-            float u = sa * rinv;
-            float d = (clo + clo) * sh * rinv;
-            float D = beta[0] * d, U = gamma[0] * u;
+            const bool two = t + 1 < tn && __float_as_uint(cols[j + 1].w) == cl;
+            const float4 c1 = two ? cols[j + 1] : c;
+            const float w0 = j < jd ? (j > lane ? row_w : 0.0f) : row_w;
+            const float w1 = two ? (j + 1 < jd ? (j + 1 > lane ? row_w : 0.0f) : row_w) : 0.0f;
+            const Col A = seed(c, w0), B = seed(c1, w1);
+            float da = A.d, ua = A.u, db = B.d, ub = B.u;
+            float Da = beta[0] * da, Ua = gamma[0] * ua, Db = beta[0] * db, Ub = gamma[0] * ub;
 #pragma unroll
             for (int k = 1; k < K; ++k) {
-                d = fmaf(-delta, u, d);
-                u += d;
-                D = fmaf(beta[k], d, D);
-                U = fmaf(gamma[k], u, U);
+                da = fmaf(-A.delta, ua, da);
+                db = fmaf(-B.delta, ub, db);
+                ua += da;
+                ub += db;
+                Da = fmaf(beta[k], da, Da);
+                Db = fmaf(beta[k], db, Db);
+                Ua = fmaf(gamma[k], ua, Ua);
+                Ub = fmaf(gamma[k], ub, Ub);
             }
-            d = fmaf(-delta, u, d);
-            D = fmaf(beta[K], d, D);
-            const float gs = fmaf(R, D, -U) * (rinv * rinv * w);
-            const float ex = gs * dx, ey = gs * dy, ez = gs * dz;
-            gx += ex;
-            gy += ey;
-            gz += ez;
-            T[t][0][lane] = -ex;
-            T[t][1][lane] = -ey;
-            T[t][2][lane] = -ez;
+            da = fmaf(-A.delta, ua, da);
+            db = fmaf(-B.delta, ub, db);
+            Da = fmaf(beta[K], da, Da);
+            Db = fmaf(beta[K], db, Db);
+            const float gsa = fmaf(A.R, Da, -Ua) * A.scale, gsb = fmaf(B.R, Db, -Ub) * B.scale;
+            const float eax = gsa * A.dx, eay = gsa * A.dy, eaz = gsa * A.dz;
+            const float ebx = gsb * B.dx, eby = gsb * B.dy, ebz = gsb * B.dz;
+            gx += eax + ebx;
+            gy += eay + eby;
+            gz += eaz + ebz;
+            T[t][0][lane] = -eax;
+            T[t][1][lane] = -eay;
+            T[t][2][lane] = -eaz;
+            if (two) {  // uniform
+                T[t + 1][0][lane] = -ebx;
+                T[t + 1][1][lane] = -eby;
+                T[t + 1][2][lane] = -ebz;
+            }
+            t += two ? 2 : 1;
         }
         __syncthreads();
         if (lane < 3 * tn) {
