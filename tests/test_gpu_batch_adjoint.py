@@ -226,6 +226,57 @@ def test_batch_autotune_and_cache(oracle, tmp_path):
     g.close()
 
 
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_ragged_and_empty_frames(oracle, persistent):
+    """Frames of 0, 1, 2, 3, 65 and 1000 atoms, empty frames first, between and last: every
+    frame's I(q) follows the FP64 oracle (an empty frame gives 0 at every q, one atom F^2),
+    and the mean is the uniform average over all frames, empty ones included."""
+    from resaxs_amd import configs
+    rng = np.random.default_rng(21)
+    q = np.linspace(0.0, 0.5, 48).astype(np.float32)
+    table = configs.factor_table(rng, q)
+    sizes = [0, 1, 2, 0, 3, 65, 1000, 0]
+    pos = [configs.in_ball(rng, m, 12.0).astype(np.float32) if m else np.zeros((0, 3), np.float32) for m in sizes]
+    typ = [configs.seeded_types(rng, m) if m else np.zeros(0, np.uint32) for m in sizes]
+    bodies = np.concatenate([configs.bodies_of(p, t) if len(p) else np.zeros((0, 4), np.float32)
+                             for p, t in zip(pos, typ)]).astype(np.float32)
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    _say(f"case: ragged batch {sizes} x {len(q)} q, persistent={persistent}")
+    e = engine_for(q, table, configs.WATER, 0.0)
+    e.set_persistent(persistent)
+    e.batch_set_frames(bodies, offsets)
+    frames, mean = e.batch_calc()
+    assert frames.shape == (len(sizes), len(q))
+    for f, m in enumerate(sizes):
+        if m == 0:
+            assert np.all(frames[f] == 0.0), f
+            continue
+        ref = oracle_iq(q, pos[f], typ[f], table)
+        err = rel_err(frames[f], ref).max()
+        assert err < 1e-5, (f, m, err)
+    assert np.allclose(mean, frames.mean(axis=0), rtol=1e-13, atol=0)
+    e.close()
+
+
+def test_zero_atoms_single_structure():
+    """A structure of no atoms: calc_curve gives 0 at every q, plain and hydrated, and an
+    empty incremental update leaves it so."""
+    from resaxs_amd import configs
+    rng = np.random.default_rng(22)
+    q = np.linspace(0.0, 0.5, 16).astype(np.float32)
+    table = configs.factor_table(rng, q)
+    empty = np.zeros((0, 4), np.float32)
+    for hydrate in (False, True):
+        _say(f"case: zero atoms, hydrate={hydrate}")
+        if hydrate:
+            e = engine_for(q, table, configs.WATER, 0.5, n=0, radii=configs.RADII, probe=configs.PROBE, n_half=32)
+        else:
+            e = engine_for(q, table, configs.WATER, 0.0, n=0)
+        assert np.all(e.calc_curve(empty) == 0.0)
+        assert np.all(e.recalc_curve(empty, 0, 0) == 0.0)
+        e.close()
+
+
 def test_batch_rejects_bad_offsets():
     """Offsets are checked before any use: offsets[0] must be 0 and they must not decrease."""
     from resaxs_amd import RsxError
