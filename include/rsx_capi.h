@@ -195,6 +195,13 @@ int rsx_batch_set_frames(rsx_engine *h, const float *bodies, const int *offsets,
 int rsx_batch_calc(rsx_engine *h, double *out_frames, double *out_mean);
 /* same on resident frames, results left on the device (no host copy); graph-captured */
 int rsx_batch_calc_dev_async(rsx_engine *h, double *out_frames_dev, double *out_mean_dev);
+
+/* Batched VJP (no reference counterpart: the reference has no adjoint):
+ * grad [atoms of all frames][3] FP32 in the frames' order as given to rsx_batch_set_frames =
+ * sum_k w[f][k] dI_f(q_k)/dx_i, f the frame of atom i, for weights w [n_frames][n_q] FP64 (the
+ * q = 0 entries are ignored).  One launch of the FP32 adjoint over every frame's pairs and a
+ * fixed-order reduction; needs a uniform q grid the recurrence covers.  Synchronous. */
+int rsx_batch_vjp(rsx_engine *h, const double *w, int n_frames, int n_q, float *out_grad);
 /* batched launch form (FP32 variants): 1 = one persistent grid of resident waves pulling
  * (frame job, q chunk) items from a work counter, longest first (default; taken only where
  * the work-queue kernel keeps the per-item kernel's occupancy); 0 = one wave per item */
@@ -261,6 +268,9 @@ int rsx_time_forward(rsx_engine *h, int reps, float *ms_total, float *ms_pair_ke
 /* average device time (ms) of the adjoint pass (kernel + fixed-order reduction) on the weights of
  * the last rsx_chi2_grad / rsx_debye_vjp call, HIP events on the engine stream */
 int rsx_time_adjoint(rsx_engine *h, int reps, float *ms);
+/* average device time (ms) of the batched adjoint pass (kernel + reduction) on the weights of
+ * the last rsx_batch_vjp call, HIP events on the engine stream (no reference counterpart) */
+int rsx_time_batch_vjp(rsx_engine *h, int reps, float *ms);
 /* average device time (ms) of one SASA count of the engine's atoms (cell list + dot occlusion
  * kernels), HIP events on the engine stream; needs a prior hydrated calc_curve (no reference
  * counterpart) */

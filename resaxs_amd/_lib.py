@@ -74,6 +74,8 @@ SIGNATURES = {
     "rsx_pdb_read": (_i, [_c.c_char_p, _i, _i, _fp, _ll, _c.POINTER(_ll), _ip, _i, _ip]),
     "rsx_batch_set_frames": (_i, [_vp, _fp, _ip, _i]),
     "rsx_batch_calc": (_i, [_vp, _dp, _dp]),
+    "rsx_batch_vjp": (_i, [_vp, _dp, _i, _i, _fp]),
+    "rsx_time_batch_vjp": (_i, [_vp, _i, _fp]),
     "rsx_batch_calc_dev_async": (_i, [_vp, _vp, _vp]),
     "rsx_set_persistent": (_i, [_vp, _i]),
     "rsx_chi2_grad": (_i, [_vp, _fp, _i, _fp, _fp, _dp, _dp, _fp, _dp]),

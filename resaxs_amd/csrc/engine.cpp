@@ -490,6 +490,18 @@ struct rsx_engine {
     std::vector<int> b_offsets;
     std::vector<float> b_bodies;     // frames as given, {x, y, z, type bits}
     bool b_layout_ok = false;        // the layout matches q, the tuning and the SASA set-up
+    uint64_t b_gen = 0;              // layouts built (the batched VJP's layout keys on it)
+    std::vector<int> b_s2c;          // caller index (position in the frames as given) per batch column
+    // batched VJP (rsx_batch_vjp): the adjoint's own q chunking and pair jobs over every frame,
+    // per-frame triangle tables, the class tables 2 F_c,k and 2 F_c,k / q_k, partials
+    uint64_t ba_gen = 0, ba_factor = 0;
+    int ba_K = 0, ba_nch = 0, ba_ldf = 0, ba_L = 0, ba_njobs = 0, ba_n = 0;
+    DBuf<float> ba_ftab, ba_cb, ba_cg, ba_invq32, ba_w, ba_rowpart, ba_colpart, ba_grad;
+    DBuf<double> ba_invq, ba_ones;
+    DBuf<ChunkConst> ba_chunks;
+    DBuf<PairJob> ba_jobs;
+    DBuf<int> ba_tri, ba_tri_off, ba_atom_frame, ba_frame_off, ba_s2c;
+    bool ba_have_w = false;
     uint64_t b_factor_stamp = 0;     // factor_stamp the class table was built from
     uint64_t b_sasa_stamp = 0;
     int b_nq = 0, b_n_types = 0;
@@ -598,6 +610,19 @@ void choose_chunks(rsx_engine* h)
     h->ldf = h->K * h->n_chunks;
 }
 
+// Seed constants of the q chunk starting at q index k0: a = q_k0 and the phases per unit r
+// of a and a + dq/2 in revolutions.
+ChunkConst chunk_const(const rsx_engine* h, int k0)
+{
+    ChunkConst c;
+    const double a = k0 < h->nq ? (double)h->q[k0] : 0.0;
+    c.a = (float)a;
+    c.a_rev = a / (2.0 * PI);
+    c.mid_rev = (a + 0.5 * h->dq) / (2.0 * PI);
+    c.pad = 0.0f;
+    return c;
+}
+
 // Per-chunk constants and the q tables ([ldf], zero padded) of a chunking of the
 // engine's q grid into n_chunks runs of K.
 void upload_q_consts(const rsx_engine* h, int K, int n_chunks, DBuf<ChunkConst>& chunks, DBuf<double>& d_invq,
@@ -605,14 +630,7 @@ void upload_q_consts(const rsx_engine* h, int K, int n_chunks, DBuf<ChunkConst>&
 {
     const int nq = h->nq, ldf = K * n_chunks;
     std::vector<ChunkConst> cc(n_chunks);
-    for (int c = 0; c < n_chunks; ++c) {
-        const int k0 = c * K;
-        const double a = k0 < nq ? (double)h->q[k0] : 0.0;
-        cc[c].a = (float)a;
-        cc[c].a_rev = a / (2.0 * PI);
-        cc[c].mid_rev = (a + 0.5 * h->dq) / (2.0 * PI);
-        cc[c].pad = 0.0f;
-    }
+    for (int c = 0; c < n_chunks; ++c) cc[c] = chunk_const(h, c * K);
     std::vector<double> invq(ldf, 0.0), qrev(ldf, 0.0);
     std::vector<float> qv(ldf, 0.0f), qsplit(2 * (size_t)ldf, 0.0f);
     for (int k = 0; k < nq; ++k) {
@@ -2339,6 +2357,7 @@ void batch_layout(rsx_engine* h)
     h->b_n_types = h->n_types;
     h->b_n_class = h->n_types * h->b_d1;
     std::vector<float4> cols(std::max(off[nf], 1));
+    h->b_s2c.assign(std::max(off[nf], 1), 0);
     h->b_pop_off.assign(nf + 1, 0);
     h->b_pop.clear();
     std::vector<int> pop(h->b_n_class, 0);
@@ -2355,6 +2374,7 @@ void batch_layout(rsx_engine* h)
         std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return cl[x] < cl[y]; });
         for (int a = 0; a < e - b; ++a) {
             const size_t i = (size_t)(b + idx[a]);
+            h->b_s2c[b + a] = (int)i;
             cols[b + a] = make_float4(bodies[4 * i], bodies[4 * i + 1], bodies[4 * i + 2], bitsf(cl[idx[a]]));
             pop[cl[idx[a]]]++;
         }
@@ -2409,6 +2429,7 @@ void batch_layout(rsx_engine* h)
     h->b_nq = h->nq;
     h->b_sasa_stamp = h->sasa_stamp;
     h->b_factor_stamp = 0;  // class factors still to come
+    h->b_gen++;
     h->b_layout_ok = true;
 }
 
@@ -2511,7 +2532,7 @@ void batch_enqueue(rsx_engine* h, double* out_frames, double* out_mean, int pers
 // the device.  The layout is redone when q, the tuning, the factor table's type count,
 // the SASA set-up or a switch to hydration made it stale; the class factors are
 // refreshed when the factor table or the water weight changed (the graph stays).
-void batch_run_dev(rsx_engine* h, double* out_frames, double* out_mean)
+void batch_prepare(rsx_engine* h)
 {
     if (h->b_frames <= 0) fail(RSX_ARGS_NOT_SET, "batch: no frames set");
     set_device(h);
@@ -2519,6 +2540,11 @@ void batch_run_dev(rsx_engine* h, double* out_frames, double* out_mean)
                        (hydration_on(h) && !h->b_hydrate) || (h->b_hydrate && h->b_sasa_stamp != h->sasa_stamp);
     if (stale) batch_layout(h);
     if (h->b_factor_stamp != h->factor_stamp) batch_factors(h);
+}
+
+void batch_run_dev(rsx_engine* h, double* out_frames, double* out_mean)
+{
+    batch_prepare(h);
     if (!h->b_exec || h->b_graph_out != out_frames || h->b_graph_mean != out_mean) {
         batch_release_graph(h);
         const bool fp32 = h->b_variant == rsx::VARIANT_FP32 || h->b_variant == rsx::VARIANT_FP32R;
@@ -2540,6 +2566,188 @@ void batch_run_dev(rsx_engine* h, double* out_frames, double* out_mean)
         h->b_graph_mean = out_mean;
     }
     hipchk(hipGraphLaunch(h->b_exec, h->stream), "graph launch");
+}
+
+// ---- batched VJP ---------------------------------------------------------------
+// grad [atoms of all frames][3] = sum_k w_f,k dI_f,k / dx_i, f the frame of atom i: one launch
+// of the FP32 adjoint kernel over every frame's triangle of pair jobs (each job carrying its
+// frame, whose weights scale the class tables inside the kernel) and one fixed-order
+// reduction.  The layout (the adjoint's q chunking and column range, the jobs, the per-frame
+// triangle tables, the class tables 2 F_c,k and 2 F_c,k / q_k) is kept until the batch layout
+// or its class factors change.
+void batch_adjoint_layout(rsx_engine* h)
+{
+    if (h->ba_gen == h->b_gen && h->ba_factor == h->b_factor_stamp && h->ba_njobs >= 0 && h->ba_K > 0) return;
+    const int nf = h->b_frames, nq = h->nq;
+    const std::vector<int>& off = h->b_offsets;
+    const int n = off[nf];
+    // q per thread: the FP32 adjoint's cost model (adjoint_layout)
+    int K = 0;
+    double best = 1e300;
+    for (int i = 0; i < rsx::adjoint_k_count(true); ++i) {
+        const int k = rsx::adjoint_k_value(true, i);
+        if (h->adj_k_req > 0 && k != h->adj_k_req) continue;
+        const double cost = ((nq + k - 1) / k) * (2.4 * k + 30.0);
+        if (cost < best) {
+            best = cost;
+            K = k;
+        }
+    }
+    if (K == 0) fail(RSX_INVALID_ARG, "batch vjp: the requested adjoint K is not available");
+    const int nch = (nq + K - 1) / K, ldfa = nch * K;
+    std::vector<ChunkConst> cc(nch);
+    for (int c = 0; c < nch; ++c) cc[c] = chunk_const(h, c * K);
+    const int ncl = std::max(1, h->b_n_class);
+    std::vector<float> ft((size_t)ncl * ldfa, 0.0f);
+    for (int c = 0; c < h->b_n_class; ++c)
+        for (int k = 0; k < nq; ++k) ft[(size_t)c * ldfa + k] = (float)h->b_f64[(size_t)c * nq + k];
+    std::vector<double> invq(ldfa, 0.0), ones(ldfa, 0.0);
+    std::vector<float> invq32(ldfa, 0.0f);
+    for (int k = 0; k < nq; ++k) {
+        invq[k] = h->q[k] != 0.0f ? 1.0 / (double)h->q[k] : 0.0;
+        invq32[k] = (float)invq[k];
+        ones[k] = 1.0;
+    }
+    // every frame's triangle (creation order: row block I, then its column ranges m)
+    const int L = h->adj_range_req > 0 ? h->adj_range_req : 128;
+    std::vector<PairJob> jobs;
+    std::vector<int> jframe, jblock, jm, tri_off(std::max(nf, 1), 0);
+    size_t tri_n = 0;
+    for (int f = 0; f < nf; ++f) {
+        const int m = off[f + 1] - off[f];
+        tri_off[f] = (int)tri_n;
+        if (m <= 0) continue;
+        triangle_jobs(jobs, h->b_cols.p + off[f], m, L, 1.0f);
+        const int T = (m + 63) / 64, stride = (m + L - 1) / L;
+        for (int I = 0; I < T; ++I)
+            for (int c0 = I * 64, mm = 0; c0 < m; c0 += L, ++mm) {
+                jframe.push_back(f);
+                jblock.push_back(I);
+                jm.push_back(mm);
+            }
+        tri_n += (size_t)T * stride;
+    }
+    if (tri_n > (size_t)INT32_MAX || jobs.size() > (size_t)INT32_MAX / 64) fail(RSX_INVALID_ARG, "batch vjp: too many jobs");
+    for (size_t j = 0; j < jobs.size(); ++j) jobs[j].pad = (int)j;
+    finalize_jobs(jobs);
+    std::vector<int> tri(std::max<size_t>(tri_n, 1), 0);
+    for (size_t sl = 0; sl < jobs.size(); ++sl) {
+        const int o = jobs[sl].pad, f = jframe[o];
+        const int stride = (off[f + 1] - off[f] + L - 1) / L;
+        tri[(size_t)tri_off[f] + (size_t)jblock[o] * stride + jm[o]] = (int)sl;
+        jobs[sl].pad = f;
+    }
+    std::vector<int> atom_frame(std::max(n, 1), 0);
+    for (int f = 0; f < nf; ++f)
+        for (int i = off[f]; i < off[f + 1]; ++i) atom_frame[i] = f;
+    const int nj = (int)jobs.size();
+    h->ba_ftab.ensure(ft.size());
+    h->ba_cb.ensure(ft.size());
+    h->ba_cg.ensure(ft.size());
+    h->ba_chunks.ensure(nch);
+    h->ba_invq.ensure(ldfa);
+    h->ba_ones.ensure(ldfa);
+    h->ba_invq32.ensure(ldfa);
+    h->ba_jobs.ensure(std::max(1, nj));
+    h->ba_tri.ensure(tri.size());
+    h->ba_tri_off.ensure(tri_off.size());
+    h->ba_atom_frame.ensure(atom_frame.size());
+    h->ba_frame_off.ensure(off.size());
+    h->ba_s2c.ensure(h->b_s2c.size());
+    h->ba_rowpart.ensure((size_t)std::max(1, nj) * nch * 64 * 3);
+    h->ba_colpart.ensure((size_t)std::max(1, nj) * nch * L * 3);
+    h->ba_grad.ensure((size_t)std::max(1, n) * 3);
+    hipchk(hipMemcpy(h->ba_ftab.p, ft.data(), ft.size() * sizeof(float), hipMemcpyHostToDevice), "upload ftab");
+    hipchk(hipMemcpy(h->ba_chunks.p, cc.data(), nch * sizeof(ChunkConst), hipMemcpyHostToDevice), "upload chunks");
+    hipchk(hipMemcpy(h->ba_invq.p, invq.data(), ldfa * sizeof(double), hipMemcpyHostToDevice), "upload invq");
+    hipchk(hipMemcpy(h->ba_ones.p, ones.data(), ldfa * sizeof(double), hipMemcpyHostToDevice), "upload ones");
+    hipchk(hipMemcpy(h->ba_invq32.p, invq32.data(), ldfa * sizeof(float), hipMemcpyHostToDevice), "upload invq32");
+    if (nj > 0)
+        hipchk(hipMemcpy(h->ba_jobs.p, jobs.data(), nj * sizeof(PairJob), hipMemcpyHostToDevice), "upload jobs");
+    hipchk(hipMemcpy(h->ba_tri.p, tri.data(), tri.size() * sizeof(int), hipMemcpyHostToDevice), "upload tri");
+    hipchk(hipMemcpy(h->ba_tri_off.p, tri_off.data(), tri_off.size() * sizeof(int), hipMemcpyHostToDevice),
+           "upload tri offsets");
+    hipchk(hipMemcpy(h->ba_atom_frame.p, atom_frame.data(), atom_frame.size() * sizeof(int), hipMemcpyHostToDevice),
+           "upload atom frames");
+    hipchk(hipMemcpy(h->ba_frame_off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice),
+           "upload frame offsets");
+    hipchk(hipMemcpy(h->ba_s2c.p, h->b_s2c.data(), h->b_s2c.size() * sizeof(int), hipMemcpyHostToDevice),
+           "upload batch slot map");
+    // class tables of the kernel's per-frame-weight form: 2 F_c,k and 2 F_c,k / q_k
+    hipchk(rsx::launch_adjoint_coef(h->ba_ftab.p, h->ba_ones.p, h->ba_invq.p, ncl, ldfa, h->ba_cb.p, h->ba_cg.p,
+                                    h->stream),
+           "adjoint class tables");
+    h->ba_K = K;
+    h->ba_nch = nch;
+    h->ba_ldf = ldfa;
+    h->ba_L = L;
+    h->ba_njobs = nj;
+    h->ba_n = n;
+    h->ba_gen = h->b_gen;
+    h->ba_factor = h->b_factor_stamp;
+    h->ba_have_w = false;
+}
+
+// the batched adjoint on the frame weights in ba_w: grad into ba_grad (caller order)
+void launch_batch_adjoint_pass(rsx_engine* h)
+{
+    rsx::AdjointArgs a;
+    a.jobs = h->ba_jobs.p;
+    a.n_chunks = h->ba_nch;
+    a.ftab = h->ba_ftab.p;
+    a.ldf = h->ba_ldf;
+    a.chunks = h->ba_chunks.p;
+    a.half_dq = 0.5 * h->dq;
+    a.wq = h->ba_ones.p;
+    a.invq = h->ba_invq.p;
+    a.invq32 = h->ba_invq32.p;
+    a.cb = h->ba_cb.p;
+    a.cg = h->ba_cg.p;
+    a.dq_rev = h->dq / (2.0 * PI);
+    a.fp32 = 1;
+    a.sin_terms = rsx::sin_series_terms(h->dq * h->b_radius);
+    a.col_range = h->ba_L;
+    a.row_partial = h->ba_rowpart.p;
+    a.col_partial = h->ba_colpart.p;
+    a.wframe = h->ba_w.p;
+    hipchk(rsx::launch_adjoint(a, h->ba_njobs, h->ba_K, h->stream), "batch adjoint kernel");
+    rsx::AdjointBatchReduceArgs r;
+    r.row_partial = h->ba_rowpart.p;
+    r.col_partial = h->ba_colpart.p;
+    r.tri_slot = h->ba_tri.p;
+    r.tri_off = h->ba_tri_off.p;
+    r.atom_frame = h->ba_atom_frame.p;
+    r.frame_off = h->ba_frame_off.p;
+    r.n = h->ba_n;
+    r.n_chunks = h->ba_nch;
+    r.col_range = h->ba_L;
+    r.slot_to_caller = h->ba_s2c.p;
+    r.grad = h->ba_grad.p;
+    hipchk(rsx::launch_adjoint_batch_reduce(r, h->stream), "batch adjoint reduce");
+}
+
+void batch_vjp_impl(rsx_engine* h, const double* w, int n_frames, int n_q, float* out_grad)
+{
+    if (!w || !out_grad) fail(RSX_INVALID_ARG, "batch vjp: null argument");
+    batch_prepare(h);
+    if (n_frames != h->b_frames || n_q != h->nq) fail(RSX_INVALID_ARG, "batch vjp: weights must be [frames][n_q]");
+    if (!h->b_uniform) fail(RSX_NOT_SUPPORTED, "batch vjp needs a uniform q grid the recurrence covers");
+    batch_adjoint_layout(h);
+    const int ldfa = h->ba_ldf;
+    std::vector<float> wf((size_t)std::max(1, n_frames) * ldfa, 0.0f);
+    for (int f = 0; f < n_frames; ++f)
+        for (int k = 0; k < n_q; ++k)
+            if (h->q[k] != 0.0f) wf[(size_t)f * ldfa + k] = (float)w[(size_t)f * n_q + k];
+    h->ba_w.ensure(wf.size());
+    hipchk(hipMemcpyAsync(h->ba_w.p, wf.data(), wf.size() * sizeof(float), hipMemcpyHostToDevice, h->stream),
+           "upload frame weights");
+    launch_batch_adjoint_pass(h);
+    h->ba_have_w = true;
+    if (h->ba_n > 0)
+        hipchk(hipMemcpyAsync(out_grad, h->ba_grad.p, (size_t)h->ba_n * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                              h->stream),
+               "download grad");
+    hipchk(hipStreamSynchronize(h->stream), "sync batch vjp");  // wf and out_grad are the caller's memory
 }
 
 // ---- adjoint -----------------------------------------------------------------
@@ -2572,14 +2780,7 @@ void adjoint_layout(rsx_engine* h)
     const int nch = (h->nq + K - 1) / K;
     const int ldfa = nch * K;
     std::vector<ChunkConst> cc(nch);
-    for (int c = 0; c < nch; ++c) {
-        const int k0 = c * K;
-        const double a = k0 < h->nq ? (double)h->q[k0] : 0.0;
-        cc[c].a = (float)a;
-        cc[c].a_rev = a / (2.0 * PI);
-        cc[c].mid_rev = (a + 0.5 * h->dq) / (2.0 * PI);
-        cc[c].pad = 0.0f;
-    }
+    for (int c = 0; c < nch; ++c) cc[c] = chunk_const(h, c * K);
     std::vector<float> ft((size_t)std::max(1, h->n_class) * ldfa, 0.0f);
     for (int c = 0; c < h->n_class; ++c)
         for (int k = 0; k < h->nq; ++k) ft[(size_t)c * ldfa + k] = (float)h->f64[(size_t)c * h->nq + k];
@@ -3102,19 +3303,24 @@ int rsx_destroy(rsx_engine* h)
                           &h->d_slab};
     for (auto* b : f4) b->release();
     h->d_qrev.release();
-    DBuf<float>* ff[] = {&h->d_ftab, &h->d_qv, &h->d_r2, &h->d_rext, &h->d_grad, &h->a_ftab, &h->a_rowpart,
-                         &h->a_colpart, &h->a_invq32, &h->a_cb, &h->a_cg, &h->b_qv, &h->b_ftab, &h->d_qsplit, &h->b_qsplit};
+    DBuf<float>* ff[] = {&h->d_ftab,    &h->d_qv,       &h->d_r2,      &h->d_rext,    &h->d_grad,    &h->a_ftab,
+                         &h->a_rowpart, &h->a_colpart,  &h->a_invq32,  &h->a_cb,      &h->a_cg,      &h->b_qv,
+                         &h->b_ftab,    &h->d_qsplit,   &h->b_qsplit,  &h->ba_ftab,   &h->ba_cb,     &h->ba_cg,
+                         &h->ba_invq32, &h->ba_w,       &h->ba_rowpart, &h->ba_colpart, &h->ba_grad};
     for (auto* b : ff) b->release();
     DBuf<double>* fd[] = {&h->d_invq, &h->d_partials, &h->d_iq, &h->d_saved_iq, &h->d_out, &h->d_wq, &h->d_q0,
                           &h->d_red, &h->b_partials, &h->b_out, &h->b_mean, &h->b_q0, &h->b_red, &h->b_invq,
-                          &h->b_qrev, &h->d_partials_new};
+                          &h->b_qrev, &h->d_partials_new, &h->ba_invq, &h->ba_ones};
     for (auto* b : fd) b->release();
     DBuf<int>* fi[] = {&h->d_slot_begin, &h->d_bucket_of, &h->d_bucket_count, &h->d_bucket_start, &h->d_bucket_fill,
                        &h->d_sorted, &h->d_s2c, &h->b_slot_begin, &h->d_local_list, &h->d_local, &h->d_tile_sum,
-                       &h->d_slab_count, &h->d_slab_pos};
+                       &h->d_slab_count, &h->d_slab_pos, &h->ba_tri, &h->ba_tri_off, &h->ba_atom_frame,
+                       &h->ba_frame_off, &h->ba_s2c};
     for (auto* b : fi) b->release();
     h->d_chunks.release();
     h->b_chunks.release();
+    h->ba_chunks.release();
+    h->ba_jobs.release();
     h->b_pin_ftab.release();
     h->b_pin_q0.release();
     for (rsx_engine::Staging* st : {&h->st_old, &h->st_new}) {
@@ -3491,6 +3697,29 @@ int rsx_batch_calc(rsx_engine* h, double* out_frames, double* out_mean)
             hipchk(hipMemcpyAsync(out_mean, h->b_mean.p, h->nq * sizeof(double), hipMemcpyDeviceToHost, h->stream),
                    "download mean");
         hipchk(hipStreamSynchronize(h->stream), "sync batch");
+    });
+}
+
+int rsx_batch_vjp(rsx_engine* h, const double* w, int n_frames, int n_q, float* out_grad)
+{
+    return guarded(h, [&] { batch_vjp_impl(h, w, n_frames, n_q, out_grad); });
+}
+
+int rsx_time_batch_vjp(rsx_engine* h, int reps, float* ms)
+{
+    return guarded(h, [&] {
+        if (!h->ba_have_w || h->ba_gen != h->b_gen || h->ba_factor != h->b_factor_stamp)
+            fail(RSX_NO_INITIAL_CALC, "time_batch_vjp needs a prior batch_vjp call on the current frames");
+        if (reps <= 0 || !ms) fail(RSX_INVALID_ARG, "reps must be > 0");
+        set_device(h);
+        launch_batch_adjoint_pass(h);  // warm-up
+        hipchk(hipEventRecord(h->ev0, h->stream), "event");
+        for (int r = 0; r < reps; ++r) launch_batch_adjoint_pass(h);
+        hipchk(hipEventRecord(h->ev1, h->stream), "event");
+        hipchk(hipEventSynchronize(h->ev1), "event sync");
+        float t = 0.0f;
+        hipchk(hipEventElapsedTime(&t, h->ev0, h->ev1), "elapsed");
+        *ms = t / reps;
     });
 }
 

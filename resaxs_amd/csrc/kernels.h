@@ -254,6 +254,10 @@ struct AdjointArgs {
     int col_range;           // columns per job (L)
     float* row_partial;      // [n_jobs * n_chunks][64][3] FP32 row-side partial gradients
     float* col_partial;      // [n_jobs * n_chunks][col_range][3] FP32 column-side partials
+    // Batched VJP (FP32 kernel only): per-frame weights [n_frames][ldf], the job's frame in its
+    // pad; cb and cg then hold 2 F_c,k and 2 F_c,k / q_k, and the kernel scales them by the
+    // job's frame weights.  Null: one weight vector, folded into cb and cg.
+    const float* wframe = nullptr;
 };
 // one wave per (job, q chunk); n_jobs = number of jobs
 hipError_t launch_adjoint(const AdjointArgs& a, int n_jobs, int K, hipStream_t s);
@@ -276,6 +280,25 @@ struct AdjointReduceArgs {
 };
 // grad of every atom: its row and column partials over all chunks, fixed order
 hipError_t launch_adjoint_reduce(const AdjointReduceArgs& a, hipStream_t s);
+
+// The same over a batch of frames, each with its own triangle of jobs: atom s (batch column
+// order) belongs to frame atom_frame[s], whose atoms are [frame_off[f], frame_off[f + 1])
+// and whose (row block, column range) -> slot table starts at tri_slot + tri_off[f] with
+// stride ceil(atoms / col_range).
+struct AdjointBatchReduceArgs {
+    const float* row_partial;
+    const float* col_partial;
+    const int* tri_slot;
+    const int* tri_off;      // [n_frames]
+    const int* atom_frame;   // [n]
+    const int* frame_off;    // [n_frames + 1]
+    int n;                   // atoms of all frames
+    int n_chunks;
+    int col_range;
+    const int* slot_to_caller;
+    float* grad;             // [n][3] in caller order
+};
+hipError_t launch_adjoint_batch_reduce(const AdjointBatchReduceArgs& a, hipStream_t s);
 
 // Adjoint weights on the device.  wq[0 .. ldfa): wq[k] = w[k] for k < n_q except the q = 0
 // entry (q0_index >= 0), 0 there and in the padding.

@@ -1752,14 +1752,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 67 ? 3 
                 cfloat* cbp = (cfloat*)(a.cb + off);
                 cfloat* cgp = (cfloat*)(a.cg + off);
                 float prev = 0.0f;
+                if (a.wframe) {  // uniform: the batch's per-frame weights scale the class tables
+                    cfloat* wfp = (cfloat*)(a.wframe + (size_t)__builtin_amdgcn_readfirstlane(jb.pad) * ldf + k0);
 #pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    if (k % 8 == 0) asm volatile("" ::: "memory");
-                    const float f = fi[k];
-                    const float bk = f * cbp[k];
-                    beta[k] = bk + prev;
-                    gamma[k] = f * cgp[k];
-                    prev = bk;
+                    for (int k = 0; k < K; ++k) {
+                        if (k % 8 == 0) asm volatile("" ::: "memory");
+                        const float f = fi[k] * wfp[k];
+                        const float bk = f * cbp[k];
+                        beta[k] = bk + prev;
+                        gamma[k] = f * cgp[k];
+                        prev = bk;
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        if (k % 8 == 0) asm volatile("" ::: "memory");
+                        const float f = fi[k];
+                        const float bk = f * cbp[k];
+                        beta[k] = bk + prev;
+                        gamma[k] = f * cgp[k];
+                        prev = bk;
+                    }
                 }
                 beta[K] = prev;
             }
@@ -1858,6 +1871,52 @@ __global__ __launch_bounds__(256) void adjoint_reduce_kernel(AdjointReduceArgs a
         const int m = rel / a.col_range;
         const int off = rel - m * a.col_range;
         const int slot = a.tri_slot[J * a.tri_stride + m];
+        const float* v = a.col_partial + (((size_t)slot * a.n_chunks + c) * a.col_range + off) * 3;
+        g0 += v[0];
+        g1 += v[1];
+        g2 += v[2];
+    }
+    g0 = wave_sum(g0);
+    g1 = wave_sum(g1);
+    g2 = wave_sum(g2);
+    if (lane == 0) {
+        const int o = a.slot_to_caller[s];
+        a.grad[3 * o] = (float)g0;
+        a.grad[3 * o + 1] = (float)g1;
+        a.grad[3 * o + 2] = (float)g2;
+    }
+}
+
+// adjoint_reduce_kernel over a batch of frames (AdjointBatchReduceArgs)
+__global__ __launch_bounds__(256) void adjoint_batch_reduce_kernel(AdjointBatchReduceArgs a)
+{
+    const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (s >= a.n) return;  // whole waves
+    const int f = a.atom_frame[s];
+    const int base = a.frame_off[f], m = a.frame_off[f + 1] - base;
+    const int ls = s - base;
+    const int stride = (m + a.col_range - 1) / a.col_range;
+    const int* __restrict__ tri = a.tri_slot + a.tri_off[f];
+    const int I = ls >> 6, ln = ls & 63;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    const int nm = (m - 64 * I + a.col_range - 1) / a.col_range;
+    const int n_row_items = nm * a.n_chunks;
+    for (int t = lane; t < n_row_items; t += 64) {
+        const int mm = t / a.n_chunks, c = t - mm * a.n_chunks;
+        const int slot = tri[I * stride + mm];
+        const float* v = a.row_partial + (((size_t)slot * a.n_chunks + c) * 64 + ln) * 3;
+        g0 += v[0];
+        g1 += v[1];
+        g2 += v[2];
+    }
+    const int n_col_items = (I + 1) * a.n_chunks;
+    for (int t = lane; t < n_col_items; t += 64) {
+        const int J = t / a.n_chunks, c = t - J * a.n_chunks;
+        const int rel = ls - 64 * J;
+        const int mm = rel / a.col_range;
+        const int off = rel - mm * a.col_range;
+        const int slot = tri[J * stride + mm];
         const float* v = a.col_partial + (((size_t)slot * a.n_chunks + c) * a.col_range + off) * 3;
         g0 += v[0];
         g1 += v[1];
@@ -2546,6 +2605,7 @@ hipError_t launch_adjoint_coef(const float* ftab, const double* wq, const double
 hipError_t launch_adjoint(const AdjointArgs& a, int n_jobs, int K, hipStream_t s)
 {
     if (n_jobs <= 0) return hipSuccess;
+    if (a.wframe && !a.fp32) return hipErrorInvalidValue;  // per-frame weights: FP32 kernel only
     if (a.fp32) {
         switch (K) {
         case 16: launch_adjoint32_k<16>(a, n_jobs, s); break;
@@ -2574,6 +2634,13 @@ hipError_t launch_adjoint_reduce(const AdjointReduceArgs& a, hipStream_t s)
 {
     if (a.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)((a.n + 3) / 4)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_batch_reduce(const AdjointBatchReduceArgs& a, hipStream_t s)
+{
+    if (a.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(adjoint_batch_reduce_kernel, dim3((unsigned)((a.n + 3) / 4)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -206,6 +206,7 @@ class Engine:
         self._check(self._L.rsx_batch_set_frames(self.h, _ptr(b, ctypes.c_float), _ptr(off, ctypes.c_int),
                                                  len(off) - 1))
         self.n_frames = len(off) - 1
+        self.n_batch_atoms = int(off[-1])
 
     def set_persistent(self, on):
         """Batched launch: persistent work-queue grid (True, default) or one wave per item."""
@@ -216,6 +217,20 @@ class Engine:
         mean = np.zeros(self.nq, dtype=np.float64)
         self._check(self._L.rsx_batch_calc(self.h, _ptr(frames, ctypes.c_double), _ptr(mean, ctypes.c_double)))
         return frames, mean
+
+    def batch_vjp(self, w):
+        """Gradients of every frame: grad [atoms of all frames][3] = sum_k w[f][k] dI_f(q_k)/dx_i
+        for weights w [n_frames][n_q] (rsx_batch_vjp)."""
+        w = np.ascontiguousarray(w, dtype=np.float64).reshape(self.n_frames, self.nq)
+        grad = np.zeros((self.n_batch_atoms, 3), dtype=np.float32)
+        self._check(self._L.rsx_batch_vjp(self.h, _ptr(w, ctypes.c_double), self.n_frames, self.nq,
+                                          _ptr(grad, ctypes.c_float)))
+        return grad
+
+    def time_batch_vjp(self, reps=3):
+        ms = ctypes.c_float()
+        self._check(self._L.rsx_time_batch_vjp(self.h, int(reps), ctypes.byref(ms)))
+        return ms.value
 
     # -- differentiable fit ---------------------------------------------------
     def chi2_grad(self, bodies, i_exp, sigma):

@@ -357,3 +357,54 @@ def test_vjp_two_atoms_closed_form():
     coef = np.sum(wv * 2 * table[:, 0] * table[:, 1] * t) / r ** 2
     assert np.allclose(g[0], coef * d, rtol=1e-4, atol=1e-6 * abs(coef) * r)
     assert np.allclose(g[1], -coef * d, rtol=1e-4, atol=1e-6 * abs(coef) * r)
+
+
+@pytest.mark.parametrize("hydrate", [False, True])
+def test_batch_vjp_matches_single_structure(oracle, hydrate):
+    """rsx_batch_vjp over ragged frames (700, 0, 2, 1500, 65 and 1 atoms) with weights of their
+    own per frame: every frame's gradient equals the single-structure VJP of that frame to
+    FP32 rounding (both FP32 adjoints; the weights enter rounded differently), sums to zero
+    over the frame (translation invariance), and the two-atom frame follows the closed form
+    2 F1 F2 [cos(qr) - sinc(qr)] (x1 - x2) / r^2."""
+    from resaxs_amd import configs
+    rng = np.random.default_rng(23)
+    q = np.linspace(0.01, 0.5, 64).astype(np.float32)
+    table = configs.factor_table(rng, q)
+    sizes = [700, 0, 2, 1500, 65, 1]
+    pos = [configs.in_ellipsoid(rng, m, configs.ellipsoid_axes(m)).astype(np.float32) if m > 2
+           else configs.in_ball(rng, m, 3.0).astype(np.float32) for m in sizes]
+    typ = [configs.seeded_types(rng, m) for m in sizes]
+    bodies = np.concatenate([configs.bodies_of(p, t) for p, t in zip(pos, typ)]).astype(np.float32)
+    offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    w = rng.normal(size=(len(sizes), len(q)))
+    kw = dict(radii=configs.RADII, probe=configs.PROBE, n_half=32) if hydrate else {}
+    ww = 0.6 if hydrate else 0.0
+    _say(f"case: batch VJP vs single-structure VJP, frames {sizes} x {len(q)} q, hydrate={hydrate}")
+    e = engine_for(q, table, configs.WATER, ww, **kw)
+    e.batch_set_frames(bodies, offsets)
+    g = e.batch_vjp(w)
+    assert g.shape == (int(offsets[-1]), 3)
+    assert e.time_batch_vjp(2) > 0.0
+    e.close()
+    for f, m in enumerate(sizes):
+        if m == 0:
+            continue
+        gf = g[offsets[f]:offsets[f + 1]].astype(np.float64)
+        s = engine_for(q, table, configs.WATER, ww, n=m, **kw)
+        ref = s.vjp(configs.bodies_of(pos[f], typ[f]), w[f]).astype(np.float64)
+        s.close()
+        scale = max(np.linalg.norm(ref), 1e-30)
+        d = np.linalg.norm(gf - ref) / scale
+        _say(f"  frame {f} ({m} atoms): |g - g_single| / |g_single| = {d:.2e}")
+        assert d < 1e-5 or (m == 1 and np.all(gf == 0.0)), (f, d)
+        assert np.linalg.norm(gf.sum(0)) <= 1e-5 * scale * np.sqrt(m) + 1e-30, f
+    if not hydrate:
+        p2, t2 = pos[2], typ[2]
+        d = p2[0].astype(np.float64) - p2[1]
+        r = np.linalg.norm(d)
+        x = q.astype(np.float64) * r
+        wk = np.where(q > 0, w[2], 0.0)
+        coef = np.sum(wk * 2 * table[:, t2[0]] * table[:, t2[1]] * (np.cos(x) - np.sin(x) / x)) / r ** 2
+        g2 = g[offsets[2]:offsets[3]].astype(np.float64)
+        assert np.allclose(g2[0], coef * d, rtol=1e-4, atol=1e-6 * abs(coef) * r)
+        assert np.allclose(g2[1], -coef * d, rtol=1e-4, atol=1e-6 * abs(coef) * r)
