@@ -352,9 +352,10 @@ def incremental_bench(hydrate, n_moves=2000, rebaseline=REBASELINE):
                       + ("; SASA recounted for the atoms within reach of the moved segment" if hydrate else "")}
 
 
-def batched_bench(reps=5):
+def batched_bench(reps=5, adjoint=True):
     """The batched ensemble: 512 frames of 1,000-15,000 atoms x 128 q, one graph-captured
-    launch per replay; per-frame I(q) and the uniform mean to device buffers."""
+    launch per replay; per-frame I(q) and the uniform mean to device buffers.  With adjoint,
+    also the batched VJP (gradients of every frame from per-frame weights dL/dI_f(q))."""
     from resaxs_amd import Engine, configs
     w = configs.ensemble_512()
     nq = len(w.q)
@@ -378,10 +379,24 @@ def batched_bench(reps=5):
     launch = e.batch_launch()
     e.dev_free(fr)
     e.dev_free(mn)
+    adj = None
+    if adjoint:
+        # the batch's gradients from per-frame weights dL/dI_f(q): one adjoint launch + one reduce
+        wts = np.random.default_rng(7).normal(size=(len(w.sizes), nq))
+        t0 = time.perf_counter()
+        e.batch_vjp(wts)
+        vjp_first_s = time.perf_counter() - t0
+        ms_adj = e.time_batch_vjp(3)
+        adj = {"ms_adjoint": round(ms_adj, 3), "ratio_fwd_adjoint_to_fwd": round((ms + ms_adj) / ms, 3),
+               "first_call_s": round(vjp_first_s, 3),
+               "timing": "HIP events on the engine stream around the adjoint kernel + per-atom reduce "
+                         "(weights resident); first_call_s is the host wall clock of the first rsx_batch_vjp "
+                         "(job layout, uploads, gradient download)"}
     e.close()
     terms = float(sum(s * (s - 1) / 2.0 for s in w.sizes)) * nq
     flops = float(sum(configs.forward_flops(int(s), nq) for s in w.sizes))
     return {"workload": f"ensemble-{len(w.sizes)}-frames-1000..15000x{nq}", "ms_per_batch": round(ms, 3),
+            "adjoint": adj,
             "terms_per_s": terms / (ms * 1e-3), "achieved_tflops": round(flops / (ms * 1e-3) / 1e12, 2),
             "frac": round(flops / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4), "layout_s": round(layout_s, 3),
             "kernel": f"variant {tuning['variant']} K={tuning['k']} L={tuning['col_range']} " + (

@@ -228,8 +228,11 @@ int rsx_debye_vjp(rsx_engine *h, const float *bodies, int n, const double *w, fl
 int rsx_debye_vjp_dev_async(rsx_engine *h, const void *bodies_dev, int n, const double *w_dev, float *grad_dev);
 int rsx_chi2_grad_dev_async(rsx_engine *h, const void *bodies_dev, int n, const float *i_exp_dev,
                             const float *sigma_dev, double *out_dev, float *grad_dev, double *iq_dev);
-/* adjoint kernel: fp32 1 = FP32 difference-form recurrence (default), 0 = FP64 recurrence;
- * k q values per thread (0 = automatic); col_range columns per job (0 = 256) */
+/* adjoint kernel: fp32 1 = FP32 difference-form recurrence (default; its class-pair coefficient
+ * form on plain layouts of at most 64 classes, K 64, 100, 128 or 200, the per-lane form otherwise),
+ * 2 = the FP32 per-lane form always, 0 = FP64 recurrence; k q values per thread (0 = automatic);
+ * col_range columns per job, a multiple of 64 (0 = automatic: 64 or 128).  Applies to
+ * rsx_batch_vjp too. */
 int rsx_set_adjoint_tuning(rsx_engine *h, int fp32, int k, int col_range);
 
 /* ---- tuning and measurement ---- */
@@ -263,6 +266,14 @@ int rsx_get_tuning(rsx_engine *h, int *k, int *col_range, int *recurrence, int *
  * the work-queue kernel runs with that many resident waves, 0 for one wave per (job, q chunk)
  * item; n_jobs = pair jobs over all frames.  RSX_NO_INITIAL_CALC before the first batch launch. */
 int rsx_batch_launch(rsx_engine *h, int *persistent_waves, int *n_jobs);
+/* the adjoint layout of the last gradient call (batch 0: rsx_chi2_grad / rsx_debye_vjp and their
+ * device forms; batch 1: rsx_batch_vjp), no reference counterpart: form 0 = FP64 recurrence,
+ * 1 = FP32 class-pair coefficient form, 2 = FP32 per-lane form; k q per thread, col_range columns
+ * per job (any pointer may be NULL).  RSX_NO_INITIAL_CALC before such a call. */
+#define RSX_ADJOINT_FP64      0
+#define RSX_ADJOINT_PAIR      1
+#define RSX_ADJOINT_PER_LANE  2
+int rsx_get_adjoint_layout(rsx_engine *h, int batch, int *form, int *k, int *col_range);
 /* average device time (ms) of the last timed region, measured with HIP events on the engine stream */
 int rsx_time_forward(rsx_engine *h, int reps, float *ms_total, float *ms_pair_kernel);
 /* average device time (ms) of the adjoint pass (kernel + fixed-order reduction) on the weights of

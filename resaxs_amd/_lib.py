@@ -92,6 +92,7 @@ SIGNATURES = {
     "rsx_time_forward": (_i, [_vp, _i, _fp, _fp]),
     "rsx_set_adjoint_tuning": (_i, [_vp, _i, _i, _i]),
     "rsx_time_adjoint": (_i, [_vp, _i, _fp]),
+    "rsx_get_adjoint_layout": (_i, [_vp, _i, _ip, _ip, _ip]),
     "rsx_time_copy": (_i, [_vp, _ll, _i, _fp]),
     "rsx_time_sasa": (_i, [_vp, _i, _fp]),
     "rsx_dev_alloc": (_i, [_vp, _c.POINTER(_vp), _ll]),

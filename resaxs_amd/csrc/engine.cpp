@@ -380,8 +380,11 @@ struct rsx_engine {
     // adjoint layout (adjoint_layout), rebuilt when layout_stamp changes
     DBuf<float> a_ftab, a_rowpart, a_colpart, a_invq32, a_cb, a_cg;
     bool adj_fp32 = true;  // adjoint kernel precision (rsx_set_adjoint_tuning)
+    bool adj_pair_ok = true;  // FP32: class-pair coefficient form where the layout allows it
     int adj_k_req = 0, adj_range_req = 0;
     bool a_fp32 = false;   // precision the cached adjoint layout was built for
+    bool a_pair = false;   // the cached layout runs the class-pair form (a_coef)
+    DBuf<float> a_coef;
     DBuf<ChunkConst> a_chunks;
     DBuf<double> a_invq;
     DBuf<PairJob> a_jobs;
@@ -496,7 +499,8 @@ struct rsx_engine {
     // per-frame triangle tables, the class tables 2 F_c,k and 2 F_c,k / q_k, partials
     uint64_t ba_gen = 0, ba_factor = 0;
     int ba_K = 0, ba_nch = 0, ba_ldf = 0, ba_L = 0, ba_njobs = 0, ba_n = 0;
-    DBuf<float> ba_ftab, ba_cb, ba_cg, ba_invq32, ba_w, ba_rowpart, ba_colpart, ba_grad;
+    bool ba_pair = false;
+    DBuf<float> ba_ftab, ba_cb, ba_cg, ba_invq32, ba_w, ba_rowpart, ba_colpart, ba_grad, ba_coef;
     DBuf<double> ba_invq, ba_ones;
     DBuf<ChunkConst> ba_chunks;
     DBuf<PairJob> ba_jobs;
@@ -2575,25 +2579,75 @@ void batch_run_dev(rsx_engine* h, double* out_frames, double* out_mean)
 // reduction.  The layout (the adjoint's q chunking and column range, the jobs, the per-frame
 // triangle tables, the class tables 2 F_c,k and 2 F_c,k / q_k) is kept until the batch layout
 // or its class factors change.
-void batch_adjoint_layout(rsx_engine* h)
-{
-    if (h->ba_gen == h->b_gen && h->ba_factor == h->b_factor_stamp && h->ba_njobs >= 0 && h->ba_K > 0) return;
-    const int nf = h->b_frames, nq = h->nq;
-    const std::vector<int>& off = h->b_offsets;
-    const int n = off[nf];
-    // q per thread: the FP32 adjoint's cost model (adjoint_layout)
+// Class-pair coefficient form of the FP32 adjoint: at most this many classes, and its table
+// (groups x classes^2 x chunks x stride floats) at most this many floats; the per-lane form
+// otherwise
+constexpr int ADJ_PAIR_MAX_CLASSES = 64;
+constexpr size_t ADJ_PAIR_MAX_FLOATS = (size_t)1 << 26;
+
+struct AdjChoice {
     int K = 0;
+    bool pair = false;
+};
+
+// q per thread of the FP32 adjoint and its form: the class-pair form where the layout is plain
+// (one class per type, d1 == 1), the classes few, the table small and a requested K one of its
+// own; the per-lane form otherwise.  Cost per (pair, chunk) in FP32-op units: class-pair
+// 2 K + 40 (its coefficients come through scalar loads, so K = 128 keeps 6 waves per SIMD),
+// per-lane 2.4 K + 30 (tables in vector registers: 3 waves per SIMD up to K = 67 and 2 at
+// K = 100, where the per-pair seeds spread over more q make up for it: 1.08 vs 1.09-1.12 ms
+// at 10k x 200).
+AdjChoice choose_adjoint32_k(const rsx_engine* h, int nq, int d1, int n_class, size_t groups)
+{
+    AdjChoice ch;
     double best = 1e300;
+    if (h->adj_pair_ok && d1 == 1 && n_class >= 1 && n_class <= ADJ_PAIR_MAX_CLASSES) {
+        for (int i = 0; i < rsx::adjoint_pair_k_count(); ++i) {
+            const int k = rsx::adjoint_pair_k_value(i);
+            if (h->adj_k_req > 0 && k != h->adj_k_req) continue;
+            const int c = (nq + k - 1) / k;
+            const size_t floats = groups * n_class * n_class * (size_t)c * rsx::adjoint_pair_stride(k);
+            if (floats > ADJ_PAIR_MAX_FLOATS) continue;
+            const double cost = c * (2.0 * k + 40.0);
+            if (cost < best) {
+                best = cost;
+                ch.K = k;
+                ch.pair = true;
+            }
+        }
+        if (ch.K > 0) return ch;
+    }
     for (int i = 0; i < rsx::adjoint_k_count(true); ++i) {
         const int k = rsx::adjoint_k_value(true, i);
         if (h->adj_k_req > 0 && k != h->adj_k_req) continue;
         const double cost = ((nq + k - 1) / k) * (2.4 * k + 30.0);
         if (cost < best) {
             best = cost;
-            K = k;
+            ch.K = k;
         }
     }
-    if (K == 0) fail(RSX_INVALID_ARG, "batch vjp: the requested adjoint K is not available");
+    if (ch.K == 0) fail(RSX_INVALID_ARG, "adjoint: the requested K is not available for this layout");
+    return ch;
+}
+
+// Columns per adjoint job: the requested range, else for the class-pair form 64 when the
+// launch is small (under 2^20 waves at 128 columns: the finer jobs balance the SIMDs, 0.98 vs
+// 1.08 ms at 10k x 200) and 128 otherwise (the same speed, half the row partials: 512 frames of
+// 1k-15k atoms x 128 q); 128 for the per-lane form.  waves128 = (job, chunk) items at 128.
+int adjoint_col_range(const rsx_engine* h, bool pair, double waves128)
+{
+    if (h->adj_range_req > 0) return h->adj_range_req;
+    return pair && waves128 < (double)(1 << 20) ? 64 : 128;
+}
+
+void batch_adjoint_layout(rsx_engine* h)
+{
+    if (h->ba_gen == h->b_gen && h->ba_factor == h->b_factor_stamp && h->ba_njobs >= 0 && h->ba_K > 0) return;
+    const int nf = h->b_frames, nq = h->nq;
+    const std::vector<int>& off = h->b_offsets;
+    const int n = off[nf];
+    const AdjChoice choice = choose_adjoint32_k(h, nq, h->b_d1, h->b_n_class, (size_t)std::max(1, nf));
+    const int K = choice.K;
     const int nch = (nq + K - 1) / K, ldfa = nch * K;
     std::vector<ChunkConst> cc(nch);
     for (int c = 0; c < nch; ++c) cc[c] = chunk_const(h, c * K);
@@ -2609,7 +2663,12 @@ void batch_adjoint_layout(rsx_engine* h)
         ones[k] = 1.0;
     }
     // every frame's triangle (creation order: row block I, then its column ranges m)
-    const int L = h->adj_range_req > 0 ? h->adj_range_req : 128;
+    double waves128 = 0.0;
+    for (int f = 0; f < nf; ++f) {
+        const double m = off[f + 1] - off[f];
+        waves128 += std::ceil(m / 64.0) * std::ceil(m / 128.0) * 0.5 * nch;
+    }
+    const int L = adjoint_col_range(h, choice.pair, waves128);
     std::vector<PairJob> jobs;
     std::vector<int> jframe, jblock, jm, tri_off(std::max(nf, 1), 0);
     size_t tri_n = 0;
@@ -2657,6 +2716,8 @@ void batch_adjoint_layout(rsx_engine* h)
     h->ba_rowpart.ensure((size_t)std::max(1, nj) * nch * 64 * 3);
     h->ba_colpart.ensure((size_t)std::max(1, nj) * nch * L * 3);
     h->ba_grad.ensure((size_t)std::max(1, n) * 3);
+    if (choice.pair)
+        h->ba_coef.ensure((size_t)std::max(1, nf) * h->b_n_class * h->b_n_class * nch * rsx::adjoint_pair_stride(K));
     hipchk(hipMemcpy(h->ba_ftab.p, ft.data(), ft.size() * sizeof(float), hipMemcpyHostToDevice), "upload ftab");
     hipchk(hipMemcpy(h->ba_chunks.p, cc.data(), nch * sizeof(ChunkConst), hipMemcpyHostToDevice), "upload chunks");
     hipchk(hipMemcpy(h->ba_invq.p, invq.data(), ldfa * sizeof(double), hipMemcpyHostToDevice), "upload invq");
@@ -2678,6 +2739,7 @@ void batch_adjoint_layout(rsx_engine* h)
                                     h->stream),
            "adjoint class tables");
     h->ba_K = K;
+    h->ba_pair = choice.pair;
     h->ba_nch = nch;
     h->ba_ldf = ldfa;
     h->ba_L = L;
@@ -2709,7 +2771,25 @@ void launch_batch_adjoint_pass(rsx_engine* h)
     a.col_range = h->ba_L;
     a.row_partial = h->ba_rowpart.p;
     a.col_partial = h->ba_colpart.p;
-    a.wframe = h->ba_w.p;
+    a.n_class = h->b_n_class;
+    if (h->ba_pair) {  // the frame weights go into the class-pair table
+        rsx::AdjointPairCoefArgs pc;
+        pc.ftab = h->ba_ftab.p;
+        pc.cb = h->ba_cb.p;
+        pc.cg = h->ba_cg.p;
+        pc.wframe = h->ba_w.p;
+        pc.groups = std::max(1, h->b_frames);
+        pc.n_class = h->b_n_class;
+        pc.n_chunks = h->ba_nch;
+        pc.K = h->ba_K;
+        pc.ldf = h->ba_ldf;
+        pc.coef = h->ba_coef.p;
+        hipchk(rsx::launch_adjoint_pair_coef(pc, h->stream), "batch adjoint class-pair table");
+        a.coef = h->ba_coef.p;
+        a.coef_grouped = 1;
+    } else {
+        a.wframe = h->ba_w.p;
+    }
     hipchk(rsx::launch_adjoint(a, h->ba_njobs, h->ba_K, h->stream), "batch adjoint kernel");
     rsx::AdjointBatchReduceArgs r;
     r.row_partial = h->ba_rowpart.p;
@@ -2760,22 +2840,26 @@ void adjoint_layout(rsx_engine* h)
     const uint64_t stamp = h->layout_stamp;
     const bool fp32 = h->adj_fp32;
     if (h->a_stamp == stamp && h->a_n == n && h->a_cols == h->d_cols.p && h->a_fp32 == fp32) return;
-    // q per thread, in FP64-op units per (pair, chunk): FP64 kernel 4K ops (two
-    // recurrences, combine, accumulate) + ~60 for distance and seeds; FP32 kernel 4K
-    // FP32 ops (~0.6 of an FP64 op each) + ~30
+    // q per thread: FP32 by choose_adjoint32_k; FP64 in FP64-op units per (pair, chunk), 4K ops
+    // (two recurrences, combine, accumulate) + ~60 for distance and seeds
     int K = 0;
-    double best = 1e300;
-    for (int i = 0; i < rsx::adjoint_k_count(fp32); ++i) {
-        const int k = rsx::adjoint_k_value(fp32, i);
-        if (h->adj_k_req > 0 && k != h->adj_k_req) continue;
-        const int c = (h->nq + k - 1) / k;
-        // (the FP32 kernel keeps 3 waves per SIMD up to K = 67 and 2 at K = 100, where the
-        // per-pair seeds spread over more q make up for it: 1.08 vs 1.09-1.12 ms at 10k x 200)
-        const double cost = fp32 ? c * (2.4 * k + 30.0) : c * (4.0 * k + 60.0);
-        if (cost < best) {
-            best = cost;
-            K = k;
+    bool pair = false;
+    if (fp32) {
+        const AdjChoice choice = choose_adjoint32_k(h, h->nq, h->d1, h->n_class, 1);
+        K = choice.K;
+        pair = choice.pair;
+    } else {
+        double best = 1e300;
+        for (int i = 0; i < rsx::adjoint_k_count(false); ++i) {
+            const int k = rsx::adjoint_k_value(false, i);
+            if (h->adj_k_req > 0 && k != h->adj_k_req) continue;
+            const double cost = ((h->nq + k - 1) / k) * (4.0 * k + 60.0);
+            if (cost < best) {
+                best = cost;
+                K = k;
+            }
         }
+        if (K == 0) fail(RSX_INVALID_ARG, "adjoint: the requested K is not available for the FP64 kernel");
     }
     const int nch = (h->nq + K - 1) / K;
     const int ldfa = nch * K;
@@ -2791,7 +2875,8 @@ void adjoint_layout(rsx_engine* h)
         invq32[k] = (float)invq[k];
     }
     // triangle jobs over the slot-ordered atoms; remember each job's (row block, range)
-    const int L = h->adj_range_req > 0 ? h->adj_range_req : 128;  // measured: 128 <= 256 < 512 at 10k x 200
+    // (per-lane form measured: 128 <= 256 < 512 at 10k x 200)
+    const int L = adjoint_col_range(h, pair, std::ceil(n / 64.0) * std::ceil(n / 128.0) * 0.5 * nch);
     std::vector<PairJob> jobs;
     triangle_jobs(jobs, h->d_cols.p, n, L, 1.0f);
     for (size_t j = 0; j < jobs.size(); ++j) jobs[j].pad = (int)j;
@@ -2825,6 +2910,7 @@ void adjoint_layout(rsx_engine* h)
     h->a_rowpart.ensure((size_t)std::max(1, nj) * nch * 64 * 3);
     h->a_colpart.ensure((size_t)std::max(1, nj) * nch * L * 3);
     h->d_grad.ensure((size_t)n * 3);
+    if (pair) h->a_coef.ensure((size_t)h->n_class * h->n_class * nch * rsx::adjoint_pair_stride(K));
     hipchk(hipMemcpy(h->a_ftab.p, ft.data(), ft.size() * sizeof(float), hipMemcpyHostToDevice), "upload ftab");
     hipchk(hipMemcpy(h->a_chunks.p, cc.data(), nch * sizeof(ChunkConst), hipMemcpyHostToDevice), "upload chunks");
     hipchk(hipMemcpy(h->a_invq.p, invq.data(), ldfa * sizeof(double), hipMemcpyHostToDevice), "upload invq");
@@ -2841,6 +2927,7 @@ void adjoint_layout(rsx_engine* h)
     h->a_stamp = stamp;
     h->a_cols = h->d_cols.p;
     h->a_fp32 = fp32;
+    h->a_pair = pair;
 }
 
 // adjoint kernel + reduction on the weights already in d_wq; grad [n][3] (caller order) -> grad
@@ -2865,10 +2952,27 @@ void launch_adjoint_pass(rsx_engine* h, float* grad = nullptr)
     a.col_range = h->a_L;
     a.row_partial = h->a_rowpart.p;
     a.col_partial = h->a_colpart.p;
+    a.n_class = h->n_class;
     if (h->a_fp32)
         hipchk(rsx::launch_adjoint_coef(h->a_ftab.p, h->d_wq.p, h->a_invq.p, std::max(1, h->n_class), h->a_ldf, h->a_cb.p,
                                         h->a_cg.p, h->stream),
                "adjoint coefficients");
+    if (h->a_pair) {
+        rsx::AdjointPairCoefArgs pc;
+        pc.ftab = h->a_ftab.p;
+        pc.cb = h->a_cb.p;
+        pc.cg = h->a_cg.p;
+        pc.wframe = nullptr;
+        pc.groups = 1;
+        pc.n_class = h->n_class;
+        pc.n_chunks = h->a_nch;
+        pc.K = h->a_K;
+        pc.ldf = h->a_ldf;
+        pc.coef = h->a_coef.p;
+        hipchk(rsx::launch_adjoint_pair_coef(pc, h->stream), "adjoint class-pair table");
+        a.coef = h->a_coef.p;
+        a.coef_grouped = 0;
+    }
     hipchk(rsx::launch_adjoint(a, h->a_njobs, h->a_K, h->stream), "adjoint kernel");
     rsx::AdjointReduceArgs r;
     r.row_partial = h->a_rowpart.p;
@@ -3306,7 +3410,8 @@ int rsx_destroy(rsx_engine* h)
     DBuf<float>* ff[] = {&h->d_ftab,    &h->d_qv,       &h->d_r2,      &h->d_rext,    &h->d_grad,    &h->a_ftab,
                          &h->a_rowpart, &h->a_colpart,  &h->a_invq32,  &h->a_cb,      &h->a_cg,      &h->b_qv,
                          &h->b_ftab,    &h->d_qsplit,   &h->b_qsplit,  &h->ba_ftab,   &h->ba_cb,     &h->ba_cg,
-                         &h->ba_invq32, &h->ba_w,       &h->ba_rowpart, &h->ba_colpart, &h->ba_grad};
+                         &h->ba_invq32, &h->ba_w,       &h->ba_rowpart, &h->ba_colpart, &h->ba_grad,
+                         &h->a_coef,    &h->ba_coef};
     for (auto* b : ff) b->release();
     DBuf<double>* fd[] = {&h->d_invq, &h->d_partials, &h->d_iq, &h->d_saved_iq, &h->d_out, &h->d_wq, &h->d_q0,
                           &h->d_red, &h->b_partials, &h->b_out, &h->b_mean, &h->b_q0, &h->b_red, &h->b_invq,
@@ -3929,15 +4034,42 @@ int rsx_set_adjoint_tuning(rsx_engine* h, int fp32, int k, int col_range)
     return guarded(h, [&] {
         if (k < 0 || col_range < 0 || (col_range > 0 && col_range % 64 != 0))
             fail(RSX_INVALID_ARG, "adjoint tuning: k >= 0, col_range a multiple of 64");
+        if (fp32 < 0 || fp32 > 2) fail(RSX_INVALID_ARG, "adjoint tuning: fp32 is 0, 1 or 2");
         if (k > 0) {
             bool ok = false;
             for (int i = 0; i < rsx::adjoint_k_count(fp32 != 0); ++i) ok = ok || rsx::adjoint_k_value(fp32 != 0, i) == k;
+            if (fp32 == 1)
+                for (int i = 0; i < rsx::adjoint_pair_k_count(); ++i) ok = ok || rsx::adjoint_pair_k_value(i) == k;
             if (!ok) fail(RSX_INVALID_ARG, "unsupported adjoint K for this precision");
         }
         h->adj_fp32 = fp32 != 0;
+        h->adj_pair_ok = fp32 == 1;
         h->adj_k_req = k;
         h->adj_range_req = col_range;
         h->a_stamp = ~0ull;  // lay out again on the next gradient call
+        h->ba_K = 0;         // and on the next batch vjp
+        h->ba_have_w = false;
+    });
+}
+
+int rsx_get_adjoint_layout(rsx_engine* h, int batch, int* form, int* k, int* col_range)
+{
+    return guarded(h, [&] {
+        int f, kk, L;
+        if (batch) {
+            if (h->ba_K <= 0) fail(RSX_NO_INITIAL_CALC, "no batch vjp layout yet");
+            f = h->ba_pair ? RSX_ADJOINT_PAIR : RSX_ADJOINT_PER_LANE;
+            kk = h->ba_K;
+            L = h->ba_L;
+        } else {
+            if (h->a_stamp == ~0ull || h->a_K <= 0) fail(RSX_NO_INITIAL_CALC, "no adjoint layout yet");
+            f = !h->a_fp32 ? RSX_ADJOINT_FP64 : h->a_pair ? RSX_ADJOINT_PAIR : RSX_ADJOINT_PER_LANE;
+            kk = h->a_K;
+            L = h->a_L;
+        }
+        if (form) *form = f;
+        if (k) *k = kk;
+        if (col_range) *col_range = L;
     });
 }
 

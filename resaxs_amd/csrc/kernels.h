@@ -258,6 +258,15 @@ struct AdjointArgs {
     // pad; cb and cg then hold 2 F_c,k and 2 F_c,k / q_k, and the kernel scales them by the
     // job's frame weights.  Null: one weight vector, folded into cb and cg.
     const float* wframe = nullptr;
+    // Classes of the factor table: a row or column record whose class is not below n_class (a
+    // type outside the table, flagged by the gather) adds nothing and reads no table entry.
+    int n_class = 0;
+    // Class-pair coefficient form (FP32 kernel, plain layouts): beta and gamma per (group, row
+    // class, column class, q chunk) from launch_adjoint_pair_coef, read through scalar loads.
+    // A group is a frame of the batch (coef_grouped: the job's frame in its pad) or the one
+    // structure.  Null: the per-lane form above.
+    const float* coef = nullptr;
+    int coef_grouped = 0;
 };
 // one wave per (job, q chunk); n_jobs = number of jobs
 hipError_t launch_adjoint(const AdjointArgs& a, int n_jobs, int K, hipStream_t s);
@@ -266,6 +275,30 @@ hipError_t launch_adjoint_coef(const float* ftab, const double* wq, const double
                                float* cg, hipStream_t s);
 int adjoint_k_count(bool fp32);
 int adjoint_k_value(bool fp32, int i);
+// q per thread of the class-pair coefficient form
+int adjoint_pair_k_count();
+int adjoint_pair_k_value(int i);
+// Floats per (group, row class, column class, chunk) entry of the class-pair table, 16-float
+// aligned: beta_k and gamma_k interleaved (2 k and 2 k + 1 for k < K), beta_K at 2 K -- one
+// stream, consumed in order, so its scalar loads run ahead of their use.
+constexpr int adjoint_pair_stride(int K)
+{
+    return (2 * K + 1 + 15) / 16 * 16;
+}
+
+// The class-pair table: for group g, row class ci, column class cl and chunk c (k = c K + s),
+// with f_k = F_ci,k (times wframe[g][k] when wframe is set) and b_k = f_k cb[cl][k]:
+// beta_s = b_s + b_{s-1} (b outside the chunk taken as 0) and gamma_s = f_k cg[cl][k], the
+// per-lane form's products with the same rounding; layout as adjoint_pair_stride states.
+struct AdjointPairCoefArgs {
+    const float* ftab;    // [n_class][ldf]
+    const float* cb;      // [n_class][ldf]
+    const float* cg;      // [n_class][ldf]
+    const float* wframe;  // [groups][ldf] or null
+    int groups, n_class, n_chunks, K, ldf;
+    float* coef;          // [groups][n_class][n_class][n_chunks][adjoint_pair_stride(K)]
+};
+hipError_t launch_adjoint_pair_coef(const AdjointPairCoefArgs& a, hipStream_t s);
 
 struct AdjointReduceArgs {
     const float* row_partial;

@@ -1575,8 +1575,9 @@ __global__ __launch_bounds__(64, 2) void adjoint_tri_kernel(const AdjointArgs a)
     const bool row_ok = lane < jb.nrow;
     const float4 ri = row_ok ? jb.rows[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t cls_i = row_ok ? __float_as_uint(ri.w) : CLS_SKIP;
-    const uint32_t ci = cls_i == CLS_SKIP ? 0u : cls_i;
-    const float row_w = row_ok ? 1.0f : 0.0f;
+    const bool row_live = row_ok && cls_i < (uint32_t)a.n_class;
+    const uint32_t ci = row_live ? cls_i : 0u;
+    const float row_w = row_live ? 1.0f : 0.0f;
     const int k0 = chunk * K;
     const ChunkConst cc = a.chunks[chunk];
     const float* __restrict__ ftab = a.ftab;
@@ -1602,15 +1603,18 @@ __global__ __launch_bounds__(64, 2) void adjoint_tri_kernel(const AdjointArgs a)
         for (int t = 0; t < tn; ++t) {
             const int j = t0 + t;
             const float4 c = cols[j];
-            const uint32_t cl = __float_as_uint(c.w);
+            const uint32_t clr = __float_as_uint(c.w);
+            const bool col_live = clr < (uint32_t)a.n_class;  // uniform
+            const uint32_t cl = col_live ? clr : 0u;
             if (cl != cur) {
                 cur = cl;
                 const float* __restrict__ fr = ftab + __builtin_amdgcn_readfirstlane((int)cl * ldf + k0);
 #pragma unroll
                 for (int k = 0; k < K; ++k) b[k] = 2.0 * wq[k] * (double)fi[k] * (double)fr[k];
             }
-            // weight: padding rows and the lower half and diagonal of the diagonal tile give 0
-            const float w = j < jd ? (j > lane ? row_w : 0.0f) : row_w;
+            // weight: padding rows, records outside the class table and the lower half and
+            // diagonal of the diagonal tile give 0
+            const float w = col_live ? (j < jd ? (j > lane ? row_w : 0.0f) : row_w) : 0.0f;
             const float dx = ri.x - c.x, dy = ri.y - c.y, dz = ri.z - c.z;
             const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
             const PairR pr = pair_r(r2);
@@ -1681,8 +1685,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 67 ? 3 
     const bool row_ok = lane < jb.nrow;
     const float4 ri = row_ok ? jb.rows[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t cls_i = row_ok ? __float_as_uint(ri.w) : CLS_SKIP;
-    const uint32_t ci = cls_i == CLS_SKIP ? 0u : cls_i;
-    const float row_w = row_ok ? 1.0f : 0.0f;
+    const bool row_live = row_ok && cls_i < (uint32_t)a.n_class;
+    const uint32_t ci = row_live ? cls_i : 0u;
+    const float row_w = row_live ? 1.0f : 0.0f;
     const int k0 = chunk * K;
     const ChunkConst cc = a.chunks[chunk];
     const float* __restrict__ ftab = a.ftab;
@@ -1741,7 +1746,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 67 ? 3 
         for (int t = 0; t < tn;) {
             const int j = t0 + t;
             const float4 c = cols[j];
-            const uint32_t cl = __float_as_uint(c.w);
+            const uint32_t clr = __float_as_uint(c.w);
+            const bool col_live = clr < (uint32_t)a.n_class;  // uniform; a dead record adds 0
+            const uint32_t cl = col_live ? clr : 0u;
             if (cl != cur) {
                 cur = cl;
                 // b_k = F_i,k cb_k with cb_k = 2 w_k F_class,k and gamma_k = F_i,k cg_k,
@@ -1776,10 +1783,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 67 ? 3 
                 }
                 beta[K] = prev;
             }
-            const bool two = t + 1 < tn && __float_as_uint(cols[j + 1].w) == cl;
+            const bool two = t + 1 < tn && __float_as_uint(cols[j + 1].w) == clr;
             const float4 c1 = two ? cols[j + 1] : c;
-            const float w0 = j < jd ? (j > lane ? row_w : 0.0f) : row_w;
-            const float w1 = two ? (j + 1 < jd ? (j + 1 > lane ? row_w : 0.0f) : row_w) : 0.0f;
+            const float rw = col_live ? row_w : 0.0f;
+            const float w0 = j < jd ? (j > lane ? rw : 0.0f) : rw;
+            const float w1 = two ? (j + 1 < jd ? (j + 1 > lane ? rw : 0.0f) : rw) : 0.0f;
             const Col A = seed(c, w0), B = seed(c1, w1);
             float da = A.d, ua = A.u, db = B.d, ub = B.u;
             float Da = beta[0] * da, Ua = gamma[0] * ua, Db = beta[0] * db, Ub = gamma[0] * ub;
@@ -1828,6 +1836,179 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(K <= 67 ? 3 
     rowpart[0] = gx;
     rowpart[1] = gy;
     rowpart[2] = gz;
+}
+
+// The FP32 adjoint with class-pair coefficients (plain layouts).  Atoms are class-sorted, so
+// the 64 rows of a job nearly always share one class; then beta_k and gamma_k are the same
+// in every lane and come from the (row class, column class, chunk) entry of a.coef through
+// scalar loads, as the scalar operand of the vector FMAs.  No table sits in vector registers:
+// K reaches 128 (one chunk, one seed per pair at 128 q) and more waves fit a SIMD than in
+// adjoint32_tri_kernel.  A job whose rows hold several classes runs each column pair once
+// per row class present, and every lane keeps the pass of its own class.  The recurrence,
+// the coefficients (built with the same rounding by adjoint_pair_coef_kernel) and the sums
+// are those of adjoint32_tri_kernel, so both give the same bits at the same K.
+template <int K, int M>
+__global__ __launch_bounds__(64) void adjoint32p_tri_kernel(const AdjointArgs a)
+{
+    constexpr int CS = adjoint_pair_stride(K);
+    const int job_id = blockIdx.x / a.n_chunks;
+    const int chunk = blockIdx.x - job_id * a.n_chunks;
+    const PairJob jb = a.jobs[job_id];
+    const int lane = threadIdx.x;
+    const bool row_ok = lane < jb.nrow;
+    const float4 ri = row_ok ? jb.rows[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t cls_i = row_ok ? __float_as_uint(ri.w) : CLS_SKIP;
+    const uint32_t ncl = (uint32_t)a.n_class;
+    const bool row_live = row_ok && cls_i < ncl;
+    const float row_w = row_live ? 1.0f : 0.0f;
+    const uint64_t rows_live = __builtin_amdgcn_ballot_w64(row_live);
+    const ChunkConst cc = a.chunks[chunk];
+    const float h_dq = (float)a.half_dq;
+    const int group = a.coef_grouped ? __builtin_amdgcn_readfirstlane(jb.pad) : 0;
+    // entry (group, rc, cl, chunk) = coef + (((group ncl + rc) ncl + cl) n_chunks + chunk) CS
+    const size_t group_base = (size_t)group * ncl * ncl;
+
+    __shared__ float T[ADJ_TC][3][65];
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    const ConstCols cols{(cv4f*)jb.cols};
+    const int ncol = jb.ncol;
+    const int jd = jb.diag ? (ncol < 64 ? ncol : 64) : 0;
+    const size_t part = (size_t)jb.slot * a.n_chunks + chunk;
+    float* __restrict__ colpart = a.col_partial + part * (size_t)a.col_range * 3;
+    struct Col {
+        float u, d, delta, R, scale, dx, dy, dz;
+    };
+    auto seed = [&](const float4 c, float w) {
+        Col o;
+        o.dx = ri.x - c.x;
+        o.dy = ri.y - c.y;
+        o.dz = ri.z - c.z;
+        const float r2 = fmaf(o.dz, o.dz, fmaf(o.dy, o.dy, o.dx * o.dx));
+        const float nzf = r2 > 1e-30f ? 1.0f : 0.0f;
+        const PairR pr = pair_r(r2);
+        const float rinv = pr.rinv, r = pr.r;
+        const double rd = (double)r;
+        const float sh = sin_small_f32<M>(r * h_dq);                    // sin(dq r / 2)
+        const float ch = __builtin_sqrtf(fmaxf(1.0f - sh * sh, 0.0f));  // cos(dq r / 2)
+        const float fa = (float)__builtin_amdgcn_fract(cc.a_rev * rd);  // a r in revolutions
+        const float sa = __builtin_amdgcn_sinf(fa);                     // sin(a r)
+        const float clo = fmaf(__builtin_amdgcn_cosf(fa), ch, sa * sh);  // cos((a - dq/2) r)
+        o.delta = 4.0f * sh * sh;
+        o.R = nzf * r * __builtin_amdgcn_rcpf(fmaxf(4.0f * sh * ch, 1e-30f));
+        o.u = sa * rinv;
+        o.d = (clo + clo) * sh * rinv;
+        o.scale = rinv * rinv * w;
+        return o;
+    };
+    for (int t0 = 0; t0 < ncol; t0 += ADJ_TC) {
+        const int tn = ncol - t0 < ADJ_TC ? ncol - t0 : ADJ_TC;
+#pragma unroll 1
+        for (int t = 0; t < tn;) {
+            const int j = t0 + t;
+            const float4 c = cols[j];
+            const uint32_t clr = __float_as_uint(c.w);
+            const bool col_live = clr < ncl;  // uniform; a dead record adds 0
+            const uint32_t cl = col_live ? clr : 0u;
+            const bool two = t + 1 < tn && __float_as_uint(cols[j + 1].w) == clr;
+            const float4 c1 = two ? cols[j + 1] : c;
+            const float rw = col_live ? row_w : 0.0f;
+            const float w0 = j < jd ? (j > lane ? rw : 0.0f) : rw;
+            const float w1 = two ? (j + 1 < jd ? (j + 1 > lane ? rw : 0.0f) : rw) : 0.0f;
+            const Col A = seed(c, w0), B = seed(c1, w1);
+            float gsa = 0.0f, gsb = 0.0f;
+            // one pass per row class present (uniform loop; nearly always one)
+            uint64_t left = rows_live;
+#pragma unroll 1
+            while (left) {
+                const int leader = (int)__builtin_ctzll(left);
+                const uint32_t rc = (uint32_t)__builtin_amdgcn_readlane((int)cls_i, leader);
+                const bool mine = cls_i == rc;
+                left &= ~__builtin_amdgcn_ballot_w64(mine);
+                const size_t entry = ((group_base + (size_t)rc * ncl + cl) * a.n_chunks + chunk) * CS;
+                cfloat* cp = (cfloat*)(a.coef + entry);  // beta_k at 2 k, gamma_k at 2 k + 1
+                float da = A.d, ua = A.u, db = B.d, ub = B.u;
+                // the recurrence runs inside each pass: held over from one pass to the next
+                // it would take 4 K registers
+                asm volatile("" : "+v"(da), "+v"(ua), "+v"(db), "+v"(ub));
+                float Da = cp[0] * da, Ua = cp[1] * ua, Db = cp[0] * db, Ub = cp[1] * ub;
+#pragma unroll
+                for (int k = 1; k < K; ++k) {
+                    da = fmaf(-A.delta, ua, da);
+                    db = fmaf(-B.delta, ub, db);
+                    ua += da;
+                    ub += db;
+                    Da = fmaf(cp[2 * k], da, Da);
+                    Db = fmaf(cp[2 * k], db, Db);
+                    Ua = fmaf(cp[2 * k + 1], ua, Ua);
+                    Ub = fmaf(cp[2 * k + 1], ub, Ub);
+                }
+                da = fmaf(-A.delta, ua, da);
+                db = fmaf(-B.delta, ub, db);
+                Da = fmaf(cp[2 * K], da, Da);
+                Db = fmaf(cp[2 * K], db, Db);
+                const float pa = fmaf(A.R, Da, -Ua) * A.scale, pb = fmaf(B.R, Db, -Ub) * B.scale;
+                gsa = mine ? pa : gsa;
+                gsb = mine ? pb : gsb;
+            }
+            const float eax = gsa * A.dx, eay = gsa * A.dy, eaz = gsa * A.dz;
+            const float ebx = gsb * B.dx, eby = gsb * B.dy, ebz = gsb * B.dz;
+            gx += eax + ebx;
+            gy += eay + eby;
+            gz += eaz + ebz;
+            T[t][0][lane] = -eax;
+            T[t][1][lane] = -eay;
+            T[t][2][lane] = -eaz;
+            if (two) {  // uniform
+                T[t + 1][0][lane] = -ebx;
+                T[t + 1][1][lane] = -eby;
+                T[t + 1][2][lane] = -ebz;
+            }
+            t += two ? 2 : 1;
+        }
+        __syncthreads();
+        if (lane < 3 * tn) {
+            const int t = lane / 3, comp = lane - 3 * (lane / 3);
+            float s = 0.0f;
+#pragma unroll 16
+            for (int l = 0; l < 64; ++l) s += T[t][comp][l];
+            colpart[(size_t)(t0 + t) * 3 + comp] = s;
+        }
+        __syncthreads();
+    }
+    float* __restrict__ rowpart = a.row_partial + (part * 64 + lane) * 3;
+    rowpart[0] = gx;
+    rowpart[1] = gy;
+    rowpart[2] = gz;
+}
+
+// the class-pair table of adjoint32p_tri_kernel (AdjointPairCoefArgs): one thread per float
+__global__ void adjoint_pair_coef_kernel(const AdjointPairCoefArgs a)
+{
+    const int K = a.K, cs = adjoint_pair_stride(K);
+    const size_t total = (size_t)a.groups * a.n_class * a.n_class * a.n_chunks * cs;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int s = (int)(t % cs);
+    size_t rest = t / cs;
+    const int chunk = (int)(rest % a.n_chunks);
+    rest /= a.n_chunks;
+    const int cl = (int)(rest % a.n_class);
+    rest /= a.n_class;
+    const int ci = (int)(rest % a.n_class);
+    const int g = (int)(rest / a.n_class);
+    const int k0 = chunk * K;
+    auto fk = [&](int k) {
+        const float f = a.ftab[(size_t)ci * a.ldf + k0 + k];
+        return a.wframe ? f * a.wframe[(size_t)g * a.ldf + k0 + k] : f;
+    };
+    auto bk = [&](int k) { return k >= 0 && k < K ? fk(k) * a.cb[(size_t)cl * a.ldf + k0 + k] : 0.0f; };
+    float v = 0.0f;  // beta_k at 2 k (k <= K), gamma_k at 2 k + 1 (k < K), 0 in the padding
+    const int k = s >> 1;
+    if ((s & 1) == 0 && k <= K)
+        v = bk(k) + bk(k - 1);
+    else if ((s & 1) == 1 && k < K)
+        v = fk(k) * a.cg[(size_t)cl * a.ldf + k0 + k];
+    a.coef[t] = v;
 }
 
 // per-class adjoint coefficients of the FP32 kernel: cb[c][k] = 2 w_k F_c,k and
@@ -1945,6 +2126,7 @@ constexpr int REC32R_KS[] = {64, 86, 96, 100, 128};
 constexpr int DIRECT_KS[] = {8, 16, 32, 48, 64};
 constexpr int ADJOINT_KS[] = {8, 16, 32, 48, 64};
 constexpr int ADJOINT32_KS[] = {16, 32, 40, 50, 64, 67, 100};
+constexpr int ADJOINT32P_KS[] = {64, 100, 128, 200};
 
 }  // namespace
 
@@ -1972,6 +2154,16 @@ int adjoint_k_count(bool fp32)
 int adjoint_k_value(bool fp32, int i)
 {
     return fp32 ? ADJOINT32_KS[i] : ADJOINT_KS[i];
+}
+
+int adjoint_pair_k_count()
+{
+    return (int)(sizeof(ADJOINT32P_KS) / sizeof(int));
+}
+
+int adjoint_pair_k_value(int i)
+{
+    return ADJOINT32P_KS[i];
 }
 
 namespace {
@@ -2590,7 +2782,27 @@ void launch_adjoint32_k(const AdjointArgs& a, int n_jobs, hipStream_t s)
         hipLaunchKernelGGL((adjoint32_tri_kernel<K, SIN_MAX_TERMS>), grid, block, 0, s, a);
 }
 
+template <int K>
+void launch_adjoint32p_k(const AdjointArgs& a, int n_jobs, hipStream_t s)
+{
+    const dim3 grid((unsigned)(n_jobs * a.n_chunks)), block(64);
+    if (a.sin_terms <= SIN_SHORT)
+        hipLaunchKernelGGL((adjoint32p_tri_kernel<K, SIN_SHORT>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((adjoint32p_tri_kernel<K, SIN_MAX_TERMS>), grid, block, 0, s, a);
+}
+
 }  // namespace
+
+hipError_t launch_adjoint_pair_coef(const AdjointPairCoefArgs& a, hipStream_t s)
+{
+    if (a.K <= 0 || a.n_chunks <= 0 || a.ldf < a.n_chunks * a.K) return hipErrorInvalidValue;
+    const size_t total = (size_t)a.groups * a.n_class * a.n_class * a.n_chunks * adjoint_pair_stride(a.K);
+    if (total == 0) return hipSuccess;
+    if ((total + 255) / 256 > 0x7fffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(adjoint_pair_coef_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_adjoint_coef(const float* ftab, const double* wq, const double* invq, int n_class, int ldf, float* cb,
                                float* cg, hipStream_t s)
@@ -2605,7 +2817,19 @@ hipError_t launch_adjoint_coef(const float* ftab, const double* wq, const double
 hipError_t launch_adjoint(const AdjointArgs& a, int n_jobs, int K, hipStream_t s)
 {
     if (n_jobs <= 0) return hipSuccess;
+    if (a.n_class <= 0) return hipErrorInvalidValue;         // every record would count as outside the table
     if (a.wframe && !a.fp32) return hipErrorInvalidValue;  // per-frame weights: FP32 kernel only
+    if (a.coef) {  // class-pair coefficients: FP32 only
+        if (!a.fp32) return hipErrorInvalidValue;
+        switch (K) {
+        case 64: launch_adjoint32p_k<64>(a, n_jobs, s); break;
+        case 100: launch_adjoint32p_k<100>(a, n_jobs, s); break;
+        case 128: launch_adjoint32p_k<128>(a, n_jobs, s); break;
+        case 200: launch_adjoint32p_k<200>(a, n_jobs, s); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     if (a.fp32) {
         switch (K) {
         case 16: launch_adjoint32_k<16>(a, n_jobs, s); break;

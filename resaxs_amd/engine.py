@@ -267,10 +267,19 @@ class Engine:
         self._check(self._L.rsx_time_forward(self.h, int(reps), ctypes.byref(tot), ctypes.byref(pair)))
         return tot.value, pair.value
 
-    def set_adjoint_tuning(self, fp32=True, k=0, col_range=0):
-        """fp32: FP32 difference-form adjoint kernel (default) or FP64 recurrence; k q per
-        thread (0 = automatic); col_range columns per job (0 = 256)."""
-        self._check(self._L.rsx_set_adjoint_tuning(self.h, int(bool(fp32)), int(k), int(col_range)))
+    def set_adjoint_tuning(self, fp32=True, k=0, col_range=0, per_lane=False):
+        """fp32: FP32 difference-form adjoint kernel (default) or FP64 recurrence; per_lane:
+        the FP32 kernel's per-lane coefficient form even where the class-pair form applies
+        (plain layouts); k q per thread (0 = automatic); col_range columns per job (0 = automatic)."""
+        mode = (2 if per_lane else 1) if fp32 else 0
+        self._check(self._L.rsx_set_adjoint_tuning(self.h, mode, int(k), int(col_range)))
+
+    def adjoint_layout(self, batch=False):
+        """The last gradient call's adjoint layout: form ("fp64", "pair" or "per_lane"), k, col_range."""
+        f, k, L = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self._L.rsx_get_adjoint_layout(self.h, int(bool(batch)), ctypes.byref(f), ctypes.byref(k),
+                                                   ctypes.byref(L)))
+        return {"form": ("fp64", "pair", "per_lane")[f.value], "k": k.value, "col_range": L.value}
 
     def time_adjoint(self, reps=5):
         ms = ctypes.c_float()

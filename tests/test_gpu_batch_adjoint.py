@@ -302,10 +302,10 @@ def _fit_case(seed=41, n=1200, nq=80):
     return rng, q, pos, types, table
 
 
-@pytest.mark.parametrize("fp32", [True, False])
-def test_adjoint_vs_analytic_and_fd(oracle, fp32):
+@pytest.mark.parametrize("form", ["pair", "per_lane", "fp64"])
+def test_adjoint_vs_analytic_and_fd(oracle, form):
     from resaxs_amd import configs
-    _say(f"case: chi2 gradient, 1200 atoms x 80 q, {'FP32' if fp32 else 'FP64'} adjoint")
+    _say(f"case: chi2 gradient, 1200 atoms x 80 q, {form} adjoint")
     rng, q, pos, types, table = _fit_case()
     F = oracle.body_factors(pos, types, q, table)
     Iref = oracle.debye(q, pos, F)
@@ -313,8 +313,9 @@ def test_adjoint_vs_analytic_and_fd(oracle, fp32):
     sigma = (0.01 * Iref).astype(np.float32)
     start = (pos + rng.normal(scale=0.5, size=pos.shape)).astype(np.float32)
     e = engine_for(q, table, n=len(pos))
-    e.set_adjoint_tuning(fp32)
+    e.set_adjoint_tuning(form != "fp64", per_lane=form == "per_lane")
     chi2, scale, grad, iq = e.chi2_grad(configs.bodies_of(start, types), i_exp, sigma)
+    assert e.adjoint_layout()["form"] == form
     xyz = start.astype(np.float64)
     Fs = oracle.body_factors(start, types, q, table)
     g_ref, I_ref, c_ref, x2_ref = oracle.chi2_grad(q, xyz, Fs, i_exp, sigma)
@@ -384,7 +385,17 @@ def test_batch_vjp_matches_single_structure(oracle, hydrate):
     e.batch_set_frames(bodies, offsets)
     g = e.batch_vjp(w)
     assert g.shape == (int(offsets[-1]), 3)
+    lay = e.adjoint_layout(batch=True)
+    assert lay["form"] == ("per_lane" if hydrate else "pair")
     assert e.time_batch_vjp(2) > 0.0
+    if not hydrate:
+        # the per-lane form at the same K: the same products and sums
+        e.set_adjoint_tuning(True, lay["k"], 0, per_lane=True)
+        g_lane = e.batch_vjp(w)
+        assert e.adjoint_layout(batch=True)["form"] == "per_lane"
+        d = np.linalg.norm(g.astype(np.float64) - g_lane) / np.linalg.norm(g_lane.astype(np.float64))
+        _say(f"  class-pair vs per-lane form, K={lay['k']}: {d:.2e}")
+        assert d < 1e-6
     e.close()
     for f, m in enumerate(sizes):
         if m == 0:
@@ -408,3 +419,53 @@ def test_batch_vjp_matches_single_structure(oracle, hydrate):
         g2 = g[offsets[2]:offsets[3]].astype(np.float64)
         assert np.allclose(g2[0], coef * d, rtol=1e-4, atol=1e-6 * abs(coef) * r)
         assert np.allclose(g2[1], -coef * d, rtol=1e-4, atol=1e-6 * abs(coef) * r)
+
+
+@pytest.mark.parametrize("n_types", [5, 40])
+def test_adjoint_class_pair_form(oracle, n_types):
+    """The FP32 adjoint's class-pair coefficient form (plain layouts; coefficients through
+    scalar loads) against its per-lane form at the same K -- the same products, recurrences
+    and sums, so the same gradient up to the rounding that contractions may move -- and at the
+    automatic K = 128 (one chunk at 128 q) against the oracle's FP64 chi^2 gradient.  With 40
+    types of ~12 atoms every 64-row block holds many classes and runs once per class."""
+    from resaxs_amd import configs
+    rng = np.random.default_rng(90 + n_types)
+    q = np.linspace(0.01, 0.5, 128).astype(np.float32)
+    if n_types == 5:
+        n = 1500
+        types = configs.seeded_types(rng, n)
+        table = configs.factor_table(rng, q)
+    else:
+        n = 480
+        types = rng.integers(0, n_types, size=n).astype(np.uint32)
+        a = rng.uniform(2.0, 10.0, size=n_types)
+        b = rng.uniform(0.5, 5.0, size=n_types)
+        table = (a[None, :] * np.exp(-b[None, :] * q.astype(np.float64)[:, None] ** 2)).astype(np.float32)
+    pos = configs.in_ellipsoid(rng, n, configs.ellipsoid_axes(n)).astype(np.float32)
+    Iref = oracle.debye(q, pos, oracle.body_factors(pos, types, q, table))
+    i_exp = (Iref * (1.0 + 0.01 * rng.normal(size=len(q)))).astype(np.float32)
+    sigma = (0.01 * Iref).astype(np.float32)
+    start = (pos + rng.normal(scale=0.5, size=pos.shape)).astype(np.float32)
+    bodies = configs.bodies_of(start, types)
+    _say(f"case: class-pair adjoint form, {n} atoms of {n_types} types x {len(q)} q")
+    e = engine_for(q, table, n=n)
+    got = {}
+    for k, per_lane in ((64, False), (64, True), (100, False), (100, True), (0, False)):
+        e.set_adjoint_tuning(True, k, 0, per_lane)
+        _, _, g, _ = e.chi2_grad(bodies, i_exp, sigma)
+        lay = e.adjoint_layout()
+        assert lay["form"] == ("per_lane" if per_lane else "pair")
+        got[(k, per_lane)] = (g.astype(np.float64), lay["k"])
+    e.close()
+    for k in (64, 100):
+        gp, gl = got[(k, False)][0], got[(k, True)][0]
+        d = np.linalg.norm(gp - gl) / np.linalg.norm(gl)
+        _say(f"  K={k}: |g_pair - g_per_lane| / |g| = {d:.2e} (bit-equal: {bool(np.array_equal(gp, gl))})")
+        assert d < 1e-6
+    g_auto, k_auto = got[(0, False)]
+    assert k_auto == 128
+    Fs = oracle.body_factors(start, types, q, table)
+    g_ref = oracle.chi2_grad(q, start.astype(np.float64), Fs, i_exp, sigma)[0]
+    d = np.linalg.norm(g_auto - g_ref) / np.linalg.norm(g_ref)
+    _say(f"  K=128 vs the oracle: {d:.2e}")
+    assert d < 1e-4

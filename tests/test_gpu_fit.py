@@ -165,6 +165,47 @@ def test_device_gradients_equal_host(oracle):
     e.close()
 
 
+def test_device_vjp_type_outside_table():
+    """A device-resident VJP on bodies that hold a type outside the factor table (the gather
+    gives its record no class): every adjoint form reads no table entry for it and adds nothing
+    for its pairs, rsx_synchronize reports the type, and the next clean call gives the host
+    gradient again."""
+    from resaxs_amd import RsxError, configs
+    rng = np.random.default_rng(82)
+    n = 600
+    q = np.linspace(0.01, 0.5, 48).astype(np.float32)
+    pos = configs.in_ellipsoid(rng, n, configs.ellipsoid_axes(n)).astype(np.float32)
+    types = configs.seeded_types(rng, n)
+    table = configs.factor_table(rng, q)
+    e = engine_for(q, table, n=n)
+    b = configs.bodies_of(pos, types)
+    bad = b.copy()
+    bad[7, 3] = np.array([table.shape[1] + 3], dtype=np.uint32).view(np.float32)[0]
+    w = np.linspace(-1.0, 1.0, len(q))
+    for form in ("pair", "per_lane", "fp64"):
+        _say(f"case: device-resident VJP with a type outside the table, {n} atoms x {len(q)} q, {form}")
+        e.set_adjoint_tuning(form != "fp64", 0, 0, form == "per_lane")
+        g_host = e.vjp(b, w)
+        assert e.adjoint_layout()["form"] == form
+        pb, pw, pg = e.dev_alloc(b.nbytes), e.dev_alloc(w.nbytes), e.dev_alloc(n * 12)
+        try:
+            e.h2d(pw, w)
+            e.h2d(pb, bad)
+            e.vjp_dev_async(pb, n, pw, pg)
+            with pytest.raises(RsxError) as ei:
+                e.synchronize()
+            assert ei.value.code == -3 and "type" in str(ei.value)
+            e.h2d(pb, b)
+            e.vjp_dev_async(pb, n, pw, pg)
+            e.synchronize()
+            g = e.d2h(np.zeros((n, 3), np.float32), pg)
+            assert np.array_equal(g, g_host)
+        finally:
+            for p in (pb, pw, pg):
+                e.dev_free(p)
+    e.close()
+
+
 def test_autograd_wrappers_on_device():
     """The torch wrappers with positions on the GPU (device-resident entry points, no host
     copies after the first layout) against the same wrappers on the CPU, and a short
