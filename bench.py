@@ -283,10 +283,15 @@ def adjoint_bench(reps=3):
     fit = e.get_tuning()
     fit_kernel = f"variant {fit['variant']} K={fit['k']}"
     ms_adj = e.time_adjoint(5)
+    lay = e.adjoint_layout()
     e.close()
     terms = n * (n - 1) / 2.0 * nq
+    w_adj = n * (n - 1) / 2.0 * (8 * nq + 19)  # adjoint FLOP: 8 per pair.q term + 19 per pair
     return {"workload": w.name, "ms_fwd_fit": round(ms_fwd_fit, 4),
             "fwd_fit_kernel": fit_kernel, "ms_adjoint": round(ms_adj, 4),
+            "adjoint_kernel": f"{lay['form']} form K={lay['k']} L={lay['col_range']}",
+            "adjoint_tflops": round(w_adj / (ms_adj * 1e-3) / 1e12, 2),
+            "adjoint_frac": round(w_adj / (ms_adj * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
             "ratio_fwd_adjoint_to_fwd": round((ms_fwd_fit + ms_adj) / ms_fwd_fit, 3),
             "ms_fwd_fastest_fp32": round(ms_fwd_default, 4),
             "fwd_fastest_kernel": f"variant {default['variant']} K={default['k']}",
@@ -387,7 +392,14 @@ def batched_bench(reps=5, adjoint=True):
         e.batch_vjp(wts)
         vjp_first_s = time.perf_counter() - t0
         ms_adj = e.time_batch_vjp(3)
+        pairs = float(sum(s * (s - 1) / 2.0 for s in w.sizes))
+        w_adj = pairs * (8 * nq + 19)  # adjoint FLOP: 8 per pair.q term + 19 per pair
+        lay = e.adjoint_layout(batch=True)
         adj = {"ms_adjoint": round(ms_adj, 3), "ratio_fwd_adjoint_to_fwd": round((ms + ms_adj) / ms, 3),
+               "terms_per_s_fwd_adjoint": pairs * nq / ((ms + ms_adj) * 1e-3),
+               "adjoint_tflops": round(w_adj / (ms_adj * 1e-3) / 1e12, 2),
+               "adjoint_frac": round(w_adj / (ms_adj * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+               "kernel": f"{lay['form']} form K={lay['k']} L={lay['col_range']}",
                "first_call_s": round(vjp_first_s, 3),
                "timing": "HIP events on the engine stream around the adjoint kernel + per-atom reduce "
                          "(weights resident); first_call_s is the host wall clock of the first rsx_batch_vjp "

@@ -207,6 +207,7 @@ class Engine:
                                                  len(off) - 1))
         self.n_frames = len(off) - 1
         self.n_batch_atoms = int(off[-1])
+        self.batch_gen = getattr(self, "batch_gen", 0) + 1  # which frames the batch holds
 
     def set_persistent(self, on):
         """Batched launch: persistent work-queue grid (True, default) or one wave per item."""

@@ -2,6 +2,10 @@
 
 DebyeIntensity: positions [N, 3] -> I(q) [n_q] (FP64); backward is the adjoint
 kernel's vector-Jacobian product sum_k w_k dI_k/dx_i.
+EnsembleIntensity: the positions of every frame of an ensemble, concatenated, and the
+frame offsets -> per-frame I(q) [n_frames, n_q] (FP64) from one batched launch; backward
+is the batched VJP (one adjoint launch over all frames).  The frames are laid out from a
+host copy of the positions on every forward call.
 Chi2Loss: positions -> chi^2 at the least-squares scale (saxs_profile.hpp:137-166
 scale, chi^2 = sum (I_exp - c I)^2 / sigma^2); forward and adjoint run in one
 fused call and backward scales the stored gradient.  The scale is held fixed
@@ -151,6 +155,36 @@ class Chi2Loss(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g):
         return ctx.grad * g.to(ctx.grad.dtype), None, None, None, None
+
+
+class EnsembleIntensity(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pos, engine, types, offsets):
+        bodies = make_bodies(_host(pos), types)
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        engine.batch_set_frames(bodies, off)
+        frames, _ = engine.batch_calc()
+        ctx.engine, ctx.bodies, ctx.offsets = engine, bodies, off
+        ctx.gen = engine.batch_gen
+        ctx.pos_meta = (pos.device, pos.dtype)
+        return torch.from_numpy(frames).to(pos.device)
+
+    @staticmethod
+    def backward(ctx, grad_frames):
+        engine = ctx.engine
+        if engine.batch_gen != ctx.gen:  # the engine's batch holds other frames now
+            engine.batch_set_frames(ctx.bodies, ctx.offsets)
+            ctx.gen = engine.batch_gen
+        w = grad_frames.detach().to("cpu", torch.float64).contiguous().numpy()
+        g = engine.batch_vjp(w)
+        dev, dt = ctx.pos_meta
+        return torch.from_numpy(g).to(dev, dt), None, None, None
+
+
+def ensemble_intensity(pos, engine, types, offsets):
+    """Per-frame I(q) [n_frames, n_q] of an ensemble whose frame f holds the atoms
+    offsets[f]..offsets[f+1] of pos [total, 3] (differentiable)."""
+    return EnsembleIntensity.apply(pos, engine, types, offsets)
 
 
 def debye_intensity(pos, engine, types):

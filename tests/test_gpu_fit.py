@@ -94,6 +94,55 @@ def test_autograd_wrappers(oracle):
     assert np.linalg.norm(grad - g_ref) / np.linalg.norm(g_ref) < 1e-4
 
 
+def test_ensemble_autograd(oracle):
+    """ensemble_intensity: per-frame I(q) of a ragged ensemble (an empty frame included) from
+    one batched launch against the oracle; backward is the batched VJP, equal to
+    Engine.batch_vjp and, frame by frame, to the single-structure VJP; a backward that runs
+    after another ensemble's forward on the same engine gets its own frames' gradient."""
+    import torch
+
+    from resaxs_amd import configs
+    from resaxs_amd.torch_debye import ensemble_intensity
+    rng = np.random.default_rng(9)
+    q = np.linspace(0.01, 0.5, 48).astype(np.float32)
+    table = configs.factor_table(rng, q)
+    sizes = [300, 0, 500, 120]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    pos = np.concatenate([configs.in_ellipsoid(rng, m, configs.ellipsoid_axes(m)) if m else np.zeros((0, 3))
+                          for m in sizes]).astype(np.float32)
+    types = np.concatenate([configs.seeded_types(rng, m) for m in sizes]).astype(np.uint32)
+    _say(f"case: ensemble autograd, frames {sizes} x {len(q)} q")
+    e = engine_for(q, table)
+    W = rng.normal(size=(len(sizes), len(q)))
+    x = torch.tensor(pos, requires_grad=True)
+    I = ensemble_intensity(x, e, types, off)
+    assert I.shape == (len(sizes), len(q))
+    for f, m in enumerate(sizes):
+        a, b = off[f], off[f + 1]
+        if m == 0:
+            assert np.all(I[f].detach().numpy() == 0.0)
+            continue
+        ref = oracle.debye(q, pos[a:b], oracle.body_factors(pos[a:b], types[a:b], q, table))
+        assert rel_err(I[f].detach().numpy(), ref).max() < 1e-5, f
+    # another ensemble's forward on the same engine before this backward
+    y = torch.tensor(pos * np.float32(0.97), requires_grad=True)
+    ensemble_intensity(y, e, types, off)
+    (I * torch.as_tensor(W)).sum().backward()
+    e.batch_set_frames(configs.bodies_of(pos, types), off)
+    assert np.array_equal(x.grad.numpy(), e.batch_vjp(W))
+    for f, m in enumerate(sizes):
+        if m < 2:
+            continue
+        a, b = off[f], off[f + 1]
+        s = engine_for(q, table, n=m)
+        g1 = s.vjp(configs.bodies_of(pos[a:b], types[a:b]), W[f]).astype(np.float64)
+        s.close()
+        d = np.linalg.norm(x.grad.numpy()[a:b] - g1) / np.linalg.norm(g1)
+        _say(f"  frame {f} ({m} atoms): |g - g_single| / |g_single| = {d:.2e}")
+        assert d < 1e-5
+    e.close()
+
+
 def test_refinement_100_steps():
     from resaxs_amd import configs
     from resaxs_amd.torch_debye import refine
