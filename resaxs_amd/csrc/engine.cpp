@@ -1455,6 +1455,15 @@ std::vector<PairJob> forward_jobs(rsx_engine* h)
     return jobs;
 }
 
+// The slot range {0, n jobs} the device-resident forward reduces over: written with every
+// forward job table that reaches d_fjobs (full_forward, retune), so it always names that table.
+void set_forward_slots(rsx_engine* h, int nj)
+{
+    int sbf[2] = {0, nj};
+    h->d_slot_begin_f.ensure(2);
+    hipchk(hipMemcpy(h->d_slot_begin_f.p, sbf, sizeof(sbf), hipMemcpyHostToDevice), "upload forward slots");
+}
+
 // The full forward into d_iq.  The job table is built (host triangle tiling, cost order,
 // upload) only when the layout's key changed: a re-baseline of the incremental path repeats
 // the forward on the same columns, whose table is still on the device.
@@ -1466,9 +1475,7 @@ void full_forward(rsx_engine* h)
         const std::vector<PairJob> jobs = forward_jobs(h);
         run_jobs(h, h->d_fjobs, jobs, h->d_iq.p, 0, sf * sf);
         h->last_jobs = jobs;
-        int sbf[2] = {0, (int)jobs.size()};
-        h->d_slot_begin_f.ensure(2);
-        hipchk(hipMemcpy(h->d_slot_begin_f.p, sbf, sizeof(sbf), hipMemcpyHostToDevice), "upload forward slots");
+        set_forward_slots(h, (int)jobs.size());
     } else {
         h->last_L = L;
         launch_jobs(h, h->d_fjobs.p, (int)h->last_jobs.size(), h->d_iq.p, 0, sf * sf);
@@ -3072,6 +3079,7 @@ void enqueue_device_forward(rsx_engine* h, const float4* bodies_dev, double* out
     enqueue_device_columns(h, bodies_dev);
     hipchk(rsx::launch_q0(h->d_cols.p, n, h->d_f0col.p, h->d_q0.p, h->stream), "q0 kernel");
     const int nj = (int)h->last_jobs.size();
+    h->d_red.ensure(std::max<size_t>(1, rsx::reduce_scratch_doubles(h->nq, 1, nj)));  // sized for this table
     const rsx::DebyeArgs a = debye_args(h, h->d_fjobs.p, h->d_partials.p);
     hipchk(rsx::launch_debye(a, nj, h->K, h->uniform, h->stream), "debye kernel");
     hipchk(rsx::launch_reduce(h->d_partials.p, h->ldf, h->nq, h->d_slot_begin_f.p, 1, out, h->nq, 0, h->d_q0.p,
@@ -3110,6 +3118,7 @@ void retune(rsx_engine* h, int K, int L)
     h->d_partials.ensure((size_t)std::max(nj, 1) * h->ldf);
     hipchk(hipMemcpy(h->d_fjobs.p, jobs.data(), nj * sizeof(PairJob), hipMemcpyHostToDevice), "upload jobs");
     h->last_jobs = jobs;
+    set_forward_slots(h, nj);
 }
 
 float time_pair_kernel(rsx_engine* h, int reps)
@@ -3996,6 +4005,7 @@ int rsx_time_forward(rsx_engine* h, int reps, float* ms_total, float* ms_pair)
         if (reps <= 0) fail(RSX_INVALID_ARG, "reps must be > 0");
         set_device(h);
         const int nj = (int)h->last_jobs.size();
+        h->d_red.ensure(std::max<size_t>(1, rsx::reduce_scratch_doubles(h->nq, 1, nj)));
         hipchk(hipMemcpy(h->d_q0.p, &h->last_q0, sizeof(double), hipMemcpyHostToDevice), "upload q0");
         const rsx::DebyeArgs a = debye_args(h, h->d_fjobs.p, h->d_partials.p);
         float pair_ms = 0.0f;

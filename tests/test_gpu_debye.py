@@ -341,6 +341,34 @@ def _dev_forward(e, bodies):
 
 
 @pytest.mark.parametrize("hydrate", [False, True])
+def test_device_forward_after_retune(tmp_path, hydrate):
+    """A device-resident forward after rsx_autotune has switched the column range, so that the
+    forward job table changes size, equals the host result on the same positions: the device
+    path reduces over the new table's slots.  Cache entries force 64, 1024 and again 64
+    columns per job (more jobs, fewer, more)."""
+    from resaxs_amd import configs
+    w = configs.sasa_ellipsoid_20000(n=3000)
+    wt = w.w if hydrate else 0.0
+    _say(f"case: device-resident forward after retuning, 3000 x 256, hydrated={hydrate}")
+    e = engine_for(w.q, w.table, w.water_index, wt, n=len(w.pos), radii=w.radii, probe=w.probe, n_half=w.n_half)
+    e.calc_curve(w.bodies)
+    probe_cache = tmp_path / "probe.txt"
+    e.autotune(str(probe_cache))
+    key = probe_cache.read_text().split()[0]
+    for L in (64, 1024, 64):
+        forced = tmp_path / f"forced_{L}.txt"
+        forced.write_text(f"{key} 128 {L} 3 1.0\n")
+        e.autotune(str(forced))
+        assert e.get_tuning()["col_range"] == L
+        I_host = e.calc_curve(w.bodies)
+        same = _dev_forward(e, w.bodies)
+        d = float(np.max(np.abs(same - I_host) / np.abs(I_host)))
+        _say(f"  {L} columns per job: device vs host {d:.2e}")
+        assert d <= 1e-12, L
+    e.close()
+
+
+@pytest.mark.parametrize("hydrate", [False, True])
 def test_device_resident_forward_on_new_positions(oracle, hydrate):
     """rsx_calc_curve_dev_async on positions other than the ones the host calc_curve laid
     out (every atom jittered by up to 0.3 A, types unchanged): the slot order and jobs are
